@@ -1,0 +1,433 @@
+"""ctypes access to the three libraries the tests compare.
+
+* oracle  -- oracle/liboracle.so, the CPU restatement (test infrastructure).
+* ref     -- oracle/_ref/libhprqp_ref.so, the reference CPU path compiled from
+             its own sources (present only where it was built).
+* gen     -- hpr-qp-c_amd/libhprqp_gen.so, the seeded workload generators.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+from dataclasses import dataclass, field
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[1]
+PKG = ROOT / "hpr-qp-c_amd"
+
+c_size_p = C.POINTER(C.c_size_t)
+c_double_p = C.POINTER(C.c_double)
+
+STATUS = {0: "OK", 1: "INVALID_ARGUMENT", 2: "ALLOCATION_FAILED", 3: "NOT_IMPLEMENTED",
+          4: "MAX_ITER", 5: "TIME_LIMIT", 6: "OPTIMAL", 7: "CONTINUE"}
+
+
+def _sz(a):
+    a = np.ascontiguousarray(a, dtype=np.uint64)
+    return a, a.ctypes.data_as(c_size_p)
+
+
+def _db(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(c_double_p)
+
+
+@dataclass
+class QP:
+    """A QP in the reference's build_from_QAbc form (CSR, size_t indices)."""
+    n: int
+    m: int
+    q_ptr: np.ndarray
+    q_ind: np.ndarray
+    q_val: np.ndarray
+    a_ptr: np.ndarray
+    a_ind: np.ndarray
+    a_val: np.ndarray
+    c: np.ndarray
+    AL: np.ndarray
+    AU: np.ndarray
+    l: np.ndarray
+    u: np.ndarray
+    obj_constant: float = 0.0
+    var_offsets: np.ndarray | None = None
+    row_offsets: np.ndarray | None = None
+    soc_rhs: np.ndarray | None = None
+    x0: np.ndarray | None = None
+
+    @property
+    def num_var_cones(self):
+        return 0 if self.var_offsets is None else len(self.var_offsets) - 1
+
+    @property
+    def num_row_cones(self):
+        return 0 if self.row_offsets is None else len(self.row_offsets) - 1
+
+    def dense_Q(self):
+        Q = np.zeros((self.n, self.n))
+        for r in range(self.n):
+            for k in range(self.q_ptr[r], self.q_ptr[r + 1]):
+                Q[r, self.q_ind[k]] += self.q_val[k]
+        return Q
+
+    def dense_A(self):
+        A = np.zeros((self.m, self.n))
+        for r in range(self.m):
+            for k in range(self.a_ptr[r], self.a_ptr[r + 1]):
+                A[r, self.a_ind[k]] += self.a_val[k]
+        return A
+
+
+def qp_from_dense(Q, A, c, AL, AU, l, u, obj_constant=0.0):
+    Q = np.atleast_2d(np.asarray(Q, dtype=float))
+    n = Q.shape[1]
+    A = np.asarray(A, dtype=float).reshape(-1, n)
+
+    def csr(M):
+        ptr, ind, val = [0], [], []
+        for row in M:
+            for j, v in enumerate(row):
+                if v != 0.0:
+                    ind.append(j)
+                    val.append(v)
+            ptr.append(len(ind))
+        return (np.array(ptr, dtype=np.uint64), np.array(ind, dtype=np.uint64), np.array(val, dtype=float))
+
+    qp, qi, qv = csr(Q)
+    ap, ai, av = csr(A)
+    return QP(n, A.shape[0], qp, qi, qv, ap, ai, av, np.asarray(c, float), np.asarray(AL, float),
+              np.asarray(AU, float), np.asarray(l, float), np.asarray(u, float), obj_constant)
+
+
+def reference_fixtures():
+    """The reference's 12 unit-test models (tests/test_support.c:79-270)."""
+    inf = math.inf
+    F = {}
+    F["STANDARD"] = qp_from_dense([[2, 0], [0, 4]], [[1, 1]], [2, 1], [2], [2], [0, 0], [10, 10])
+    F["STANDARD_NOC"] = qp_from_dense([[2, 0], [0, 4]], [[1, 1]], [2, 1], [2], [2], [-inf, -inf], [inf, inf])
+    F["SCALE"] = qp_from_dense([[4, 0], [0, 9]], [[16, 25]], [2, 4], [2], [6], [0, -1], [10, 3])
+    F["RESIDUALS"] = qp_from_dense([[2, 0], [0, 4]], [[1, 2]], [1, -1], [0], [100], [0, 0], [10, 10])
+    F["DENSE_Q"] = qp_from_dense([[2, 1], [1, 2]], [[1, 2]], [0, 0], [-1], [1], [-10, -10], [10, 10])
+    F["CONTROL_NONDIAG"] = qp_from_dense([[2, .5], [.5, 3]], [[1, -1]], [1, -.5], [0], [1.5], [-10, -10], [10, 10])
+    F["EMPTY_A"] = qp_from_dense([[2, 0], [0, 3]], np.zeros((0, 2)), [1, -.5], [], [], [-inf, -inf], [inf, inf])
+    F["TALL_DIAG"] = qp_from_dense([[3, 0], [0, 5]], [[1, 1], [1, -1], [2, .5]], [1.5, -.5], [1, -.5, .25],
+                                   [1, -.5, .25], [0, -1], [2, 3])
+    F["DISPATCH_1D"] = qp_from_dense([[1]], np.zeros((0, 1)), [0], [], [], [-1], [1])
+    F["GPU_SPMV_1D"] = qp_from_dense([[2]], [[3]], [0], [-1], [1], [-10], [10])
+    F["GPU_ITERATION_STANDARD"] = qp_from_dense([[2, .5], [.5, 3]], [[1, -2]], [.5, -.25], [-.5], [.75],
+                                                [-1, -1], [1, 1])
+    F["ZERO_1D"] = qp_from_dense(np.zeros((1, 1)), np.zeros((0, 1)), [0], [], [], [-1], [1])
+    return F
+
+
+# --------------------------------------------------------------------------- generator
+
+class _GenProblem(C.Structure):
+    _fields_ = [("n", C.c_size_t), ("m", C.c_size_t), ("m_eq", C.c_size_t),
+                ("q_nnz", C.c_size_t), ("a_nnz", C.c_size_t),
+                ("q_ptr", c_size_p), ("q_ind", c_size_p), ("q_val", c_double_p),
+                ("a_ptr", c_size_p), ("a_ind", c_size_p), ("a_val", c_double_p),
+                ("c", c_double_p), ("AL", c_double_p), ("AU", c_double_p), ("l", c_double_p),
+                ("u", c_double_p), ("x0", c_double_p),
+                ("num_var_cones", C.c_size_t), ("var_offsets", c_size_p),
+                ("num_row_cones", C.c_size_t), ("row_offsets", c_size_p), ("soc_rhs", c_double_p)]
+
+
+_gen = None
+
+
+def gen_lib():
+    global _gen
+    if _gen is None:
+        path = PKG / "libhprqp_gen.so"
+        if not path.exists():
+            raise RuntimeError(f"{path} is not built; run __graft_entry__.build()")
+        _gen = C.CDLL(str(path))
+        _gen.qpgen_random_qp.argtypes = [C.c_uint64] + [C.c_size_t] * 8 + [C.c_double, C.POINTER(_GenProblem)]
+        _gen.qpgen_soc_qp.argtypes = [C.c_uint64] + [C.c_size_t] * 5 + [C.c_double, C.POINTER(_GenProblem)]
+        _gen.qpgen_free.argtypes = [C.POINTER(_GenProblem)]
+    return _gen
+
+
+def _take(gp: _GenProblem) -> QP:
+    def arr(p, n, dt):
+        if n == 0 or not p:
+            return np.zeros(0, dtype=dt)
+        return np.ctypeslib.as_array(p, shape=(n,)).astype(dt, copy=True)
+    n, m = gp.n, gp.m
+    qp = QP(n, m, arr(gp.q_ptr, n + 1, np.uint64), arr(gp.q_ind, gp.q_nnz, np.uint64), arr(gp.q_val, gp.q_nnz, float),
+            arr(gp.a_ptr, m + 1, np.uint64), arr(gp.a_ind, gp.a_nnz, np.uint64), arr(gp.a_val, gp.a_nnz, float),
+            arr(gp.c, n, float), arr(gp.AL, m, float), arr(gp.AU, m, float), arr(gp.l, n, float),
+            arr(gp.u, n, float), 0.0, x0=arr(gp.x0, n, float))
+    if gp.num_var_cones:
+        qp.var_offsets = arr(gp.var_offsets, gp.num_var_cones + 1, np.uint64)
+    if gp.num_row_cones:
+        qp.row_offsets = arr(gp.row_offsets, gp.num_row_cones + 1, np.uint64)
+        qp.soc_rhs = arr(gp.soc_rhs, m, float)
+    return qp
+
+
+def random_qp(seed=0, n=2000, m_eq=600, m_in=400, a_row_nnz=10, heavy_rows=0, heavy_nnz=0,
+              b_rows=500, b_row_nnz=5, box=10.0) -> QP:
+    g = gen_lib()
+    gp = _GenProblem()
+    rc = g.qpgen_random_qp(seed, n, m_eq, m_in, a_row_nnz, heavy_rows, heavy_nnz, b_rows, b_row_nnz, box,
+                           C.byref(gp))
+    if rc != 0:
+        raise RuntimeError("qpgen_random_qp failed")
+    try:
+        return _take(gp)
+    finally:
+        g.qpgen_free(C.byref(gp))
+
+
+def soc_qp(seed=0, n_box=100, cones=10, cone_dim=10, m=50, a_row_nnz=10, box=5.0) -> QP:
+    g = gen_lib()
+    gp = _GenProblem()
+    if g.qpgen_soc_qp(seed, n_box, cones, cone_dim, m, a_row_nnz, box, C.byref(gp)) != 0:
+        raise RuntimeError("qpgen_soc_qp failed")
+    try:
+        return _take(gp)
+    finally:
+        g.qpgen_free(C.byref(gp))
+
+
+# --------------------------------------------------------------------------- oracle
+
+class OrParams(C.Structure):
+    _fields_ = [("stoptol", C.c_double), ("max_iter", C.c_size_t), ("time_limit", C.c_double),
+                ("check_iter", C.c_size_t), ("print_frequency", C.c_int), ("use_scaling", C.c_int)]
+
+
+class OrResults(C.Structure):
+    _fields_ = [("status", C.c_int), ("iter", C.c_size_t), ("iter_4", C.c_size_t), ("iter_6", C.c_size_t),
+                ("time", C.c_double), ("time_4", C.c_double), ("time_6", C.c_double),
+                ("primal_obj", C.c_double), ("residuals", C.c_double), ("gap", C.c_double),
+                ("err_rp", C.c_double), ("err_rd", C.c_double), ("dual_obj", C.c_double)]
+
+
+_oracle = None
+
+
+def oracle_lib():
+    global _oracle
+    if _oracle is None:
+        path = ROOT / "oracle" / "liboracle.so"
+        if not path.exists():
+            raise RuntimeError(f"{path} is not built; run __graft_entry__.build()")
+        L = C.CDLL(str(path))
+        L.or_params_defaults.argtypes = [C.POINTER(OrParams)]
+        L.or_session_create.argtypes = ([C.c_size_t, C.c_size_t] + [C.c_size_t, c_size_p, c_size_p, c_double_p] * 2
+                                        + [c_double_p] * 5 + [C.c_double, C.c_size_t, c_size_p, C.c_size_t,
+                                                              c_size_p, c_double_p, C.POINTER(OrParams),
+                                                              C.POINTER(C.c_void_p)])
+        L.or_session_destroy.argtypes = [C.c_void_p]
+        L.or_session_advance.argtypes = [C.c_void_p, C.c_size_t, C.POINTER(C.c_int)]
+        L.or_session_solve.argtypes = [C.c_void_p, C.POINTER(OrResults)] + [c_double_p] * 4
+        L.or_session_results.argtypes = [C.c_void_p, C.POINTER(OrResults)] + [c_double_p] * 4
+        L.or_session_n.argtypes = [C.c_void_p]
+        L.or_session_n.restype = C.c_size_t
+        L.or_session_m.argtypes = [C.c_void_p]
+        L.or_session_m.restype = C.c_size_t
+        L.or_session_read.argtypes = [C.c_void_p, C.c_char_p, c_double_p, C.c_size_t]
+        L.or_session_read.restype = C.c_size_t
+        L.or_session_scalar.argtypes = [C.c_void_p, C.c_char_p]
+        L.or_session_scalar.restype = C.c_double
+        L.or_session_matrix.argtypes = [C.c_void_p, C.c_char_p, c_size_p, c_size_p, c_double_p]
+        L.or_session_matrix.restype = C.c_size_t
+        L.or_csr_matvec.argtypes = [C.c_size_t, c_size_p, c_size_p, c_double_p, c_double_p, c_double_p]
+        L.or_csr_transpose.argtypes = [C.c_size_t, C.c_size_t, c_size_p, c_size_p, c_double_p, c_size_p, c_size_p,
+                                       c_double_p]
+        L.or_soc_project.argtypes = [c_double_p, c_double_p, C.c_size_t]
+        L.or_preprocess.argtypes = ([C.c_size_t, C.c_size_t] + [C.c_size_t, c_size_p, c_size_p, c_double_p] * 2
+                                    + [c_double_p, c_double_p, c_size_p, c_size_p, c_size_p, c_double_p,
+                                       c_size_p, c_size_p, c_size_p, c_size_p, c_double_p])
+        _oracle = L
+    return _oracle
+
+
+def oracle_params(**kw) -> OrParams:
+    p = OrParams()
+    oracle_lib().or_params_defaults(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+class OracleSession:
+    def __init__(self, qp: QP, **params):
+        L = oracle_lib()
+        self._keep = []
+
+        def sz(a):
+            arr, p = _sz(a if a is not None else np.zeros(1, np.uint64))
+            self._keep.append(arr)
+            return p
+
+        def db(a):
+            arr, p = _db(a if a is not None and len(a) else np.zeros(1))
+            self._keep.append(arr)
+            return p
+        self.params = oracle_params(**params)
+        h = C.c_void_p()
+        st = L.or_session_create(qp.n, qp.m, len(qp.q_val), sz(qp.q_ptr), sz(qp.q_ind), db(qp.q_val),
+                                 len(qp.a_val), sz(qp.a_ptr), sz(qp.a_ind), db(qp.a_val), db(qp.c), db(qp.AL),
+                                 db(qp.AU), db(qp.l), db(qp.u), qp.obj_constant, qp.num_var_cones,
+                                 sz(qp.var_offsets) if qp.num_var_cones else None, qp.num_row_cones,
+                                 sz(qp.row_offsets) if qp.num_row_cones else None,
+                                 db(qp.soc_rhs) if qp.num_row_cones else None, C.byref(self.params), C.byref(h))
+        self.status = st
+        self.h = h if st == 0 else None
+        self.n_in, self.m_in = qp.n, qp.m
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            oracle_lib().or_session_destroy(self.h)
+            self.h = None
+
+    @property
+    def n(self):
+        return oracle_lib().or_session_n(self.h)
+
+    @property
+    def m(self):
+        return oracle_lib().or_session_m(self.h)
+
+    def advance(self, bodies=1):
+        fin = C.c_int(0)
+        st = oracle_lib().or_session_advance(self.h, bodies, C.byref(fin))
+        return st, bool(fin.value)
+
+    def read(self, name):
+        L = oracle_lib()
+        ln = L.or_session_read(self.h, name.encode(), None, 0)
+        out = np.zeros(ln)
+        if ln:
+            L.or_session_read(self.h, name.encode(), out.ctypes.data_as(c_double_p), ln)
+        return out
+
+    def scalar(self, name):
+        return oracle_lib().or_session_scalar(self.h, name.encode())
+
+    def matrix(self, name):
+        L = oracle_lib()
+        rows = {"Q": self.n, "A": self.m, "AT": self.n}[name]
+        nnz = L.or_session_matrix(self.h, name.encode(), None, None, None)
+        ptr = np.zeros(rows + 1, np.uint64)
+        ind = np.zeros(max(nnz, 1), np.uint64)
+        val = np.zeros(max(nnz, 1))
+        L.or_session_matrix(self.h, name.encode(), ptr.ctypes.data_as(c_size_p), ind.ctypes.data_as(c_size_p),
+                            val.ctypes.data_as(c_double_p))
+        return ptr, ind[:nnz], val[:nnz]
+
+    def results(self):
+        r = OrResults()
+        n, m = self.n, self.m
+        x, y, z, w = np.zeros(max(n, 1)), np.zeros(max(m, 1)), np.zeros(max(n, 1)), np.zeros(max(n, 1))
+        oracle_lib().or_session_results(self.h, C.byref(r), x.ctypes.data_as(c_double_p),
+                                        y.ctypes.data_as(c_double_p), z.ctypes.data_as(c_double_p),
+                                        w.ctypes.data_as(c_double_p))
+        return r, x[:n], y[:m], z[:n], w[:n]
+
+    def solve(self):
+        self.advance((1 << 63))
+        return self.results()
+
+
+# --------------------------------------------------------------------------- reference (compiled from source)
+
+REF_PATH = ROOT / "oracle" / "_ref" / "libhprqp_ref.so"
+_ref = None
+
+
+def ref_available():
+    return REF_PATH.exists()
+
+
+def ref_lib():
+    global _ref
+    if _ref is None:
+        L = C.CDLL(str(REF_PATH))
+        vp = C.c_void_p
+        L.hprqp_model_create.argtypes = [C.POINTER(vp)]
+        L.hprqp_model_destroy.argtypes = [vp]
+        L.hprqp_model_build_from_qabc_csr.argtypes = ([vp] + [C.c_size_t] * 3 + [c_size_p, c_size_p, c_double_p]
+                                                      + [C.c_size_t] * 3 + [c_size_p, c_size_p, c_double_p]
+                                                      + [c_double_p] * 5 + [C.c_double])
+        L.hprqp_model_num_variables.argtypes = [vp]
+        L.hprqp_model_num_variables.restype = C.c_size_t
+        L.hprqp_model_num_constraints.argtypes = [vp]
+        L.hprqp_model_num_constraints.restype = C.c_size_t
+        L.hprqp_params_create.argtypes = [C.POINTER(vp)]
+        L.hprqp_params_destroy.argtypes = [vp]
+        for nm, t in [("stoptol", C.c_double), ("max_iter", C.c_size_t), ("use_gpu", C.c_int),
+                      ("time_limit", C.c_double), ("check_iter", C.c_size_t), ("print_frequency", C.c_int),
+                      ("use_scaling", C.c_int), ("verbose", C.c_int)]:
+            getattr(L, "hprqp_params_set_" + nm).argtypes = [vp, t]
+        L.hprqp_results_create.argtypes = [C.POINTER(vp)]
+        L.hprqp_results_destroy.argtypes = [vp]
+        L.hprqp_optimize.argtypes = [vp, vp, vp]
+        for nm in ["status", "num_variables", "num_constraints", "iter", "iter_4", "iter_6"]:
+            f = getattr(L, "hprqp_results_" + nm)
+            f.argtypes = [vp]
+            f.restype = C.c_size_t if nm != "status" else C.c_int
+        for nm in ["time", "time_4", "time_6", "objective", "residual", "gap"]:
+            f = getattr(L, "hprqp_results_" + nm)
+            f.argtypes = [vp]
+            f.restype = C.c_double
+        for nm in "xyzw":
+            f = getattr(L, f"hprqp_results_{nm}_data")
+            f.argtypes = [vp]
+            f.restype = c_double_p
+        _ref = L
+    return _ref
+
+
+def binding_solve(L, qp: QP, **params):
+    """Solve through the hprqp_binding.h API of library L (reference or product)."""
+    vp = C.c_void_p
+    model, prm, res = vp(), vp(), vp()
+    L.hprqp_model_create(C.byref(model))
+    L.hprqp_params_create(C.byref(prm))
+    L.hprqp_results_create(C.byref(res))
+    try:
+        keep = []
+
+        def sz(a):
+            arr, p = _sz(a if len(a) else np.zeros(1, np.uint64))
+            keep.append(arr)
+            return p
+
+        def db(a):
+            arr, p = _db(a if len(a) else np.zeros(1))
+            keep.append(arr)
+            return p
+        st = L.hprqp_model_build_from_qabc_csr(model, qp.n, qp.n, len(qp.q_val), sz(qp.q_ptr), sz(qp.q_ind),
+                                               db(qp.q_val), qp.m, qp.n, len(qp.a_val), sz(qp.a_ptr), sz(qp.a_ind),
+                                               db(qp.a_val), db(qp.c), db(qp.AL), db(qp.AU), db(qp.l), db(qp.u),
+                                               qp.obj_constant)
+        if st != 0:
+            return {"status": STATUS[st], "build_failed": True}
+        params.setdefault("verbose", 0)
+        for k, v in params.items():
+            getattr(L, "hprqp_params_set_" + k)(prm, v)
+        st = L.hprqp_optimize(model, prm, res)
+        n = L.hprqp_results_num_variables(res)
+        m = L.hprqp_results_num_constraints(res)
+
+        def vec(nm, ln):
+            p = getattr(L, f"hprqp_results_{nm}_data")(res)
+            return np.ctypeslib.as_array(p, shape=(ln,)).copy() if (ln and p) else np.zeros(0)
+        out = {"rc": STATUS.get(st, st), "status": STATUS.get(L.hprqp_results_status(res)),
+               "iter": L.hprqp_results_iter(res), "iter_4": L.hprqp_results_iter_4(res),
+               "iter_6": L.hprqp_results_iter_6(res), "time": L.hprqp_results_time(res),
+               "objective": L.hprqp_results_objective(res), "residual": L.hprqp_results_residual(res),
+               "gap": L.hprqp_results_gap(res), "x": vec("x", n), "y": vec("y", m), "z": vec("z", n),
+               "w": vec("w", n), "n": n, "m": m}
+        return out
+    finally:
+        L.hprqp_results_destroy(res)
+        L.hprqp_params_destroy(prm)
+        L.hprqp_model_destroy(model)
