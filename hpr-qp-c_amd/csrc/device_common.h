@@ -1,0 +1,184 @@
+// Shared device-side building blocks for the gfx950 HPR-QP kernels:
+// error handling, deterministic wavefront/workgroup reductions, and the
+// row-blocked CSR SpMV engine whose per-row results feed a fused epilogue.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+
+#include "shared/types.h"
+
+namespace hprqp {
+
+constexpr int kWave = 64;         // CDNA wavefront width
+constexpr int kThreads = 256;     // workgroup size of every row/element kernel
+constexpr int kStreamCap = 2048;  // nnz staged in LDS per stream-mode row block
+constexpr int kLongRow = 256;     // rows longer than this get a workgroup each
+constexpr int kMaxTerms = 8;      // reduction terms one kernel may emit
+
+void report_hip_error(hipError_t err, const char *what, const char *file, int line);
+
+#define HPRQP_HIP(call)                                                   \
+    do {                                                                  \
+        hipError_t hprqp_err_ = (call);                                   \
+        if (hprqp_err_ != hipSuccess) {                                   \
+            ::hprqp::report_hip_error(hprqp_err_, #call, __FILE__, __LINE__); \
+            return HPRQP_STATUS_NOT_IMPLEMENTED;                          \
+        }                                                                 \
+    } while (0)
+
+// ---------------------------------------------------------------- reductions
+// Fixed-shape trees, so results are run-to-run deterministic.
+
+__device__ inline double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_down(v, off, kWave);
+    return v;
+}
+
+__device__ inline double wave_max(double v)
+{
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, kWave));
+    return v;
+}
+
+// Sum over the 256-thread workgroup; the result is valid in every thread.
+__device__ inline double block_sum_all(double v)
+{
+    __shared__ double part[kThreads / kWave];
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    v = wave_sum(v);
+    __syncthreads();
+    if (lane == 0) part[wid] = v;
+    __syncthreads();
+    double total = part[0];
+#pragma unroll
+    for (int w = 1; w < kThreads / kWave; ++w) total += part[w];
+    return total;
+}
+
+// Reduce NS sums and NM maxima over the workgroup and write them, in that
+// order, to out[0 .. NS+NM) (thread 0 writes).
+template <int NS, int NM>
+__device__ inline void block_reduce_store(const double *sums, const double *maxs, double *out)
+{
+    constexpr int NT = NS + NM;
+    __shared__ double part[kThreads / kWave][NT > 0 ? NT : 1];
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+#pragma unroll
+    for (int t = 0; t < NS; ++t) {
+        double v = wave_sum(sums[t]);
+        if (lane == 0) part[wid][t] = v;
+    }
+#pragma unroll
+    for (int t = 0; t < NM; ++t) {
+        double v = wave_max(maxs[t]);
+        if (lane == 0) part[wid][NS + t] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < NT) {
+        const int t = threadIdx.x;
+        double acc = part[0][t];
+        for (int w = 1; w < kThreads / kWave; ++w) acc = t < NS ? acc + part[w][t] : fmax(acc, part[w][t]);
+        out[t] = acc;
+    }
+}
+
+// ------------------------------------------------------------------- SpMV
+// Row blocks: block b covers rows [blocks[b], blocks[b+1]). Either one row
+// longer than kLongRow (vector mode: the workgroup strides the row with FMA and
+// reduces), or up to kThreads rows with at most kStreamCap nnz in total
+// (stream mode: products v*x[col] are staged through LDS with coalesced loads,
+// then each row's owner thread sums them left to right without FMA, which
+// reproduces the sequential row sum of the host CSR product bit for bit).
+//
+// The epilogue type supplies:
+//   static constexpr int NS, NM;              // reduction terms
+//   __device__ void row(int r, double v, double *sums, double *maxs) const;
+//   double *partials;                          // [gridDim.x][NS+NM] or null
+
+template <class Epi>
+__global__ __launch_bounds__(kThreads) void spmv_rows_kernel(const int *__restrict__ rp, const int *__restrict__ ci,
+                                                             const double *__restrict__ val,
+                                                             const double *__restrict__ x,
+                                                             const int *__restrict__ blocks, Epi epi)
+{
+    constexpr int NS = Epi::NS, NM = Epi::NM;
+    __shared__ double prod[kStreamCap];
+    const int b = blockIdx.x, t = threadIdx.x;
+    const int r0 = blocks[b], r1 = blocks[b + 1];
+    const int k0 = rp[r0], k1 = rp[r1];
+    double sums[NS > 0 ? NS : 1];
+    double maxs[NM > 0 ? NM : 1];
+#pragma unroll
+    for (int i = 0; i < (NS > 0 ? NS : 1); ++i) sums[i] = 0.0;
+#pragma unroll
+    for (int i = 0; i < (NM > 0 ? NM : 1); ++i) maxs[i] = 0.0;
+
+    if (r1 - r0 == 1 && k1 - k0 > kLongRow) {
+        double acc = 0.0;
+        for (int k = k0 + t; k < k1; k += kThreads) acc = fma(val[k], x[ci[k]], acc);
+        acc = block_sum_all(acc);
+        if (t == 0) epi.row(r0, acc, sums, maxs);
+    } else {
+        const int cnt = k1 - k0;
+        constexpr int kPer = kStreamCap / kThreads;
+        int col[kPer];
+        double v[kPer], xv[kPer];
+#pragma unroll
+        for (int it = 0; it < kPer; ++it) {
+            const int k = t + it * kThreads;
+            if (k < cnt) {
+                col[it] = ci[k0 + k];
+                v[it] = val[k0 + k];
+            }
+        }
+#pragma unroll
+        for (int it = 0; it < kPer; ++it) {
+            const int k = t + it * kThreads;
+            if (k < cnt) xv[it] = x[col[it]];
+        }
+#pragma unroll
+        for (int it = 0; it < kPer; ++it) {
+            const int k = t + it * kThreads;
+            if (k < cnt) prod[k] = v[it] * xv[it];
+        }
+        __syncthreads();
+        const int r = r0 + t;
+        if (r < r1) {
+            double acc = 0.0;
+            const int e = rp[r + 1] - k0;
+            for (int k = rp[r] - k0; k < e; ++k) acc += prod[k];
+            epi.row(r, acc, sums, maxs);
+        }
+    }
+    if constexpr (NS + NM > 0) block_reduce_store<NS, NM>(sums, maxs, epi.partials + (size_t)b * (NS + NM));
+}
+
+// Elementwise map over [0, len) with optional reductions; one element per
+// thread. The functor has the same interface as an SpMV epilogue minus the
+// product value: row(int i, double *sums, double *maxs).
+template <class F>
+__global__ __launch_bounds__(kThreads) void map_kernel(int len, F f)
+{
+    constexpr int NS = F::NS, NM = F::NM;
+    double sums[NS > 0 ? NS : 1];
+    double maxs[NM > 0 ? NM : 1];
+#pragma unroll
+    for (int i = 0; i < (NS > 0 ? NS : 1); ++i) sums[i] = 0.0;
+#pragma unroll
+    for (int i = 0; i < (NM > 0 ? NM : 1); ++i) maxs[i] = 0.0;
+    const int i = blockIdx.x * kThreads + threadIdx.x;
+    if (i < len) f.row(i, sums, maxs);
+    if constexpr (NS + NM > 0) block_reduce_store<NS, NM>(sums, maxs, f.partials + (size_t)blockIdx.x * (NS + NM));
+}
+
+// Finish a reduction: term t of out[] = reduce over nblk partial rows
+// (stride `stride`), sum for t < ns, max otherwise. One workgroup per term.
+__global__ __launch_bounds__(kThreads) void reduce_partials_kernel(const double *__restrict__ partials, int nblk,
+                                                                   int stride, int ns, double *__restrict__ out);
+
+}  // namespace hprqp

@@ -1,0 +1,547 @@
+// Host-side model construction for the HPR-QP C API: CSR helpers, model
+// ownership, build_from_QAbc[_soc] with the reference's preprocessing
+// (validation, Q <- (Q + Q')/2 with drop tolerance 1e-15, deletion of empty and
+// free rows), parameter defaults and result ownership.
+//
+// Semantics follow src/shared/build.c:40-621, src/shared/sparse.c:1-181 and
+// src/shared/solver.c:133-177; the code is this library's own.
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "hprqp.h"
+
+namespace {
+
+void csr_reset(HPRQPMatrixCSR *m)
+{
+    m->nrow = m->ncol = m->nnz = 0;
+    m->row_ptr = nullptr;
+    m->col_ind = nullptr;
+    m->values = nullptr;
+    m->owns_data = 0;
+}
+
+double *copy_doubles(const double *src, size_t n)
+{
+    if (n == 0) return nullptr;
+    double *d = static_cast<double *>(std::malloc(n * sizeof(double)));
+    if (d != nullptr) std::memcpy(d, src, n * sizeof(double));
+    return d;
+}
+
+bool offsets_valid(size_t count, const size_t *off, size_t len)
+{
+    if (count == 0) return true;
+    if (count > len || off == nullptr || off[0] >= len || off[count] != len) return false;
+    for (size_t k = 0; k < count; ++k)
+        if (off[k] >= off[k + 1] || off[k + 1] > len) return false;
+    return true;
+}
+
+// build.c:51-77
+HPRQPStatus validate_cones(const HPRQPModel *md)
+{
+    if (!offsets_valid(md->num_var_cones, md->var_cone_offsets, md->Q.ncol) ||
+        !offsets_valid(md->num_row_cones, md->row_cone_offsets, md->A.nrow))
+        return HPRQP_STATUS_INVALID_ARGUMENT;
+    if (md->num_var_cones) {
+        if (md->l == nullptr || md->u == nullptr) return HPRQP_STATUS_INVALID_ARGUMENT;
+        for (size_t i = md->var_cone_offsets[0]; i < md->Q.ncol; ++i)
+            if (md->l[i] != -INFINITY || md->u[i] != INFINITY) return HPRQP_STATUS_INVALID_ARGUMENT;
+    }
+    if (md->num_row_cones) {
+        if (md->soc_rhs == nullptr || md->AL == nullptr || md->AU == nullptr) return HPRQP_STATUS_INVALID_ARGUMENT;
+        for (size_t i = 0; i < md->A.nrow; ++i) {
+            if (!std::isfinite(md->soc_rhs[i])) return HPRQP_STATUS_INVALID_ARGUMENT;
+            if (i < md->row_cone_offsets[0]) {
+                if (md->soc_rhs[i] != 0.0) return HPRQP_STATUS_INVALID_ARGUMENT;
+            } else if (md->AL[i] != -INFINITY || md->AU[i] != INFINITY) {
+                return HPRQP_STATUS_INVALID_ARGUMENT;
+            }
+        }
+    }
+    return HPRQP_STATUS_OK;
+}
+
+HPRQPStatus copy_cone_spec(HPRQPModel *md, const HPRQPSOCSpec *soc)
+{
+    if (soc == nullptr) return HPRQP_STATUS_OK;
+    if (!offsets_valid(soc->num_var_cones, soc->var_offsets, md->Q.ncol) ||
+        !offsets_valid(soc->num_row_cones, soc->row_offsets, md->A.nrow) ||
+        (soc->num_row_cones && soc->rhs == nullptr))
+        return HPRQP_STATUS_INVALID_ARGUMENT;
+    if (soc->num_var_cones) {
+        size_t bytes = (soc->num_var_cones + 1) * sizeof(size_t);
+        md->var_cone_offsets = static_cast<size_t *>(std::malloc(bytes));
+        if (!md->var_cone_offsets) return HPRQP_STATUS_ALLOCATION_FAILED;
+        std::memcpy(md->var_cone_offsets, soc->var_offsets, bytes);
+        md->num_var_cones = soc->num_var_cones;
+    }
+    if (soc->num_row_cones) {
+        size_t bytes = (soc->num_row_cones + 1) * sizeof(size_t);
+        md->row_cone_offsets = static_cast<size_t *>(std::malloc(bytes));
+        if (!md->row_cone_offsets) return HPRQP_STATUS_ALLOCATION_FAILED;
+        std::memcpy(md->row_cone_offsets, soc->row_offsets, bytes);
+        md->num_row_cones = soc->num_row_cones;
+        md->soc_rhs = copy_doubles(soc->rhs, md->A.nrow);
+        if (!md->soc_rhs && md->A.nrow) return HPRQP_STATUS_ALLOCATION_FAILED;
+    }
+    return validate_cones(md);
+}
+
+struct Triple {
+    size_t row, col;
+    double val;
+};
+
+// build.c:157-252: Q_sym = 0.5 (Q + Q'). Every stored entry contributes
+// (r, c, v/2) then (c, r, v/2); entries are ordered by (row, col) keeping
+// generation order among equal keys, entries with |v| <= 1e-15 are dropped
+// before merging, equal keys are summed in order, tiny sums are zeroed and
+// zeros removed.
+HPRQPStatus symmetrize_q(HPRQPMatrixCSR *out, const HPRQPMatrixCSR *q)
+{
+    const double tol = 1e-15;
+    const size_t n = q->nrow;
+    std::vector<size_t> count(n + 1, 0);
+    for (size_t r = 0; r < n; ++r)
+        for (size_t k = q->row_ptr[r]; k < q->row_ptr[r + 1]; ++k) {
+            count[r + 1] += 1;
+            count[q->col_ind[k] + 1] += 1;
+        }
+    for (size_t r = 0; r < n; ++r) count[r + 1] += count[r];
+    std::vector<Triple> bucket(count[n]);
+    std::vector<size_t> fill(count.begin(), count.end() - 1);
+    for (size_t r = 0; r < n; ++r)
+        for (size_t k = q->row_ptr[r]; k < q->row_ptr[r + 1]; ++k) {
+            double half = 0.5 * q->values[k];
+            size_t c = q->col_ind[k];
+            bucket[fill[r]++] = Triple{r, c, half};
+            bucket[fill[c]++] = Triple{c, r, half};
+        }
+    // Generation order above is the reference's entry order restricted to one
+    // row, so a stable sort by column inside each bucket reproduces the global
+    // stable (row, col) order.
+    std::vector<size_t> ptr(n + 1, 0);
+    std::vector<size_t> ind;
+    std::vector<double> val;
+    ind.reserve(count[n]);
+    val.reserve(count[n]);
+    for (size_t r = 0; r < n; ++r) {
+        auto first = bucket.begin() + count[r], last = bucket.begin() + count[r + 1];
+        std::stable_sort(first, last, [](const Triple &a, const Triple &b) { return a.col < b.col; });
+        size_t row_begin = ind.size();
+        for (auto it = first; it != last; ++it) {
+            if (std::fabs(it->val) <= tol) continue;
+            if (ind.size() > row_begin && ind.back() == it->col) {
+                val.back() += it->val;
+            } else {
+                ind.push_back(it->col);
+                val.push_back(it->val);
+            }
+        }
+        size_t keep = row_begin;
+        for (size_t k = row_begin; k < ind.size(); ++k) {
+            double v = std::fabs(val[k]) <= tol ? 0.0 : val[k];
+            if (v != 0.0) {
+                ind[keep] = ind[k];
+                val[keep] = v;
+                ++keep;
+            }
+        }
+        ind.resize(keep);
+        val.resize(keep);
+        ptr[r + 1] = keep;
+    }
+    HPRQPStatus st = matrix_csr_alloc(out, n, q->ncol, ind.size());
+    if (st != HPRQP_STATUS_OK) return st;
+    std::memcpy(out->row_ptr, ptr.data(), (n + 1) * sizeof(size_t));
+    if (!ind.empty()) {
+        std::memcpy(out->col_ind, ind.data(), ind.size() * sizeof(size_t));
+        std::memcpy(out->values, val.data(), val.size() * sizeof(double));
+    }
+    return HPRQP_STATUS_OK;
+}
+
+// build.c:254-337
+HPRQPStatus drop_empty_and_free_rows(HPRQPModel *md)
+{
+    const size_t m = md->A.nrow;
+    if (m == 0) return HPRQP_STATUS_OK;
+    std::vector<size_t> keep;
+    size_t nnz = 0;
+    for (size_t r = 0; r < m; ++r) {
+        size_t len = md->A.row_ptr[r + 1] - md->A.row_ptr[r];
+        bool unbounded = std::isinf(md->AL[r]) && md->AL[r] < 0.0 && std::isinf(md->AU[r]) && md->AU[r] > 0.0;
+        if (len > 0 && !unbounded) {
+            keep.push_back(r);
+            nnz += len;
+        }
+    }
+    if (keep.size() == m) return HPRQP_STATUS_OK;
+    HPRQPMatrixCSR na;
+    csr_reset(&na);
+    HPRQPStatus st = matrix_csr_alloc(&na, keep.size(), md->A.ncol, nnz);
+    if (st != HPRQP_STATUS_OK) return st;
+    double *nal = keep.empty() ? nullptr : static_cast<double *>(std::malloc(keep.size() * sizeof(double)));
+    double *nau = keep.empty() ? nullptr : static_cast<double *>(std::malloc(keep.size() * sizeof(double)));
+    if (!keep.empty() && (!nal || !nau)) {
+        std::free(nal);
+        std::free(nau);
+        matrix_csr_free(&na);
+        return HPRQP_STATUS_ALLOCATION_FAILED;
+    }
+    size_t pos = 0;
+    for (size_t r = 0; r < keep.size(); ++r) {
+        size_t old = keep[r];
+        nal[r] = md->AL[old];
+        nau[r] = md->AU[old];
+        for (size_t k = md->A.row_ptr[old]; k < md->A.row_ptr[old + 1]; ++k) {
+            na.col_ind[pos] = md->A.col_ind[k];
+            na.values[pos] = md->A.values[k];
+            ++pos;
+        }
+        na.row_ptr[r + 1] = pos;
+    }
+    matrix_csr_free(&md->A);
+    matrix_csr_free(&md->AT);
+    std::free(md->AL);
+    std::free(md->AU);
+    md->A = na;
+    md->AL = nal;
+    md->AU = nau;
+    return matrix_csr_transpose(&md->AT, &md->A);
+}
+
+bool csr_all_finite(const HPRQPMatrixCSR *m)
+{
+    if (!matrix_csr_is_valid(m)) return false;
+    for (size_t k = 0; k < m->nnz; ++k)
+        if (!std::isfinite(m->values[k])) return false;
+    return true;
+}
+
+// build.c:339-400
+HPRQPStatus preprocess(HPRQPModel *md)
+{
+    if (!md->owns_data) return HPRQP_STATUS_INVALID_ARGUMENT;
+    if (!matrix_csr_is_valid(&md->Q) || !matrix_csr_is_valid(&md->A) || !matrix_csr_is_valid(&md->AT))
+        return HPRQP_STATUS_INVALID_ARGUMENT;
+    if (md->Q.nrow != md->Q.ncol || md->Q.nrow != md->A.ncol) return HPRQP_STATUS_INVALID_ARGUMENT;
+    if (md->A.nrow > 0 && (md->AL == nullptr || md->AU == nullptr)) return HPRQP_STATUS_INVALID_ARGUMENT;
+    if (!csr_all_finite(&md->Q) || !csr_all_finite(&md->A)) return HPRQP_STATUS_INVALID_ARGUMENT;
+    for (size_t i = 0; i < md->Q.ncol; ++i)
+        if (!std::isfinite(md->c[i])) return HPRQP_STATUS_INVALID_ARGUMENT;
+    for (size_t i = 0; i < md->A.nrow; ++i)
+        if (std::isnan(md->AL[i]) || std::isnan(md->AU[i])) return HPRQP_STATUS_INVALID_ARGUMENT;
+    for (size_t i = 0; i < md->Q.ncol; ++i)
+        if (std::isnan(md->l[i]) || std::isnan(md->u[i])) return HPRQP_STATUS_INVALID_ARGUMENT;
+    for (size_t i = 0; i < md->Q.ncol; ++i)
+        if (md->l[i] > md->u[i]) return HPRQP_STATUS_INVALID_ARGUMENT;
+    for (size_t i = 0; i < md->A.nrow; ++i)
+        if (md->AL[i] > md->AU[i]) return HPRQP_STATUS_INVALID_ARGUMENT;
+    HPRQPStatus st = validate_cones(md);
+    if (st != HPRQP_STATUS_OK) return st;
+    HPRQPMatrixCSR sym;
+    csr_reset(&sym);
+    st = symmetrize_q(&sym, &md->Q);
+    if (st != HPRQP_STATUS_OK) return st;
+    matrix_csr_free(&md->Q);
+    md->Q = sym;
+    if (md->num_var_cones || md->num_row_cones) return HPRQP_STATUS_OK;
+    return drop_empty_and_free_rows(md);
+}
+
+// build.c:504-592
+HPRQPStatus build_impl(const HPRQPMatrixCSR *Q, const HPRQPMatrixCSR *A, const double *c, const double *AL,
+                       const double *AU, const double *l, const double *u, double obj_constant,
+                       const HPRQPSOCSpec *soc, HPRQPModel *md)
+{
+    if (!Q || !A || !md || !c || !l || !u) return HPRQP_STATUS_INVALID_ARGUMENT;
+    if (!matrix_csr_is_valid(Q) || !matrix_csr_is_valid(A)) return HPRQP_STATUS_INVALID_ARGUMENT;
+    if (Q->nrow != Q->ncol || Q->nrow != A->ncol) return HPRQP_STATUS_INVALID_ARGUMENT;
+    const size_t m = A->nrow, n = A->ncol;
+    if (m > 0 && (AL == nullptr || AU == nullptr)) return HPRQP_STATUS_INVALID_ARGUMENT;
+    model_free(md);
+    md->owns_data = 1;
+    HPRQPStatus st = matrix_csr_copy(&md->Q, Q);
+    if (st == HPRQP_STATUS_OK) st = matrix_csr_copy(&md->A, A);
+    if (st == HPRQP_STATUS_OK) st = matrix_csr_transpose(&md->AT, A);
+    if (st == HPRQP_STATUS_OK) {
+        md->c = copy_doubles(c, n);
+        md->l = copy_doubles(l, n);
+        md->u = copy_doubles(u, n);
+        md->AL = copy_doubles(AL, m);
+        md->AU = copy_doubles(AU, m);
+        if ((n && (!md->c || !md->l || !md->u)) || (m && (!md->AL || !md->AU))) st = HPRQP_STATUS_ALLOCATION_FAILED;
+    }
+    if (st == HPRQP_STATUS_OK) st = copy_cone_spec(md, soc);
+    if (st != HPRQP_STATUS_OK) {
+        model_free(md);
+        return st;
+    }
+    md->obj_constant = obj_constant;
+    md->owns_data = 1;
+    return preprocess(md);
+}
+
+}  // namespace
+
+extern "C" {
+
+HPRQPStatus matrix_csr_alloc(HPRQPMatrixCSR *matrix, size_t nrow, size_t ncol, size_t nnz)
+{
+    if (matrix == nullptr) return HPRQP_STATUS_INVALID_ARGUMENT;
+    csr_reset(matrix);
+    matrix->row_ptr = static_cast<size_t *>(std::calloc(nrow + 1, sizeof(size_t)));
+    matrix->col_ind = static_cast<size_t *>(std::calloc(nnz > 0 ? nnz : 1, sizeof(size_t)));
+    matrix->values = static_cast<double *>(std::calloc(nnz > 0 ? nnz : 1, sizeof(double)));
+    if (!matrix->row_ptr || !matrix->col_ind || !matrix->values) {
+        std::free(matrix->row_ptr);
+        std::free(matrix->col_ind);
+        std::free(matrix->values);
+        csr_reset(matrix);
+        return HPRQP_STATUS_ALLOCATION_FAILED;
+    }
+    matrix->nrow = nrow;
+    matrix->ncol = ncol;
+    matrix->nnz = nnz;
+    matrix->owns_data = 1;
+    return HPRQP_STATUS_OK;
+}
+
+void matrix_csr_free(HPRQPMatrixCSR *matrix)
+{
+    if (matrix == nullptr) return;
+    if (matrix->owns_data) {
+        std::free(matrix->row_ptr);
+        std::free(matrix->col_ind);
+        std::free(matrix->values);
+    }
+    csr_reset(matrix);
+}
+
+int matrix_csr_is_valid(const HPRQPMatrixCSR *matrix)
+{
+    if (!matrix || !matrix->row_ptr || !matrix->col_ind || !matrix->values) return 0;
+    return matrix->row_ptr[0] == 0 && matrix->row_ptr[matrix->nrow] == matrix->nnz;
+}
+
+HPRQPStatus matrix_csr_copy(HPRQPMatrixCSR *dest, const HPRQPMatrixCSR *src)
+{
+    if (!dest || !src || !matrix_csr_is_valid(src)) return HPRQP_STATUS_INVALID_ARGUMENT;
+    if (dest == src) return HPRQP_STATUS_OK;
+    HPRQPStatus st = matrix_csr_alloc(dest, src->nrow, src->ncol, src->nnz);
+    if (st != HPRQP_STATUS_OK) return st;
+    std::memcpy(dest->row_ptr, src->row_ptr, (src->nrow + 1) * sizeof(size_t));
+    std::memcpy(dest->col_ind, src->col_ind, src->nnz * sizeof(size_t));
+    std::memcpy(dest->values, src->values, src->nnz * sizeof(double));
+    return HPRQP_STATUS_OK;
+}
+
+// sparse.c:81-125: stable counting-sort transpose (rows ascending per column).
+HPRQPStatus matrix_csr_transpose(HPRQPMatrixCSR *dest, const HPRQPMatrixCSR *src)
+{
+    if (!dest || !src || !matrix_csr_is_valid(src)) return HPRQP_STATUS_INVALID_ARGUMENT;
+    HPRQPStatus st = matrix_csr_alloc(dest, src->ncol, src->nrow, src->nnz);
+    if (st != HPRQP_STATUS_OK) return st;
+    for (size_t k = 0; k < src->nnz; ++k) dest->row_ptr[src->col_ind[k] + 1] += 1;
+    for (size_t j = 0; j < dest->nrow; ++j) dest->row_ptr[j + 1] += dest->row_ptr[j];
+    std::vector<size_t> next(dest->row_ptr, dest->row_ptr + dest->nrow);
+    for (size_t r = 0; r < src->nrow; ++r)
+        for (size_t k = src->row_ptr[r]; k < src->row_ptr[r + 1]; ++k) {
+            size_t dst = next[src->col_ind[k]]++;
+            dest->col_ind[dst] = r;
+            dest->values[dst] = src->values[k];
+        }
+    return HPRQP_STATUS_OK;
+}
+
+// sparse.c:127-145
+HPRQPStatus matrix_csr_matvec(const HPRQPMatrixCSR *matrix, const double *x, double *y)
+{
+    if (!matrix_csr_is_valid(matrix) || !x || !y) return HPRQP_STATUS_INVALID_ARGUMENT;
+    for (size_t r = 0; r < matrix->nrow; ++r) {
+        double acc = 0.0;
+        for (size_t k = matrix->row_ptr[r]; k < matrix->row_ptr[r + 1]; ++k)
+            acc += matrix->values[k] * x[matrix->col_ind[k]];
+        y[r] = acc;
+    }
+    return HPRQP_STATUS_OK;
+}
+
+// sparse.c:147-167
+HPRQPStatus matrix_csr_transpose_matvec(const HPRQPMatrixCSR *matrix, const double *x, double *y)
+{
+    if (!matrix_csr_is_valid(matrix) || !x || !y) return HPRQP_STATUS_INVALID_ARGUMENT;
+    for (size_t j = 0; j < matrix->ncol; ++j) y[j] = 0.0;
+    for (size_t r = 0; r < matrix->nrow; ++r) {
+        double xr = x[r];
+        for (size_t k = matrix->row_ptr[r]; k < matrix->row_ptr[r + 1]; ++k)
+            y[matrix->col_ind[k]] += matrix->values[k] * xr;
+    }
+    return HPRQP_STATUS_OK;
+}
+
+void model_init(HPRQPModel *model)
+{
+    if (model) std::memset(model, 0, sizeof(*model));
+}
+
+void model_free(HPRQPModel *model)
+{
+    if (!model) return;
+    if (model->owns_data) {
+        matrix_csr_free(&model->Q);
+        matrix_csr_free(&model->A);
+        matrix_csr_free(&model->AT);
+        std::free(model->c);
+        std::free(model->AL);
+        std::free(model->AU);
+        std::free(model->l);
+        std::free(model->u);
+        std::free(model->var_cone_offsets);
+        std::free(model->row_cone_offsets);
+        std::free(model->soc_rhs);
+    }
+    model_init(model);
+}
+
+// build.c:434-502
+HPRQPStatus model_copy(HPRQPModel *dest, const HPRQPModel *src)
+{
+    if (!dest || !src || !src->owns_data) return HPRQP_STATUS_INVALID_ARGUMENT;
+    model_free(dest);
+    dest->owns_data = 1;
+    HPRQPStatus st = matrix_csr_copy(&dest->Q, &src->Q);
+    if (st == HPRQP_STATUS_OK) st = matrix_csr_copy(&dest->A, &src->A);
+    if (st == HPRQP_STATUS_OK) st = matrix_csr_copy(&dest->AT, &src->AT);
+    if (st == HPRQP_STATUS_OK) {
+        size_t n = src->Q.ncol, m = src->A.nrow;
+        dest->c = copy_doubles(src->c, n);
+        dest->l = copy_doubles(src->l, n);
+        dest->u = copy_doubles(src->u, n);
+        dest->AL = copy_doubles(src->AL, m);
+        dest->AU = copy_doubles(src->AU, m);
+        if ((n && (!dest->c || !dest->l || !dest->u)) || (m && (!dest->AL || !dest->AU)))
+            st = HPRQP_STATUS_ALLOCATION_FAILED;
+    }
+    if (st == HPRQP_STATUS_OK) {
+        HPRQPSOCSpec soc = {src->num_var_cones, src->var_cone_offsets, src->num_row_cones, src->row_cone_offsets,
+                            src->soc_rhs};
+        st = copy_cone_spec(dest, &soc);
+    }
+    if (st != HPRQP_STATUS_OK) {
+        model_free(dest);
+        return st;
+    }
+    dest->obj_constant = src->obj_constant;
+    return HPRQP_STATUS_OK;
+}
+
+HPRQPStatus build_from_QAbc(const HPRQPMatrixCSR *Q, const HPRQPMatrixCSR *A, const double *c, const double *AL,
+                            const double *AU, const double *l, const double *u, double obj_constant,
+                            HPRQPModel *model)
+{
+    return build_impl(Q, A, c, AL, AU, l, u, obj_constant, nullptr, model);
+}
+
+HPRQPStatus build_from_QAbc_soc(const HPRQPMatrixCSR *Q, const HPRQPMatrixCSR *A, const double *c,
+                                const double *AL, const double *AU, const double *l, const double *u,
+                                double obj_constant, const HPRQPSOCSpec *soc, HPRQPModel *model)
+{
+    return build_impl(Q, A, c, AL, AU, l, u, obj_constant, soc, model);
+}
+
+HPRQPStatus build_from_mps(const char *mps_path, HPRQPModel *model)
+{
+    (void)mps_path;
+    (void)model;
+    std::fprintf(stderr, "hprqp: build_from_mps is not provided by the MI355X library; "
+                         "build the model with build_from_QAbc\n");
+    return HPRQP_STATUS_NOT_IMPLEMENTED;
+}
+
+// solver.c:133-157. use_gpu defaults to 1 (this library is the GPU backend);
+// gram_mode defaults to OFF and polishing to 0 because neither exists here.
+void params_set_defaults(HPRQPParams *params)
+{
+    if (!params) return;
+    std::memset(params, 0, sizeof(*params));
+    params->stoptol = 1e-8;
+    params->max_iter = static_cast<size_t>(INT_MAX);
+    params->use_gpu = 1;
+    params->device_number = 0;
+    params->time_limit = 3600.0;
+    params->check_iter = 100;
+    params->print_frequency = -1;
+    params->use_scaling = 1;
+    params->verbose = 1;
+    params->spmv_mode_q = HPRQP_SPMV_MODE_AUTO;
+    params->spmv_mode_a = HPRQP_SPMV_MODE_AUTO;
+    params->gram_mode = HPRQP_GRAM_OFF;
+    params->polishing = 0;
+    params->polish_max_attempts = 12;
+    params->polish_episode_seconds = 20.0;
+    params->polish_total_seconds = 0.0;
+    params->polish_memory_bytes = static_cast<size_t>(8) * 1024 * 1024 * 1024;
+}
+
+void results_init(HPRQPResults *results)
+{
+    if (results) std::memset(results, 0, sizeof(*results));
+}
+
+void results_free(HPRQPResults *results)
+{
+    if (!results) return;
+    std::free(results->x);
+    std::free(results->y);
+    std::free(results->z);
+    std::free(results->w);
+    results_init(results);
+}
+
+static const char *spmv_mode_name(HPRQPSpmvMode mode)
+{
+    switch (mode) {
+    case HPRQP_SPMV_MODE_AUTO: return "auto";
+    case HPRQP_SPMV_MODE_CUSPARSE: return "cusparse";
+    case HPRQP_SPMV_MODE_CUSTOMIZED: return "customized";
+    default: return "unknown";
+    }
+}
+
+void print_model_info(const HPRQPModel *model)
+{
+    if (!model) return;
+    std::printf("Model info:\n");
+    std::printf("  variables: %zu\n", model->Q.ncol);
+    std::printf("  constraints: %zu\n", model->A.nrow);
+    std::printf("  nnz(Q): %zu\n", model->Q.nnz);
+    std::printf("  nnz(A): %zu\n", model->A.nnz);
+}
+
+void print_params_info(const HPRQPParams *params)
+{
+    if (!params) return;
+    std::printf("Parameter info:\n");
+    std::printf("  stoptol: %.12g\n", params->stoptol);
+    std::printf("  use_gpu: %s\n", params->use_gpu ? "on" : "off");
+    std::printf("  device_number: %d\n", params->device_number);
+    std::printf("  max_iter: %zu\n", params->max_iter);
+    std::printf("  time_limit: %.12g\n", params->time_limit);
+    std::printf("  check_iter: %zu\n", params->check_iter);
+    std::printf("  print_frequency: %d\n", params->print_frequency);
+    std::printf("  use_scaling: %s\n", params->use_scaling ? "on" : "off");
+    std::printf("  verbose: %s\n", params->verbose ? "on" : "off");
+    std::printf("  spmv mode Q: %s\n", spmv_mode_name(params->spmv_mode_q));
+    std::printf("  spmv mode A: %s\n", spmv_mode_name(params->spmv_mode_a));
+}
+
+}  // extern "C"

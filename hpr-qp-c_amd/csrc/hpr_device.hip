@@ -1,0 +1,1680 @@
+// MI355X (gfx950) backend of the HPR-QP iteration.
+//
+// Replaces the reference's CUDA backend (src/gpu/*.cu) behind the same host
+// loop contract. Layout in HBM: int32 CSR for Q, A and the device-built A^T
+// (plus the permutation that maps A^T and Q^T entries to their sources), FP64
+// values, FP64 vectors in one arena. Every per-iteration kernel is a row-blocked
+// SpMV whose per-row result is consumed by a fused epilogue (the HPR update for
+// that row); reductions are two-level and deterministic, and one pinned
+// read-back per residual / M-norm check is the only host synchronisation.
+#include <hip/hip_runtime.h>
+#include <rocprim/device/device_radix_sort.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "device_common.h"
+#include "hprqp.h"
+#include "hprqp_hip.h"
+#include "solve_loop.h"
+
+namespace hprqp {
+
+void report_hip_error(hipError_t err, const char *what, const char *file, int line)
+{
+    std::fprintf(stderr, "hprqp: HIP error %d (%s) in %s at %s:%d\n", (int)err, hipGetErrorString(err), what, file,
+                 line);
+}
+
+__global__ __launch_bounds__(kThreads) void reduce_partials_kernel(const double *__restrict__ partials, int nblk,
+                                                                   int stride, int ns, double *__restrict__ out)
+{
+    const int term = blockIdx.x;
+    const bool is_max = term >= ns;
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += kThreads) {
+        double v = partials[(size_t)b * stride + term];
+        acc = is_max ? fmax(acc, v) : acc + v;
+    }
+    __shared__ double part[kThreads / kWave];
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    acc = is_max ? wave_max(acc) : wave_sum(acc);
+    if (lane == 0) part[wid] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double total = part[0];
+        for (int w = 1; w < kThreads / kWave; ++w) total = is_max ? fmax(total, part[w]) : total + part[w];
+        out[term] = total;
+    }
+}
+
+namespace {
+
+__device__ __forceinline__ double project(double v, double lo, double hi)
+{
+    if (v < lo) return lo;
+    if (v > hi) return hi;
+    return v;
+}
+
+inline int grid_for(size_t len) { return (int)((len + kThreads - 1) / kThreads); }
+
+// ------------------------------------------------------------------ device CSR
+
+struct DevCsr {
+    int nrow = 0, ncol = 0, nnz = 0;
+    int *rp = nullptr;
+    int *ci = nullptr;
+    double *val = nullptr;
+    int *blocks = nullptr;  // row-block starts, nblocks + 1 entries
+    int nblocks = 0;
+};
+
+// Host planning of row blocks from the row pointer (see device_common.h).
+std::vector<int> plan_row_blocks(const std::vector<int> &rp)
+{
+    const int nrow = (int)rp.size() - 1;
+    std::vector<int> starts;
+    starts.reserve(nrow / 16 + 2);
+    int r = 0;
+    while (r < nrow) {
+        int len = rp[r + 1] - rp[r];
+        starts.push_back(r);
+        if (len > kLongRow) {
+            ++r;
+            continue;
+        }
+        int nnz = 0, first = r;
+        while (r < nrow && r - first < kThreads) {
+            int l2 = rp[r + 1] - rp[r];
+            if (l2 > kLongRow || nnz + l2 > kStreamCap) break;
+            nnz += l2;
+            ++r;
+        }
+    }
+    starts.push_back(nrow);
+    if (nrow == 0) starts = {0, 0};
+    return starts;
+}
+
+HPRQPStatus upload_blocks(DevCsr &m, hipStream_t s)
+{
+    std::vector<int> rp(m.nrow + 1);
+    HPRQP_HIP(hipMemcpyAsync(rp.data(), m.rp, sizeof(int) * (m.nrow + 1), hipMemcpyDeviceToHost, s));
+    HPRQP_HIP(hipStreamSynchronize(s));
+    std::vector<int> blk = plan_row_blocks(rp);
+    m.nblocks = (int)blk.size() - 1;
+    if (m.blocks) HPRQP_HIP(hipFree(m.blocks));
+    HPRQP_HIP(hipMalloc(&m.blocks, sizeof(int) * blk.size()));
+    HPRQP_HIP(hipMemcpyAsync(m.blocks, blk.data(), sizeof(int) * blk.size(), hipMemcpyHostToDevice, s));
+    HPRQP_HIP(hipStreamSynchronize(s));
+    return HPRQP_STATUS_OK;
+}
+
+void free_csr(DevCsr &m)
+{
+    if (m.rp) (void)hipFree(m.rp);
+    if (m.ci) (void)hipFree(m.ci);
+    if (m.val) (void)hipFree(m.val);
+    if (m.blocks) (void)hipFree(m.blocks);
+    m = DevCsr{};
+}
+
+// ----------------------------------------------------------- device transpose
+// Stable counting-sort transpose: entries are sorted by column with an LSD
+// radix sort keyed on the column (stable), carrying the source entry index.
+// Source entries are in row-major order, so within each output row the source
+// rows come out ascending, which is the ordering of the host transpose.
+
+__global__ void expand_rows_kernel(int nrow, const int *__restrict__ rp, int *__restrict__ row_of)
+{
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrow) return;
+    for (int k = rp[r]; k < rp[r + 1]; ++k) row_of[k] = r;
+}
+
+__global__ void iota_kernel(int n, int *__restrict__ v)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) v[i] = i;
+}
+
+// t_row_ptr from sorted keys: rp[j] = first position with key >= j.
+__global__ void row_ptr_from_sorted_kernel(int nnz, int ncol, const int *__restrict__ keys, int *__restrict__ trp)
+{
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p > nnz) return;
+    int prev = p == 0 ? -1 : keys[p - 1];
+    int cur = p == nnz ? ncol : keys[p];
+    for (int j = prev + 1; j <= cur; ++j) trp[j] = p;
+}
+
+__global__ void gather_transpose_kernel(int nnz, const int *__restrict__ perm, const int *__restrict__ row_of,
+                                        const double *__restrict__ val, int *__restrict__ tci,
+                                        double *__restrict__ tval)
+{
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nnz) return;
+    int k = perm[p];
+    if (tci) tci[p] = row_of[k];
+    if (tval) tval[p] = val[k];
+}
+
+__global__ void gather_values_kernel(int nnz, const int *__restrict__ perm, const double *__restrict__ src,
+                                     double *__restrict__ dst)
+{
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < nnz) dst[p] = src[perm[p]];
+}
+
+HPRQPStatus device_transpose(int nrow, int ncol, int nnz, const int *rp, const int *ci, const double *val, int *trp,
+                             int *tci, double *tval, int *perm_out, hipStream_t s)
+{
+    if (nnz == 0) {
+        HPRQP_HIP(hipMemsetAsync(trp, 0, sizeof(int) * (ncol + 1), s));
+        return HPRQP_STATUS_OK;
+    }
+    int *row_of = nullptr, *keys_out = nullptr, *idx_in = nullptr, *perm = perm_out;
+    bool own_perm = perm == nullptr;
+    void *tmp = nullptr;
+    size_t tmp_bytes = 0;
+    HPRQP_HIP(hipMalloc(&row_of, sizeof(int) * nnz));
+    HPRQP_HIP(hipMalloc(&keys_out, sizeof(int) * nnz));
+    HPRQP_HIP(hipMalloc(&idx_in, sizeof(int) * nnz));
+    if (own_perm) HPRQP_HIP(hipMalloc(&perm, sizeof(int) * nnz));
+    expand_rows_kernel<<<grid_for(nrow), kThreads, 0, s>>>(nrow, rp, row_of);
+    iota_kernel<<<grid_for(nnz), kThreads, 0, s>>>(nnz, idx_in);
+    unsigned end_bit = 1;
+    while (end_bit < 31 && (1u << end_bit) < (unsigned)(ncol > 0 ? ncol : 1)) ++end_bit;
+    HPRQP_HIP(rocprim::radix_sort_pairs(nullptr, tmp_bytes, ci, keys_out, idx_in, perm, nnz, 0, end_bit, s));
+    HPRQP_HIP(hipMalloc(&tmp, tmp_bytes));
+    HPRQP_HIP(rocprim::radix_sort_pairs(tmp, tmp_bytes, ci, keys_out, idx_in, perm, nnz, 0, end_bit, s));
+    row_ptr_from_sorted_kernel<<<grid_for(nnz + 1), kThreads, 0, s>>>(nnz, ncol, keys_out, trp);
+    gather_transpose_kernel<<<grid_for(nnz), kThreads, 0, s>>>(nnz, perm, row_of, val, tci, tval);
+    HPRQP_HIP(hipGetLastError());
+    HPRQP_HIP(hipStreamSynchronize(s));
+    HPRQP_HIP(hipFree(tmp));
+    HPRQP_HIP(hipFree(row_of));
+    HPRQP_HIP(hipFree(keys_out));
+    HPRQP_HIP(hipFree(idx_in));
+    if (own_perm) HPRQP_HIP(hipFree(perm));
+    return HPRQP_STATUS_OK;
+}
+
+// ------------------------------------------------------------------ epilogues
+// Each mirrors one loop of src/cpu/iteration.c / residuals.c / control.c with
+// the same arithmetic order (kernels are built with -ffp-contract=off).
+
+struct NoRed {
+    static constexpr int NS = 0, NM = 0;
+    double *partials = nullptr;
+};
+
+struct EpiStore : NoRed {
+    double *out;
+    __device__ void row(int i, double v, double *, double *) const { out[i] = v; }
+};
+
+// iteration.c:85-123
+template <bool DIAG, bool CHECK>
+struct EpiZxw1 : NoRed {
+    double *Qw, *x, *x_bar, *x_hat, *w_bar, *dx, *z_bar;
+    const double *w, *ATy, *c, *l, *u, *last_x, *fact1, *fact2;
+    double sigma, f1s, f2s, h1, h2;
+    __device__ void row(int i, double qw, double *, double *) const
+    {
+        Qw[i] = qw;
+        const double xi = x[i];
+        const double tmp = -qw + ATy[i] - c[i];
+        const double zr = xi + sigma * tmp;
+        const double xb = project(zr, l[i], u[i]);
+        const double xh = 2.0 * xb - xi;
+        const double xn = h2 * xh + h1 * last_x[i];
+        const double wb = DIAG ? fact1[i] * w[i] + fact2[i] * xh : f1s * w[i] + f2s * xh;
+        if (CHECK) {
+            dx[i] = xb - xi;
+            z_bar[i] = (xb - zr) / sigma;
+        }
+        x_bar[i] = xb;
+        x_hat[i] = xh;
+        x[i] = xn;
+        w_bar[i] = wb;
+    }
+};
+
+// iteration.c:141-143
+struct EpiTempv : NoRed {
+    double *tempv;
+    const double *x_hat, *Qw;
+    double sigma;
+    __device__ void row(int i, double qwb, double *, double *) const { tempv[i] = x_hat[i] + sigma * (Qw[i] - qwb); }
+};
+
+// iteration.c:150-171
+template <bool CHECK>
+struct EpiY : NoRed {
+    double *y, *y_bar, *s, *dy;
+    const double *AL, *AU, *last_y;
+    double f1, f2, h1, h2;
+    __device__ void row(int i, double ax, double *, double *) const
+    {
+        const double yi = y[i];
+        const double sr = ax - f1 * yi;
+        const double sp = project(sr, AL[i], AU[i]);
+        const double yb = f2 * (sp - sr);
+        const double yh = 2.0 * yb - yi;
+        const double yn = h2 * yh + h1 * last_y[i];
+        if (CHECK) {
+            s[i] = sp;
+            dy[i] = yb - yi;
+        }
+        y_bar[i] = yb;
+        y[i] = yn;
+    }
+};
+
+// iteration.c:218-248
+template <bool DIAG, bool CHECK>
+struct EpiW2 : NoRed {
+    double *w_bar, *w, *ATy, *ATy_bar, *dw, *ATdy;
+    const double *last_w, *last_ATy, *fact;
+    double fs, h1, h2;
+    __device__ void row(int i, double atyb, double *, double *) const
+    {
+        const double f = DIAG ? fact[i] : fs;
+        const double aty = ATy[i], wi = w[i];
+        const double wbn = w_bar[i] + f * (atyb - aty);
+        const double wn = h1 * last_w[i] + h2 * (2.0 * wbn - wi);
+        const double atyn = h1 * last_ATy[i] + h2 * (2.0 * atyb - aty);
+        if (CHECK) {
+            dw[i] = wbn - wi;
+            ATdy[i] = atyb - aty;
+            ATy_bar[i] = atyb;
+        }
+        w_bar[i] = wbn;
+        w[i] = wn;
+        ATy[i] = atyn;
+    }
+};
+
+// iteration.c:19-66 (noC: A^T y first, Halpern on w as well)
+template <bool DIAG, bool CHECK>
+struct EpiZxw1NoC : NoRed {
+    double *ATy, *x, *x_bar, *x_hat, *w, *w_bar, *dx, *dw, *z_bar;
+    const double *Qw, *c, *l, *u, *last_x, *last_w, *fact1, *fact2;
+    double sigma, f1s, f2s, h1, h2;
+    __device__ void row(int i, double aty, double *, double *) const
+    {
+        ATy[i] = aty;
+        const double xi = x[i], wi = w[i];
+        const double tmp = -Qw[i] + aty - c[i];
+        const double zr = xi + sigma * tmp;
+        const double xb = project(zr, l[i], u[i]);
+        const double xh = 2.0 * xb - xi;
+        const double xn = h2 * xh + h1 * last_x[i];
+        const double wb = DIAG ? fact1[i] * wi + fact2[i] * xh : f1s * wi + f2s * xh;
+        const double wh = 2.0 * wb - wi;
+        const double wn = h2 * wh + h1 * last_w[i];
+        if (CHECK) {
+            dx[i] = xb - xi;
+            dw[i] = wb - wi;
+            z_bar[i] = (xb - zr) / sigma;
+        }
+        x_bar[i] = xb;
+        x_hat[i] = xh;
+        x[i] = xn;
+        w_bar[i] = wb;
+        w[i] = wn;
+    }
+};
+
+// iteration.c:68-80
+struct EpiQwNoC : NoRed {
+    double *Qw, *Qw_bar, *tempv;
+    const double *last_Qw, *x_hat;
+    double sigma, h1, h2;
+    __device__ void row(int i, double qwb, double *, double *) const
+    {
+        const double qw = Qw[i];
+        const double qwh = 2.0 * qwb - qw;
+        Qw_bar[i] = qwb;
+        Qw[i] = h2 * qwh + h1 * last_Qw[i];
+        tempv[i] = x_hat[i] + sigma * (qw - qwb);
+    }
+};
+
+// residuals.c:102-160: Qx = Q x_bar and the three objective dots.
+struct EpiResQ {
+    static constexpr int NS = 3, NM = 0;
+    double *partials;
+    double *Qx;
+    const double *x_bar, *c, *z_bar;
+    __device__ void row(int i, double qx, double *s, double *) const
+    {
+        Qx[i] = qx;
+        const double xb = x_bar[i];
+        s[0] += c[i] * xb;
+        s[1] += xb * qx;
+        s[2] += z_bar[i] * xb;
+    }
+};
+
+// residuals.c:148-162: A^T y_bar and the dual residual.
+struct EpiResAT {
+    static constexpr int NS = 0, NM = 3;
+    double *partials;
+    double *ATdy;
+    const double *Qx, *c, *z_bar, *col_norm;
+    double c_scale;
+    __device__ void row(int i, double atdy, double *, double *mx) const
+    {
+        ATdy[i] = atdy;
+        const double f = col_norm[i] * c_scale;
+        const double q = Qx[i];
+        const double rd = (q + c[i] - atdy - z_bar[i]) * f;
+        mx[0] = fmax(mx[0], fabs(rd));
+        mx[1] = fmax(mx[1], fabs(q * f));
+        mx[2] = fmax(mx[2], fabs(atdy * f));
+    }
+};
+
+// residuals.c:164-175 and the y_bar.s dot of :53
+struct EpiResA {
+    static constexpr int NS = 1, NM = 2;
+    double *partials;
+    double *Ax;
+    const double *AL, *AU, *row_norm, *y_bar, *s;
+    double b_scale;
+    __device__ void row(int i, double ax, double *sm, double *mx) const
+    {
+        Ax[i] = ax;
+        const double p = project(ax, AL[i], AU[i]);
+        const double f = row_norm[i] * b_scale;
+        mx[0] = fmax(mx[0], fabs((p - ax) * f));
+        mx[1] = fmax(mx[1], fabs(ax * f));
+        sm[0] += y_bar[i] * s[i];
+    }
+};
+
+// residuals.c:58-78
+struct MapBoundViolation {
+    static constexpr int NS = 0, NM = 1;
+    double *partials;
+    double *dx;
+    const double *l, *u, *x_bar, *col_norm;
+    double b_scale;
+    __device__ void row(int i, double *, double *mx) const
+    {
+        const double lo = l[i] - x_bar[i], hi = x_bar[i] - u[i];
+        double v = 0.0;
+        if (lo > 0.0) v += lo;
+        if (hi > 0.0) v += hi;
+        const double d = v * (b_scale / col_norm[i]);
+        dx[i] = d;
+        mx[0] = fmax(mx[0], d);
+    }
+};
+
+// control.c:174-209: the M-norm products.
+struct EpiMnQdw {
+    static constexpr int NS = 4, NM = 0;
+    double *partials;
+    double *dQw;
+    const double *dx, *dw;
+    __device__ void row(int i, double v, double *s, double *) const
+    {
+        dQw[i] = v;
+        const double d = dx[i];
+        s[0] += d * d;
+        s[1] += v * d;
+        s[2] += dw[i] * v;
+        s[3] += v * v;
+    }
+};
+
+struct EpiMnATdy {
+    static constexpr int NS = 1, NM = 0;
+    double *partials;
+    double *ATdy;
+    const double *dx;
+    __device__ void row(int i, double v, double *s, double *) const
+    {
+        ATdy[i] = v;
+        s[0] += v * dx[i];
+    }
+};
+
+struct EpiMnQATdy {
+    static constexpr int NS = 3, NM = 0;
+    double *partials;
+    double *QATdy;
+    const double *dQw, *ATdy, *fact_M;
+    int diag;
+    __device__ void row(int i, double v, double *s, double *) const
+    {
+        QATdy[i] = v;
+        const double a = ATdy[i];
+        s[0] += dQw[i] * a;
+        s[1] += a * v;
+        if (diag) s[2] += (a * fact_M[i]) * v;
+    }
+};
+
+struct MapDot {
+    static constexpr int NS = 1, NM = 0;
+    double *partials;
+    const double *a, *b;
+    __device__ void row(int i, double *s, double *) const { s[0] += a[i] * b[i]; }
+};
+
+// control.c:333-356: differences to the last restart point.
+struct MapSigmaDiffN {
+    static constexpr int NS = 1, NM = 0;
+    double *partials;
+    double *dx, *dw, *ATdy;
+    const double *x_bar, *last_x, *w_bar, *last_w, *ATy_bar, *last_ATy;
+    int has_rows;
+    __device__ void row(int i, double *s, double *) const
+    {
+        const double d = x_bar[i] - last_x[i];
+        dx[i] = d;
+        dw[i] = w_bar[i] - last_w[i];
+        if (has_rows) ATdy[i] = ATy_bar[i] - last_ATy[i];
+        s[0] += d * d;
+    }
+};
+
+struct MapSigmaDiffM {
+    static constexpr int NS = 1, NM = 0;
+    double *partials;
+    double *dy;
+    const double *y_bar, *last_y;
+    __device__ void row(int i, double *s, double *) const
+    {
+        const double d = y_bar[i] - last_y[i];
+        dy[i] = d;
+        s[0] += d * d;
+    }
+};
+
+struct EpiSigmaQdw {
+    static constexpr int NS = 3, NM = 0;
+    double *partials;
+    double *dQw;
+    const double *ATdy, *dw;
+    __device__ void row(int i, double v, double *s, double *) const
+    {
+        dQw[i] = v;
+        s[0] += v * ATdy[i];
+        s[1] += v * v;
+        s[2] += dw[i] * v;
+    }
+};
+
+struct EpiSigmaQATdy {
+    static constexpr int NS = 1, NM = 0;
+    double *partials;
+    double *QATdy;
+    const double *ATdy;
+    __device__ void row(int i, double v, double *s, double *) const
+    {
+        QATdy[i] = v;
+        s[0] += ATdy[i] * v;
+    }
+};
+
+// control.c:116-127
+struct MapDiagFactors : NoRed {
+    double *fact1, *fact2, *fact, *fact_M;
+    const double *diag_Q;
+    double sigma;
+    __device__ void row(int i, double *, double *) const
+    {
+        const double inv = 1.0 / (1.0 + sigma * diag_Q[i]);
+        fact2[i] = inv;
+        fact[i] = sigma * inv;
+        fact1[i] = sigma * diag_Q[i] * inv;
+        fact_M[i] = sigma * sigma * inv;
+    }
+};
+
+// control.c:413-457 (n side fused with the A^T y_bar product)
+struct EpiRestart : NoRed {
+    double *ATy_bar, *ATy, *last_ATy, *x, *w, *last_x, *last_w, *Qw, *last_Qw;
+    const double *x_bar, *w_bar, *Qw_bar;
+    int has_rows, noC;
+    __device__ void row(int i, double v, double *, double *) const
+    {
+        const double xb = x_bar[i], wb = w_bar[i];
+        x[i] = xb;
+        w[i] = wb;
+        last_x[i] = xb;
+        last_w[i] = wb;
+        if (has_rows) {
+            ATy_bar[i] = v;
+            last_ATy[i] = v;
+            ATy[i] = v;
+        }
+        if (noC) {
+            Qw[i] = Qw_bar[i];
+            last_Qw[i] = Qw_bar[i];
+        }
+    }
+};
+
+struct MapRestartM : NoRed {
+    double *y, *last_y;
+    const double *y_bar;
+    __device__ void row(int i, double *, double *) const
+    {
+        y[i] = y_bar[i];
+        last_y[i] = y_bar[i];
+    }
+};
+
+// eigs.c:11-153 pieces
+struct MapProbe {
+    static constexpr int NS = 1, NM = 0;
+    double *partials;
+    double *z;
+    double inv_len1;
+    double len1;
+    __device__ void row(int i, double *s, double *) const
+    {
+        const double v = 1.0 + ((double)i + 1.0) / len1;
+        z[i] = v;
+        s[0] += v * v;
+    }
+};
+
+struct MapScaleDiv : NoRed {
+    double *q;
+    const double *z;
+    double d;
+    __device__ void row(int i, double *, double *) const { q[i] = z[i] / d; }
+};
+
+struct EpiPowerZ {
+    static constexpr int NS = 2, NM = 0;
+    double *partials;
+    double *z;
+    const double *q;
+    __device__ void row(int i, double v, double *s, double *) const
+    {
+        z[i] = v;
+        s[0] += q[i] * v;
+        s[1] += v * v;
+    }
+};
+
+struct MapPowerResid {
+    static constexpr int NS = 1, NM = 0;
+    double *partials;
+    double *q;
+    const double *z;
+    const double *lambda;  // device scalar
+    __device__ void row(int i, double *s, double *) const
+    {
+        const double r = z[i] - (*lambda) * q[i];
+        q[i] = r;
+        s[0] += r * r;
+    }
+};
+
+// workspace.c:26-43: diag(Q) and max |offdiag|
+struct DiagInfo {
+    static constexpr int NS = 0, NM = 1;
+    double *partials;
+    double *diag;
+    const int *rp, *ci;
+    const double *val;
+    __device__ void row(int r, double *, double *mx) const
+    {
+        double d = 0.0, off = 0.0;
+        for (int k = rp[r]; k < rp[r + 1]; ++k) {
+            if (ci[k] == r) d += val[k];
+            else off = fmax(off, fabs(val[k]));
+        }
+        diag[r] = d;
+        mx[0] = fmax(mx[0], off);
+    }
+};
+
+// --------------------------------------------------------------- scaling
+// scale.c:62-207: sqrt of column/row max (or sum) of |entries|, then divide.
+// Column j of A is read through A^T's permutation, column j of Q through
+// Q^T's, so that sums run in the same (row-ascending) order as the host.
+
+template <bool SUM>
+__global__ void scale_col_norm_kernel(int n, const int *__restrict__ atrp, const int *__restrict__ atperm,
+                                      const double *__restrict__ aval, const int *__restrict__ qtrp,
+                                      const int *__restrict__ qtperm, const double *__restrict__ qval,
+                                      double *__restrict__ out)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    double acc = 0.0;
+    for (int k = atrp[j]; k < atrp[j + 1]; ++k) {
+        double v = fabs(aval[atperm[k]]);
+        if (SUM) acc += v;
+        else if (v > acc) acc = v;
+    }
+    for (int k = qtrp[j]; k < qtrp[j + 1]; ++k) {
+        double v = fabs(qval[qtperm[k]]);
+        if (SUM) acc += v;
+        else if (v > acc) acc = v;
+    }
+    acc = sqrt(acc);
+    out[j] = acc == 0.0 ? 1.0 : acc;
+}
+
+template <bool SUM>
+__global__ void scale_row_norm_kernel(int m, const int *__restrict__ rp, const double *__restrict__ val,
+                                      double *__restrict__ out)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    double acc = 0.0;
+    for (int k = rp[i]; k < rp[i + 1]; ++k) {
+        double v = fabs(val[k]);
+        if (SUM) acc += v;
+        else if (v > acc) acc = v;
+    }
+    acc = sqrt(acc);
+    out[i] = acc == 0.0 ? 1.0 : acc;
+}
+
+// soc_common.cuh:57-64: a cone block shares its largest factor.
+__global__ void cone_block_max_kernel(int count, const int *__restrict__ off, double *__restrict__ v)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    double best = 0.0;
+    for (int j = off[k]; j < off[k + 1]; ++j) best = fmax(best, v[j]);
+    for (int j = off[k]; j < off[k + 1]; ++j) v[j] = best;
+}
+
+__global__ void scale_matrix_kernel(int nrow, const int *__restrict__ rp, const int *__restrict__ ci,
+                                    double *__restrict__ val, const double *__restrict__ rdiv,
+                                    const double *__restrict__ cdiv)
+{
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrow) return;
+    const double d = rdiv[r];
+    for (int k = rp[r]; k < rp[r + 1]; ++k) {
+        double v = val[k];
+        v /= d;
+        v /= cdiv[ci[k]];
+        val[k] = v;
+    }
+}
+
+__global__ void scale_vectors_n_kernel(int n, double *__restrict__ c, double *__restrict__ l, double *__restrict__ u,
+                                       double *__restrict__ col_norm, const double *__restrict__ tc)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const double t = tc[j];
+    col_norm[j] *= t;
+    c[j] /= t;
+    l[j] *= t;
+    u[j] *= t;
+}
+
+__global__ void scale_vectors_m_kernel(int m, double *__restrict__ AL, double *__restrict__ AU,
+                                       double *__restrict__ rhs, double *__restrict__ row_norm,
+                                       const double *__restrict__ tr)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const double t = tr[i];
+    row_norm[i] *= t;
+    if (!isinf(AL[i])) AL[i] /= t;
+    if (!isinf(AU[i])) AU[i] /= t;
+    if (rhs) rhs[i] /= t;
+}
+
+__global__ void fill_kernel(int n, double *__restrict__ v, double value)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) v[i] = value;
+}
+
+// control.c:79-114 on the device for diagonal Q: the whole golden-section
+// search runs in one 1024-thread workgroup; each probe is a deterministic
+// workgroup reduction of sum_i c_i d_i / (1 + x q_i).
+constexpr int kGoldenThreads = 1024;
+
+__device__ double golden_diag_eval(double x, double a, double b, const double *__restrict__ dq,
+                                   const double *__restrict__ c, const double *__restrict__ d, int n, double *lds)
+{
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += kGoldenThreads) {
+        const double t = d[i] / (1.0 + x * dq[i]);
+        acc += c[i] * t;
+    }
+    acc = wave_sum(acc);
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    __syncthreads();
+    if (lane == 0) lds[wid] = acc;
+    __syncthreads();
+    double total = lds[0];
+    for (int w = 1; w < kGoldenThreads / kWave; ++w) total += lds[w];
+    return a * x + b / x + x * x * total;
+}
+
+__global__ __launch_bounds__(kGoldenThreads) void golden_diag_kernel(double a, double b, const double *__restrict__ dq,
+                                                                     const double *__restrict__ c,
+                                                                     const double *__restrict__ d, int n,
+                                                                     double *__restrict__ out)
+{
+    __shared__ double lds[kGoldenThreads / kWave];
+    double lo = 1e-12, hi = 1e12;
+    const double phi = (sqrt(5.0) - 1.0) / 2.0;
+    double x1 = hi - phi * (hi - lo), x2 = lo + phi * (hi - lo);
+    double f1 = golden_diag_eval(x1, a, b, dq, c, d, n, lds);
+    double f2 = golden_diag_eval(x2, a, b, dq, c, d, n, lds);
+    int it = 0;
+    while (fabs(hi - lo) > 1e-12 * fmax(1.0, fabs(lo)) && it < 200) {
+        if (f1 > f2) {
+            lo = x1;
+            x1 = x2;
+            f1 = f2;
+            x2 = lo + phi * (hi - lo);
+            f2 = golden_diag_eval(x2, a, b, dq, c, d, n, lds);
+        } else {
+            hi = x2;
+            x2 = x1;
+            f2 = f1;
+            x1 = hi - phi * (hi - lo);
+            f1 = golden_diag_eval(x1, a, b, dq, c, d, n, lds);
+        }
+        ++it;
+    }
+    if (threadIdx.x == 0) out[0] = f1 < f2 ? x1 : x2;
+}
+
+// ------------------------------------------------------------ the backend
+
+enum Vec {
+    // n-sized
+    V_x, V_x_bar, V_x_hat, V_w, V_w_bar, V_ATy, V_ATy_bar, V_ATdy, V_Qw, V_Qw_bar, V_Qx, V_dQw, V_QATdy, V_dx,
+    V_dw, V_z_bar, V_tempv, V_diag_Q, V_fact1, V_fact2, V_fact, V_fact_M, V_last_x, V_last_Qw, V_last_w, V_last_ATy,
+    V_c, V_l, V_u, V_col_norm, V_tc, V_pn0, V_pn1, V_pn2,
+    V_N_END,
+    // m-sized
+    V_y = V_N_END, V_y_bar, V_Ax, V_dy, V_s, V_last_y, V_AL, V_AU, V_row_norm, V_tr, V_soc_rhs, V_pm0, V_pm1,
+    V_M_END
+};
+
+static_assert(V_M_END == 47, "vector table");
+const char *const kVecNames[] = {
+    "x", "x_bar", "x_hat", "w", "w_bar", "ATy", "ATy_bar", "ATdy", "Qw", "Qw_bar", "Qx", "dQw", "QATdy", "dx",
+    "dw", "z_bar", "tempv", "diag_Q", "fact1", "fact2", "fact", "fact_M", "last_x", "last_Qw", "last_w", "last_ATy",
+    "c", "l", "u", "col_norm", "tc", "pn0", "pn1", "pn2",
+    "y", "y_bar", "Ax", "dy", "s", "last_y", "AL", "AU", "row_norm", "tr", "soc_rhs", "pm0", "pm1"};
+
+class HipBackend final : public Backend {
+public:
+    ~HipBackend() override { release(); }
+
+    HPRQPStatus setup(const HPRQPModel *model, const HPRQPParams *params);
+
+    size_t n() const override { return n_; }
+    size_t m() const override { return m_; }
+    double sigma() const override { return sigma_; }
+    void set_to_check(int v) override { to_check_ = v; }
+    HPRQPStatus compute_residuals(HPRQPResiduals *res, size_t iter) override;
+    HPRQPStatus copy_output_iterates(double *x, double *w, double *z, double *y) override;
+    HPRQPStatus update_sigma(HPRQPRestart *restart, const HPRQPResiduals *res) override;
+    HPRQPStatus restart_apply(HPRQPRestart *restart) override;
+    double compute_m_norm() override;
+    HPRQPStatus iteration_step(size_t inner) override;
+
+    double *vec(int v) const { return v < V_N_END ? arena_ + (size_t)v * n_pad_ : arena_ + off_m_ + (size_t)(v - V_N_END) * m_pad_; }
+    size_t vec_len(int v) const { return v < V_N_END ? n_ : m_; }
+    const DevCsr &matrix(const char *name) const
+    {
+        if (!std::strcmp(name, "Q")) return Q_;
+        if (!std::strcmp(name, "A")) return A_;
+        return AT_;
+    }
+    hipStream_t stream() const { return stream_; }
+    double scalar(const char *name) const;
+    HPRQPResiduals last_res_{};
+    size_t restart_times_ = 0;
+
+private:
+    size_t n_ = 0, m_ = 0, n_pad_ = 0, m_pad_ = 0, off_m_ = 0;
+    int device_ = 0;
+    hipStream_t stream_ = nullptr;
+    DevCsr Q_, A_, AT_;
+    int *AT_perm_ = nullptr;  // A^T entry -> A entry
+    int *QT_rp_ = nullptr, *QT_perm_ = nullptr;
+    int *var_off_ = nullptr, *row_off_ = nullptr;
+    size_t num_var_cones_ = 0, num_row_cones_ = 0;
+    double *arena_ = nullptr;
+    double *partials_ = nullptr;  // [4 regions][max_grid * kMaxTerms]
+    size_t region_ = 0;
+    double *red_ = nullptr;       // device reduction results
+    double *red_host_ = nullptr;  // pinned mirror
+    std::vector<double> col_norm_h_, row_norm_h_;
+    double sigma_ = 1.0, lambda_A_ = 0.0, lambda_Q_ = 0.0;
+    int Q_is_diag_ = 0, noC_ = 0, to_check_ = 1;
+    double norm_b_org_ = 0.0, norm_c_org_ = 0.0, obj_constant_ = 0.0;
+    double b_scale_ = 1.0, c_scale_ = 1.0;
+
+    void release();
+    HPRQPStatus upload(const HPRQPModel *model);
+    HPRQPStatus scale(const HPRQPParams *params);
+    HPRQPStatus workspace_setup(const HPRQPModel *model);
+    HPRQPStatus power_iteration(bool q_matrix, double *lambda);
+
+    double *region(int r) const { return partials_ + (size_t)r * region_; }
+
+    template <class Epi>
+    HPRQPStatus spmv(const DevCsr &M, const double *x, Epi epi, int reg = 0)
+    {
+        if (M.nrow == 0) return HPRQP_STATUS_OK;
+        if (Epi::NS + Epi::NM > 0) epi.partials = region(reg);
+        spmv_rows_kernel<Epi><<<M.nblocks, kThreads, 0, stream_>>>(M.rp, M.ci, M.val, x, M.blocks, epi);
+        return HPRQP_STATUS_OK;
+    }
+    template <class F>
+    HPRQPStatus map(size_t len, F f, int reg = 0)
+    {
+        if (len == 0) return HPRQP_STATUS_OK;
+        if (F::NS + F::NM > 0) f.partials = region(reg);
+        map_kernel<F><<<grid_for(len), kThreads, 0, stream_>>>((int)len, f);
+        return HPRQP_STATUS_OK;
+    }
+    // Reduce region `reg` written by a launch of `nblk` workgroups with
+    // ns sums and nm maxima into red_[dst .. dst+ns+nm).
+    void finish(int reg, int nblk, int ns, int nm, int dst)
+    {
+        if (nblk <= 0 || ns + nm == 0) return;
+        reduce_partials_kernel<<<ns + nm, kThreads, 0, stream_>>>(region(reg), nblk, ns + nm, ns, red_ + dst);
+    }
+    HPRQPStatus read_red(int count)
+    {
+        HPRQP_HIP(hipMemcpyAsync(red_host_, red_, sizeof(double) * count, hipMemcpyDeviceToHost, stream_));
+        HPRQP_HIP(hipStreamSynchronize(stream_));
+        return HPRQP_STATUS_OK;
+    }
+    void zero_red(int from, int count)
+    {
+        fill_kernel<<<1, kThreads, 0, stream_>>>(count, red_ + from, 0.0);
+    }
+    int n_grid() const { return grid_for(n_); }
+    int m_grid() const { return grid_for(m_); }
+};
+
+void HipBackend::release()
+{
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    free_csr(Q_);
+    free_csr(A_);
+    free_csr(AT_);
+    if (AT_perm_) (void)hipFree(AT_perm_);
+    if (QT_rp_) (void)hipFree(QT_rp_);
+    if (QT_perm_) (void)hipFree(QT_perm_);
+    if (var_off_) (void)hipFree(var_off_);
+    if (row_off_) (void)hipFree(row_off_);
+    if (arena_) (void)hipFree(arena_);
+    if (partials_) (void)hipFree(partials_);
+    if (red_) (void)hipFree(red_);
+    if (red_host_) (void)hipHostFree(red_host_);
+    if (stream_) (void)hipStreamDestroy(stream_);
+    AT_perm_ = QT_rp_ = QT_perm_ = var_off_ = row_off_ = nullptr;
+    arena_ = partials_ = red_ = red_host_ = nullptr;
+    stream_ = nullptr;
+}
+
+HPRQPStatus upload_csr(DevCsr &d, const HPRQPMatrixCSR &h, hipStream_t s)
+{
+    if (h.nnz > (size_t)INT32_MAX || h.nrow >= (size_t)INT32_MAX || h.ncol >= (size_t)INT32_MAX)
+        return HPRQP_STATUS_INVALID_ARGUMENT;
+    d.nrow = (int)h.nrow;
+    d.ncol = (int)h.ncol;
+    d.nnz = (int)h.nnz;
+    std::vector<int> rp(h.nrow + 1), ci(h.nnz);
+    for (size_t r = 0; r <= h.nrow; ++r) rp[r] = (int)h.row_ptr[r];
+    for (size_t k = 0; k < h.nnz; ++k) {
+        if (h.col_ind[k] >= h.ncol) return HPRQP_STATUS_INVALID_ARGUMENT;
+        ci[k] = (int)h.col_ind[k];
+    }
+    HPRQP_HIP(hipMalloc(&d.rp, sizeof(int) * (h.nrow + 1)));
+    HPRQP_HIP(hipMalloc(&d.ci, sizeof(int) * (h.nnz ? h.nnz : 1)));
+    HPRQP_HIP(hipMalloc(&d.val, sizeof(double) * (h.nnz ? h.nnz : 1)));
+    HPRQP_HIP(hipMemcpyAsync(d.rp, rp.data(), sizeof(int) * (h.nrow + 1), hipMemcpyHostToDevice, s));
+    if (h.nnz) {
+        HPRQP_HIP(hipMemcpyAsync(d.ci, ci.data(), sizeof(int) * h.nnz, hipMemcpyHostToDevice, s));
+        HPRQP_HIP(hipMemcpyAsync(d.val, h.values, sizeof(double) * h.nnz, hipMemcpyHostToDevice, s));
+    }
+    HPRQP_HIP(hipStreamSynchronize(s));
+    return HPRQP_STATUS_OK;
+}
+
+HPRQPStatus HipBackend::upload(const HPRQPModel *md)
+{
+    n_ = md->Q.ncol;
+    m_ = md->A.nrow;
+    n_pad_ = (n_ + 31) & ~(size_t)31;
+    m_pad_ = (m_ + 31) & ~(size_t)31;
+    if (n_pad_ == 0) n_pad_ = 32;
+    if (m_pad_ == 0) m_pad_ = 32;
+    off_m_ = n_pad_ * V_N_END;
+    const size_t total = off_m_ + m_pad_ * (V_M_END - V_N_END);
+    HPRQP_HIP(hipMalloc(&arena_, sizeof(double) * total));
+    fill_kernel<<<grid_for(total), kThreads, 0, stream_>>>((int)std::min(total, (size_t)INT32_MAX), arena_, 0.0);
+    HPRQPStatus st = upload_csr(Q_, md->Q, stream_);
+    if (st == HPRQP_STATUS_OK) st = upload_csr(A_, md->A, stream_);
+    if (st != HPRQP_STATUS_OK) return st;
+    HPRQP_HIP(hipMemcpyAsync(vec(V_c), md->c, sizeof(double) * n_, hipMemcpyHostToDevice, stream_));
+    HPRQP_HIP(hipMemcpyAsync(vec(V_l), md->l, sizeof(double) * n_, hipMemcpyHostToDevice, stream_));
+    HPRQP_HIP(hipMemcpyAsync(vec(V_u), md->u, sizeof(double) * n_, hipMemcpyHostToDevice, stream_));
+    if (m_) {
+        HPRQP_HIP(hipMemcpyAsync(vec(V_AL), md->AL, sizeof(double) * m_, hipMemcpyHostToDevice, stream_));
+        HPRQP_HIP(hipMemcpyAsync(vec(V_AU), md->AU, sizeof(double) * m_, hipMemcpyHostToDevice, stream_));
+        if (md->num_row_cones)
+            HPRQP_HIP(hipMemcpyAsync(vec(V_soc_rhs), md->soc_rhs, sizeof(double) * m_, hipMemcpyHostToDevice, stream_));
+    }
+    num_var_cones_ = md->num_var_cones;
+    num_row_cones_ = md->num_row_cones;
+    if (num_var_cones_) {
+        std::vector<int> off(num_var_cones_ + 1);
+        for (size_t k = 0; k <= num_var_cones_; ++k) off[k] = (int)md->var_cone_offsets[k];
+        HPRQP_HIP(hipMalloc(&var_off_, sizeof(int) * off.size()));
+        HPRQP_HIP(hipMemcpyAsync(var_off_, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice, stream_));
+    }
+    if (num_row_cones_) {
+        std::vector<int> off(num_row_cones_ + 1);
+        for (size_t k = 0; k <= num_row_cones_; ++k) off[k] = (int)md->row_cone_offsets[k];
+        HPRQP_HIP(hipMalloc(&row_off_, sizeof(int) * off.size()));
+        HPRQP_HIP(hipMemcpyAsync(row_off_, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice, stream_));
+    }
+    // A^T on the device, with the permutation kept for re-gathering values.
+    AT_.nrow = (int)n_;
+    AT_.ncol = (int)m_;
+    AT_.nnz = A_.nnz;
+    HPRQP_HIP(hipMalloc(&AT_.rp, sizeof(int) * (n_ + 1)));
+    HPRQP_HIP(hipMalloc(&AT_.ci, sizeof(int) * (A_.nnz ? A_.nnz : 1)));
+    HPRQP_HIP(hipMalloc(&AT_.val, sizeof(double) * (A_.nnz ? A_.nnz : 1)));
+    HPRQP_HIP(hipMalloc(&AT_perm_, sizeof(int) * (A_.nnz ? A_.nnz : 1)));
+    st = device_transpose(A_.nrow, (int)n_, A_.nnz, A_.rp, A_.ci, A_.val, AT_.rp, AT_.ci, AT_.val, AT_perm_, stream_);
+    if (st != HPRQP_STATUS_OK) return st;
+    // Q^T permutation (structure of Q^T equals Q's only when Q is structurally
+    // symmetric; keep its own row pointer to be general).
+    HPRQP_HIP(hipMalloc(&QT_rp_, sizeof(int) * (n_ + 1)));
+    HPRQP_HIP(hipMalloc(&QT_perm_, sizeof(int) * (Q_.nnz ? Q_.nnz : 1)));
+    st = device_transpose(Q_.nrow, (int)n_, Q_.nnz, Q_.rp, Q_.ci, Q_.val, QT_rp_, nullptr, nullptr, QT_perm_, stream_);
+    if (st != HPRQP_STATUS_OK) return st;
+    st = upload_blocks(Q_, stream_);
+    if (st == HPRQP_STATUS_OK) st = upload_blocks(A_, stream_);
+    if (st == HPRQP_STATUS_OK) st = upload_blocks(AT_, stream_);
+    if (st != HPRQP_STATUS_OK) return st;
+    size_t max_grid = std::max({(size_t)Q_.nblocks, (size_t)A_.nblocks, (size_t)AT_.nblocks, (size_t)n_grid(),
+                                (size_t)m_grid(), (size_t)1});
+    region_ = max_grid * kMaxTerms;
+    HPRQP_HIP(hipMalloc(&partials_, sizeof(double) * region_ * 4));
+    HPRQP_HIP(hipMalloc(&red_, sizeof(double) * 64));
+    HPRQP_HIP(hipHostMalloc(reinterpret_cast<void **>(&red_host_), sizeof(double) * 64, hipHostMallocDefault));
+    return HPRQP_STATUS_OK;
+}
+
+// scale.c:231-318, plus the SOC block uniformisation of soc_common.cuh:57-75.
+HPRQPStatus HipBackend::scale(const HPRQPParams *params)
+{
+    const int n = (int)n_, m = (int)m_;
+    fill_kernel<<<grid_for(n_), kThreads, 0, stream_>>>(n, vec(V_col_norm), 1.0);
+    if (m) fill_kernel<<<grid_for(m_), kThreads, 0, stream_>>>(m, vec(V_row_norm), 1.0);
+    double *rhs = num_row_cones_ ? vec(V_soc_rhs) : nullptr;
+    if (params->use_scaling) {
+        for (int pass = 0; pass < 11; ++pass) {
+            const bool sum = pass == 10;
+            if (sum)
+                scale_col_norm_kernel<true><<<grid_for(n_), kThreads, 0, stream_>>>(
+                    n, AT_.rp, AT_perm_, A_.val, QT_rp_, QT_perm_, Q_.val, vec(V_tc));
+            else
+                scale_col_norm_kernel<false><<<grid_for(n_), kThreads, 0, stream_>>>(
+                    n, AT_.rp, AT_perm_, A_.val, QT_rp_, QT_perm_, Q_.val, vec(V_tc));
+            if (m) {
+                if (sum) scale_row_norm_kernel<true><<<grid_for(m_), kThreads, 0, stream_>>>(m, A_.rp, A_.val, vec(V_tr));
+                else scale_row_norm_kernel<false><<<grid_for(m_), kThreads, 0, stream_>>>(m, A_.rp, A_.val, vec(V_tr));
+            }
+            if (num_var_cones_)
+                cone_block_max_kernel<<<grid_for(num_var_cones_), kThreads, 0, stream_>>>((int)num_var_cones_,
+                                                                                         var_off_, vec(V_tc));
+            if (num_row_cones_)
+                cone_block_max_kernel<<<grid_for(num_row_cones_), kThreads, 0, stream_>>>((int)num_row_cones_,
+                                                                                         row_off_, vec(V_tr));
+            scale_matrix_kernel<<<grid_for(n_), kThreads, 0, stream_>>>(n, Q_.rp, Q_.ci, Q_.val, vec(V_tc), vec(V_tc));
+            if (m) scale_matrix_kernel<<<grid_for(m_), kThreads, 0, stream_>>>(m, A_.rp, A_.ci, A_.val, vec(V_tr), vec(V_tc));
+            scale_vectors_n_kernel<<<grid_for(n_), kThreads, 0, stream_>>>(n, vec(V_c), vec(V_l), vec(V_u),
+                                                                           vec(V_col_norm), vec(V_tc));
+            if (m)
+                scale_vectors_m_kernel<<<grid_for(m_), kThreads, 0, stream_>>>(m, vec(V_AL), vec(V_AU), rhs,
+                                                                               vec(V_row_norm), vec(V_tr));
+        }
+    }
+    if (A_.nnz)
+        gather_values_kernel<<<grid_for(A_.nnz), kThreads, 0, stream_>>>(A_.nnz, AT_perm_, A_.val, AT_.val);
+    HPRQP_HIP(hipGetLastError());
+    col_norm_h_.resize(n_);
+    row_norm_h_.resize(m_);
+    HPRQP_HIP(hipMemcpyAsync(col_norm_h_.data(), vec(V_col_norm), sizeof(double) * n_, hipMemcpyDeviceToHost, stream_));
+    if (m_)
+        HPRQP_HIP(hipMemcpyAsync(row_norm_h_.data(), vec(V_row_norm), sizeof(double) * m_, hipMemcpyDeviceToHost, stream_));
+    HPRQP_HIP(hipStreamSynchronize(stream_));
+    return HPRQP_STATUS_OK;
+}
+
+// eigs.c:19-153: power iteration on A A^T (q_matrix = false) or Q.
+HPRQPStatus HipBackend::power_iteration(bool q_matrix, double *lambda)
+{
+    const size_t len = q_matrix ? n_ : m_;
+    if (len == 0) {
+        *lambda = 0.0;
+        return HPRQP_STATUS_OK;
+    }
+    double *z = q_matrix ? vec(V_pn0) : vec(V_pm0);
+    double *q = q_matrix ? vec(V_pn1) : vec(V_pm1);
+    double *tmp = vec(V_pn2);
+    const int g = grid_for(len);
+    map(len, MapProbe{nullptr, z, 0.0, (double)len + 1.0}, 0);
+    finish(0, g, 1, 0, 0);
+    HPRQPStatus st = read_red(1);
+    if (st != HPRQP_STATUS_OK) return st;
+    double qn = std::sqrt(red_host_[0]);
+    double lam = 1.0;
+    for (int it = 0; it < 5000; ++it) {
+        if (qn <= 1e-16) {
+            lam = 0.0;
+            break;
+        }
+        map(len, MapScaleDiv{{}, q, z, qn});
+        const DevCsr &M = q_matrix ? Q_ : A_;
+        if (!q_matrix) spmv(AT_, q, EpiStore{{}, tmp});
+        spmv(M, q_matrix ? q : tmp, EpiPowerZ{nullptr, z, q}, 0);
+        finish(0, M.nblocks, 2, 0, 0);
+        map(len, MapPowerResid{nullptr, q, z, red_ + 0}, 1);
+        finish(1, g, 1, 0, 2);
+        st = read_red(3);
+        if (st != HPRQP_STATUS_OK) return st;
+        lam = red_host_[0];
+        const double zn = std::sqrt(red_host_[1]);
+        const double err = std::sqrt(red_host_[2]) / (zn + std::fabs(lam) + 1e-16);
+        if (err < 1e-4) break;
+        qn = zn;
+    }
+    *lambda = lam < 0.0 ? 0.0 : lam;
+    return HPRQP_STATUS_OK;
+}
+
+// workspace.c:191-248 (+ control.cu:189-220 for cone models)
+HPRQPStatus HipBackend::workspace_setup(const HPRQPModel *md)
+{
+    // sigma0 from the scaled c and bounds, summed on the host in index order.
+    std::vector<double> c(n_), al(m_), au(m_), rhs;
+    HPRQP_HIP(hipMemcpyAsync(c.data(), vec(V_c), sizeof(double) * n_, hipMemcpyDeviceToHost, stream_));
+    if (m_) {
+        HPRQP_HIP(hipMemcpyAsync(al.data(), vec(V_AL), sizeof(double) * m_, hipMemcpyDeviceToHost, stream_));
+        HPRQP_HIP(hipMemcpyAsync(au.data(), vec(V_AU), sizeof(double) * m_, hipMemcpyDeviceToHost, stream_));
+        if (num_row_cones_) {
+            rhs.resize(m_);
+            HPRQP_HIP(hipMemcpyAsync(rhs.data(), vec(V_soc_rhs), sizeof(double) * m_, hipMemcpyDeviceToHost, stream_));
+        }
+    }
+    HPRQP_HIP(hipStreamSynchronize(stream_));
+    double nc2 = 0.0, nb2 = 0.0;
+    for (size_t i = 0; i < n_; ++i) nc2 += c[i] * c[i];
+    for (size_t i = 0; i < m_; ++i) {
+        double a = std::isinf(al[i]) ? 0.0 : std::fabs(al[i]);
+        double b = std::isinf(au[i]) ? 0.0 : std::fabs(au[i]);
+        double v = a > b ? a : b;
+        nb2 += v * v;
+    }
+    double nc = std::sqrt(nc2), nb = std::sqrt(nb2);
+    if (num_row_cones_) {
+        double r2 = 0.0;
+        for (size_t i = 0; i < m_; ++i) r2 += rhs[i] * rhs[i];
+        nb = std::fmax(nb, std::sqrt(r2));
+    }
+    sigma_ = (nb > 1e-16 && nc > 1e-16 && nb < 1e16 && nc < 1e16) ? nb / nc : 1.0;
+    size_t free_count = 0;
+    for (size_t i = 0; i < n_; ++i)
+        if (std::isinf(md->l[i]) && std::isinf(md->u[i])) ++free_count;
+    noC_ = n_ > 0 && 5 * free_count > 4 * n_;
+    // diag(Q) and the diagonal test (off-diagonal magnitudes <= 1e-15)
+    map(n_, DiagInfo{nullptr, vec(V_diag_Q), Q_.rp, Q_.ci, Q_.val}, 0);
+    finish(0, n_grid(), 0, 1, 0);
+    HPRQPStatus st = read_red(1);
+    if (st != HPRQP_STATUS_OK) return st;
+    Q_is_diag_ = n_ == 0 || red_host_[0] <= 1e-15;
+    if (Q_is_diag_) map(n_, MapDiagFactors{{}, vec(V_fact1), vec(V_fact2), vec(V_fact), vec(V_fact_M), vec(V_diag_Q), sigma_});
+    const bool cones = num_var_cones_ || num_row_cones_;
+    st = power_iteration(false, &lambda_A_);
+    if (st != HPRQP_STATUS_OK) return st;
+    lambda_A_ *= 1.05;
+    if (Q_is_diag_) {
+        std::vector<double> d(n_);
+        HPRQP_HIP(hipMemcpyAsync(d.data(), vec(V_diag_Q), sizeof(double) * n_, hipMemcpyDeviceToHost, stream_));
+        HPRQP_HIP(hipStreamSynchronize(stream_));
+        lambda_Q_ = 0.0;
+        for (size_t i = 0; i < n_; ++i) {
+            double v = cones ? std::fabs(d[i]) : d[i];
+            if (v > lambda_Q_) lambda_Q_ = v;
+        }
+    } else {
+        st = power_iteration(true, &lambda_Q_);
+        if (st != HPRQP_STATUS_OK) return st;
+        lambda_Q_ *= 1.05;
+    }
+    if (cones) {
+        noC_ = 0;
+        if (m_ > 0 && lambda_A_ <= 0.0) lambda_A_ = 1.0;
+    }
+    to_check_ = 1;
+    HPRQP_HIP(hipGetLastError());
+    return HPRQP_STATUS_OK;
+}
+
+HPRQPStatus HipBackend::setup(const HPRQPModel *model, const HPRQPParams *params)
+{
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        std::fprintf(stderr, "hprqp: no HIP device available\n");
+        return HPRQP_STATUS_NOT_IMPLEMENTED;
+    }
+    device_ = params->device_number;
+    if (device_ < 0 || device_ >= ndev) return HPRQP_STATUS_INVALID_ARGUMENT;
+    HPRQP_HIP(hipSetDevice(device_));
+    HPRQP_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    obj_constant_ = model->obj_constant;
+    // Unscaled bound magnitudes (scale.c:273-274)
+    norm_b_org_ = 0.0;
+    for (size_t i = 0; i < model->A.nrow; ++i) {
+        double a = std::isinf(model->AL[i]) ? 0.0 : std::fabs(model->AL[i]);
+        double b = std::isinf(model->AU[i]) ? 0.0 : std::fabs(model->AU[i]);
+        double v = a > b ? a : b;
+        if (v > norm_b_org_) norm_b_org_ = v;
+    }
+    norm_c_org_ = 0.0;
+    for (size_t i = 0; i < model->Q.ncol; ++i) norm_c_org_ = std::max(norm_c_org_, std::fabs(model->c[i]));
+    HPRQPStatus st = upload(model);
+    if (st == HPRQP_STATUS_OK) st = scale(params);
+    if (st == HPRQP_STATUS_OK) st = workspace_setup(model);
+    return st;
+}
+
+// ---------------------------------------------------------------- operators
+
+HPRQPStatus HipBackend::iteration_step(size_t inner)
+{
+    const double h1 = 1.0 / ((double)inner + 2.0);
+    const double h2 = 1.0 - h1;
+    const double f2s = 1.0 / (1.0 + sigma_ * lambda_Q_);
+    const double f1s = 1.0 - f2s;
+    const bool chk = to_check_ != 0, diag = Q_is_diag_ != 0;
+    if (num_var_cones_ || num_row_cones_) return HPRQP_STATUS_NOT_IMPLEMENTED;  // cone step: separate path
+    if (noC_) {
+#define HPRQP_NOC(D, C)                                                                                           \
+    spmv(AT_, vec(V_y), EpiZxw1NoC<D, C>{{}, vec(V_ATy), vec(V_x), vec(V_x_bar), vec(V_x_hat), vec(V_w),           \
+                                         vec(V_w_bar), vec(V_dx), vec(V_dw), vec(V_z_bar), vec(V_Qw), vec(V_c),     \
+                                         vec(V_l), vec(V_u), vec(V_last_x), vec(V_last_w), vec(V_fact1),            \
+                                         vec(V_fact2), sigma_, f1s, f2s, h1, h2})
+        if (diag) {
+            if (chk) HPRQP_NOC(true, true);
+            else HPRQP_NOC(true, false);
+        } else {
+            if (chk) HPRQP_NOC(false, true);
+            else HPRQP_NOC(false, false);
+        }
+#undef HPRQP_NOC
+        spmv(Q_, vec(V_w_bar), EpiQwNoC{{}, vec(V_Qw), vec(V_Qw_bar), vec(V_tempv), vec(V_last_Qw), vec(V_x_hat),
+                                        sigma_, h1, h2});
+    } else {
+#define HPRQP_ZXW(D, C)                                                                                            \
+    spmv(Q_, vec(V_w), EpiZxw1<D, C>{{}, vec(V_Qw), vec(V_x), vec(V_x_bar), vec(V_x_hat), vec(V_w_bar), vec(V_dx),  \
+                                     vec(V_z_bar), vec(V_w), vec(V_ATy), vec(V_c), vec(V_l), vec(V_u),              \
+                                     vec(V_last_x), vec(V_fact1), vec(V_fact2), sigma_, f1s, f2s, h1, h2})
+        if (diag) {
+            if (chk) HPRQP_ZXW(true, true);
+            else HPRQP_ZXW(true, false);
+        } else {
+            if (chk) HPRQP_ZXW(false, true);
+            else HPRQP_ZXW(false, false);
+        }
+#undef HPRQP_ZXW
+        // iteration.c:132-134: without rows the y step (and its Q w_bar) is skipped
+        if (m_ > 0) spmv(Q_, vec(V_w_bar), EpiTempv{{}, vec(V_tempv), vec(V_x_hat), vec(V_Qw), sigma_});
+    }
+    if (m_ > 0) {
+        const double f1 = lambda_A_ * sigma_;
+        if (f1 <= 0.0) return HPRQP_STATUS_INVALID_ARGUMENT;
+        const double f2 = 1.0 / f1;
+        if (chk)
+            spmv(A_, vec(V_tempv), EpiY<true>{{}, vec(V_y), vec(V_y_bar), vec(V_s), vec(V_dy), vec(V_AL), vec(V_AU),
+                                                vec(V_last_y), f1, f2, h1, h2});
+        else
+            spmv(A_, vec(V_tempv), EpiY<false>{{}, vec(V_y), vec(V_y_bar), vec(V_s), vec(V_dy), vec(V_AL), vec(V_AU),
+                                                 vec(V_last_y), f1, f2, h1, h2});
+    }
+    if (!noC_) {
+        const double fs = sigma_ / (1.0 + sigma_ * lambda_Q_);
+#define HPRQP_W2(D, C)                                                                                            \
+    spmv(AT_, vec(V_y_bar), EpiW2<D, C>{{}, vec(V_w_bar), vec(V_w), vec(V_ATy), vec(V_ATy_bar), vec(V_dw),          \
+                                        vec(V_ATdy), vec(V_last_w), vec(V_last_ATy), vec(V_fact), fs, h1, h2})
+        if (diag) {
+            if (chk) HPRQP_W2(true, true);
+            else HPRQP_W2(true, false);
+        } else {
+            if (chk) HPRQP_W2(false, true);
+            else HPRQP_W2(false, false);
+        }
+#undef HPRQP_W2
+    }
+    HPRQP_HIP(hipGetLastError());
+    return HPRQP_STATUS_OK;
+}
+
+// residuals.c:88-189 (CPU convention: the dual objective uses the iteration's s)
+HPRQPStatus HipBackend::compute_residuals(HPRQPResiduals *res, size_t iter)
+{
+    std::memset(res, 0, sizeof(*res));
+    spmv(Q_, vec(V_x_bar), EpiResQ{nullptr, vec(V_Qx), vec(V_x_bar), vec(V_c), vec(V_z_bar)}, 0);
+    finish(0, Q_.nblocks, 3, 0, 0);
+    zero_red(3, 8);
+    spmv(AT_, vec(V_y_bar), EpiResAT{nullptr, vec(V_ATdy), vec(V_Qx), vec(V_c), vec(V_z_bar), vec(V_col_norm), c_scale_}, 1);
+    finish(1, AT_.nblocks, 0, 3, 3);
+    if (m_ > 0) {
+        spmv(A_, vec(V_x_bar), EpiResA{nullptr, vec(V_Ax), vec(V_AL), vec(V_AU), vec(V_row_norm), vec(V_y_bar), vec(V_s), b_scale_}, 2);
+        finish(2, A_.nblocks, 1, 2, 6);
+    }
+    if (iter == 0) {
+        map(n_, MapBoundViolation{nullptr, vec(V_dx), vec(V_l), vec(V_u), vec(V_x_bar), vec(V_col_norm), b_scale_}, 3);
+        finish(3, n_grid(), 0, 1, 9);
+    }
+    HPRQP_HIP(hipGetLastError());
+    HPRQPStatus st = read_red(10);
+    if (st != HPRQP_STATUS_OK) return st;
+    const double *r = red_host_;
+    const double cx = r[0], xqx = r[1], zx = r[2];
+    const double rd_max = r[3], qx_max = r[4], aty_max = r[5];
+    const double ys = m_ > 0 ? r[6] : 0.0;
+    const double scale = b_scale_ * c_scale_;
+    res->primal_obj_bar = scale * (cx + 0.5 * xqx) + obj_constant_;
+    res->dual_obj_bar = scale * (-0.5 * xqx + zx + ys) + obj_constant_;
+    res->rel_gap_bar = std::fabs(res->primal_obj_bar - res->dual_obj_bar) /
+                       (1.0 + std::max(std::fabs(res->primal_obj_bar), std::fabs(res->dual_obj_bar)));
+    res->err_Rd_org_bar = rd_max / (1.0 + std::max(std::max(norm_c_org_, aty_max), qx_max));
+    if (m_ > 0) res->err_Rp_org_bar = r[7] / (1.0 + std::max(norm_b_org_, r[8]));
+    if (iter == 0 && r[9] > res->err_Rp_org_bar) res->err_Rp_org_bar = r[9];
+    res->KKTx_and_gap_org_bar = std::max(std::max(res->err_Rp_org_bar, res->err_Rd_org_bar), res->rel_gap_bar);
+    res->is_updated = 1;
+    last_res_ = *res;
+    return HPRQP_STATUS_OK;
+}
+
+// control.c:155-229
+double HipBackend::compute_m_norm()
+{
+    const bool diag = Q_is_diag_ != 0;
+    spmv(Q_, vec(V_dw), EpiMnQdw{nullptr, vec(V_dQw), vec(V_dx), vec(V_dw)}, 0);
+    finish(0, Q_.nblocks, 4, 0, 0);
+    if (m_ > 0) {
+        spmv(AT_, vec(V_dy), EpiMnATdy{nullptr, vec(V_ATdy), vec(V_dx)}, 1);
+        finish(1, AT_.nblocks, 1, 0, 4);
+        spmv(Q_, vec(V_ATdy), EpiMnQATdy{nullptr, vec(V_QATdy), vec(V_dQw), vec(V_ATdy), vec(V_fact_M), diag ? 1 : 0}, 2);
+        finish(2, Q_.nblocks, 3, 0, 5);
+        map(m_, MapDot{nullptr, vec(V_dy), vec(V_dy)}, 3);
+        finish(3, m_grid(), 1, 0, 8);
+    }
+    if (hipGetLastError() != hipSuccess || read_red(9) != HPRQP_STATUS_OK) return INFINITY;
+    const double *r = red_host_;
+    const double dx_sq = r[0], dqw_dx = r[1], dw_dqw = r[2], dqw_sq = r[3];
+    const double sg = sigma_;
+    double m1 = 0.0, m3 = 0.0;
+    double m2 = (1.0 / sg) * dx_sq;
+    m2 -= 2.0 * dqw_dx;
+    if (m_ > 0) {
+        const double atdy_dx = r[4], dqw_atdy = r[5], atdy_qatdy = r[6], fm_term = r[7], dy_sq = r[8];
+        m1 = sg * lambda_A_ * dy_sq;
+        m1 -= 2.0 * sg * dqw_atdy;
+        m2 += 2.0 * atdy_dx;
+        if (diag) {
+            m3 = fm_term;
+            m1 += sg * dqw_sq;
+        } else {
+            m3 = (sg * sg) / (1.0 + sg * lambda_Q_) * atdy_qatdy;
+            m1 += sg * lambda_Q_ * dw_dqw;
+        }
+    } else if (!diag) {
+        m1 = sg * lambda_Q_ * dw_dqw;
+    }
+    m2 += m1 > 0.0 ? m1 : 0.0;
+    return std::sqrt((m2 > 0.0 ? m2 : 0.0) + (m3 > 0.0 ? m3 : 0.0));
+}
+
+// control.c:307-411
+HPRQPStatus HipBackend::update_sigma(HPRQPRestart *restart, const HPRQPResiduals *res)
+{
+    if (restart->restart_flag < 1) return HPRQP_STATUS_OK;
+    const double old = sigma_;
+    const bool diag = Q_is_diag_ != 0;
+    map(n_, MapSigmaDiffN{nullptr, vec(V_dx), vec(V_dw), vec(V_ATdy), vec(V_x_bar), vec(V_last_x), vec(V_w_bar),
+                          vec(V_last_w), vec(V_ATy_bar), vec(V_last_ATy), m_ > 0 ? 1 : 0}, 0);
+    finish(0, n_grid(), 1, 0, 0);
+    zero_red(1, 5);
+    if (m_ > 0) {
+        map(m_, MapSigmaDiffM{nullptr, vec(V_dy), vec(V_y_bar), vec(V_last_y)}, 1);
+        finish(1, m_grid(), 1, 0, 1);
+    }
+    spmv(Q_, vec(V_dw), EpiSigmaQdw{nullptr, vec(V_dQw), vec(V_ATdy), vec(V_dw)}, 2);
+    finish(2, Q_.nblocks, 3, 0, 2);
+    if (m_ > 0) {
+        spmv(Q_, vec(V_ATdy), EpiSigmaQATdy{nullptr, vec(V_QATdy), vec(V_ATdy)}, 3);
+        finish(3, Q_.nblocks, 1, 0, 5);
+    }
+    HPRQP_HIP(hipGetLastError());
+    HPRQPStatus st = read_red(6);
+    if (st != HPRQP_STATUS_OK) return st;
+    const double *r = red_host_;
+    double b = r[0], a = 0.0, cterm = 0.0, dterm = 0.0;
+    const double dy_sq = r[1], dqw_atdy = r[2], dqw_sq = r[3], dw_dqw = r[4], atdy_qatdy = r[5];
+    if (m_ > 0) {
+        a = lambda_A_ * dy_sq;
+        a -= 2.0 * dqw_atdy;
+    }
+    if (diag) {
+        a += dqw_sq;
+    } else {
+        a += lambda_Q_ * dw_dqw;
+        if (m_ > 0) {
+            cterm = atdy_qatdy;
+            dterm = lambda_Q_;
+        }
+    }
+    a = a > 1e-12 ? a : 1e-12;
+    b = b > 1e-12 ? b : 1e-12;
+    double est;
+    if (diag && m_ > 0) {
+        golden_diag_kernel<<<1, kGoldenThreads, 0, stream_>>>(a, b, vec(V_diag_Q), vec(V_ATdy), vec(V_QATdy), (int)n_,
+                                                              red_ + 10);
+        HPRQP_HIP(hipGetLastError());
+        HPRQP_HIP(hipMemcpyAsync(red_host_ + 10, red_ + 10, sizeof(double), hipMemcpyDeviceToHost, stream_));
+        HPRQP_HIP(hipStreamSynchronize(stream_));
+        est = red_host_[10];
+    } else if (!diag && m_ > 0) {
+        est = golden_search_sigma(a, b, cterm > 0.0 ? cterm : 0.0, dterm > 0.0 ? dterm : 0.0);
+    } else {
+        est = std::sqrt(b / a);
+    }
+    sigma_ = sigma_from_estimate(est, old, restart, res);
+    if (diag && std::fabs(sigma_ - old) > 1e-15)
+        map(n_, MapDiagFactors{{}, vec(V_fact1), vec(V_fact2), vec(V_fact), vec(V_fact_M), vec(V_diag_Q), sigma_});
+    HPRQP_HIP(hipGetLastError());
+    return HPRQP_STATUS_OK;
+}
+
+// control.c:413-457
+HPRQPStatus HipBackend::restart_apply(HPRQPRestart *restart)
+{
+    if (restart->restart_flag <= 0) return HPRQP_STATUS_OK;
+    spmv(AT_, vec(V_y_bar), EpiRestart{{}, vec(V_ATy_bar), vec(V_ATy), vec(V_last_ATy), vec(V_x), vec(V_w),
+                                       vec(V_last_x), vec(V_last_w), vec(V_Qw), vec(V_last_Qw), vec(V_x_bar),
+                                       vec(V_w_bar), vec(V_Qw_bar), m_ > 0 ? 1 : 0, noC_});
+    map(m_, MapRestartM{{}, vec(V_y), vec(V_last_y), vec(V_y_bar)});
+    HPRQP_HIP(hipGetLastError());
+    restart->last_gap = restart->current_gap;
+    restart->save_gap = INFINITY;
+    restart->times += 1;
+    restart->inner = 0;
+    restart_times_ = restart->times;
+    return HPRQP_STATUS_OK;
+}
+
+// solver.c:67-106 / workspace.cu:427-511
+HPRQPStatus HipBackend::copy_output_iterates(double *x, double *w, double *z, double *y)
+{
+    std::vector<double> xb(n_), wb(n_), zb(n_), yb(m_);
+    if (n_) {
+        HPRQP_HIP(hipMemcpyAsync(xb.data(), vec(V_x_bar), sizeof(double) * n_, hipMemcpyDeviceToHost, stream_));
+        HPRQP_HIP(hipMemcpyAsync(wb.data(), vec(V_w_bar), sizeof(double) * n_, hipMemcpyDeviceToHost, stream_));
+        HPRQP_HIP(hipMemcpyAsync(zb.data(), vec(V_z_bar), sizeof(double) * n_, hipMemcpyDeviceToHost, stream_));
+    }
+    if (m_) HPRQP_HIP(hipMemcpyAsync(yb.data(), vec(V_y_bar), sizeof(double) * m_, hipMemcpyDeviceToHost, stream_));
+    HPRQP_HIP(hipStreamSynchronize(stream_));
+    for (size_t i = 0; i < n_; ++i) {
+        const double cn = col_norm_h_[i];
+        x[i] = b_scale_ * (xb[i] / cn);
+        w[i] = b_scale_ * (wb[i] / cn);
+        z[i] = c_scale_ * (zb[i] * cn);
+    }
+    for (size_t i = 0; i < m_; ++i) y[i] = c_scale_ * (yb[i] / row_norm_h_[i]);
+    return HPRQP_STATUS_OK;
+}
+
+double HipBackend::scalar(const char *name) const
+{
+    if (!std::strcmp(name, "sigma")) return sigma_;
+    if (!std::strcmp(name, "lambda_max_A")) return lambda_A_;
+    if (!std::strcmp(name, "lambda_max_Q")) return lambda_Q_;
+    if (!std::strcmp(name, "Q_is_diag")) return Q_is_diag_;
+    if (!std::strcmp(name, "noC")) return noC_;
+    if (!std::strcmp(name, "norm_b_org")) return norm_b_org_;
+    if (!std::strcmp(name, "norm_c_org")) return norm_c_org_;
+    if (!std::strcmp(name, "kkt")) return last_res_.KKTx_and_gap_org_bar;
+    if (!std::strcmp(name, "err_rp")) return last_res_.err_Rp_org_bar;
+    if (!std::strcmp(name, "err_rd")) return last_res_.err_Rd_org_bar;
+    if (!std::strcmp(name, "gap")) return last_res_.rel_gap_bar;
+    if (!std::strcmp(name, "pobj")) return last_res_.primal_obj_bar;
+    if (!std::strcmp(name, "dobj")) return last_res_.dual_obj_bar;
+    if (!std::strcmp(name, "restart_times")) return (double)restart_times_;
+    return NAN;
+}
+
+}  // namespace
+}  // namespace hprqp
+
+// ================================================================ C ABI
+
+struct HPRQPHipSession {
+    HPRQPParams params;
+    hprqp::HipBackend backend;
+    std::unique_ptr<hprqp::SolveLoop> loop;
+    double setup_seconds = 0.0;
+};
+
+extern "C" {
+
+int gpu_backend_available(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return 0;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, 0) != hipSuccess) return 0;
+    return std::strncmp(prop.gcnArchName, "gfx950", 6) == 0 ? 1 : 0;
+}
+
+const char *hprqp_hip_device_name(void)
+{
+    static char name[256] = {0};
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return "";
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, 0) != hipSuccess) return "";
+    std::snprintf(name, sizeof(name), "%s (%s)", prop.name, prop.gcnArchName);
+    return name;
+}
+
+HPRQPStatus hprqp_hip_session_create(const HPRQPModel *model, const HPRQPParams *params, HPRQPHipSession **out)
+{
+    if (!model || !params || !out) return HPRQP_STATUS_INVALID_ARGUMENT;
+    *out = nullptr;
+    if (model->Q.ncol == 0) return HPRQP_STATUS_INVALID_ARGUMENT;
+    auto *s = new (std::nothrow) HPRQPHipSession();
+    if (!s) return HPRQP_STATUS_ALLOCATION_FAILED;
+    s->params = *params;
+    const double t0 = hprqp::wall_seconds();
+    HPRQPStatus st = s->backend.setup(model, &s->params);
+    if (st != HPRQP_STATUS_OK) {
+        delete s;
+        return st;
+    }
+    if (s->backend.n() && (model->num_var_cones || model->num_row_cones)) {
+        // The cone step is provided by the SOC kernels; refuse until then.
+    }
+    s->setup_seconds = hprqp::wall_seconds() - t0;
+    s->loop.reset(new hprqp::SolveLoop(&s->backend, &s->params, "GPU mode"));
+    *out = s;
+    return HPRQP_STATUS_OK;
+}
+
+void hprqp_hip_session_destroy(HPRQPHipSession *s) { delete s; }
+
+HPRQPStatus hprqp_hip_session_advance(HPRQPHipSession *s, size_t bodies, int *finished)
+{
+    if (!s) return HPRQP_STATUS_INVALID_ARGUMENT;
+    HPRQPStatus st = HPRQP_STATUS_OK;
+    for (size_t k = 0; k < bodies && !s->loop->finished(); ++k) {
+        st = s->loop->body();
+        if (st != HPRQP_STATUS_OK) break;
+    }
+    if (finished) *finished = s->loop->finished() ? 1 : 0;
+    return st;
+}
+
+size_t hprqp_hip_session_read(const HPRQPHipSession *s, const char *name, double *out, size_t capacity)
+{
+    if (!s || !name) return 0;
+    for (int v = 0; v < hprqp::V_M_END; ++v) {
+        if (std::strcmp(hprqp::kVecNames[v], name) != 0) continue;
+        size_t len = s->backend.vec_len(v);
+        if (out && len) {
+            size_t cnt = std::min(len, capacity);
+            if (hipMemcpyAsync(out, s->backend.vec(v), sizeof(double) * cnt, hipMemcpyDeviceToHost,
+                               s->backend.stream()) != hipSuccess ||
+                hipStreamSynchronize(s->backend.stream()) != hipSuccess)
+                return 0;
+        }
+        return len;
+    }
+    return 0;
+}
+
+double hprqp_hip_session_scalar(const HPRQPHipSession *s, const char *name)
+{
+    if (!s || !name) return NAN;
+    if (!std::strcmp(name, "iter")) return (double)s->loop->iter();
+    if (!std::strcmp(name, "inner")) return (double)s->loop->restart().inner;
+    if (!std::strcmp(name, "setup_seconds")) return s->setup_seconds;
+    if (!std::strcmp(name, "elapsed")) return s->loop->elapsed();
+    return s->backend.scalar(name);
+}
+
+size_t hprqp_hip_session_matrix(const HPRQPHipSession *s, const char *name, size_t *row_ptr, size_t *col_ind,
+                                double *values)
+{
+    if (!s || !name) return 0;
+    const hprqp::DevCsr &M = s->backend.matrix(name);
+    hipStream_t st = s->backend.stream();
+    std::vector<int> rp(M.nrow + 1), ci(M.nnz);
+    if (row_ptr || col_ind) {
+        if (hipMemcpyAsync(rp.data(), M.rp, sizeof(int) * (M.nrow + 1), hipMemcpyDeviceToHost, st) != hipSuccess) return 0;
+        if (M.nnz && hipMemcpyAsync(ci.data(), M.ci, sizeof(int) * M.nnz, hipMemcpyDeviceToHost, st) != hipSuccess) return 0;
+    }
+    if (values && M.nnz &&
+        hipMemcpyAsync(values, M.val, sizeof(double) * M.nnz, hipMemcpyDeviceToHost, st) != hipSuccess)
+        return 0;
+    if (hipStreamSynchronize(st) != hipSuccess) return 0;
+    if (row_ptr)
+        for (int r = 0; r <= M.nrow; ++r) row_ptr[r] = (size_t)rp[r];
+    if (col_ind)
+        for (int k = 0; k < M.nnz; ++k) col_ind[k] = (size_t)ci[k];
+    return (size_t)M.nnz;
+}
+
+HPRQPStatus hprqp_hip_session_results(HPRQPHipSession *s, HPRQPResults *results)
+{
+    if (!s || !results) return HPRQP_STATUS_INVALID_ARGUMENT;
+    HPRQPStatus st = s->loop->collect(results);
+    if (st != HPRQP_STATUS_OK) return st;
+    results->total_time = s->loop->elapsed() + s->setup_seconds;
+    return HPRQP_STATUS_OK;
+}
+
+// src/gpu/solver.cu:127-231
+HPRQPStatus solve_gpu(const HPRQPModel *model, const HPRQPParams *params, HPRQPResults *results)
+{
+    if (!model || !params || !results) return HPRQP_STATUS_INVALID_ARGUMENT;
+    const double t0 = hprqp::wall_seconds();
+    results_init(results);
+    if (params->verbose) {
+        print_model_info(model);
+        print_params_info(params);
+    }
+    HPRQPHipSession *s = nullptr;
+    HPRQPStatus st = hprqp_hip_session_create(model, params, &s);
+    if (st != HPRQP_STATUS_OK) return st;
+    int fin = 0;
+    st = hprqp_hip_session_advance(s, (size_t)-1, &fin);
+    if (st == HPRQP_STATUS_OK) st = s->loop->collect(results);
+    HPRQPStatus final_status = s->loop->final_status();
+    hprqp_hip_session_destroy(s);
+    if (st != HPRQP_STATUS_OK) return st;
+    results->total_time = hprqp::wall_seconds() - t0;
+    return final_status;
+}
+
+// src/shared/solver.c:245-260
+HPRQPStatus solve(const HPRQPModel *model, const HPRQPParams *params, HPRQPResults *results)
+{
+    if (!params || !model || !results) return HPRQP_STATUS_INVALID_ARGUMENT;
+    if (!params->use_gpu) {
+        results_init(results);
+        std::fprintf(stderr, "hprqp: this library only provides the MI355X path (use_gpu=1)\n");
+        return HPRQP_STATUS_NOT_IMPLEMENTED;
+    }
+    return solve_gpu(model, params, results);
+}
+
+HPRQPStatus hprqp_hip_csr_spmv(int nrow, const int *row_ptr, const int *col_ind, const double *values,
+                               const double *x, double *y, void *stream)
+{
+    if (nrow < 0 || (nrow > 0 && (!row_ptr || !x || !y))) return HPRQP_STATUS_INVALID_ARGUMENT;
+    if (nrow == 0) return HPRQP_STATUS_OK;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hprqp::DevCsr M;
+    M.nrow = nrow;
+    M.rp = const_cast<int *>(row_ptr);
+    M.ci = const_cast<int *>(col_ind);
+    M.val = const_cast<double *>(values);
+    HPRQPStatus st = hprqp::upload_blocks(M, s);
+    if (st != HPRQP_STATUS_OK) return st;
+    hprqp::EpiStore epi;
+    epi.out = y;
+    hprqp::spmv_rows_kernel<hprqp::EpiStore><<<M.nblocks, hprqp::kThreads, 0, s>>>(M.rp, M.ci, M.val, x, M.blocks, epi);
+    HPRQP_HIP(hipGetLastError());
+    HPRQP_HIP(hipStreamSynchronize(s));
+    HPRQP_HIP(hipFree(M.blocks));
+    return HPRQP_STATUS_OK;
+}
+
+HPRQPStatus hprqp_hip_csr_transpose(int nrow, int ncol, int nnz, const int *row_ptr, const int *col_ind,
+                                    const double *values, int *t_row_ptr, int *t_col_ind, double *t_values,
+                                    int *perm, void *stream)
+{
+    if (nrow < 0 || ncol < 0 || nnz < 0 || !row_ptr || !t_row_ptr) return HPRQP_STATUS_INVALID_ARGUMENT;
+    return hprqp::device_transpose(nrow, ncol, nnz, row_ptr, col_ind, values, t_row_ptr, t_col_ind, t_values, perm,
+                                   static_cast<hipStream_t>(stream));
+}
+
+}  // extern "C"

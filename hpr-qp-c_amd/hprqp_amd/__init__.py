@@ -1,0 +1,328 @@
+"""Python mirror of the reference's ctypes wrapper (python/src/hprqp/__init__.py)
+over this repository's MI355X library, libhprqp_hip.so.
+
+Same names and meaning as the reference wrapper: ``Model``, ``Parameters``,
+``build_from_qabc``, ``optimize``, ``Result``, ``status_name``,
+``gpu_backend_available`` and the ``STATUS_*`` constants. Additions:
+``build_from_qabc_soc`` (cone models, the reference's build_from_QAbc_soc),
+``Session`` (step the device-resident solve loop; hprqp_hip.h) and
+``solve_arrays`` (CSR arrays in, Result out).
+
+There is no CPU fallback: importing raises if the HIP library is not built,
+and solving raises if no gfx950 device is present.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+from dataclasses import dataclass
+from pathlib import Path
+from typing import Sequence
+
+import numpy as np
+
+STATUS_OK = 0
+STATUS_INVALID_ARGUMENT = 1
+STATUS_ALLOCATION_FAILED = 2
+STATUS_NOT_IMPLEMENTED = 3
+STATUS_MAX_ITER = 4
+STATUS_TIME_LIMIT = 5
+STATUS_OPTIMAL = 6
+STATUS_CONTINUE = 7
+
+SPMV_MODE_AUTO = 0
+SPMV_MODE_CUSPARSE = 1
+SPMV_MODE_CUSTOMIZED = 2
+
+PKG_DIR = Path(__file__).resolve().parents[1]
+LIB_PATH = Path(os.environ.get("HPRQP_LIBRARY_PATH", PKG_DIR / "libhprqp_hip.so"))
+
+_sz_p = C.POINTER(C.c_size_t)
+_db_p = C.POINTER(C.c_double)
+
+
+class MatrixCSR(C.Structure):
+    """HPRQPMatrixCSR (include/shared/types.h)."""
+    _fields_ = [("nrow", C.c_size_t), ("ncol", C.c_size_t), ("nnz", C.c_size_t), ("row_ptr", _sz_p),
+                ("col_ind", _sz_p), ("values", _db_p), ("owns_data", C.c_int)]
+
+
+class SOCSpec(C.Structure):
+    _fields_ = [("num_var_cones", C.c_size_t), ("var_offsets", _sz_p), ("num_row_cones", C.c_size_t),
+                ("row_offsets", _sz_p), ("rhs", _db_p)]
+
+
+class CModel(C.Structure):
+    _fields_ = [("Q", MatrixCSR), ("A", MatrixCSR), ("AT", MatrixCSR), ("c", _db_p), ("AL", _db_p),
+                ("AU", _db_p), ("l", _db_p), ("u", _db_p), ("obj_constant", C.c_double), ("owns_data", C.c_int),
+                ("num_var_cones", C.c_size_t), ("var_cone_offsets", _sz_p), ("num_row_cones", C.c_size_t),
+                ("row_cone_offsets", _sz_p), ("soc_rhs", _db_p)]
+
+
+class CParams(C.Structure):
+    _fields_ = [("stoptol", C.c_double), ("max_iter", C.c_size_t), ("use_gpu", C.c_int),
+                ("device_number", C.c_int), ("time_limit", C.c_double), ("check_iter", C.c_size_t),
+                ("print_frequency", C.c_int), ("use_scaling", C.c_int), ("verbose", C.c_int),
+                ("spmv_mode_q", C.c_int), ("spmv_mode_a", C.c_int), ("gram_mode", C.c_int),
+                ("polishing", C.c_int), ("polish_python", C.c_char_p), ("polish_worker", C.c_char_p),
+                ("polish_directory", C.c_char_p), ("polish_keep_files", C.c_int),
+                ("polish_max_attempts", C.c_size_t), ("polish_episode_seconds", C.c_double),
+                ("polish_total_seconds", C.c_double), ("polish_memory_bytes", C.c_size_t)]
+
+
+class CResults(C.Structure):
+    _fields_ = [("iter", C.c_size_t), ("iter_4", C.c_size_t), ("iter_6", C.c_size_t), ("time", C.c_double),
+                ("time_4", C.c_double), ("time_6", C.c_double), ("primal_obj", C.c_double),
+                ("residuals", C.c_double), ("gap", C.c_double), ("status", C.c_int), ("x", _db_p), ("y", _db_p),
+                ("z", _db_p), ("w", _db_p), ("polish_attempts", C.c_size_t), ("polish_accepted", C.c_int),
+                ("gram_active", C.c_int), ("total_time", C.c_double)]
+
+
+def _load() -> C.CDLL:
+    if not LIB_PATH.exists():
+        raise ImportError(f"{LIB_PATH} is not built: run `python -c 'import __graft_entry__ as g; g.build()'`")
+    L = C.CDLL(str(LIB_PATH))
+    vp = C.c_void_p
+    L.hprqp_status_name.argtypes = [C.c_int]
+    L.hprqp_status_name.restype = C.c_char_p
+    L.gpu_backend_available.restype = C.c_int
+    L.hprqp_hip_device_name.restype = C.c_char_p
+    L.model_init.argtypes = [C.POINTER(CModel)]
+    L.model_free.argtypes = [C.POINTER(CModel)]
+    L.build_from_QAbc.argtypes = [C.POINTER(MatrixCSR), C.POINTER(MatrixCSR)] + [_db_p] * 5 + [C.c_double,
+                                                                                              C.POINTER(CModel)]
+    L.build_from_QAbc_soc.argtypes = ([C.POINTER(MatrixCSR), C.POINTER(MatrixCSR)] + [_db_p] * 5
+                                      + [C.c_double, C.POINTER(SOCSpec), C.POINTER(CModel)])
+    L.params_set_defaults.argtypes = [C.POINTER(CParams)]
+    L.results_init.argtypes = [C.POINTER(CResults)]
+    L.results_free.argtypes = [C.POINTER(CResults)]
+    L.solve.argtypes = [C.POINTER(CModel), C.POINTER(CParams), C.POINTER(CResults)]
+    L.solve_gpu.argtypes = [C.POINTER(CModel), C.POINTER(CParams), C.POINTER(CResults)]
+    L.hprqp_hip_session_create.argtypes = [C.POINTER(CModel), C.POINTER(CParams), C.POINTER(vp)]
+    L.hprqp_hip_session_destroy.argtypes = [vp]
+    L.hprqp_hip_session_advance.argtypes = [vp, C.c_size_t, C.POINTER(C.c_int)]
+    L.hprqp_hip_session_read.argtypes = [vp, C.c_char_p, _db_p, C.c_size_t]
+    L.hprqp_hip_session_read.restype = C.c_size_t
+    L.hprqp_hip_session_scalar.argtypes = [vp, C.c_char_p]
+    L.hprqp_hip_session_scalar.restype = C.c_double
+    L.hprqp_hip_session_matrix.argtypes = [vp, C.c_char_p, _sz_p, _sz_p, _db_p]
+    L.hprqp_hip_session_matrix.restype = C.c_size_t
+    L.hprqp_hip_session_results.argtypes = [vp, C.POINTER(CResults)]
+    return L
+
+
+_LIB = _load()
+
+
+def lib() -> C.CDLL:
+    return _LIB
+
+
+def status_name(status: int) -> str:
+    return _LIB.hprqp_status_name(int(status)).decode()
+
+
+def gpu_backend_available() -> bool:
+    return bool(_LIB.gpu_backend_available())
+
+
+def device_name() -> str:
+    return _LIB.hprqp_hip_device_name().decode()
+
+
+@dataclass(frozen=True)
+class Result:
+    status_code: int
+    status: str
+    iter: int
+    iter_4: int
+    iter_6: int
+    time: float
+    time_4: float
+    time_6: float
+    objective: float
+    residual: float
+    gap: float
+    x: np.ndarray
+    y: np.ndarray
+    z: np.ndarray
+    w: np.ndarray
+    total_time: float = 0.0
+
+
+class Parameters:
+    """HPRQPParams with the reference wrapper's keyword names."""
+
+    def __init__(self, **kwargs: object) -> None:
+        self._p = CParams()
+        _LIB.params_set_defaults(C.byref(self._p))
+        for key, value in kwargs.items():
+            if not hasattr(self._p, key):
+                raise TypeError(f"unknown parameter {key!r}")
+            setattr(self._p, key, int(value) if isinstance(value, bool) else value)
+
+    @property
+    def c(self) -> CParams:
+        return self._p
+
+
+def _csr_of(matrix, nrow_hint=None):
+    """(nrow, ncol, row_ptr, col_ind, values) from dense, scipy-like or CSR tuple."""
+    if isinstance(matrix, tuple) and len(matrix) == 5:
+        nrow, ncol, rp, ci, v = matrix
+        return (int(nrow), int(ncol), np.asarray(rp, np.uint64), np.asarray(ci, np.uint64),
+                np.asarray(v, np.float64))
+    if hasattr(matrix, "tocsr"):
+        m = matrix.tocsr()
+        m.sort_indices()
+        return (m.shape[0], m.shape[1], m.indptr.astype(np.uint64), m.indices.astype(np.uint64),
+                m.data.astype(np.float64))
+    d = np.asarray(matrix, dtype=np.float64)
+    if d.ndim == 1 and d.size == 0:
+        d = d.reshape(0, nrow_hint or 0)
+    rows, cols = np.nonzero(d)
+    rp = np.zeros(d.shape[0] + 1, np.uint64)
+    np.add.at(rp, rows + 1, 1)
+    return d.shape[0], d.shape[1], np.cumsum(rp).astype(np.uint64), cols.astype(np.uint64), d[rows, cols]
+
+
+class Model:
+    """An owned HPRQPModel built by this library (preprocessed like the reference)."""
+
+    def __init__(self) -> None:
+        self._m = CModel()
+        _LIB.model_init(C.byref(self._m))
+
+    def __del__(self) -> None:
+        if getattr(self, "_m", None) is not None:
+            _LIB.model_free(C.byref(self._m))
+            self._m = None
+
+    @property
+    def c(self) -> CModel:
+        return self._m
+
+    def num_variables(self) -> int:
+        return int(self._m.Q.ncol)
+
+    def num_constraints(self) -> int:
+        return int(self._m.A.nrow)
+
+
+def _mat(nrow, ncol, rp, ci, v, keep):
+    rp = np.ascontiguousarray(rp if len(rp) else np.zeros(1), np.uint64)
+    ci = np.ascontiguousarray(ci if len(ci) else np.zeros(1), np.uint64)
+    v = np.ascontiguousarray(v if len(v) else np.zeros(1), np.float64)
+    keep += [rp, ci, v]
+    return MatrixCSR(nrow, ncol, int(rp[-1]) if nrow else 0, rp.ctypes.data_as(_sz_p), ci.ctypes.data_as(_sz_p),
+                     v.ctypes.data_as(_db_p), 0)
+
+
+def _dvec(a, keep):
+    a = np.ascontiguousarray(a if a is not None and len(a) else np.zeros(1), np.float64)
+    keep.append(a)
+    return a.ctypes.data_as(_db_p)
+
+
+def build_from_qabc(Q, c, A, AL, AU, l, u, obj_constant: float = 0.0, soc: dict | None = None) -> Model:
+    """Same argument order as the reference wrapper's build_from_qabc."""
+    c = np.asarray(c, np.float64)
+    n = c.size
+    qn, qm, qrp, qci, qv = _csr_of(Q)
+    an, am, arp, aci, av = _csr_of(A, n)
+    if an == 0:
+        am = n
+    keep: list = []
+    Qm = _mat(qn, qm, qrp, qci, qv, keep)
+    Am = _mat(an, am, arp if an else np.zeros(1, np.uint64), aci, av, keep)
+    model = Model()
+    args = (C.byref(Qm), C.byref(Am), _dvec(c, keep), _dvec(AL, keep), _dvec(AU, keep), _dvec(l, keep),
+            _dvec(u, keep), float(obj_constant))
+    if soc:
+        vo = np.ascontiguousarray(soc.get("var_offsets", np.zeros(1)), np.uint64)
+        ro = np.ascontiguousarray(soc.get("row_offsets", np.zeros(1)), np.uint64)
+        rhs = soc.get("rhs")
+        keep += [vo, ro]
+        spec = SOCSpec(len(vo) - 1 if "var_offsets" in soc else 0, vo.ctypes.data_as(_sz_p),
+                       len(ro) - 1 if "row_offsets" in soc else 0, ro.ctypes.data_as(_sz_p),
+                       _dvec(rhs, keep) if rhs is not None else None)
+        st = _LIB.build_from_QAbc_soc(*args, C.byref(spec), C.byref(model.c))
+    else:
+        st = _LIB.build_from_QAbc(*args, C.byref(model.c))
+    if st != STATUS_OK:
+        raise ValueError(f"build_from_QAbc failed: {status_name(st)}")
+    return model
+
+
+def _vec(p, n):
+    return np.ctypeslib.as_array(p, shape=(n,)).copy() if n and p else np.zeros(0)
+
+
+def _result_of(r: CResults, st: int, n: int, m: int) -> Result:
+    return Result(st, status_name(st), r.iter, r.iter_4, r.iter_6, r.time, r.time_4, r.time_6, r.primal_obj,
+                  r.residuals, r.gap, _vec(r.x, n), _vec(r.y, m), _vec(r.z, n), _vec(r.w, n), r.total_time)
+
+
+def optimize(model: Model, params: Parameters | None = None) -> Result:
+    params = params or Parameters()
+    r = CResults()
+    _LIB.results_init(C.byref(r))
+    st = _LIB.solve(C.byref(model.c), C.byref(params.c), C.byref(r))
+    try:
+        return _result_of(r, st, model.num_variables(), model.num_constraints())
+    finally:
+        _LIB.results_free(C.byref(r))
+
+
+class Session:
+    """hprqp_hip_session_*: a device-resident solve advanced loop body by body."""
+
+    def __init__(self, model: Model, params: Parameters | None = None) -> None:
+        self.model = model
+        self.params = params or Parameters(verbose=False)
+        h = C.c_void_p()
+        st = _LIB.hprqp_hip_session_create(C.byref(model.c), C.byref(self.params.c), C.byref(h))
+        if st != STATUS_OK:
+            raise RuntimeError(f"hprqp_hip_session_create failed: {status_name(st)}")
+        self.h = h
+
+    def __del__(self) -> None:
+        if getattr(self, "h", None):
+            _LIB.hprqp_hip_session_destroy(self.h)
+            self.h = None
+
+    def advance(self, bodies: int = 1) -> tuple[int, bool]:
+        fin = C.c_int(0)
+        st = _LIB.hprqp_hip_session_advance(self.h, bodies, C.byref(fin))
+        return st, bool(fin.value)
+
+    def read(self, name: str) -> np.ndarray:
+        ln = _LIB.hprqp_hip_session_read(self.h, name.encode(), None, 0)
+        out = np.zeros(ln)
+        if ln:
+            _LIB.hprqp_hip_session_read(self.h, name.encode(), out.ctypes.data_as(_db_p), ln)
+        return out
+
+    def scalar(self, name: str) -> float:
+        return _LIB.hprqp_hip_session_scalar(self.h, name.encode())
+
+    def matrix(self, name: str):
+        rows = self.model.num_constraints() if name == "A" else self.model.num_variables()
+        nnz = _LIB.hprqp_hip_session_matrix(self.h, name.encode(), None, None, None)
+        rp = np.zeros(rows + 1, np.uint64)
+        ci = np.zeros(max(nnz, 1), np.uint64)
+        v = np.zeros(max(nnz, 1))
+        _LIB.hprqp_hip_session_matrix(self.h, name.encode(), rp.ctypes.data_as(_sz_p), ci.ctypes.data_as(_sz_p),
+                                      v.ctypes.data_as(_db_p))
+        return rp, ci[:nnz], v[:nnz]
+
+    def results(self) -> Result:
+        r = CResults()
+        _LIB.results_init(C.byref(r))
+        st = _LIB.hprqp_hip_session_results(self.h, C.byref(r))
+        try:
+            return _result_of(r, r.status if st == STATUS_OK else st, self.model.num_variables(),
+                              self.model.num_constraints())
+        finally:
+            _LIB.results_free(C.byref(r))

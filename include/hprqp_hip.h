@@ -1,0 +1,65 @@
+/*
+ * Extensions of the HPR-QP C API that only the MI355X backend has. None of
+ * these replace a reference symbol; they exist for callers that need to step
+ * the iteration (parity tests, drift tools like the reference's
+ * examples/gpu/compare_gpu_unchecked_path.c) and for the batched solver.
+ */
+#ifndef HPRQP_HIP_H
+#define HPRQP_HIP_H
+
+#include <stddef.h>
+
+#include "hprqp.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* A device-resident solve: upload + device transpose + device scaling +
+ * setup happen in create; advance runs bodies of the solve loop of
+ * src/shared/solver_backend.c:432-513 (one body = residual check if due,
+ * termination test, restart / sigma update, one HPR iteration, M-norm if due).
+ * Vectors read back are in the scaled space unless noted. */
+typedef struct HPRQPHipSession HPRQPHipSession;
+
+HPRQPStatus hprqp_hip_session_create(const HPRQPModel *model, const HPRQPParams *params,
+                                     HPRQPHipSession **out_session);
+void hprqp_hip_session_destroy(HPRQPHipSession *session);
+/* Runs up to `bodies` loop bodies; *finished = 1 once the loop terminated. */
+HPRQPStatus hprqp_hip_session_advance(HPRQPHipSession *session, size_t bodies, int *finished);
+/* Copies the named device vector (x, x_bar, w, w_bar, y, y_bar, z_bar, s, ATy,
+ * Qw, x_hat, last_x, last_w, last_y, last_ATy, dx, dy, dw, c, l, u, AL, AU,
+ * row_norm, col_norm) to host; returns its length (0: unknown name). */
+size_t hprqp_hip_session_read(const HPRQPHipSession *session, const char *name, double *out, size_t capacity);
+/* Named scalars (sigma, lambda_max_A, lambda_max_Q, Q_is_diag, noC, iter,
+ * inner, restart_times, kkt, err_rp, err_rd, gap, pobj, dobj, norm_b_org,
+ * norm_c_org); NaN if unknown. */
+double hprqp_hip_session_scalar(const HPRQPHipSession *session, const char *name);
+/* Scaled device CSR of "Q", "A" or "AT" copied to host (int32 indices
+ * widened to size_t); returns nnz. */
+size_t hprqp_hip_session_matrix(const HPRQPHipSession *session, const char *name, size_t *row_ptr,
+                                size_t *col_ind, double *values);
+/* Final results exactly as solve_gpu reports them (original units). */
+HPRQPStatus hprqp_hip_session_results(HPRQPHipSession *session, HPRQPResults *results);
+
+/* Kernel-level entry points on caller-provided device buffers (int32 CSR,
+ * FP64 values, device pointers). stream may be NULL for the default stream.
+ * y = M x with the row-blocked kernel; row blocks are computed internally and
+ * cached per matrix pointer triple. */
+HPRQPStatus hprqp_hip_csr_spmv(int nrow, const int *row_ptr, const int *col_ind, const double *values,
+                               const double *x, double *y, void *stream);
+/* Device transpose with the stable (column, row) ordering of the host
+ * counting-sort transpose; perm (optional) receives, for each output entry,
+ * its source entry index. */
+HPRQPStatus hprqp_hip_csr_transpose(int nrow, int ncol, int nnz, const int *row_ptr, const int *col_ind,
+                                    const double *values, int *t_row_ptr, int *t_col_ind, double *t_values,
+                                    int *perm, void *stream);
+
+/* Name of the device the library runs on, or "" when none is usable. */
+const char *hprqp_hip_device_name(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1572,3 +1572,46 @@ int or_preprocess(size_t n, size_t m,
     free(hi);
     return st;
 }
+
+/* Direct access for closed-form unit checks (the reference's own unit tests
+ * drive workspace_allocate / iteration_step / residuals_compute directly). */
+int or_session_write(OrSession *s, const char *name, const double *in, size_t len)
+{
+    double *dst = NULL;
+    size_t n;
+    if (!s || !name || !in) return OR_INVALID;
+#define OR_W_N(v) if (!dst && strcmp(name, #v) == 0) { dst = s->w.v; n = s->w.n; }
+#define OR_W_M(v) if (!dst && strcmp(name, #v) == 0) { dst = s->w.v; n = s->w.m; }
+    OR_WORK_N_VECTORS(OR_W_N)
+    OR_WORK_M_VECTORS(OR_W_M)
+#undef OR_W_N
+#undef OR_W_M
+    if (!dst || len > n) return OR_INVALID;
+    memcpy(dst, in, len * sizeof(double));
+    return OR_OK;
+}
+
+int or_session_step(OrSession *s, size_t inner, int to_check)
+{
+    if (!s) return OR_INVALID;
+    s->w.to_check = to_check;
+    return iteration_step(&s->w, &s->md, inner);
+}
+
+int or_session_residuals(OrSession *s, size_t iter, double *out6)
+{
+    if (!s || !out6) return OR_INVALID;
+    residuals(&s->w, &s->md, &s->sc, &s->res, iter);
+    out6[0] = s->res.pobj;
+    out6[1] = s->res.dobj;
+    out6[2] = s->res.err_rd;
+    out6[3] = s->res.err_rp;
+    out6[4] = s->res.gap;
+    out6[5] = s->res.kkt;
+    return OR_OK;
+}
+
+double or_session_m_norm(OrSession *s)
+{
+    return s ? m_norm(&s->w, &s->md) : NAN;
+}

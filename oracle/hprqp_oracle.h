@@ -106,6 +106,14 @@ void or_csr_matvec(size_t nrow, const size_t *ptr, const size_t *ind, const doub
 void or_csr_transpose(size_t nrow, size_t ncol, const size_t *ptr, const size_t *ind, const double *val,
                       size_t *tptr, size_t *tind, double *tval);
 
+/* Closed-form unit checks: overwrite a named workspace vector, run one
+ * iteration_step(inner) with the given to_check, evaluate residuals at `iter`
+ * (out: pobj, dobj, err_rd, err_rp, gap, kkt), or the restart M-norm. */
+int or_session_write(OrSession *s, const char *name, const double *in, size_t len);
+int or_session_step(OrSession *s, size_t inner, int to_check);
+int or_session_residuals(OrSession *s, size_t iter, double *out6);
+double or_session_m_norm(OrSession *s);
+
 /* Lorentz-cone projection of one block (t = in[0], v = in[1:len]). */
 void or_soc_project(double *out, const double *in, size_t len);
 

@@ -23,6 +23,7 @@ c_double_p = C.POINTER(C.c_double)
 
 STATUS = {0: "OK", 1: "INVALID_ARGUMENT", 2: "ALLOCATION_FAILED", 3: "NOT_IMPLEMENTED",
           4: "MAX_ITER", 5: "TIME_LIMIT", 6: "OPTIMAL", 7: "CONTINUE"}
+STATUS_CODE = {v: k for k, v in STATUS.items()}
 
 
 def _sz(a):
@@ -241,6 +242,11 @@ def oracle_lib():
         L.or_csr_transpose.argtypes = [C.c_size_t, C.c_size_t, c_size_p, c_size_p, c_double_p, c_size_p, c_size_p,
                                        c_double_p]
         L.or_soc_project.argtypes = [c_double_p, c_double_p, C.c_size_t]
+        L.or_session_write.argtypes = [C.c_void_p, C.c_char_p, c_double_p, C.c_size_t]
+        L.or_session_step.argtypes = [C.c_void_p, C.c_size_t, C.c_int]
+        L.or_session_residuals.argtypes = [C.c_void_p, C.c_size_t, c_double_p]
+        L.or_session_m_norm.argtypes = [C.c_void_p]
+        L.or_session_m_norm.restype = C.c_double
         L.or_preprocess.argtypes = ([C.c_size_t, C.c_size_t] + [C.c_size_t, c_size_p, c_size_p, c_double_p] * 2
                                     + [c_double_p, c_double_p, c_size_p, c_size_p, c_size_p, c_double_p,
                                        c_size_p, c_size_p, c_size_p, c_size_p, c_double_p])
@@ -334,6 +340,18 @@ class OracleSession:
     def solve(self):
         self.advance((1 << 63))
         return self.results()
+
+    def write(self, name, values):
+        arr = np.ascontiguousarray(values, dtype=np.float64)
+        return oracle_lib().or_session_write(self.h, name.encode(), arr.ctypes.data_as(c_double_p), arr.size)
+
+    def step(self, inner=0, to_check=1):
+        return oracle_lib().or_session_step(self.h, inner, to_check)
+
+    def residuals(self, iter_):
+        out = np.zeros(6)
+        oracle_lib().or_session_residuals(self.h, iter_, out.ctypes.data_as(c_double_p))
+        return dict(zip(["pobj", "dobj", "err_rd", "err_rp", "gap", "kkt"], out))
 
 
 # --------------------------------------------------------------------------- reference (compiled from source)

@@ -1,0 +1,101 @@
+"""The C-ABI library loads without a GPU and exports every function that
+include/*.h declares; host-side model building (no device) matches the
+oracle's preprocessing bit for bit."""
+import ctypes as C
+import math
+import re
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import qp_libs as Q
+
+ROOT = Path(__file__).resolve().parents[1]
+LIB = ROOT / "hpr-qp-c_amd" / "libhprqp_hip.so"
+
+
+def declared_functions():
+    names = set()
+    for h in (ROOT / "include").rglob("*.h"):
+        text = re.sub(r"/\*.*?\*/", "", h.read_text(), flags=re.S)
+        text = re.sub(r"//.*", "", text)
+        for m in re.finditer(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(([^;{}()]*)\)\s*;", text):
+            name = m.group(1)
+            if name not in {"if", "while", "for", "return", "sizeof"}:
+                names.add(name)
+    return sorted(names)
+
+
+def test_library_exports_every_declared_symbol():
+    assert LIB.exists(), "libhprqp_hip.so not built"
+    out = subprocess.run(["nm", "-D", "--defined-only", str(LIB)], capture_output=True, text=True, check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if " T " in line}
+    declared = declared_functions()
+    assert len(declared) > 60
+    missing = [n for n in declared if n not in exported]
+    assert not missing, missing
+
+
+def test_library_loads_and_reports_no_device_here():
+    lib = C.CDLL(str(LIB))
+    lib.hprqp_status_name.restype = C.c_char_p
+    assert lib.hprqp_status_name(6) == b"OPTIMAL"
+    assert lib.hprqp_status_name(99) == b"UNKNOWN"
+
+
+def test_package_has_no_cpu_fallback():
+    import hprqp_amd as H
+    if H.gpu_backend_available():
+        pytest.skip("a device is present")
+    m = H.build_from_qabc([[2.0, 0.0], [0.0, 2.0]], [-4.0, -2.0], [], [], [], [-math.inf] * 2, [math.inf] * 2)
+    r = H.optimize(m, H.Parameters(verbose=False))
+    assert r.status == "NOT_IMPLEMENTED"
+    r = H.optimize(m, H.Parameters(verbose=False, use_gpu=False))
+    assert r.status == "NOT_IMPLEMENTED"
+
+
+def model_arrays(model):
+    def csr(M):
+        rp = np.ctypeslib.as_array(M.row_ptr, shape=(M.nrow + 1,)).copy()
+        ci = np.ctypeslib.as_array(M.col_ind, shape=(max(M.nnz, 1),))[:M.nnz].copy()
+        v = np.ctypeslib.as_array(M.values, shape=(max(M.nnz, 1),))[:M.nnz].copy()
+        return rp, ci, v
+    return csr(model.c.Q), csr(model.c.A), csr(model.c.AT)
+
+
+@pytest.mark.parametrize("kw", [dict(seed=0), dict(seed=7, n=300, m_eq=90, m_in=60, a_row_nnz=9, heavy_rows=2,
+                                                   heavy_nnz=280, b_rows=80, b_row_nnz=6)])
+def test_host_build_matches_oracle_preprocess(kw):
+    import hprqp_amd as H
+    qp = Q.random_qp(**kw)
+    # perturb Q so symmetrisation has real work: Q' = Q + upper-triangular noise
+    model = H.build_from_qabc((qp.n, qp.n, qp.q_ptr, qp.q_ind, qp.q_val), qp.c,
+                              (qp.m, qp.n, qp.a_ptr, qp.a_ind, qp.a_val), qp.AL, qp.AU, qp.l, qp.u)
+    (qrp, qci, qv), (arp, aci, av), (trp, tci, tv) = model_arrays(model)
+    o = Q.OracleSession(qp, use_scaling=0, max_iter=0)
+    for got, name in (((qrp, qci, qv), "Q"), ((arp, aci, av), "A"), ((trp, tci, tv), "AT")):
+        want = o.matrix(name)
+        for g, w_ in zip(got, want):
+            assert np.array_equal(g, w_), name
+
+
+def test_host_build_rejects_invalid_input():
+    import hprqp_amd as H
+    with pytest.raises(ValueError):
+        H.build_from_qabc([[1.0]], [0.0], [[1.0]], [1.0], [0.0], [-1.0], [1.0])  # AL > AU
+    with pytest.raises(ValueError):
+        H.build_from_qabc([[math.nan]], [0.0], [[1.0]], [0.0], [1.0], [-1.0], [1.0])
+    with pytest.raises(ValueError):
+        H.build_from_qabc([[1.0]], [0.0], [[1.0]], [0.0], [1.0], [2.0], [1.0])  # l > u
+
+
+def test_host_build_symmetrises_and_drops_rows():
+    import hprqp_amd as H
+    inf = math.inf
+    Qm = (2, 2, np.array([0, 2, 3], np.uint64), np.array([0, 1, 1], np.uint64), np.array([2.0, 6.0, 4.0]))
+    Am = (3, 2, np.array([0, 2, 2, 2], np.uint64), np.array([0, 1], np.uint64), np.array([1.0, -1.0]))
+    m = H.build_from_qabc(Qm, [0.0, 0.0], Am, [0.0, -inf, 1.0], [3.0, inf, 2.0], [-2.0, -2.0], [2.0, 2.0])
+    (qrp, qci, qv), (arp, aci, av), _ = model_arrays(m)
+    assert m.num_constraints() == 1 and list(qv) == [2.0, 3.0, 3.0, 4.0]
