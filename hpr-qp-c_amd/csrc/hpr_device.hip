@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "device_common.h"
+#include "epilogues.h"
 #include "hprqp.h"
 #include "hprqp_hip.h"
 #include "solve_loop.h"
@@ -57,13 +58,6 @@ __global__ __launch_bounds__(kThreads) void reduce_partials_kernel(const double 
 
 namespace {
 
-__device__ __forceinline__ double project(double v, double lo, double hi)
-{
-    if (v < lo) return lo;
-    if (v > hi) return hi;
-    return v;
-}
-
 inline int grid_for(size_t len) { return (int)((len + kThreads - 1) / kThreads); }
 
 // ------------------------------------------------------------------ device CSR
@@ -76,6 +70,8 @@ struct DevCsr {
     int *blocks = nullptr;  // row-block starts, nblocks + 1 entries
     int nblocks = 0;
 };
+
+}  // namespace
 
 // Host planning of row blocks from the row pointer (see device_common.h).
 std::vector<int> plan_row_blocks(const std::vector<int> &rp)
@@ -103,6 +99,8 @@ std::vector<int> plan_row_blocks(const std::vector<int> &rp)
     if (nrow == 0) starts = {0, 0};
     return starts;
 }
+
+namespace {
 
 HPRQPStatus upload_blocks(DevCsr &m, hipStream_t s)
 {
@@ -208,442 +206,176 @@ HPRQPStatus device_transpose(int nrow, int ncol, int nnz, const int *rp, const i
     return HPRQP_STATUS_OK;
 }
 
-// ------------------------------------------------------------------ epilogues
-// Each mirrors one loop of src/cpu/iteration.c / residuals.c / control.c with
-// the same arithmetic order (kernels are built with -ffp-contract=off).
+// ------------------------------------------------------------- cone (SOC)
+// src/gpu/soc_common.cuh:7-34: Lorentz projection of one block (t, v):
+// ||v|| by a max-scaled compensated sum; identity inside, zero in the polar
+// cone, ((||v||+t)/2, (1+t/||v||)/2 v) otherwise.
+__device__ void soc_project_block(double *__restrict__ out, const double *__restrict__ in, int len)
+{
+    const double t = in[0];
+    double mx = 0.0, sum = 0.0, comp = 0.0;
+    for (int j = 1; j < len; ++j) mx = fmax(mx, fabs(in[j]));
+    if (mx > 0.0) {
+        for (int j = 1; j < len; ++j) {
+            const double sc = in[j] / mx;
+            const double term = sc * sc - comp;
+            const double nx = sum + term;
+            comp = (nx - sum) - term;
+            sum = nx;
+        }
+    }
+    const double norm = mx * sqrt(sum);
+    if (norm <= t) {
+        for (int j = 0; j < len; ++j) out[j] = in[j];
+    } else if (norm <= -t) {
+        for (int j = 0; j < len; ++j) out[j] = 0.0;
+    } else {
+        const double alpha = 0.5 + 0.5 * (t / norm);
+        out[0] = 0.5 * norm + 0.5 * t;
+        for (int j = 1; j < len; ++j) out[j] = alpha * in[j];
+    }
+}
 
-struct NoRed {
-    static constexpr int NS = 0, NM = 0;
-    double *partials = nullptr;
+__device__ __forceinline__ double box(double v, double lo, double hi) { return fmin(fmax(v, lo), hi); }
+
+// soc_iteration.cuh:14-26 for one variable, given its projected value.
+struct SocFinishX {
+    double *x, *x_hat, *w_bar, *dx, *z_bar;
+    const double *tempv, *w, *last_x, *fact1, *fact2;
+    double sigma, f1s, f2s, a, b;
+    int diag;
+    __device__ void apply(int i, double xb) const
+    {
+        const double xi = x[i];
+        const double xh = 2.0 * xb - xi;
+        const double f2 = diag ? fact2[i] : f2s;
+        const double f1 = diag ? fact1[i] : f1s;
+        dx[i] = xb - xi;
+        z_bar[i] = (xb - tempv[i]) / sigma;
+        x_hat[i] = xh;
+        x[i] = a * last_x[i] + b * xh;
+        w_bar[i] = f1 * w[i] + f2 * xh;
+    }
 };
 
-struct EpiStore : NoRed {
-    double *out;
-    __device__ void row(int i, double v, double *, double *) const { out[i] = v; }
-};
-
-// iteration.c:85-123
-template <bool DIAG, bool CHECK>
-struct EpiZxw1 : NoRed {
-    double *Qw, *x, *x_bar, *x_hat, *w_bar, *dx, *z_bar;
-    const double *w, *ATy, *c, *l, *u, *last_x, *fact1, *fact2;
-    double sigma, f1s, f2s, h1, h2;
+// soc_iteration.cuh:7-12 (raw x) fused with Q w; interval prefix projected
+// and finished in place, cone entries left for the cone kernel.
+struct EpiSocX : NoRed {
+    SocFinishX fin;
+    double *Qw, *tempv, *x_bar;
+    const double *xv, *ATy, *c, *l, *u;
+    int lin_n;
     __device__ void row(int i, double qw, double *, double *) const
     {
         Qw[i] = qw;
-        const double xi = x[i];
-        const double tmp = -qw + ATy[i] - c[i];
-        const double zr = xi + sigma * tmp;
-        const double xb = project(zr, l[i], u[i]);
-        const double xh = 2.0 * xb - xi;
-        const double xn = h2 * xh + h1 * last_x[i];
-        const double wb = DIAG ? fact1[i] * w[i] + fact2[i] * xh : f1s * w[i] + f2s * xh;
-        if (CHECK) {
-            dx[i] = xb - xi;
-            z_bar[i] = (xb - zr) / sigma;
+        const double raw = xv[i] + fin.sigma * (-qw + ATy[i] - c[i]);
+        tempv[i] = raw;
+        if (i < lin_n) {
+            const double xb = box(raw, l[i], u[i]);
+            x_bar[i] = xb;
+            fin.apply(i, xb);
         }
-        x_bar[i] = xb;
-        x_hat[i] = xh;
-        x[i] = xn;
-        w_bar[i] = wb;
     }
 };
 
-// iteration.c:141-143
-struct EpiTempv : NoRed {
-    double *tempv;
-    const double *x_hat, *Qw;
-    double sigma;
-    __device__ void row(int i, double qwb, double *, double *) const { tempv[i] = x_hat[i] + sigma * (Qw[i] - qwb); }
-};
+__global__ void soc_cone_x_kernel(int count, const int *__restrict__ off, SocFinishX fin, double *__restrict__ x_bar)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    const int b = off[k], e = off[k + 1];
+    soc_project_block(x_bar + b, fin.tempv + b, e - b);
+    for (int i = b; i < e; ++i) fin.apply(i, x_bar[i]);
+}
 
-// iteration.c:150-171
-template <bool CHECK>
-struct EpiY : NoRed {
-    double *y, *y_bar, *s, *dy;
-    const double *AL, *AU, *last_y;
-    double f1, f2, h1, h2;
-    __device__ void row(int i, double ax, double *, double *) const
+// soc_iteration.cuh:28-41 for one row
+struct SocFinishY {
+    double *y, *y_bar, *dy;
+    const double *s, *Rp, *last_y;
+    double tau, a, b;
+    __device__ void apply(int i) const
     {
         const double yi = y[i];
-        const double sr = ax - f1 * yi;
-        const double sp = project(sr, AL[i], AU[i]);
-        const double yb = f2 * (sp - sr);
-        const double yh = 2.0 * yb - yi;
-        const double yn = h2 * yh + h1 * last_y[i];
-        if (CHECK) {
-            s[i] = sp;
-            dy[i] = yb - yi;
-        }
+        const double yb = (s[i] - Rp[i]) / tau;
+        dy[i] = yb - yi;
         y_bar[i] = yb;
-        y[i] = yn;
+        y[i] = a * last_y[i] + b * (2.0 * yb - yi);
     }
 };
 
-// iteration.c:218-248
-template <bool DIAG, bool CHECK>
-struct EpiW2 : NoRed {
-    double *w_bar, *w, *ATy, *ATy_bar, *dw, *ATdy;
-    const double *last_w, *last_ATy, *fact;
-    double fs, h1, h2;
-    __device__ void row(int i, double atyb, double *, double *) const
+struct EpiSocY : NoRed {
+    SocFinishY fin;
+    double *Rp, *s;
+    const double *rhs, *AL, *AU;
+    int lin_m;
+    __device__ void row(int i, double ax, double *, double *) const
     {
-        const double f = DIAG ? fact[i] : fs;
-        const double aty = ATy[i], wi = w[i];
-        const double wbn = w_bar[i] + f * (atyb - aty);
-        const double wn = h1 * last_w[i] + h2 * (2.0 * wbn - wi);
-        const double atyn = h1 * last_ATy[i] + h2 * (2.0 * atyb - aty);
-        if (CHECK) {
-            dw[i] = wbn - wi;
-            ATdy[i] = atyb - aty;
-            ATy_bar[i] = atyb;
+        const double raw = ax - (rhs ? rhs[i] : 0.0) - fin.tau * fin.y[i];
+        Rp[i] = raw;
+        if (i < lin_m) {
+            s[i] = box(raw, AL[i], AU[i]);
+            fin.apply(i);
         }
-        w_bar[i] = wbn;
-        w[i] = wn;
-        ATy[i] = atyn;
     }
 };
 
-// iteration.c:19-66 (noC: A^T y first, Halpern on w as well)
-template <bool DIAG, bool CHECK>
-struct EpiZxw1NoC : NoRed {
-    double *ATy, *x, *x_bar, *x_hat, *w, *w_bar, *dx, *dw, *z_bar;
-    const double *Qw, *c, *l, *u, *last_x, *last_w, *fact1, *fact2;
-    double sigma, f1s, f2s, h1, h2;
-    __device__ void row(int i, double aty, double *, double *) const
-    {
-        ATy[i] = aty;
-        const double xi = x[i], wi = w[i];
-        const double tmp = -Qw[i] + aty - c[i];
-        const double zr = xi + sigma * tmp;
-        const double xb = project(zr, l[i], u[i]);
-        const double xh = 2.0 * xb - xi;
-        const double xn = h2 * xh + h1 * last_x[i];
-        const double wb = DIAG ? fact1[i] * wi + fact2[i] * xh : f1s * wi + f2s * xh;
-        const double wh = 2.0 * wb - wi;
-        const double wn = h2 * wh + h1 * last_w[i];
-        if (CHECK) {
-            dx[i] = xb - xi;
-            dw[i] = wb - wi;
-            z_bar[i] = (xb - zr) / sigma;
-        }
-        x_bar[i] = xb;
-        x_hat[i] = xh;
-        x[i] = xn;
-        w_bar[i] = wb;
-        w[i] = wn;
-    }
-};
+__global__ void soc_cone_y_kernel(int count, const int *__restrict__ off, SocFinishY fin, double *__restrict__ s)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    const int b = off[k], e = off[k + 1];
+    soc_project_block(s + b, fin.Rp + b, e - b);
+    for (int i = b; i < e; ++i) fin.apply(i);
+}
 
-// iteration.c:68-80
-struct EpiQwNoC : NoRed {
-    double *Qw, *Qw_bar, *tempv;
-    const double *last_Qw, *x_hat;
-    double sigma, h1, h2;
-    __device__ void row(int i, double qwb, double *, double *) const
-    {
-        const double qw = Qw[i];
-        const double qwh = 2.0 * qwb - qw;
-        Qw_bar[i] = qwb;
-        Qw[i] = h2 * qwh + h1 * last_Qw[i];
-        tempv[i] = x_hat[i] + sigma * (qw - qwb);
-    }
-};
+// Projection of a mixed vector (interval prefix then cones) for residuals.
+__global__ void soc_project_mixed_kernel(int len, int lin, const double *__restrict__ in, const double *__restrict__ lo,
+                                         const double *__restrict__ hi, int count, const int *__restrict__ off,
+                                         double *__restrict__ out)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < lin) out[t] = box(in[t], lo[t], hi[t]);
+    if (t < count) soc_project_block(out + off[t], in + off[t], off[t + 1] - off[t]);
+}
 
-// residuals.c:102-160: Qx = Q x_bar and the three objective dots.
-struct EpiResQ {
-    static constexpr int NS = 3, NM = 0;
+// soc_residuals.cuh:5-9 + the dot products of :194-205 on the rows
+struct EpiSocResA {
+    static constexpr int NS = 2, NM = 1;
     double *partials;
-    double *Qx;
-    const double *x_bar, *c, *z_bar;
-    __device__ void row(int i, double qx, double *s, double *) const
-    {
-        Qx[i] = qx;
-        const double xb = x_bar[i];
-        s[0] += c[i] * xb;
-        s[1] += xb * qx;
-        s[2] += z_bar[i] * xb;
-    }
-};
-
-// residuals.c:148-162: A^T y_bar and the dual residual.
-struct EpiResAT {
-    static constexpr int NS = 0, NM = 3;
-    double *partials;
-    double *ATdy;
-    const double *Qx, *c, *z_bar, *col_norm;
-    double c_scale;
-    __device__ void row(int i, double atdy, double *, double *mx) const
-    {
-        ATdy[i] = atdy;
-        const double f = col_norm[i] * c_scale;
-        const double q = Qx[i];
-        const double rd = (q + c[i] - atdy - z_bar[i]) * f;
-        mx[0] = fmax(mx[0], fabs(rd));
-        mx[1] = fmax(mx[1], fabs(q * f));
-        mx[2] = fmax(mx[2], fabs(atdy * f));
-    }
-};
-
-// residuals.c:164-175 and the y_bar.s dot of :53
-struct EpiResA {
-    static constexpr int NS = 1, NM = 2;
-    double *partials;
-    double *Ax;
-    const double *AL, *AU, *row_norm, *y_bar, *s;
+    double *Ax, *arg;
+    const double *rhs, *row_norm, *y_bar, *s;
     double b_scale;
     __device__ void row(int i, double ax, double *sm, double *mx) const
     {
         Ax[i] = ax;
-        const double p = project(ax, AL[i], AU[i]);
-        const double f = row_norm[i] * b_scale;
-        mx[0] = fmax(mx[0], fabs((p - ax) * f));
-        mx[1] = fmax(mx[1], fabs(ax * f));
+        const double r = rhs ? rhs[i] : 0.0;
+        arg[i] = ax - r;
+        mx[0] = fmax(mx[0], fabs(ax * (row_norm[i] * b_scale)));
         sm[0] += y_bar[i] * s[i];
+        sm[1] += y_bar[i] * r;
     }
 };
 
-// residuals.c:58-78
-struct MapBoundViolation {
+// soc_residuals.cuh:11-18
+struct MapSocRowResid {
     static constexpr int NS = 0, NM = 1;
     double *partials;
-    double *dx;
-    const double *l, *u, *x_bar, *col_norm;
+    const double *proj, *arg, *row_norm;
     double b_scale;
     __device__ void row(int i, double *, double *mx) const
     {
-        const double lo = l[i] - x_bar[i], hi = x_bar[i] - u[i];
-        double v = 0.0;
-        if (lo > 0.0) v += lo;
-        if (hi > 0.0) v += hi;
-        const double d = v * (b_scale / col_norm[i]);
-        dx[i] = d;
-        mx[0] = fmax(mx[0], d);
+        mx[0] = fmax(mx[0], fabs((proj[i] - arg[i]) * (row_norm[i] * b_scale)));
     }
 };
 
-// control.c:174-209: the M-norm products.
-struct EpiMnQdw {
-    static constexpr int NS = 4, NM = 0;
-    double *partials;
-    double *dQw;
-    const double *dx, *dw;
-    __device__ void row(int i, double v, double *s, double *) const
-    {
-        dQw[i] = v;
-        const double d = dx[i];
-        s[0] += d * d;
-        s[1] += v * d;
-        s[2] += dw[i] * v;
-        s[3] += v * v;
-    }
-};
-
-struct EpiMnATdy {
-    static constexpr int NS = 1, NM = 0;
-    double *partials;
-    double *ATdy;
-    const double *dx;
-    __device__ void row(int i, double v, double *s, double *) const
-    {
-        ATdy[i] = v;
-        s[0] += v * dx[i];
-    }
-};
-
-struct EpiMnQATdy {
-    static constexpr int NS = 3, NM = 0;
-    double *partials;
-    double *QATdy;
-    const double *dQw, *ATdy, *fact_M;
-    int diag;
-    __device__ void row(int i, double v, double *s, double *) const
-    {
-        QATdy[i] = v;
-        const double a = ATdy[i];
-        s[0] += dQw[i] * a;
-        s[1] += a * v;
-        if (diag) s[2] += (a * fact_M[i]) * v;
-    }
-};
-
-struct MapDot {
-    static constexpr int NS = 1, NM = 0;
-    double *partials;
-    const double *a, *b;
-    __device__ void row(int i, double *s, double *) const { s[0] += a[i] * b[i]; }
-};
-
-// control.c:333-356: differences to the last restart point.
-struct MapSigmaDiffN {
-    static constexpr int NS = 1, NM = 0;
-    double *partials;
-    double *dx, *dw, *ATdy;
-    const double *x_bar, *last_x, *w_bar, *last_w, *ATy_bar, *last_ATy;
-    int has_rows;
-    __device__ void row(int i, double *s, double *) const
-    {
-        const double d = x_bar[i] - last_x[i];
-        dx[i] = d;
-        dw[i] = w_bar[i] - last_w[i];
-        if (has_rows) ATdy[i] = ATy_bar[i] - last_ATy[i];
-        s[0] += d * d;
-    }
-};
-
-struct MapSigmaDiffM {
-    static constexpr int NS = 1, NM = 0;
-    double *partials;
-    double *dy;
-    const double *y_bar, *last_y;
-    __device__ void row(int i, double *s, double *) const
-    {
-        const double d = y_bar[i] - last_y[i];
-        dy[i] = d;
-        s[0] += d * d;
-    }
-};
-
-struct EpiSigmaQdw {
-    static constexpr int NS = 3, NM = 0;
-    double *partials;
-    double *dQw;
-    const double *ATdy, *dw;
-    __device__ void row(int i, double v, double *s, double *) const
-    {
-        dQw[i] = v;
-        s[0] += v * ATdy[i];
-        s[1] += v * v;
-        s[2] += dw[i] * v;
-    }
-};
-
-struct EpiSigmaQATdy {
-    static constexpr int NS = 1, NM = 0;
-    double *partials;
-    double *QATdy;
-    const double *ATdy;
-    __device__ void row(int i, double v, double *s, double *) const
-    {
-        QATdy[i] = v;
-        s[0] += ATdy[i] * v;
-    }
-};
-
-// control.c:116-127
-struct MapDiagFactors : NoRed {
-    double *fact1, *fact2, *fact, *fact_M;
-    const double *diag_Q;
-    double sigma;
-    __device__ void row(int i, double *, double *) const
-    {
-        const double inv = 1.0 / (1.0 + sigma * diag_Q[i]);
-        fact2[i] = inv;
-        fact[i] = sigma * inv;
-        fact1[i] = sigma * diag_Q[i] * inv;
-        fact_M[i] = sigma * sigma * inv;
-    }
-};
-
-// control.c:413-457 (n side fused with the A^T y_bar product)
-struct EpiRestart : NoRed {
-    double *ATy_bar, *ATy, *last_ATy, *x, *w, *last_x, *last_w, *Qw, *last_Qw;
-    const double *x_bar, *w_bar, *Qw_bar;
-    int has_rows, noC;
-    __device__ void row(int i, double v, double *, double *) const
-    {
-        const double xb = x_bar[i], wb = w_bar[i];
-        x[i] = xb;
-        w[i] = wb;
-        last_x[i] = xb;
-        last_w[i] = wb;
-        if (has_rows) {
-            ATy_bar[i] = v;
-            last_ATy[i] = v;
-            ATy[i] = v;
-        }
-        if (noC) {
-            Qw[i] = Qw_bar[i];
-            last_Qw[i] = Qw_bar[i];
-        }
-    }
-};
-
-struct MapRestartM : NoRed {
-    double *y, *last_y;
-    const double *y_bar;
-    __device__ void row(int i, double *, double *) const
-    {
-        y[i] = y_bar[i];
-        last_y[i] = y_bar[i];
-    }
-};
-
-// eigs.c:11-153 pieces
-struct MapProbe {
-    static constexpr int NS = 1, NM = 0;
-    double *partials;
-    double *z;
-    double inv_len1;
-    double len1;
-    __device__ void row(int i, double *s, double *) const
-    {
-        const double v = 1.0 + ((double)i + 1.0) / len1;
-        z[i] = v;
-        s[0] += v * v;
-    }
-};
-
-struct MapScaleDiv : NoRed {
-    double *q;
-    const double *z;
-    double d;
-    __device__ void row(int i, double *, double *) const { q[i] = z[i] / d; }
-};
-
-struct EpiPowerZ {
-    static constexpr int NS = 2, NM = 0;
-    double *partials;
-    double *z;
-    const double *q;
-    __device__ void row(int i, double v, double *s, double *) const
-    {
-        z[i] = v;
-        s[0] += q[i] * v;
-        s[1] += v * v;
-    }
-};
-
-struct MapPowerResid {
-    static constexpr int NS = 1, NM = 0;
-    double *partials;
-    double *q;
-    const double *z;
-    const double *lambda;  // device scalar
-    __device__ void row(int i, double *s, double *) const
-    {
-        const double r = z[i] - (*lambda) * q[i];
-        q[i] = r;
-        s[0] += r * r;
-    }
-};
-
-// workspace.c:26-43: diag(Q) and max |offdiag|
-struct DiagInfo {
+// soc_residuals.cuh:20-24
+struct MapSocVarResid {
     static constexpr int NS = 0, NM = 1;
     double *partials;
-    double *diag;
-    const int *rp, *ci;
-    const double *val;
-    __device__ void row(int r, double *, double *mx) const
+    const double *proj, *x_bar, *col_norm;
+    double b_scale;
+    __device__ void row(int i, double *, double *mx) const
     {
-        double d = 0.0, off = 0.0;
-        for (int k = rp[r]; k < rp[r + 1]; ++k) {
-            if (ci[k] == r) d += val[k];
-            else off = fmax(off, fabs(val[k]));
-        }
-        diag[r] = d;
-        mx[0] = fmax(mx[0], off);
+        mx[0] = fmax(mx[0], fabs((proj[i] - x_bar[i]) * b_scale / col_norm[i]));
     }
 };
 
@@ -811,15 +543,17 @@ enum Vec {
     V_N_END,
     // m-sized
     V_y = V_N_END, V_y_bar, V_Ax, V_dy, V_s, V_last_y, V_AL, V_AU, V_row_norm, V_tr, V_soc_rhs, V_pm0, V_pm1,
+    V_Rp,
     V_M_END
 };
 
-static_assert(V_M_END == 47, "vector table");
+static_assert(V_M_END == 48, "vector table");
+constexpr int V_Rp_soc() { return V_Rp; }
 const char *const kVecNames[] = {
     "x", "x_bar", "x_hat", "w", "w_bar", "ATy", "ATy_bar", "ATdy", "Qw", "Qw_bar", "Qx", "dQw", "QATdy", "dx",
     "dw", "z_bar", "tempv", "diag_Q", "fact1", "fact2", "fact", "fact_M", "last_x", "last_Qw", "last_w", "last_ATy",
     "c", "l", "u", "col_norm", "tc", "pn0", "pn1", "pn2",
-    "y", "y_bar", "Ax", "dy", "s", "last_y", "AL", "AU", "row_norm", "tr", "soc_rhs", "pm0", "pm1"};
+    "y", "y_bar", "Ax", "dy", "s", "last_y", "AL", "AU", "row_norm", "tr", "soc_rhs", "pm0", "pm1", "Rp"};
 
 class HipBackend final : public Backend {
 public:
@@ -851,6 +585,17 @@ public:
     HPRQPResiduals last_res_{};
     size_t restart_times_ = 0;
 
+    // Per-kernel timing of the iteration kernels with HIP events recorded on
+    // this backend's own stream (profile()).
+    enum Tag { T_ZXW1, T_TEMPV, T_Y, T_W2, T_ZXW1_NOC, T_QW_NOC, T_COUNT };
+    HPRQPStatus profile(size_t iterations, double *mean_ms, double *bytes, size_t *launches);
+    double algorithmic_bytes(int tag) const;
+    static const char *tag_name(int tag)
+    {
+        static const char *names[] = {"zxw1_q", "tempv_q", "y_a", "w2_at", "zxw1_noc_at", "qw_noc_q"};
+        return names[tag];
+    }
+
 private:
     size_t n_ = 0, m_ = 0, n_pad_ = 0, m_pad_ = 0, off_m_ = 0;
     int device_ = 0;
@@ -870,21 +615,45 @@ private:
     int Q_is_diag_ = 0, noC_ = 0, to_check_ = 1;
     double norm_b_org_ = 0.0, norm_c_org_ = 0.0, obj_constant_ = 0.0;
     double b_scale_ = 1.0, c_scale_ = 1.0;
+    bool profiling_ = false;
+    std::vector<hipEvent_t> ev_pool_;
+    std::vector<int> ev_tags_;
+    size_t ev_used_ = 0;
+    void prof_begin(int tag)
+    {
+        if (ev_used_ + 2 > ev_pool_.size()) return;
+        (void)hipEventRecord(ev_pool_[ev_used_], stream_);
+        ev_tags_[ev_used_] = tag;
+        ++ev_used_;
+    }
+    void prof_end()
+    {
+        if (ev_used_ == 0 || ev_used_ + 1 > ev_pool_.size() || ev_tags_[ev_used_ - 1] < 0) return;
+        (void)hipEventRecord(ev_pool_[ev_used_], stream_);
+        ev_tags_[ev_used_] = -1;
+        ++ev_used_;
+    }
 
     void release();
     HPRQPStatus upload(const HPRQPModel *model);
     HPRQPStatus scale(const HPRQPParams *params);
     HPRQPStatus workspace_setup(const HPRQPModel *model);
     HPRQPStatus power_iteration(bool q_matrix, double *lambda);
+    HPRQPStatus soc_iteration_step(size_t inner);
+    HPRQPStatus soc_residuals(HPRQPResiduals *res);
+    size_t lin_n_ = 0, lin_m_ = 0;
 
     double *region(int r) const { return partials_ + (size_t)r * region_; }
 
     template <class Epi>
-    HPRQPStatus spmv(const DevCsr &M, const double *x, Epi epi, int reg = 0)
+    HPRQPStatus spmv(const DevCsr &M, const double *x, Epi epi, int reg = 0, int tag = -1)
     {
         if (M.nrow == 0) return HPRQP_STATUS_OK;
         if (Epi::NS + Epi::NM > 0) epi.partials = region(reg);
+        const bool timed = profiling_ && tag >= 0;
+        if (timed) prof_begin(tag);
         spmv_rows_kernel<Epi><<<M.nblocks, kThreads, 0, stream_>>>(M.rp, M.ci, M.val, x, M.blocks, epi);
+        if (timed) prof_end();
         return HPRQP_STATUS_OK;
     }
     template <class F>
@@ -988,6 +757,8 @@ HPRQPStatus HipBackend::upload(const HPRQPModel *md)
     }
     num_var_cones_ = md->num_var_cones;
     num_row_cones_ = md->num_row_cones;
+    lin_n_ = num_var_cones_ ? md->var_cone_offsets[0] : n_;
+    lin_m_ = num_row_cones_ ? md->row_cone_offsets[0] : m_;
     if (num_var_cones_) {
         std::vector<int> off(num_var_cones_ + 1);
         for (size_t k = 0; k <= num_var_cones_; ++k) off[k] = (int)md->var_cone_offsets[k];
@@ -1223,13 +994,13 @@ HPRQPStatus HipBackend::iteration_step(size_t inner)
     const double f2s = 1.0 / (1.0 + sigma_ * lambda_Q_);
     const double f1s = 1.0 - f2s;
     const bool chk = to_check_ != 0, diag = Q_is_diag_ != 0;
-    if (num_var_cones_ || num_row_cones_) return HPRQP_STATUS_NOT_IMPLEMENTED;  // cone step: separate path
+    if (num_var_cones_ || num_row_cones_) return soc_iteration_step(inner);
     if (noC_) {
 #define HPRQP_NOC(D, C)                                                                                           \
     spmv(AT_, vec(V_y), EpiZxw1NoC<D, C>{{}, vec(V_ATy), vec(V_x), vec(V_x_bar), vec(V_x_hat), vec(V_w),           \
                                          vec(V_w_bar), vec(V_dx), vec(V_dw), vec(V_z_bar), vec(V_Qw), vec(V_c),     \
                                          vec(V_l), vec(V_u), vec(V_last_x), vec(V_last_w), vec(V_fact1),            \
-                                         vec(V_fact2), sigma_, f1s, f2s, h1, h2})
+                                         vec(V_fact2), sigma_, f1s, f2s, h1, h2}, 0, T_ZXW1_NOC)
         if (diag) {
             if (chk) HPRQP_NOC(true, true);
             else HPRQP_NOC(true, false);
@@ -1239,12 +1010,12 @@ HPRQPStatus HipBackend::iteration_step(size_t inner)
         }
 #undef HPRQP_NOC
         spmv(Q_, vec(V_w_bar), EpiQwNoC{{}, vec(V_Qw), vec(V_Qw_bar), vec(V_tempv), vec(V_last_Qw), vec(V_x_hat),
-                                        sigma_, h1, h2});
+                                        sigma_, h1, h2}, 0, T_QW_NOC);
     } else {
 #define HPRQP_ZXW(D, C)                                                                                            \
     spmv(Q_, vec(V_w), EpiZxw1<D, C>{{}, vec(V_Qw), vec(V_x), vec(V_x_bar), vec(V_x_hat), vec(V_w_bar), vec(V_dx),  \
                                      vec(V_z_bar), vec(V_w), vec(V_ATy), vec(V_c), vec(V_l), vec(V_u),              \
-                                     vec(V_last_x), vec(V_fact1), vec(V_fact2), sigma_, f1s, f2s, h1, h2})
+                                     vec(V_last_x), vec(V_fact1), vec(V_fact2), sigma_, f1s, f2s, h1, h2}, 0, T_ZXW1)
         if (diag) {
             if (chk) HPRQP_ZXW(true, true);
             else HPRQP_ZXW(true, false);
@@ -1254,7 +1025,7 @@ HPRQPStatus HipBackend::iteration_step(size_t inner)
         }
 #undef HPRQP_ZXW
         // iteration.c:132-134: without rows the y step (and its Q w_bar) is skipped
-        if (m_ > 0) spmv(Q_, vec(V_w_bar), EpiTempv{{}, vec(V_tempv), vec(V_x_hat), vec(V_Qw), sigma_});
+        if (m_ > 0) spmv(Q_, vec(V_w_bar), EpiTempv{{}, vec(V_tempv), vec(V_x_hat), vec(V_Qw), sigma_}, 0, T_TEMPV);
     }
     if (m_ > 0) {
         const double f1 = lambda_A_ * sigma_;
@@ -1262,16 +1033,16 @@ HPRQPStatus HipBackend::iteration_step(size_t inner)
         const double f2 = 1.0 / f1;
         if (chk)
             spmv(A_, vec(V_tempv), EpiY<true>{{}, vec(V_y), vec(V_y_bar), vec(V_s), vec(V_dy), vec(V_AL), vec(V_AU),
-                                                vec(V_last_y), f1, f2, h1, h2});
+                                                vec(V_last_y), f1, f2, h1, h2}, 0, T_Y);
         else
             spmv(A_, vec(V_tempv), EpiY<false>{{}, vec(V_y), vec(V_y_bar), vec(V_s), vec(V_dy), vec(V_AL), vec(V_AU),
-                                                 vec(V_last_y), f1, f2, h1, h2});
+                                                 vec(V_last_y), f1, f2, h1, h2}, 0, T_Y);
     }
     if (!noC_) {
         const double fs = sigma_ / (1.0 + sigma_ * lambda_Q_);
 #define HPRQP_W2(D, C)                                                                                            \
     spmv(AT_, vec(V_y_bar), EpiW2<D, C>{{}, vec(V_w_bar), vec(V_w), vec(V_ATy), vec(V_ATy_bar), vec(V_dw),          \
-                                        vec(V_ATdy), vec(V_last_w), vec(V_last_ATy), vec(V_fact), fs, h1, h2})
+                                        vec(V_ATdy), vec(V_last_w), vec(V_last_ATy), vec(V_fact), fs, h1, h2}, 0, T_W2)
         if (diag) {
             if (chk) HPRQP_W2(true, true);
             else HPRQP_W2(true, false);
@@ -1289,6 +1060,7 @@ HPRQPStatus HipBackend::iteration_step(size_t inner)
 HPRQPStatus HipBackend::compute_residuals(HPRQPResiduals *res, size_t iter)
 {
     std::memset(res, 0, sizeof(*res));
+    if (num_var_cones_ || num_row_cones_) return soc_residuals(res);
     spmv(Q_, vec(V_x_bar), EpiResQ{nullptr, vec(V_Qx), vec(V_x_bar), vec(V_c), vec(V_z_bar)}, 0);
     finish(0, Q_.nblocks, 3, 0, 0);
     zero_red(3, 8);
@@ -1318,6 +1090,102 @@ HPRQPStatus HipBackend::compute_residuals(HPRQPResiduals *res, size_t iter)
     if (m_ > 0) res->err_Rp_org_bar = r[7] / (1.0 + std::max(norm_b_org_, r[8]));
     if (iter == 0 && r[9] > res->err_Rp_org_bar) res->err_Rp_org_bar = r[9];
     res->KKTx_and_gap_org_bar = std::max(std::max(res->err_Rp_org_bar, res->err_Rd_org_bar), res->rel_gap_bar);
+    res->is_updated = 1;
+    last_res_ = *res;
+    return HPRQP_STATUS_OK;
+}
+
+// src/gpu/soc_iteration.cuh:45-82: the generic cone step with full writes.
+HPRQPStatus HipBackend::soc_iteration_step(size_t inner)
+{
+    const double a = 1.0 / ((double)inner + 2.0), b = 1.0 - a;
+    const double f2s = 1.0 / (1.0 + sigma_ * lambda_Q_);
+    const double f1s = 1.0 - f2s;
+    SocFinishX fx{vec(V_x), vec(V_x_hat), vec(V_w_bar), vec(V_dx), vec(V_z_bar), vec(V_tempv), vec(V_w),
+                  vec(V_last_x), vec(V_fact1), vec(V_fact2), sigma_, f1s, f2s, a, b, Q_is_diag_};
+    EpiSocX ex;
+    ex.fin = fx;
+    ex.Qw = vec(V_Qw);
+    ex.tempv = vec(V_tempv);
+    ex.x_bar = vec(V_x_bar);
+    ex.xv = vec(V_x);
+    ex.ATy = vec(V_ATy);
+    ex.c = vec(V_c);
+    ex.l = vec(V_l);
+    ex.u = vec(V_u);
+    ex.lin_n = (int)lin_n_;
+    spmv(Q_, vec(V_w), ex, 0, T_ZXW1);
+    if (num_var_cones_)
+        soc_cone_x_kernel<<<grid_for(num_var_cones_), kThreads, 0, stream_>>>((int)num_var_cones_, var_off_, fx,
+                                                                              vec(V_x_bar));
+    spmv(Q_, vec(V_w_bar), EpiTempv{{}, vec(V_tempv), vec(V_x_hat), vec(V_Qw), sigma_}, 0, T_TEMPV);
+    if (m_ > 0) {
+        const double tau = sigma_ * lambda_A_;
+        SocFinishY fy{vec(V_y), vec(V_y_bar), vec(V_dy), vec(V_s), vec(V_Rp_soc()), vec(V_last_y), tau, a, b};
+        EpiSocY ey;
+        ey.fin = fy;
+        ey.Rp = vec(V_Rp_soc());
+        ey.s = vec(V_s);
+        ey.rhs = num_row_cones_ ? vec(V_soc_rhs) : nullptr;
+        ey.AL = vec(V_AL);
+        ey.AU = vec(V_AU);
+        ey.lin_m = (int)lin_m_;
+        spmv(A_, vec(V_tempv), ey, 0, T_Y);
+        if (num_row_cones_)
+            soc_cone_y_kernel<<<grid_for(num_row_cones_), kThreads, 0, stream_>>>((int)num_row_cones_, row_off_, fy,
+                                                                                  vec(V_s));
+    }
+    const double fs = sigma_ / (1.0 + sigma_ * lambda_Q_);
+    if (Q_is_diag_)
+        spmv(AT_, vec(V_y_bar), EpiW2<true, true>{{}, vec(V_w_bar), vec(V_w), vec(V_ATy), vec(V_ATy_bar), vec(V_dw),
+                                                  vec(V_ATdy), vec(V_last_w), vec(V_last_ATy), vec(V_fact), fs, a, b},
+             0, T_W2);
+    else
+        spmv(AT_, vec(V_y_bar), EpiW2<false, true>{{}, vec(V_w_bar), vec(V_w), vec(V_ATy), vec(V_ATy_bar), vec(V_dw),
+                                                   vec(V_ATdy), vec(V_last_w), vec(V_last_ATy), vec(V_fact), fs, a, b},
+             0, T_W2);
+    HPRQP_HIP(hipGetLastError());
+    return HPRQP_STATUS_OK;
+}
+
+// src/gpu/soc_residuals.cuh:27-78 (the iteration s pairs with y_bar; the
+// variable-domain violation is audited at every check).
+HPRQPStatus HipBackend::soc_residuals(HPRQPResiduals *res)
+{
+    const int vc = (int)num_var_cones_, rc = (int)num_row_cones_;
+    spmv(Q_, vec(V_x_bar), EpiResQ{nullptr, vec(V_Qx), vec(V_x_bar), vec(V_c), vec(V_z_bar)}, 0);
+    finish(0, Q_.nblocks, 3, 0, 0);
+    zero_red(3, 8);
+    spmv(AT_, vec(V_y_bar), EpiResAT{nullptr, vec(V_ATdy), vec(V_Qx), vec(V_c), vec(V_z_bar), vec(V_col_norm), c_scale_}, 1);
+    finish(1, AT_.nblocks, 0, 3, 3);
+    if (m_ > 0) {
+        spmv(A_, vec(V_x_bar), EpiSocResA{nullptr, vec(V_Ax), vec(V_Rp_soc()), rc ? vec(V_soc_rhs) : nullptr,
+                                          vec(V_row_norm), vec(V_y_bar), vec(V_s), b_scale_}, 2);
+        finish(2, A_.nblocks, 2, 1, 6);
+        soc_project_mixed_kernel<<<grid_for(std::max(lin_m_, (size_t)rc)), kThreads, 0, stream_>>>(
+            (int)m_, (int)lin_m_, vec(V_Rp_soc()), vec(V_AL), vec(V_AU), rc, row_off_, vec(V_tr));
+        map(m_, MapSocRowResid{nullptr, vec(V_tr), vec(V_Rp_soc()), vec(V_row_norm), b_scale_}, 3);
+        finish(3, m_grid(), 0, 1, 9);
+    }
+    soc_project_mixed_kernel<<<grid_for(std::max(lin_n_, (size_t)vc)), kThreads, 0, stream_>>>(
+        (int)n_, (int)lin_n_, vec(V_x_bar), vec(V_l), vec(V_u), vc, var_off_, vec(V_tc));
+    map(n_, MapSocVarResid{nullptr, vec(V_tc), vec(V_x_bar), vec(V_col_norm), b_scale_}, 0);
+    finish(0, n_grid(), 0, 1, 10);
+    HPRQP_HIP(hipGetLastError());
+    HPRQPStatus st = read_red(11);
+    if (st != HPRQP_STATUS_OK) return st;
+    const double *r = red_host_;
+    const double cx = r[0], xqx = r[1], zx = r[2];
+    const double ys = m_ > 0 ? r[6] : 0.0, yb = m_ > 0 ? r[7] : 0.0;
+    const double scale = b_scale_ * c_scale_;
+    res->primal_obj_bar = scale * (cx + 0.5 * xqx) + obj_constant_;
+    res->dual_obj_bar = scale * (-0.5 * xqx + zx + ys + yb) + obj_constant_;
+    res->rel_gap_bar = std::fabs(res->primal_obj_bar - res->dual_obj_bar) /
+                       (1.0 + std::fmax(std::fabs(res->primal_obj_bar), std::fabs(res->dual_obj_bar)));
+    res->err_Rd_org_bar = r[3] / (1.0 + std::fmax(norm_c_org_, std::fmax(r[4], r[5])));
+    if (m_ > 0) res->err_Rp_org_bar = r[9] / (1.0 + std::fmax(norm_b_org_, r[8]));
+    res->err_Rp_org_bar = std::fmax(res->err_Rp_org_bar, r[10]);
+    res->KKTx_and_gap_org_bar = std::fmax(res->err_Rp_org_bar, std::fmax(res->err_Rd_org_bar, res->rel_gap_bar));
     res->is_updated = 1;
     last_res_ = *res;
     return HPRQP_STATUS_OK;
@@ -1460,6 +1328,68 @@ HPRQPStatus HipBackend::copy_output_iterates(double *x, double *w, double *z, do
     }
     for (size_t i = 0; i < m_; ++i) y[i] = c_scale_ * (yb[i] / row_norm_h_[i]);
     return HPRQP_STATUS_OK;
+}
+
+// Algorithmic HBM bytes of one launch (non-check variant): the matrix
+// stream (8 B value + 4 B column index per nonzero, 4 B per row pointer), the
+// gathered vector once (8 B per element of its length), and 8 B per element
+// for every dense vector read or written by the epilogue.
+double HipBackend::algorithmic_bytes(int tag) const
+{
+    const double n = (double)n_, m = (double)m_;
+    const double nq = (double)Q_.nnz, na = (double)A_.nnz;
+    const double diag = Q_is_diag_ ? 2.0 : 0.0;
+    switch (tag) {
+    case T_ZXW1:  // reads w(gather), w, ATy, c, l, u, x, last_x [+fact1,fact2]; writes Qw, x_bar, x_hat, x, w_bar
+        return 12.0 * nq + 4.0 * (n + 1) + 8.0 * n + 8.0 * n * (7.0 + diag + 5.0);
+    case T_TEMPV:  // reads w_bar(gather), x_hat, Qw; writes tempv
+        return 12.0 * nq + 4.0 * (n + 1) + 8.0 * n + 8.0 * n * 3.0;
+    case T_Y:  // reads tempv(gather), y, last_y, AL, AU; writes y_bar, y
+        return 12.0 * na + 4.0 * (m + 1) + 8.0 * n + 8.0 * m * 6.0;
+    case T_W2:  // reads y_bar(gather), w_bar, ATy, w, last_w, last_ATy [+fact]; writes w_bar, w, ATy
+        return 12.0 * na + 4.0 * (n + 1) + 8.0 * m + 8.0 * n * (5.0 + (Q_is_diag_ ? 1.0 : 0.0) + 3.0);
+    case T_ZXW1_NOC:  // reads y(gather), Qw, c, l, u, x, w, last_x, last_w [+2]; writes ATy, x_bar, x_hat, x, w_bar, w
+        return 12.0 * na + 4.0 * (n + 1) + 8.0 * m + 8.0 * n * (8.0 + diag + 6.0);
+    case T_QW_NOC:  // reads w_bar(gather), Qw, last_Qw, x_hat; writes Qw_bar, Qw, tempv
+        return 12.0 * nq + 4.0 * (n + 1) + 8.0 * n + 8.0 * n * 6.0;
+    default:
+        return 0.0;
+    }
+}
+
+HPRQPStatus HipBackend::profile(size_t iterations, double *mean_ms, double *bytes, size_t *launches)
+{
+    const size_t need = iterations * 8 + 8;
+    ev_pool_.resize(need);
+    ev_tags_.assign(need, -1);
+    for (auto &e : ev_pool_) HPRQP_HIP(hipEventCreate(&e));
+    ev_used_ = 0;
+    const int saved_check = to_check_;
+    to_check_ = 0;
+    profiling_ = true;
+    HPRQPStatus st = HPRQP_STATUS_OK;
+    for (size_t k = 0; k < iterations && st == HPRQP_STATUS_OK; ++k) st = iteration_step(k);
+    profiling_ = false;
+    to_check_ = saved_check;
+    HPRQP_HIP(hipStreamSynchronize(stream_));
+    double total[T_COUNT] = {0};
+    size_t count[T_COUNT] = {0};
+    for (size_t e = 0; e + 1 < ev_used_; ++e) {
+        const int tag = ev_tags_[e];
+        if (tag < 0 || ev_tags_[e + 1] != -1) continue;
+        float ms = 0.0f;
+        HPRQP_HIP(hipEventElapsedTime(&ms, ev_pool_[e], ev_pool_[e + 1]));
+        total[tag] += ms;
+        count[tag] += 1;
+    }
+    for (auto &e : ev_pool_) (void)hipEventDestroy(e);
+    ev_pool_.clear();
+    for (int t = 0; t < T_COUNT; ++t) {
+        mean_ms[t] = count[t] ? total[t] / (double)count[t] : 0.0;
+        bytes[t] = algorithmic_bytes(t);
+        launches[t] = count[t];
+    }
+    return st;
 }
 
 double HipBackend::scalar(const char *name) const
@@ -1644,6 +1574,16 @@ HPRQPStatus solve(const HPRQPModel *model, const HPRQPParams *params, HPRQPResul
         return HPRQP_STATUS_NOT_IMPLEMENTED;
     }
     return solve_gpu(model, params, results);
+}
+
+int hprqp_hip_session_profile(HPRQPHipSession *s, size_t iterations, int capacity, char *names, double *mean_ms,
+                              double *bytes_per_launch, size_t *launches)
+{
+    using B = hprqp::HipBackend;
+    if (!s || capacity < (int)B::T_COUNT || !names || !mean_ms || !bytes_per_launch || !launches) return -1;
+    if (s->backend.profile(iterations, mean_ms, bytes_per_launch, launches) != HPRQP_STATUS_OK) return -1;
+    for (int t = 0; t < (int)B::T_COUNT; ++t) std::snprintf(names + 32 * t, 32, "%s", B::tag_name(t));
+    return (int)B::T_COUNT;
 }
 
 HPRQPStatus hprqp_hip_csr_spmv(int nrow, const int *row_ptr, const int *col_ind, const double *values,

@@ -326,3 +326,60 @@ class Session:
                               self.model.num_constraints())
         finally:
             _LIB.results_free(C.byref(r))
+
+
+_LIB.hprqp_hip_batch_create.argtypes = [C.POINTER(CModel), C.c_size_t, C.POINTER(CParams), C.POINTER(C.c_void_p)]
+_LIB.hprqp_hip_batch_solve.argtypes = [C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]
+_LIB.hprqp_hip_batch_seconds.argtypes = [C.c_void_p]
+_LIB.hprqp_hip_batch_seconds.restype = C.c_double
+_LIB.hprqp_hip_batch_result.argtypes = [C.c_void_p, C.c_size_t, C.POINTER(CResults)]
+_LIB.hprqp_hip_batch_destroy.argtypes = [C.c_void_p]
+
+
+class Batch:
+    """hprqp_hip_batch_*: many independent QPs in one persistent launch."""
+
+    def __init__(self, models: Sequence[Model], params: Parameters | None = None) -> None:
+        self.models = list(models)
+        self.params = params or Parameters(verbose=False)
+        arr = (CModel * len(self.models))(*[m.c for m in self.models])
+        self._arr = arr
+        h = C.c_void_p()
+        st = _LIB.hprqp_hip_batch_create(arr, len(self.models), C.byref(self.params.c), C.byref(h))
+        if st != STATUS_OK:
+            raise RuntimeError(f"hprqp_hip_batch_create failed: {status_name(st)}")
+        self.h = h
+
+    def __del__(self) -> None:
+        if getattr(self, "h", None):
+            _LIB.hprqp_hip_batch_destroy(self.h)
+            self.h = None
+
+    def solve(self, bodies_per_launch: int = 100) -> int:
+        rem = C.c_size_t(0)
+        st = _LIB.hprqp_hip_batch_solve(self.h, bodies_per_launch, C.byref(rem))
+        if st != STATUS_OK:
+            raise RuntimeError(f"hprqp_hip_batch_solve failed: {status_name(st)}")
+        return int(rem.value)
+
+    @property
+    def seconds(self) -> float:
+        return _LIB.hprqp_hip_batch_seconds(self.h)
+
+    def result(self, k: int) -> Result:
+        r = CResults()
+        _LIB.results_init(C.byref(r))
+        st = _LIB.hprqp_hip_batch_result(self.h, k, C.byref(r))
+        try:
+            if st != STATUS_OK:
+                raise RuntimeError(f"hprqp_hip_batch_result failed: {status_name(st)}")
+            m = self.models[k]
+            return _result_of(r, r.status, m.num_variables(), m.num_constraints())
+        finally:
+            _LIB.results_free(C.byref(r))
+
+
+def solve_batch(models: Sequence[Model], params: Parameters | None = None, bodies_per_launch: int = 100):
+    b = Batch(models, params)
+    b.solve(bodies_per_launch)
+    return [b.result(k) for k in range(len(b.models))], b.seconds

@@ -42,6 +42,14 @@ size_t hprqp_hip_session_matrix(const HPRQPHipSession *session, const char *name
 /* Final results exactly as solve_gpu reports them (original units). */
 HPRQPStatus hprqp_hip_session_results(HPRQPHipSession *session, HPRQPResults *results);
 
+/* Times `iterations` HPR iterations (non-check variant, advancing the
+ * session's iterate) with HIP events on the session's stream. For each
+ * iteration kernel kind k < return value: names + 32*k (NUL-terminated),
+ * mean_ms[k] per launch, bytes_per_launch[k] (algorithmic HBM bytes),
+ * launches[k]. capacity >= 8. Returns -1 on error. */
+int hprqp_hip_session_profile(HPRQPHipSession *session, size_t iterations, int capacity, char *names,
+                              double *mean_ms, double *bytes_per_launch, size_t *launches);
+
 /* Kernel-level entry points on caller-provided device buffers (int32 CSR,
  * FP64 values, device pointers). stream may be NULL for the default stream.
  * y = M x with the row-blocked kernel; row blocks are computed internally and
@@ -54,6 +62,23 @@ HPRQPStatus hprqp_hip_csr_spmv(int nrow, const int *row_ptr, const int *col_ind,
 HPRQPStatus hprqp_hip_csr_transpose(int nrow, int ncol, int nnz, const int *row_ptr, const int *col_ind,
                                     const double *values, int *t_row_ptr, int *t_col_ind, double *t_values,
                                     int *perm, void *stream);
+
+/* Batched solver: `count` independent QPs (built with build_from_QAbc; no
+ * cones, not the noC variant) solved by one persistent launch with one
+ * workgroup per instance and the whole HPR state machine on the device.
+ * create uploads, scales and runs setup + the iteration-0 check on the
+ * device; solve relaunches the chunk kernel (bodies_per_launch loop bodies
+ * per instance per launch) until every instance terminates (stoptol,
+ * max_iter from params) or params->time_limit passes; result returns
+ * instance k in original units. No counterpart in the reference, which
+ * solves one QP per call. */
+typedef struct HPRQPHipBatch HPRQPHipBatch;
+HPRQPStatus hprqp_hip_batch_create(const HPRQPModel *models, size_t count, const HPRQPParams *params,
+                                   HPRQPHipBatch **out_batch);
+HPRQPStatus hprqp_hip_batch_solve(HPRQPHipBatch *batch, size_t bodies_per_launch, size_t *remaining);
+double hprqp_hip_batch_seconds(const HPRQPHipBatch *batch);
+HPRQPStatus hprqp_hip_batch_result(const HPRQPHipBatch *batch, size_t index, HPRQPResults *results);
+void hprqp_hip_batch_destroy(HPRQPHipBatch *batch);
 
 /* Name of the device the library runs on, or "" when none is usable. */
 const char *hprqp_hip_device_name(void);

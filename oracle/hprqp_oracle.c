@@ -653,7 +653,7 @@ static double power_iteration(const Model *md, int which)
 }
 
 /* workspace.c:191-248 (+ control.cu:189-220 for cone models) */
-static int work_setup(Work *w, const Model *md)
+static int work_setup(Work *w, const Model *md, const OrParams *p)
 {
     size_t free_cnt = 0;
     double nb = 0.0, nc, nb2 = 0.0;
@@ -674,7 +674,7 @@ static int work_setup(Work *w, const Model *md)
     w->Q_is_diag = q_diag_check(&md->Q, w->diag_Q);
     if (w->Q_is_diag) diag_factors(w);
     /* eigs.c:155-185 */
-    w->lambda_max_A = power_iteration(md, 0) * 1.05;
+    w->lambda_max_A = p->lambda_A_given > 0.0 ? p->lambda_A_given : power_iteration(md, 0) * 1.05;
     if (w->Q_is_diag) {
         w->lambda_max_Q = 0.0;
         for (size_t i = 0; i < md->n; ++i) {
@@ -682,7 +682,7 @@ static int work_setup(Work *w, const Model *md)
             if (d > w->lambda_max_Q) w->lambda_max_Q = d;
         }
     } else {
-        w->lambda_max_Q = power_iteration(md, 1) * 1.05;
+        w->lambda_max_Q = p->lambda_Q_given > 0.0 ? p->lambda_Q_given : power_iteration(md, 1) * 1.05;
     }
     if (w->soc) {
         w->noC = 0;
@@ -1380,6 +1380,8 @@ void or_params_defaults(OrParams *p)
     p->check_iter = 100;
     p->print_frequency = -1;
     p->use_scaling = 1;
+    p->lambda_A_given = 0.0;
+    p->lambda_Q_given = 0.0;
 }
 
 int or_session_create(size_t n, size_t m,
@@ -1401,7 +1403,7 @@ int or_session_create(size_t n, size_t m,
     st = model_build(&s->md, n, m, q_nnz, q_ptr, q_ind, q_val, a_nnz, a_ptr, a_ind, a_val, c, AL, AU, l, u,
                      obj_constant, num_var_cones, var_offsets, num_row_cones, row_offsets, soc_rhs);
     if (st == OR_OK) st = scale_model(&s->md, params->use_scaling, &s->sc);
-    if (st == OR_OK) st = work_setup(&s->w, &s->md);
+    if (st == OR_OK) st = work_setup(&s->w, &s->md, params);
     if (st != OR_OK) {
         or_session_destroy(s);
         return st;

@@ -30,6 +30,10 @@ typedef struct OrParams {
     size_t check_iter;
     int print_frequency;
     int use_scaling;
+    /* > 0: use this lambda_max estimate instead of running the power
+     * iteration (only the CPU-baseline timing uses this, to bound setup). */
+    double lambda_A_given;
+    double lambda_Q_given;
 } OrParams;
 
 typedef struct OrResults {

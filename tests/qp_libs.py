@@ -199,7 +199,8 @@ def soc_qp(seed=0, n_box=100, cones=10, cone_dim=10, m=50, a_row_nnz=10, box=5.0
 
 class OrParams(C.Structure):
     _fields_ = [("stoptol", C.c_double), ("max_iter", C.c_size_t), ("time_limit", C.c_double),
-                ("check_iter", C.c_size_t), ("print_frequency", C.c_int), ("use_scaling", C.c_int)]
+                ("check_iter", C.c_size_t), ("print_frequency", C.c_int), ("use_scaling", C.c_int),
+                ("lambda_A_given", C.c_double), ("lambda_Q_given", C.c_double)]
 
 
 class OrResults(C.Structure):
@@ -449,3 +450,154 @@ def binding_solve(L, qp: QP, **params):
         L.hprqp_results_destroy(res)
         L.hprqp_params_destroy(prm)
         L.hprqp_model_destroy(model)
+
+
+# --------------------------------------------------------------------------- cone fixtures
+
+def _csr_dense(M):
+    M = np.asarray(M, dtype=float)
+    rows, cols = np.nonzero(M)
+    ptr = np.zeros(M.shape[0] + 1, np.uint64)
+    np.add.at(ptr, rows + 1, 1)
+    return np.cumsum(ptr).astype(np.uint64), cols.astype(np.uint64), M[rows, cols]
+
+
+def qp_dense_soc(Qd, Ad, c, AL, AU, l, u, var_offsets=None, row_offsets=None, rhs=None):
+    Qd = np.atleast_2d(np.asarray(Qd, float))
+    n = Qd.shape[1]
+    Ad = np.asarray(Ad, float).reshape(-1, n)
+    qp_, qi, qv = _csr_dense(Qd)
+    ap, ai, av = _csr_dense(Ad)
+    qp = QP(n, Ad.shape[0], qp_, qi, qv, ap, ai, av, np.asarray(c, float), np.asarray(AL, float),
+            np.asarray(AU, float), np.asarray(l, float), np.asarray(u, float))
+    if var_offsets is not None:
+        qp.var_offsets = np.asarray(var_offsets, np.uint64)
+    if row_offsets is not None:
+        qp.row_offsets = np.asarray(row_offsets, np.uint64)
+        qp.soc_rhs = np.asarray(rhs, float)
+    return qp
+
+
+def soc_mixed_model(q_kind):
+    """tests/gpu/test_gpu_soc.cu:115-150: 8 variables (box prefix 2, cones
+    [2,5) and [5,8)), 7 rows (two linear, an empty one, cones [2,4), [4,7))
+    built around a known primal-dual point x = (0.25,-0.5,5,3,4,2,0,0)."""
+    n, m = 8, 7
+    inf = math.inf
+    a = np.zeros((m, n))
+    for i in range(m):
+        for j in range(n):
+            a[i, j] = 0.0 if i == 3 else (((i * 3 + j * 5 + 1) % 11) - 5) / 7.0
+    q = np.zeros((n, n))
+    for j in range(n):
+        if q_kind:
+            q[j, j] = 1.0 + j / 8.0
+        if q_kind == 2 and j + 1 < n:
+            q[j, j] += 0.25
+            q[j + 1, j + 1] += 0.25
+            q[j, j + 1] = q[j + 1, j] = -0.25
+    x = np.array([0.25, -0.5, 5, 3, 4, 2, 0, 0])
+    y = np.array([0.2, 0.3, 2, -2, 0, 0, 0])
+    z = np.array([0.4, 0, 1, -0.6, -0.8, 0, 0, 0])
+    ax, aty, qx = a @ x, a.T @ y, q @ x
+    c = aty + z - qx
+    lower = np.full(n, -inf)
+    upper = np.full(n, inf)
+    lower[0], upper[0], lower[1], upper[1] = x[0], 1.0, -1.0, 1.0
+    al = np.full(m, -inf)
+    au = np.full(m, inf)
+    al[0] = au[0] = ax[0]
+    al[1] = ax[1]
+    slack = np.array([ax[0], ax[1], 1, 1, 3, 0, 0])
+    rhs = np.zeros(m)
+    rhs[2:] = ax[2:] - slack[2:]
+    return qp_dense_soc(q, a, c, al, au, lower, upper, [2, 5, 8], [2, 4, 7], rhs)
+
+
+def soc_simple_models():
+    """tests/gpu/test_gpu_soc.cu:234-273: name -> (qp, expected x or None)."""
+    inf = math.inf
+    out = {}
+    out["proj_cone"] = (qp_dense_soc(np.eye(3), np.zeros((0, 3)), [0, -3, -4], [], [], [-inf] * 3, [inf] * 3,
+                                     var_offsets=[0, 3]), [2.5, 1.5, 2.0])
+    out["affine_cone"] = (qp_dense_soc([[0.0]], [[1], [0], [0]], [1], [-inf] * 3, [inf] * 3, [-inf], [inf],
+                                       row_offsets=[0, 3], rhs=[2, -3, -4]), [7.0])
+    out["constant_cone"] = (qp_dense_soc([[1.0]], np.zeros((2, 1)), [-1], [-inf] * 2, [inf] * 2, [-inf], [inf],
+                                         row_offsets=[0, 2], rhs=[-3.0, 0.0]), [1.0])
+    out["constant_cone_infeasible"] = (qp_dense_soc([[1.0]], np.zeros((2, 1)), [-1], [-inf] * 2, [inf] * 2,
+                                                    [-inf], [inf], row_offsets=[0, 2], rhs=[1.0, 0.0]), None)
+    return out
+
+
+def cone_project(v):
+    v = np.asarray(v, float)
+    t, x = v[0], v[1:]
+    nx = np.linalg.norm(x)
+    if nx <= t:
+        return v.copy()
+    if nx <= -t:
+        return np.zeros_like(v)
+    return np.concatenate([[(nx + t) / 2], (nx + t) / (2 * nx) * x])
+
+
+def soc_certificate(qp: QP, x, y, z):
+    """Independent original-units optimality certificate
+    (tests/gpu/test_gpu_soc.cu:152-196): primal/dual cone residuals,
+    stationarity and the relative gap using support values."""
+    Qd, Ad = qp.dense_Q(), qp.dense_A()
+    Qs = 0.5 * (Qd + Qd.T)
+    n, m = qp.n, qp.m
+    lin_n = int(qp.var_offsets[0]) if qp.num_var_cones else n
+    lin_m = int(qp.row_offsets[0]) if qp.num_row_cones else m
+    qx, ax, aty = Qs @ x, Ad @ x, Ad.T @ y
+    rd = qx + qp.c - aty - z
+    primal = dual = support = 0.0
+    for j in range(lin_n):
+        primal = max(primal, qp.l[j] - x[j], x[j] - qp.u[j])
+        if z[j] > 0 and math.isfinite(qp.l[j]):
+            support += z[j] * qp.l[j]
+        elif z[j] < 0 and math.isfinite(qp.u[j]):
+            support += z[j] * qp.u[j]
+        else:
+            dual = max(dual, abs(z[j]))
+    for i in range(lin_m):
+        primal = max(primal, qp.AL[i] - ax[i], ax[i] - qp.AU[i])
+        if y[i] > 0 and math.isfinite(qp.AL[i]):
+            support += y[i] * qp.AL[i]
+        elif y[i] < 0 and math.isfinite(qp.AU[i]):
+            support += y[i] * qp.AU[i]
+        else:
+            dual = max(dual, abs(y[i]))
+    for k in range(qp.num_var_cones):
+        b, e = int(qp.var_offsets[k]), int(qp.var_offsets[k + 1])
+        primal = max(primal, np.max(np.abs(cone_project(x[b:e]) - x[b:e])))
+        dual = max(dual, np.max(np.abs(cone_project(z[b:e]) - z[b:e])))
+    slack = ax.copy()
+    for i in range(lin_m, m):
+        slack[i] -= qp.soc_rhs[i]
+        support += y[i] * qp.soc_rhs[i]
+    for k in range(qp.num_row_cones):
+        b, e = int(qp.row_offsets[k]), int(qp.row_offsets[k + 1])
+        primal = max(primal, np.max(np.abs(cone_project(slack[b:e]) - slack[b:e])))
+        dual = max(dual, np.max(np.abs(cone_project(y[b:e]) - y[b:e])))
+    pobj = 0.5 * x @ qx + qp.c @ x + qp.obj_constant
+    dobj = -0.5 * x @ qx + support + qp.obj_constant
+    gap = abs(pobj - dobj) / (1 + max(abs(pobj), abs(dobj)))
+    inf_ = lambda a: float(np.max(np.abs(a))) if len(a) else 0.0
+    stat = inf_(rd) / (1 + max(inf_(qp.c), inf_(qx), inf_(aty)))
+    return dict(primal=primal, dual=dual, stationarity=stat, gap=gap)
+
+
+def batched_instances(count=4096, seed=0, n_lo=50, n_hi=2000):
+    """The batched workload: n ~ U{n_lo..n_hi}, m = round(n/2) (60% equality),
+    A at min(10, n) nnz/row, Q = B'DB + 1e-2 I with B (n/4) x n at 4 nnz/row,
+    box [-10, 10]; instance k uses generator seed `seed * 1_000_003 + k`."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        n = int(rng.integers(n_lo, n_hi + 1))
+        m = int(round(n / 2))
+        m_eq = int(round(0.6 * m))
+        out.append(random_qp(seed=seed * 1_000_003 + k, n=n, m_eq=m_eq, m_in=m - m_eq, a_row_nnz=min(10, n),
+                             heavy_rows=0, heavy_nnz=0, b_rows=max(1, n // 4), b_row_nnz=4, box=10.0))
+    return out
