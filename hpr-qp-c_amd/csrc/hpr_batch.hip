@@ -58,9 +58,9 @@ struct State {
 enum NV {
     N_x, N_x_bar, N_x_hat, N_w, N_w_bar, N_ATy, N_ATy_bar, N_ATdy, N_Qw, N_Qx, N_dQw, N_QATdy, N_dx, N_dw,
     N_z_bar, N_tempv, N_diag, N_fact1, N_fact2, N_fact, N_fact_M, N_last_x, N_last_w, N_last_ATy, N_c, N_l, N_u,
-    N_col, N_tc, N_pq, N_pz, N_pt, N_COUNT
+    N_col, N_tc, N_pq, N_pz, N_pt, N_Qw_bar, N_last_Qw, N_COUNT
 };
-enum MV { M_y, M_y_bar, M_dy, M_s, M_last_y, M_AL, M_AU, M_row, M_tr, M_pq, M_pz, M_COUNT };
+enum MV { M_y, M_y_bar, M_dy, M_s, M_last_y, M_AL, M_AU, M_row, M_tr, M_pq, M_pz, M_e0, M_e1, M_e2, M_e3, M_COUNT };
 
 struct Arrays {
     const Inst *inst;
@@ -645,7 +645,8 @@ __device__ void wg_restart_apply(WG &g, double *sums, double *maxs)
     const int m = g.I->m;
     wg_spmv(g.AT.rp, g.AT.ci, g.AT.val, g.M(M_y_bar), g.AT.blk, g.AT.nblk,
             EpiRestart{{}, g.N(N_ATy_bar), g.N(N_ATy), g.N(N_last_ATy), g.N(N_x), g.N(N_w), g.N(N_last_x),
-                       g.N(N_last_w), g.N(N_Qw), g.N(N_pq), g.N(N_x_bar), g.N(N_w_bar), g.N(N_pq), m > 0 ? 1 : 0, 0},
+                       g.N(N_last_w), g.N(N_Qw), g.N(N_last_Qw), g.N(N_x_bar), g.N(N_w_bar), g.N(N_Qw_bar),
+                       m > 0 ? 1 : 0, g.I->noC},
             sums, maxs, g.prod);
     wg_map(m, MapRestartM{{}, g.M(M_y), g.M(M_last_y), g.M(M_y_bar)}, sums, maxs);
     if (threadIdx.x == 0) {
@@ -657,7 +658,7 @@ __device__ void wg_restart_apply(WG &g, double *sums, double *maxs)
     __syncthreads();
 }
 
-// iteration.c:250-293 (general Q: the noC variant is not used in batches)
+// iteration.c:250-293 (general variant)
 template <bool DIAG, bool CHECK>
 __device__ void wg_step_t(WG &g, double h1, double h2, double *sums, double *maxs)
 {
@@ -686,10 +687,45 @@ __device__ void wg_step_t(WG &g, double h1, double h2, double *sums, double *max
             sums, maxs, g.prod);
 }
 
+// iteration.c:19-83 and 176-216 (noC: A^T y first, Halpern on w and Qw)
+template <bool DIAG, bool CHECK>
+__device__ void wg_step_noc_t(WG &g, double h1, double h2, double *sums, double *maxs)
+{
+    const State &S = *g.S;
+    const int m = g.I->m;
+    const double sg = S.sigma;
+    const double f2s = 1.0 / (1.0 + sg * S.lam_q), f1s = 1.0 - f2s;
+    wg_spmv(g.AT.rp, g.AT.ci, g.AT.val, g.M(M_y), g.AT.blk, g.AT.nblk,
+            EpiZxw1NoC<DIAG, CHECK>{{}, g.N(N_ATy), g.N(N_x), g.N(N_x_bar), g.N(N_x_hat), g.N(N_w), g.N(N_w_bar),
+                                    g.N(N_dx), g.N(N_dw), g.N(N_z_bar), g.N(N_Qw), g.N(N_c), g.N(N_l), g.N(N_u),
+                                    g.N(N_last_x), g.N(N_last_w), g.N(N_fact1), g.N(N_fact2), sg, f1s, f2s, h1, h2},
+            sums, maxs, g.prod);
+    wg_spmv(g.Q.rp, g.Q.ci, g.Q.val, g.N(N_w_bar), g.Q.blk, g.Q.nblk,
+            EpiQwNoC{{}, g.N(N_Qw), g.N(N_Qw_bar), g.N(N_tempv), g.N(N_last_Qw), g.N(N_x_hat), sg, h1, h2}, sums,
+            maxs, g.prod);
+    if (m > 0) {
+        const double f1 = S.lam_a * sg, f2 = 1.0 / f1;
+        wg_spmv(g.A.rp, g.A.ci, g.A.val, g.N(N_tempv), g.A.blk, g.A.nblk,
+                EpiY<CHECK>{{}, g.M(M_y), g.M(M_y_bar), g.M(M_s), g.M(M_dy), g.M(M_AL), g.M(M_AU), g.M(M_last_y), f1,
+                            f2, h1, h2},
+                sums, maxs, g.prod);
+    }
+}
+
 __device__ void wg_step(WG &g, long long inner, double *sums, double *maxs)
 {
     const double h1 = 1.0 / ((double)inner + 2.0), h2 = 1.0 - h1;
     const bool diag = g.S->Q_is_diag, chk = g.S->to_check;
+    if (g.I->noC) {
+        if (diag) {
+            if (chk) wg_step_noc_t<true, true>(g, h1, h2, sums, maxs);
+            else wg_step_noc_t<true, false>(g, h1, h2, sums, maxs);
+        } else {
+            if (chk) wg_step_noc_t<false, true>(g, h1, h2, sums, maxs);
+            else wg_step_noc_t<false, false>(g, h1, h2, sums, maxs);
+        }
+        return;
+    }
     if (diag) {
         if (chk) wg_step_t<true, true>(g, h1, h2, sums, maxs);
         else wg_step_t<true, false>(g, h1, h2, sums, maxs);
@@ -772,6 +808,173 @@ __device__ void wg_bind(WG &g, const Arrays &ar, const Params &P, int k, State *
     g.qt_perm = ar.perm + ar.qt_perm_base + I->q_nz;
     for (int v = 0; v < N_COUNT; ++v) g.vn[v] = ar.vn + (long long)v * ar.sum_n + I->vn;
     for (int v = 0; v < M_COUNT; ++v) g.vm[v] = ar.vm + (long long)v * ar.sum_m + I->vm;
+}
+
+
+// ------------------------------------------------------------- adjoint
+// Implicit differentiation of the converged KKT point of one instance.
+// Active set from the projections: row i is active when y_bar_i != 0 or it is
+// an equality row; variable j is bound-active when z_bar_j != 0 or l_j = u_j.
+// In the scaled space (Q~ = D^-1 Q D^-1, A~ = R^-1 A D^-1) solve
+//   [Q~_FF  A~_SF'] [u~_F]   [(g/d)_F]
+//   [A~_SF    0   ] [v~_S] = [   0   ]
+// matrix-free by MINRES (the conjugate-residual form, one operator application
+// per step) with the workgroup SpMV engine, then recover the bound
+// multipliers v~_B = (g/d)_B - (Q~ u~ + A~' v~)_B and unscale:
+// u = u~/d, v_rows = v~/r, v_bounds = d v~_B.
+
+struct AdjMask {
+    const double *z_bar, *l, *u, *y_bar, *AL, *AU;
+    __device__ bool free_var(int j) const { return z_bar[j] == 0.0 && l[j] != u[j]; }
+    __device__ bool active_row(int i) const { return y_bar[i] != 0.0 || AL[i] == AU[i]; }
+};
+
+struct EpiAdjN : NoRed {  // yn = mask_F (tmp + A~' xm)
+    double *yn;
+    const double *tmp;
+    AdjMask mk;
+    __device__ void row(int i, double v, double *, double *) const { yn[i] = mk.free_var(i) ? tmp[i] + v : 0.0; }
+};
+
+struct EpiAddStore : NoRed {  // out = tmp + value
+    double *out;
+    const double *tmp;
+    __device__ void row(int i, double v, double *, double *) const { out[i] = tmp[i] + v; }
+};
+
+struct EpiAdjM : NoRed {  // ym = mask_S (A~ xn)
+    double *ym;
+    AdjMask mk;
+    __device__ void row(int i, double v, double *, double *) const { ym[i] = mk.active_row(i) ? v : 0.0; }
+};
+
+__device__ void adj_op(WG &g, const AdjMask &mk, const double *xn, const double *xm, double *yn, double *ym,
+                       double *sums, double *maxs)
+{
+    double *tmp = g.N(N_pt);
+    wg_spmv(g.Q.rp, g.Q.ci, g.Q.val, xn, g.Q.blk, g.Q.nblk, EpiStore{{}, tmp}, sums, maxs, g.prod);
+    wg_spmv(g.AT.rp, g.AT.ci, g.AT.val, xm, g.AT.blk, g.AT.nblk, EpiAdjN{{}, yn, tmp, mk}, sums, maxs, g.prod);
+    if (g.I->m > 0) wg_spmv(g.A.rp, g.A.ci, g.A.val, xn, g.A.blk, g.A.nblk, EpiAdjM{{}, ym, mk}, sums, maxs, g.prod);
+}
+
+__device__ double adj_dot(WG &g, const double *an, const double *am, const double *bn, const double *bm,
+                          double *sums, double *maxs)
+{
+    const int n = g.I->n, m = g.I->m;
+    for (int i = threadIdx.x; i < n; i += kThreads) sums[0] += an[i] * bn[i];
+    for (int i = threadIdx.x; i < m; i += kThreads) sums[0] += am[i] * bm[i];
+    __syncthreads();
+    wg_reduce<1, 0>(sums, maxs, g.red);
+    const double v = g.red[0];
+    __syncthreads();
+    return v;
+}
+
+// y += a x over both parts
+__device__ void adj_axpy(WG &g, double a, const double *xn, const double *xm, double *yn, double *ym)
+{
+    const int n = g.I->n, m = g.I->m;
+    for (int i = threadIdx.x; i < n; i += kThreads) yn[i] += a * xn[i];
+    for (int i = threadIdx.x; i < m; i += kThreads) ym[i] += a * xm[i];
+    __syncthreads();
+}
+
+__device__ void adj_copy(WG &g, const double *xn, const double *xm, double *yn, double *ym)
+{
+    const int n = g.I->n, m = g.I->m;
+    for (int i = threadIdx.x; i < n; i += kThreads) yn[i] = xn[i];
+    for (int i = threadIdx.x; i < m; i += kThreads) ym[i] = xm[i];
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(kThreads) void batch_adjoint_kernel(Arrays ar, Params P, int count,
+                                                                 const double *__restrict__ gvec,
+                                                                 double *__restrict__ u_out,
+                                                                 double *__restrict__ vrow_out,
+                                                                 double *__restrict__ vbnd_out, double tol,
+                                                                 int max_steps, int *__restrict__ steps_out)
+{
+    __shared__ double prod[kStreamCap];
+    __shared__ double red[kMaxTerms];
+    __shared__ State S;
+    const int k = blockIdx.x;
+    if (k >= count) return;
+    if (threadIdx.x == 0) S = ar.state[k];
+    __syncthreads();
+    WG g;
+    wg_bind(g, ar, P, k, &S, prod, red);
+    const int n = g.I->n, m = g.I->m, t = threadIdx.x;
+    double sums[kMaxTerms] = {0}, maxs[kMaxTerms] = {0};
+    const AdjMask mk{g.N(N_z_bar), g.N(N_l), g.N(N_u), g.M(M_y_bar), g.M(M_AL), g.M(M_AU)};
+    const double *d = g.N(N_col), *rr = g.M(M_row);
+    const double *gk = gvec + g.I->vn;
+    double *bn = g.N(N_tc);
+    double *xn = g.N(N_pq), *xm = g.M(M_pq);
+    double *rn = g.N(N_pz), *rm = g.M(M_pz);
+    double *pn[3] = {g.N(N_dx), g.N(N_dw), g.N(N_tempv)};
+    double *pm[3] = {g.M(M_tr), g.M(M_dy), g.M(M_e0)};
+    double *sn[3] = {g.N(N_dQw), g.N(N_QATdy), g.N(N_Qx)};
+    double *sm[3] = {g.M(M_e1), g.M(M_e2), g.M(M_e3)};
+    for (int i = t; i < n; i += kThreads) {
+        const double b = mk.free_var(i) ? gk[i] / d[i] : 0.0;
+        bn[i] = b;
+        xn[i] = 0.0;
+        rn[i] = b;
+        pn[0][i] = b;
+    }
+    for (int i = t; i < m; i += kThreads) {
+        xm[i] = 0.0;
+        rm[i] = 0.0;
+        pm[0][i] = 0.0;
+    }
+    __syncthreads();
+    const double bnorm = sqrt(adj_dot(g, bn, rm, bn, rm, sums, maxs));
+    adj_op(g, mk, pn[0], pm[0], sn[0], sm[0], sums, maxs);
+    adj_copy(g, pn[0], pm[0], pn[1], pm[1]);
+    adj_copy(g, sn[0], sm[0], sn[1], sm[1]);
+    // ring indices: i0 = newest (p0,s0), i1 = p1,s1, i2 = p2,s2
+    int i0 = 0, i1 = 1, i2 = 2;
+    int step = 0;
+    if (bnorm > 0.0) {
+        for (step = 1; step <= max_steps; ++step) {
+            // rotate: p2 <- p1, p1 <- p0; the next p0 is written into p2's old buffer
+            const int o2 = i2;
+            i2 = i1;
+            i1 = i0;
+            i0 = o2;
+            const double ss = adj_dot(g, sn[i1], sm[i1], sn[i1], sm[i1], sums, maxs);
+            if (!(ss > 0.0)) break;
+            const double alpha = adj_dot(g, rn, rm, sn[i1], sm[i1], sums, maxs) / ss;
+            adj_axpy(g, alpha, pn[i1], pm[i1], xn, xm);
+            adj_axpy(g, -alpha, sn[i1], sm[i1], rn, rm);
+            const double rnorm = sqrt(adj_dot(g, rn, rm, rn, rm, sums, maxs));
+            if (rnorm <= tol * bnorm) break;
+            adj_copy(g, sn[i1], sm[i1], pn[i0], pm[i0]);
+            adj_op(g, mk, sn[i1], sm[i1], sn[i0], sm[i0], sums, maxs);
+            const double beta1 = adj_dot(g, sn[i0], sm[i0], sn[i1], sm[i1], sums, maxs) / ss;
+            adj_axpy(g, -beta1, pn[i1], pm[i1], pn[i0], pm[i0]);
+            adj_axpy(g, -beta1, sn[i1], sm[i1], sn[i0], sm[i0]);
+            if (step > 1) {
+                const double s22 = adj_dot(g, sn[i2], sm[i2], sn[i2], sm[i2], sums, maxs);
+                if (s22 > 0.0) {
+                    const double beta2 = adj_dot(g, sn[i0], sm[i0], sn[i2], sm[i2], sums, maxs) / s22;
+                    adj_axpy(g, -beta2, pn[i2], pm[i2], pn[i0], pm[i0]);
+                    adj_axpy(g, -beta2, sn[i2], sm[i2], sn[i0], sm[i0]);
+                }
+            }
+        }
+    }
+    // bound multipliers: v~_B = b~_B - (Q~ u~ + A~' v~)_B with b~ = g/d
+    double *tn = g.N(N_Qw_bar), *tmp = g.N(N_pt);
+    wg_spmv(g.Q.rp, g.Q.ci, g.Q.val, xn, g.Q.blk, g.Q.nblk, EpiStore{{}, tmp}, sums, maxs, g.prod);
+    wg_spmv(g.AT.rp, g.AT.ci, g.AT.val, xm, g.AT.blk, g.AT.nblk, EpiAddStore{{}, tn, tmp}, sums, maxs, g.prod);
+    for (int i = t; i < n; i += kThreads) {
+        const bool fr = mk.free_var(i);
+        u_out[g.I->vn + i] = xn[i] / d[i];
+        vbnd_out[g.I->vn + i] = fr ? 0.0 : d[i] * (gk[i] / d[i] - tn[i]);
+    }
+    for (int i = t; i < m; i += kThreads) vrow_out[g.I->vm + i] = xm[i] / rr[i];
+    if (t == 0) steps_out[k] = step;
 }
 
 __global__ __launch_bounds__(kThreads) void batch_setup_kernel(Arrays ar, Params P, int count)
@@ -913,13 +1116,6 @@ HPRQPStatus hprqp_hip_batch_create(const HPRQPModel *models, size_t count, const
             delete b;
             return md.num_var_cones || md.num_row_cones ? HPRQP_STATUS_NOT_IMPLEMENTED : HPRQP_STATUS_INVALID_ARGUMENT;
         }
-        size_t free_count = 0;
-        for (size_t i = 0; i < md.Q.ncol; ++i)
-            if (std::isinf(md.l[i]) && std::isinf(md.u[i])) ++free_count;
-        if (5 * free_count > 4 * md.Q.ncol) {  // the noC variant is not batched
-            delete b;
-            return HPRQP_STATUS_NOT_IMPLEMENTED;
-        }
         sum_n += (long long)md.Q.ncol;
         sum_m += (long long)md.A.nrow;
         sum_q += (long long)md.Q.nnz;
@@ -951,7 +1147,12 @@ HPRQPStatus hprqp_hip_batch_create(const HPRQPModel *models, size_t count, const
         I.a_nz = anz;
         I.vn = vno;
         I.vm = vmo;
-        I.noC = 0;
+        {
+            size_t free_count = 0;
+            for (size_t i = 0; i < n; ++i)
+                if (std::isinf(md.l[i]) && std::isinf(md.u[i])) ++free_count;
+            I.noC = 5 * free_count > 4 * n ? 1 : 0;  // workspace.c:45-61
+        }
         I.obj_constant = md.obj_constant;
         double nb = 0.0, nc = 0.0;
         for (size_t i = 0; i < m; ++i) {
@@ -1131,5 +1332,47 @@ HPRQPStatus hprqp_hip_batch_result(const HPRQPHipBatch *b, size_t k, HPRQPResult
 }
 
 void hprqp_hip_batch_destroy(HPRQPHipBatch *b) { delete b; }
+
+}  // extern "C"
+
+extern "C" {
+
+// Adjoint of every instance of a solved batch: for upstream gradients g
+// (dL/dx in original units, instances concatenated) returns u (Sum n), the
+// row multipliers v_rows (Sum m) and the bound multipliers v_bounds (Sum n) of
+// the active-set KKT system K [u; v] = [g; 0]; steps[k] = MINRES steps used.
+// Host arrays in and out.
+HPRQPStatus hprqp_hip_batch_adjoint(HPRQPHipBatch *b, const double *dl_dx, double tol, int max_steps, double *u,
+                                    double *v_rows, double *v_bounds, int *steps)
+{
+    using namespace hprqp::batch;
+    if (!b || !dl_dx || !u || !v_rows || !v_bounds || max_steps <= 0) return HPRQP_STATUS_INVALID_ARGUMENT;
+    const size_t sn = (size_t)b->ar.sum_n, sm = (size_t)b->ar.sum_m;
+    double *d_g = nullptr, *d_u = nullptr, *d_vr = nullptr, *d_vb = nullptr;
+    int *d_steps = nullptr;
+    HPRQP_HIP(hipMalloc((void **)&d_g, sizeof(double) * sn));
+    HPRQP_HIP(hipMalloc((void **)&d_u, sizeof(double) * sn));
+    HPRQP_HIP(hipMalloc((void **)&d_vb, sizeof(double) * sn));
+    HPRQP_HIP(hipMalloc((void **)&d_vr, sizeof(double) * sm));
+    HPRQP_HIP(hipMalloc((void **)&d_steps, sizeof(int) * b->count));
+    HPRQP_HIP(hipMemcpy(d_g, dl_dx, sizeof(double) * sn, hipMemcpyHostToDevice));
+    HPRQP_HIP(hipMemset(d_vr, 0, sizeof(double) * sm));
+    batch_adjoint_kernel<<<(unsigned)b->count, hprqp::kThreads, 0, b->stream>>>(b->ar, b->P, (int)b->count, d_g, d_u,
+                                                                                d_vr, d_vb, tol, max_steps, d_steps);
+    HPRQP_HIP(hipGetLastError());
+    HPRQP_HIP(hipStreamSynchronize(b->stream));
+    HPRQP_HIP(hipMemcpy(u, d_u, sizeof(double) * sn, hipMemcpyDeviceToHost));
+    HPRQP_HIP(hipMemcpy(v_bounds, d_vb, sizeof(double) * sn, hipMemcpyDeviceToHost));
+    size_t real_m = 0;
+    for (size_t k = 0; k < b->count; ++k) real_m += b->m[k];
+    HPRQP_HIP(hipMemcpy(v_rows, d_vr, sizeof(double) * real_m, hipMemcpyDeviceToHost));
+    if (steps) HPRQP_HIP(hipMemcpy(steps, d_steps, sizeof(int) * b->count, hipMemcpyDeviceToHost));
+    (void)hipFree(d_g);
+    (void)hipFree(d_u);
+    (void)hipFree(d_vr);
+    (void)hipFree(d_vb);
+    (void)hipFree(d_steps);
+    return HPRQP_STATUS_OK;
+}
 
 }  // extern "C"

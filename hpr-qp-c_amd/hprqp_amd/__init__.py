@@ -383,3 +383,31 @@ def solve_batch(models: Sequence[Model], params: Parameters | None = None, bodie
     b = Batch(models, params)
     b.solve(bodies_per_launch)
     return [b.result(k) for k in range(len(b.models))], b.seconds
+
+
+_LIB.hprqp_hip_batch_adjoint.argtypes = [C.c_void_p, _db_p, C.c_double, C.c_int, _db_p, _db_p, _db_p,
+                                         C.POINTER(C.c_int)]
+
+
+def batch_adjoint(batch: Batch, dl_dx: Sequence[np.ndarray], tol: float = 1e-12, max_steps: int = 20000):
+    """Per instance (u, v_rows, v_bounds, steps) of the active-set KKT adjoint
+    K [u; v] = [dl_dx; 0] (see include/hprqp_hip.h)."""
+    ns = [m.num_variables() for m in batch.models]
+    ms = [m.num_constraints() for m in batch.models]
+    g = np.ascontiguousarray(np.concatenate([np.asarray(a, np.float64) for a in dl_dx]))
+    assert g.size == sum(ns)
+    u = np.zeros(sum(ns))
+    vb = np.zeros(sum(ns))
+    vr = np.zeros(max(sum(ms), 1))
+    steps = np.zeros(len(ns), np.int32)
+    st = _LIB.hprqp_hip_batch_adjoint(batch.h, g.ctypes.data_as(_db_p), tol, max_steps, u.ctypes.data_as(_db_p),
+                                      vr.ctypes.data_as(_db_p), vb.ctypes.data_as(_db_p),
+                                      steps.ctypes.data_as(C.POINTER(C.c_int)))
+    if st != STATUS_OK:
+        raise RuntimeError(f"hprqp_hip_batch_adjoint failed: {status_name(st)}")
+    out, on, om = [], 0, 0
+    for k, (n, m) in enumerate(zip(ns, ms)):
+        out.append((u[on:on + n].copy(), vr[om:om + m].copy(), vb[on:on + n].copy(), int(steps[k])))
+        on += n
+        om += m
+    return out

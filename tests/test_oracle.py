@@ -252,3 +252,25 @@ def test_cone_projection_against_closed_form_and_moreau():
         L.or_soc_project(q.ctypes.data_as(Q.c_double_p), mv.ctypes.data_as(Q.c_double_p), d)
         assert np.allclose(p - q, v, rtol=1e-12, atol=1e-12 * np.abs(v).max())
         assert abs(p @ q) <= 1e-9 * max(1.0, np.abs(v).max() ** 2)
+
+
+def test_sigma_search_sensitivity():
+    """Why iterates after the first restarts are compared loosely: with lambda_A
+    changed by one ulp the oracle's own trajectory agrees to ~1e-15 through
+    iteration 150 but sigma differs by ~1e-8 relative after the second
+    restart, because the golden-section search (control.c:45-77) resolves the
+    minimiser only to about sqrt(eps)."""
+    qp = Q.random_qp(seed=1, n=60, m_eq=20, m_in=10, a_row_nnz=5, b_rows=15, b_row_nnz=3)
+    base = Q.OracleSession(qp)
+    la, lq = base.scalar("lambda_max_A"), base.scalar("lambda_max_Q")
+
+    def run(k, lam):
+        o = Q.OracleSession(qp, max_iter=k, lambda_A_given=lam, lambda_Q_given=lq)
+        r = o.solve()
+        return r[1], o.scalar("sigma")
+    x1, _ = run(150, la)
+    x2, _ = run(150, la * (1 + 2.0 ** -52))
+    assert np.max(np.abs(x1 - x2)) <= 1e-13 * np.max(np.abs(x1))
+    _, s1 = run(250, la)
+    _, s2 = run(250, la * (1 + 2.0 ** -52))
+    assert 1e-12 < abs(s1 - s2) / s1 < 1e-6
