@@ -35,6 +35,8 @@ sys.path.insert(0, str(ROOT / "tests"))
 
 HBM_PEAK_GBS = 8000.0
 
+SOC_WORKLOAD = dict(seed=0, n_box=20_000, cones=20_000, cone_dim=10, m=100_000, a_row_nnz=40, box=5.0)
+
 WORKLOADS = {
     "plain_n1M": dict(seed=0, n=1_000_000, m_eq=300_000, m_in=200_000, a_row_nnz=20, heavy_rows=5_000,
                       heavy_nnz=2_000, b_rows=200_000, b_row_nnz=8, box=10.0),
@@ -77,6 +79,14 @@ def profile_kernels(H, model, iters):
 
 def cpu_baseline(qp, gpu_iters, lam_a, lam_q, sample_bodies):
     import qp_libs as Q
+    if sample_bodies >= 10 ** 9:  # small workload: the whole solve on one core
+        t0 = time.time()
+        o = Q.OracleSession(qp)
+        r = o.solve()[0]
+        dt = time.time() - t0
+        return dict(value=r.time, unit="s (solve loop to 1e-8)", cores=1, kind="port",
+                    sample=f"oracle, full solve ({r.iter} iterations, {Q.STATUS[r.status]}); "
+                           f"setup+solve {dt:.3f} s")
     t0 = time.time()
     o = Q.OracleSession(qp, lambda_A_given=lam_a, lambda_Q_given=lam_q)
     setup = time.time() - t0
@@ -96,7 +106,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="plain_n1M", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="plain_n1M",
+                    choices=sorted(WORKLOADS) + ["soc_n220k", "batched_4096", "layer_1024"])
+    ap.add_argument("--batch-count", type=int, default=4096)
+    ap.add_argument("--seq-sample", type=int, default=32)
     ap.add_argument("--stoptol", type=float, default=1e-8)
     ap.add_argument("--profile-iters", type=int, default=20)
     ap.add_argument("--cpu-sample", type=int, default=10)
@@ -112,13 +125,30 @@ def main():
     if not H.gpu_backend_available():
         raise SystemExit("no gfx950 device visible")
     log("device:", H.device_name())
+    if args.workload == "batched_4096":
+        return bench_batched(H, Q, args)
+    if args.workload == "layer_1024":
+        return bench_layer(H, Q, args)
+    if args.workload == "soc_n220k":
+        qp = Q.soc_qp(**SOC_WORKLOAD)
+        log(f"generated soc_n220k: n={qp.n} m={qp.m} nnzA={len(qp.a_val)} cones={qp.num_var_cones}")
+        return bench_single(H, Q, args, qp, "soc_n220k", soc=True)
     kw = WORKLOADS[args.workload]
     t = time.time()
     qp = Q.random_qp(**kw)
     log(f"generated {args.workload}: n={qp.n} m={qp.m} nnzA={len(qp.a_val)} nnzQ={len(qp.q_val)} "
         f"in {time.time() - t:.1f}s")
+    return bench_single(H, Q, args, qp, args.workload)
+
+
+def bench_single(H, Q, args, qp, workload, soc=False):
     t = time.time()
-    model = build_model(H, qp)
+    if soc:
+        model = H.build_from_qabc((qp.n, qp.n, qp.q_ptr, qp.q_ind, qp.q_val), qp.c,
+                                  (qp.m, qp.n, qp.a_ptr, qp.a_ind, qp.a_val), qp.AL, qp.AU, qp.l, qp.u,
+                                  soc={"var_offsets": qp.var_offsets})
+    else:
+        model = build_model(H, qp)
     log(f"host build/preprocess {time.time() - t:.1f}s (nnzQ after symmetrisation {model.c.Q.nnz})")
 
     for w in range(args.warmup):
@@ -152,7 +182,8 @@ def main():
     cpu = None
     if not args.no_cpu:
         s = H.Session(model, H.Parameters(verbose=False))
-        cpu = cpu_baseline(qp, iters, s.scalar("lambda_max_A"), s.scalar("lambda_max_Q"), args.cpu_sample)
+        cpu = cpu_baseline(qp, iters, s.scalar("lambda_max_A"), s.scalar("lambda_max_Q"),
+                           args.cpu_sample if qp.n > 100_000 else 10 ** 9)
         del s
         log("cpu_baseline:", cpu)
 
@@ -161,7 +192,7 @@ def main():
         "value": round(value, 4), "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(value * 1e3, 2), "higher_is_better": False, "scaling": "strong",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded generator)",
-        "config": {"workload": args.workload, "n": qp.n, "m": qp.m, "nnzA": int(len(qp.a_val)),
+        "config": {"workload": workload, "n": qp.n, "m": qp.m, "nnzA": int(len(qp.a_val)),
                    "nnzQ": int(model.c.Q.nnz), "stoptol": args.stoptol, "iterations": iters,
                    "status": results[-1].status, "objective": results[-1].objective,
                    "kkt": results[-1].residual, "time_to_1e-6_s": results[-1].time_6,
@@ -170,6 +201,71 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    print(json.dumps(line), flush=True)
+
+
+def bench_batched(H, Q, args):
+    """Batched workload: args.batch_count QPs with n ~ U{50..2000} in one
+    persistent launch (stoptol 1e-6), against sequential plain GPU solves of a
+    sample of the same instances."""
+    t = time.time()
+    qps = Q.batched_instances(count=args.batch_count, seed=0)
+    models = [build_model(H, qp) for qp in qps]
+    log(f"built {len(models)} instances in {time.time() - t:.1f}s; sum n={sum(q.n for q in qps)}")
+    params = H.Parameters(verbose=False, stoptol=1e-6)
+    b = H.Batch(models[:8], params)
+    b.solve(50)
+    del b
+    t = time.time()
+    b = H.Batch(models, params)
+    setup = time.time() - t
+    left = b.solve(100)
+    secs = b.seconds
+    res = [b.result(k) for k in range(len(models))]
+    ok = sum(r.status == "OPTIMAL" for r in res)
+    qps_s = len(models) / secs
+    log(f"batched: {ok}/{len(models)} optimal, {secs:.3f}s solve ({setup:.2f}s setup), {qps_s:.1f} QPs/s, left={left}")
+    sample = list(range(0, len(models), max(1, len(models) // args.seq_sample)))[:args.seq_sample]
+    t = time.time()
+    seq_iters = 0
+    for k in sample:
+        r = H.optimize(models[k], params)
+        seq_iters += r.iter
+    seq = time.time() - t
+    seq_qps = len(sample) / seq
+    log(f"sequential plain GPU solves: {len(sample)} in {seq:.2f}s -> {seq_qps:.1f} QPs/s")
+    line = {"metric": "batched QPs/s at 1e-6 (1 GPU)", "value": round(qps_s, 2), "unit": "QPs/s", "n_gpus": 1,
+            "steps": 1, "warmup": 1, "ms_per_step": round(secs * 1e3, 2), "higher_is_better": True,
+            "scaling": "strong", "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded generator)",
+            "config": {"workload": f"batched_{len(models)}", "optimal": ok, "solve_s": secs, "setup_s": setup,
+                       "sequential_plain_gpu_qps": round(seq_qps, 3), "speedup_vs_sequential": round(qps_s / seq_qps, 2),
+                       "sequential_sample": len(sample)}}
+    print(json.dumps(line), flush=True)
+
+
+def bench_layer(H, Q, args):
+    """Differentiable layer workload: 1024 dense QPs (n=256, m=320); forward to
+    1e-8 and backward (adjoint) times."""
+    import torch
+    from hprqp_amd.layer import QPLayer
+    count = min(args.batch_count, 1024)
+    Qd, c, A, AL, AU, l, u = Q.layer_instances(count)
+    T = lambda a: torch.tensor(a, dtype=torch.float64, requires_grad=True)
+    ts = [T(a) for a in (Qd, c, A, AL, AU, l, u)]
+    t = time.time()
+    x = QPLayer.apply(*ts, 1e-8)
+    fwd = time.time() - t
+    g = torch.tensor(np.random.default_rng(0).normal(size=x.shape))
+    t = time.time()
+    (x * g).sum().backward()
+    bwd = time.time() - t
+    log(f"layer: forward {fwd:.3f}s backward {bwd:.3f}s")
+    line = {"metric": "QP layer fwd+bwd ms (1 GPU)", "value": round((fwd + bwd) * 1e3, 2), "unit": "ms",
+            "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": round((fwd + bwd) * 1e3, 2),
+            "higher_is_better": False, "scaling": "strong", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic (seeded generator)",
+            "config": {"workload": f"layer_{count}", "forward_ms": fwd * 1e3, "backward_ms": bwd * 1e3,
+                       "backward_over_forward": bwd / fwd}}
     print(json.dumps(line), flush=True)
 
 

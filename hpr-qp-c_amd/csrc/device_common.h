@@ -119,32 +119,47 @@ __global__ __launch_bounds__(kThreads) void spmv_rows_kernel(const int *__restri
     for (int i = 0; i < (NM > 0 ? NM : 1); ++i) maxs[i] = 0.0;
 
     if (r1 - r0 == 1 && k1 - k0 > kLongRow) {
-        double acc = 0.0;
-        for (int k = k0 + t; k < k1; k += kThreads) acc = fma(val[k], x[ci[k]], acc);
-        acc = block_sum_all(acc);
+        // vector mode: 4 independent gathers in flight per thread, then a tree
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+        int k = k0 + t;
+        for (; k + 3 * kThreads < k1; k += 4 * kThreads) {
+            const int c0 = __builtin_nontemporal_load(ci + k), c1 = __builtin_nontemporal_load(ci + k + kThreads);
+            const int c2 = __builtin_nontemporal_load(ci + k + 2 * kThreads);
+            const int c3 = __builtin_nontemporal_load(ci + k + 3 * kThreads);
+            const double v0 = __builtin_nontemporal_load(val + k), v1 = __builtin_nontemporal_load(val + k + kThreads);
+            const double v2 = __builtin_nontemporal_load(val + k + 2 * kThreads);
+            const double v3 = __builtin_nontemporal_load(val + k + 3 * kThreads);
+            const double x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
+            a0 = fma(v0, x0, a0);
+            a1 = fma(v1, x1, a1);
+            a2 = fma(v2, x2, a2);
+            a3 = fma(v3, x3, a3);
+        }
+        for (; k < k1; k += kThreads) a0 = fma(__builtin_nontemporal_load(val + k), x[__builtin_nontemporal_load(ci + k)], a0);
+        double acc = block_sum_all((a0 + a1) + (a2 + a3));
         if (t == 0) epi.row(r0, acc, sums, maxs);
     } else {
         const int cnt = k1 - k0;
         constexpr int kPer = kStreamCap / kThreads;
-        int col[kPer];
-        double v[kPer], xv[kPer];
+        if (cnt > 0) {
+            // Branch-free load phase: indices are clamped into the block so all
+            // 2*kPer stream loads and kPer gathers issue before any use.
+            int col[kPer];
+            double v[kPer], xv[kPer];
+            const int last = cnt - 1;
 #pragma unroll
-        for (int it = 0; it < kPer; ++it) {
-            const int k = t + it * kThreads;
-            if (k < cnt) {
-                col[it] = ci[k0 + k];
-                v[it] = val[k0 + k];
+            for (int it = 0; it < kPer; ++it) {
+                const int k = min(t + it * kThreads, last);
+                col[it] = __builtin_nontemporal_load(ci + k0 + k);
+                v[it] = __builtin_nontemporal_load(val + k0 + k);
             }
-        }
 #pragma unroll
-        for (int it = 0; it < kPer; ++it) {
-            const int k = t + it * kThreads;
-            if (k < cnt) xv[it] = x[col[it]];
-        }
+            for (int it = 0; it < kPer; ++it) xv[it] = x[col[it]];
 #pragma unroll
-        for (int it = 0; it < kPer; ++it) {
-            const int k = t + it * kThreads;
-            if (k < cnt) prod[k] = v[it] * xv[it];
+            for (int it = 0; it < kPer; ++it) {
+                const int k = t + it * kThreads;
+                if (k < cnt) prod[k] = v[it] * xv[it];
+            }
         }
         __syncthreads();
         const int r = r0 + t;
