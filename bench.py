@@ -114,6 +114,9 @@ def main():
     ap.add_argument("--profile-iters", type=int, default=20)
     ap.add_argument("--cpu-sample", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile-only", action="store_true",
+                    help="set up the workload, run --profile-iters timed iterations and a 256 MiB device copy "
+                         "(the reference kernel for counter calibration); no solve, no JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU sharding of this "
@@ -141,6 +144,17 @@ def main():
     return bench_single(H, Q, args, qp, args.workload)
 
 
+def calibration_copy():
+    """One device-to-device copy of 256 MiB (reads 256 MiB, writes 256 MiB):
+    the known-size kernel against which FETCH_SIZE / WRITE_SIZE are scaled."""
+    import torch
+    src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    log("calibration copy: 268435456 bytes read, 268435456 bytes written")
+
+
 def bench_single(H, Q, args, qp, workload, soc=False):
     t = time.time()
     if soc:
@@ -150,6 +164,13 @@ def bench_single(H, Q, args, qp, workload, soc=False):
     else:
         model = build_model(H, qp)
     log(f"host build/preprocess {time.time() - t:.1f}s (nnzQ after symmetrisation {model.c.Q.nnz})")
+    if args.profile_only:
+        prof = profile_kernels(H, model, args.profile_iters)
+        for p in prof:
+            log(f"kernel {p['kernel']}: {p['mean_ms'] * 1e3:.1f} us/launch, {p['bytes'] / 1e6:.1f} MB, "
+                f"{p['gbs']:.0f} GB/s")
+        calibration_copy()
+        return
 
     for w in range(args.warmup):
         r = H.optimize(model, H.Parameters(verbose=False, max_iter=50, stoptol=args.stoptol))

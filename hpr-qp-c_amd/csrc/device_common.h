@@ -100,11 +100,18 @@ __device__ inline void block_reduce_store(const double *sums, const double *maxs
 //   __device__ void row(int r, double v, double *sums, double *maxs) const;
 //   double *partials;                          // [gridDim.x][NS+NM] or null
 
-template <class Epi>
+//
+// Column panels: a matrix may be split into column panels (each its own CSR
+// over the same rows). Panel p > 0 is run with CARRY_IN: a row's sum starts
+// from the partial sum the previous panel left in `carry`, so with columns
+// ascending inside each row the left-to-right order, and hence the bits, are
+// those of the unsplit product; each panel gathers only a slice of x.
+template <class Epi, bool CARRY_IN = false>
 __global__ __launch_bounds__(kThreads) void spmv_rows_kernel(const int *__restrict__ rp, const int *__restrict__ ci,
                                                              const double *__restrict__ val,
                                                              const double *__restrict__ x,
-                                                             const int *__restrict__ blocks, Epi epi)
+                                                             const int *__restrict__ blocks, Epi epi,
+                                                             const double *__restrict__ carry = nullptr)
 {
     constexpr int NS = Epi::NS, NM = Epi::NM;
     __shared__ double prod[kStreamCap];
@@ -137,6 +144,7 @@ __global__ __launch_bounds__(kThreads) void spmv_rows_kernel(const int *__restri
         }
         for (; k < k1; k += kThreads) a0 = fma(__builtin_nontemporal_load(val + k), x[__builtin_nontemporal_load(ci + k)], a0);
         double acc = block_sum_all((a0 + a1) + (a2 + a3));
+        if (CARRY_IN) acc = carry[r0] + acc;
         if (t == 0) epi.row(r0, acc, sums, maxs);
     } else {
         const int cnt = k1 - k0;
@@ -164,7 +172,7 @@ __global__ __launch_bounds__(kThreads) void spmv_rows_kernel(const int *__restri
         __syncthreads();
         const int r = r0 + t;
         if (r < r1) {
-            double acc = 0.0;
+            double acc = CARRY_IN ? carry[r] : 0.0;
             const int e = rp[r + 1] - k0;
             for (int k = rp[r] - k0; k < e; ++k) acc += prod[k];
             epi.row(r, acc, sums, maxs);

@@ -29,6 +29,12 @@ struct EpiStore : NoRed {
     __device__ void row(int i, double v, double *, double *) const { out[i] = v; }
 };
 
+// Partial row sum handed to the next column panel.
+struct EpiCarry : NoRed {
+    double *carry;
+    __device__ void row(int i, double v, double *, double *) const { carry[i] = v; }
+};
+
 // iteration.c:85-123
 template <bool DIAG, bool CHECK>
 struct EpiZxw1 : NoRed {

@@ -116,6 +116,32 @@ HPRQPStatus upload_blocks(DevCsr &m, hipStream_t s)
     return HPRQP_STATUS_OK;
 }
 
+// Column panels of one matrix (see spmv_rows_kernel): part[p] holds the
+// entries whose column lies in [p*width, (p+1)*width), rows unchanged;
+// src[p][k] is the entry's index in the full matrix (values are gathered
+// through it after scaling).
+constexpr int kMaxPanels = 4;
+struct Panels {
+    int P = 1;
+    int width = 0;
+    DevCsr part[kMaxPanels];
+    int *src[kMaxPanels] = {nullptr, nullptr, nullptr, nullptr};
+};
+
+void free_csr(DevCsr &m);
+HPRQPStatus upload_blocks(DevCsr &m, hipStream_t s);
+
+void free_panels(Panels &pn)
+{
+    for (int p = 0; p < kMaxPanels; ++p) {
+        free_csr(pn.part[p]);
+        if (pn.src[p]) (void)hipFree(pn.src[p]);
+        pn.src[p] = nullptr;
+    }
+    pn.P = 1;
+    pn.width = 0;
+}
+
 void free_csr(DevCsr &m)
 {
     if (m.rp) (void)hipFree(m.rp);
@@ -123,6 +149,71 @@ void free_csr(DevCsr &m)
     if (m.val) (void)hipFree(m.val);
     if (m.blocks) (void)hipFree(m.blocks);
     m = DevCsr{};
+}
+
+__global__ void gather_values_kernel(int nnz, const int *__restrict__ perm, const double *__restrict__ src,
+                                     double *__restrict__ dst);
+
+// Split M into P column panels (structure planned on the host from M's
+// indices; values filled by fill_panel_values once they are final).
+HPRQPStatus build_panels(const DevCsr &M, int P, Panels &pn, hipStream_t s)
+{
+    free_panels(pn);
+    pn.P = P;
+    pn.width = (M.ncol + P - 1) / P;
+    if (P <= 1) return HPRQP_STATUS_OK;
+    std::vector<int> rp(M.nrow + 1), ci(M.nnz);
+    HPRQP_HIP(hipMemcpyAsync(rp.data(), M.rp, sizeof(int) * (M.nrow + 1), hipMemcpyDeviceToHost, s));
+    if (M.nnz) HPRQP_HIP(hipMemcpyAsync(ci.data(), M.ci, sizeof(int) * M.nnz, hipMemcpyDeviceToHost, s));
+    HPRQP_HIP(hipStreamSynchronize(s));
+    std::vector<std::vector<int>> prp(P, std::vector<int>(M.nrow + 1, 0)), pci(P), psrc(P);
+    for (int r = 0; r < M.nrow; ++r)
+        for (int k = rp[r]; k < rp[r + 1]; ++k) prp[ci[k] / pn.width][r + 1] += 1;
+    for (int p = 0; p < P; ++p) {
+        for (int r = 0; r < M.nrow; ++r) prp[p][r + 1] += prp[p][r];
+        pci[p].resize(prp[p][M.nrow]);
+        psrc[p].resize(prp[p][M.nrow]);
+    }
+    std::vector<std::vector<int>> fill(P);
+    for (int p = 0; p < P; ++p) fill[p].assign(prp[p].begin(), prp[p].end() - 1);
+    for (int r = 0; r < M.nrow; ++r)
+        for (int k = rp[r]; k < rp[r + 1]; ++k) {
+            const int p = ci[k] / pn.width;
+            const int dst = fill[p][r]++;
+            pci[p][dst] = ci[k];
+            psrc[p][dst] = k;
+        }
+    for (int p = 0; p < P; ++p) {
+        DevCsr &d = pn.part[p];
+        d.nrow = M.nrow;
+        d.ncol = M.ncol;
+        d.nnz = (int)pci[p].size();
+        const size_t nz = std::max((size_t)d.nnz, (size_t)1);
+        HPRQP_HIP(hipMalloc(&d.rp, sizeof(int) * (M.nrow + 1)));
+        HPRQP_HIP(hipMalloc(&d.ci, sizeof(int) * nz));
+        HPRQP_HIP(hipMalloc(&d.val, sizeof(double) * nz));
+        HPRQP_HIP(hipMalloc(&pn.src[p], sizeof(int) * nz));
+        HPRQP_HIP(hipMemcpyAsync(d.rp, prp[p].data(), sizeof(int) * (M.nrow + 1), hipMemcpyHostToDevice, s));
+        if (d.nnz) {
+            HPRQP_HIP(hipMemcpyAsync(d.ci, pci[p].data(), sizeof(int) * d.nnz, hipMemcpyHostToDevice, s));
+            HPRQP_HIP(hipMemcpyAsync(pn.src[p], psrc[p].data(), sizeof(int) * d.nnz, hipMemcpyHostToDevice, s));
+        }
+        HPRQP_HIP(hipStreamSynchronize(s));
+        HPRQPStatus st = upload_blocks(d, s);
+        if (st != HPRQP_STATUS_OK) return st;
+    }
+    return HPRQP_STATUS_OK;
+}
+
+HPRQPStatus fill_panel_values(const DevCsr &M, Panels &pn, hipStream_t s)
+{
+    if (pn.P <= 1) return HPRQP_STATUS_OK;
+    for (int p = 0; p < pn.P; ++p)
+        if (pn.part[p].nnz)
+            gather_values_kernel<<<grid_for(pn.part[p].nnz), kThreads, 0, s>>>(pn.part[p].nnz, pn.src[p], M.val,
+                                                                              pn.part[p].val);
+    HPRQP_HIP(hipGetLastError());
+    return HPRQP_STATUS_OK;
 }
 
 // ----------------------------------------------------------- device transpose
@@ -539,21 +630,21 @@ enum Vec {
     // n-sized
     V_x, V_x_bar, V_x_hat, V_w, V_w_bar, V_ATy, V_ATy_bar, V_ATdy, V_Qw, V_Qw_bar, V_Qx, V_dQw, V_QATdy, V_dx,
     V_dw, V_z_bar, V_tempv, V_diag_Q, V_fact1, V_fact2, V_fact, V_fact_M, V_last_x, V_last_Qw, V_last_w, V_last_ATy,
-    V_c, V_l, V_u, V_col_norm, V_tc, V_pn0, V_pn1, V_pn2,
+    V_c, V_l, V_u, V_col_norm, V_tc, V_pn0, V_pn1, V_pn2, V_carry_n,
     V_N_END,
     // m-sized
     V_y = V_N_END, V_y_bar, V_Ax, V_dy, V_s, V_last_y, V_AL, V_AU, V_row_norm, V_tr, V_soc_rhs, V_pm0, V_pm1,
-    V_Rp,
+    V_Rp, V_carry_m,
     V_M_END
 };
 
-static_assert(V_M_END == 48, "vector table");
+static_assert(V_M_END == 50, "vector table");
 constexpr int V_Rp_soc() { return V_Rp; }
 const char *const kVecNames[] = {
     "x", "x_bar", "x_hat", "w", "w_bar", "ATy", "ATy_bar", "ATdy", "Qw", "Qw_bar", "Qx", "dQw", "QATdy", "dx",
     "dw", "z_bar", "tempv", "diag_Q", "fact1", "fact2", "fact", "fact_M", "last_x", "last_Qw", "last_w", "last_ATy",
-    "c", "l", "u", "col_norm", "tc", "pn0", "pn1", "pn2",
-    "y", "y_bar", "Ax", "dy", "s", "last_y", "AL", "AU", "row_norm", "tr", "soc_rhs", "pm0", "pm1", "Rp"};
+    "c", "l", "u", "col_norm", "tc", "pn0", "pn1", "pn2", "carry_n",
+    "y", "y_bar", "Ax", "dy", "s", "last_y", "AL", "AU", "row_norm", "tr", "soc_rhs", "pm0", "pm1", "Rp", "carry_m"};
 
 class HipBackend final : public Backend {
 public:
@@ -601,6 +692,7 @@ private:
     int device_ = 0;
     hipStream_t stream_ = nullptr;
     DevCsr Q_, A_, AT_;
+    Panels Qp_, Ap_, ATp_;
     int *AT_perm_ = nullptr;  // A^T entry -> A entry
     int *QT_rp_ = nullptr, *QT_perm_ = nullptr;
     int *var_off_ = nullptr, *row_off_ = nullptr;
@@ -645,6 +737,13 @@ private:
 
     double *region(int r) const { return partials_ + (size_t)r * region_; }
 
+    const Panels &panels_of(const DevCsr &M) const { return &M == &Q_ ? Qp_ : (&M == &A_ ? Ap_ : ATp_); }
+    // nblocks of the launch that carries the reduction terms (the last panel)
+    int red_blocks(const DevCsr &M) const
+    {
+        const Panels &pn = panels_of(M);
+        return pn.P > 1 ? pn.part[pn.P - 1].nblocks : M.nblocks;
+    }
     template <class Epi>
     HPRQPStatus spmv(const DevCsr &M, const double *x, Epi epi, int reg = 0, int tag = -1)
     {
@@ -652,10 +751,28 @@ private:
         if (Epi::NS + Epi::NM > 0) epi.partials = region(reg);
         const bool timed = profiling_ && tag >= 0;
         if (timed) prof_begin(tag);
-        spmv_rows_kernel<Epi><<<M.nblocks, kThreads, 0, stream_>>>(M.rp, M.ci, M.val, x, M.blocks, epi);
+        const Panels &pn = panels_of(M);
+        if (pn.P <= 1) {
+            spmv_rows_kernel<Epi><<<M.nblocks, kThreads, 0, stream_>>>(M.rp, M.ci, M.val, x, M.blocks, epi);
+        } else {
+            double *carry = M.nrow == (int)n_ ? vec(V_carry_n) : vec(V_carry_m);
+            for (int p = 0; p + 1 < pn.P; ++p) {
+                const DevCsr &d = pn.part[p];
+                if (p == 0)
+                    spmv_rows_kernel<EpiCarry, false><<<d.nblocks, kThreads, 0, stream_>>>(
+                        d.rp, d.ci, d.val, x, d.blocks, EpiCarry{{}, carry}, carry);
+                else
+                    spmv_rows_kernel<EpiCarry, true><<<d.nblocks, kThreads, 0, stream_>>>(
+                        d.rp, d.ci, d.val, x, d.blocks, EpiCarry{{}, carry}, carry);
+            }
+            const DevCsr &d = pn.part[pn.P - 1];
+            spmv_rows_kernel<Epi, true><<<d.nblocks, kThreads, 0, stream_>>>(d.rp, d.ci, d.val, x, d.blocks, epi,
+                                                                              carry);
+        }
         if (timed) prof_end();
         return HPRQP_STATUS_OK;
     }
+    HPRQPStatus autotune_panels();
     template <class F>
     HPRQPStatus map(size_t len, F f, int reg = 0)
     {
@@ -688,6 +805,9 @@ private:
 void HipBackend::release()
 {
     if (stream_) (void)hipStreamSynchronize(stream_);
+    free_panels(Qp_);
+    free_panels(Ap_);
+    free_panels(ATp_);
     free_csr(Q_);
     free_csr(A_);
     free_csr(AT_);
@@ -847,6 +967,80 @@ HPRQPStatus HipBackend::scale(const HPRQPParams *params)
     return HPRQP_STATUS_OK;
 }
 
+// Panel-count autotuner: for Q, A and A^T try 1, 2 and 4 column panels
+// (only when the gathered vector exceeds 1 MiB and panels stay >= 32k
+// columns wide), time three plain products of each with HIP events on this
+// stream, keep the fastest. HPRQP_PANELS=1|2|4 forces a count.
+HPRQPStatus HipBackend::autotune_panels()
+{
+    const char *force = std::getenv("HPRQP_PANELS");
+    const int forced = force ? std::atoi(force) : 0;
+    struct Item {
+        const DevCsr *M;
+        Panels *pn;
+        double *x, *y;
+    } items[3] = {{&Q_, &Qp_, vec(V_pn0), vec(V_pn1)},
+                  {&A_, &Ap_, vec(V_pn0), vec(V_pm0)},
+                  {&AT_, &ATp_, vec(V_pm1), vec(V_pn2)}};
+    hipEvent_t e0, e1;
+    HPRQP_HIP(hipEventCreate(&e0));
+    HPRQP_HIP(hipEventCreate(&e1));
+    for (Item &it : items) {
+        const DevCsr &M = *it.M;
+        std::vector<int> cands = {1};
+        if (forced == 1 || forced == 2 || forced == 4) {
+            cands = {forced};
+        } else if ((size_t)M.ncol * sizeof(double) > ((size_t)1 << 20) && M.nnz > 0) {
+            if (M.ncol >= 2 * 32768) cands.push_back(2);
+            if (M.ncol >= 4 * 32768) cands.push_back(4);
+        }
+        int best = 1;
+        float best_ms = 1e30f;
+        for (int P : cands) {
+            HPRQPStatus st = build_panels(M, P, *it.pn, stream_);
+            if (st == HPRQP_STATUS_OK) st = fill_panel_values(M, *it.pn, stream_);
+            if (st != HPRQP_STATUS_OK) return st;
+            if (cands.size() == 1) {
+                best = P;
+                break;
+            }
+            float ms_min = 1e30f;
+            for (int rep = 0; rep < 3; ++rep) {
+                HPRQP_HIP(hipEventRecord(e0, stream_));
+                spmv(M, it.x, EpiStore{{}, it.y});
+                HPRQP_HIP(hipEventRecord(e1, stream_));
+                HPRQP_HIP(hipEventSynchronize(e1));
+                float ms = 0.0f;
+                HPRQP_HIP(hipEventElapsedTime(&ms, e0, e1));
+                ms_min = std::min(ms_min, ms);
+            }
+            if (ms_min < best_ms) {
+                best_ms = ms_min;
+                best = P;
+            }
+        }
+        if (it.pn->P != best) {
+            HPRQPStatus st = build_panels(M, best, *it.pn, stream_);
+            if (st == HPRQP_STATUS_OK) st = fill_panel_values(M, *it.pn, stream_);
+            if (st != HPRQP_STATUS_OK) return st;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    // the reduction partials must cover the largest panel launch
+    size_t need = region_ / kMaxTerms;
+    for (const Panels *pn : {&Qp_, &Ap_, &ATp_})
+        for (int p = 0; p < pn->P; ++p) need = std::max(need, (size_t)pn->part[p].nblocks);
+    if (need * kMaxTerms > region_) {
+        HPRQP_HIP(hipStreamSynchronize(stream_));
+        HPRQP_HIP(hipFree(partials_));
+        region_ = need * kMaxTerms;
+        HPRQP_HIP(hipMalloc(&partials_, sizeof(double) * region_ * 4));
+    }
+    HPRQP_HIP(hipStreamSynchronize(stream_));
+    return HPRQP_STATUS_OK;
+}
+
 // eigs.c:19-153: power iteration on A A^T (q_matrix = false) or Q.
 HPRQPStatus HipBackend::power_iteration(bool q_matrix, double *lambda)
 {
@@ -874,7 +1068,7 @@ HPRQPStatus HipBackend::power_iteration(bool q_matrix, double *lambda)
         const DevCsr &M = q_matrix ? Q_ : A_;
         if (!q_matrix) spmv(AT_, q, EpiStore{{}, tmp});
         spmv(M, q_matrix ? q : tmp, EpiPowerZ{nullptr, z, q}, 0);
-        finish(0, M.nblocks, 2, 0, 0);
+        finish(0, red_blocks(M), 2, 0, 0);
         map(len, MapPowerResid{nullptr, q, z, red_ + 0}, 1);
         finish(1, g, 1, 0, 2);
         st = read_red(3);
@@ -981,6 +1175,7 @@ HPRQPStatus HipBackend::setup(const HPRQPModel *model, const HPRQPParams *params
     for (size_t i = 0; i < model->Q.ncol; ++i) norm_c_org_ = std::max(norm_c_org_, std::fabs(model->c[i]));
     HPRQPStatus st = upload(model);
     if (st == HPRQP_STATUS_OK) st = scale(params);
+    if (st == HPRQP_STATUS_OK) st = autotune_panels();
     if (st == HPRQP_STATUS_OK) st = workspace_setup(model);
     return st;
 }
@@ -1062,13 +1257,13 @@ HPRQPStatus HipBackend::compute_residuals(HPRQPResiduals *res, size_t iter)
     std::memset(res, 0, sizeof(*res));
     if (num_var_cones_ || num_row_cones_) return soc_residuals(res);
     spmv(Q_, vec(V_x_bar), EpiResQ{nullptr, vec(V_Qx), vec(V_x_bar), vec(V_c), vec(V_z_bar)}, 0);
-    finish(0, Q_.nblocks, 3, 0, 0);
+    finish(0, red_blocks(Q_), 3, 0, 0);
     zero_red(3, 8);
     spmv(AT_, vec(V_y_bar), EpiResAT{nullptr, vec(V_ATdy), vec(V_Qx), vec(V_c), vec(V_z_bar), vec(V_col_norm), c_scale_}, 1);
-    finish(1, AT_.nblocks, 0, 3, 3);
+    finish(1, red_blocks(AT_), 0, 3, 3);
     if (m_ > 0) {
         spmv(A_, vec(V_x_bar), EpiResA{nullptr, vec(V_Ax), vec(V_AL), vec(V_AU), vec(V_row_norm), vec(V_y_bar), vec(V_s), b_scale_}, 2);
-        finish(2, A_.nblocks, 1, 2, 6);
+        finish(2, red_blocks(A_), 1, 2, 6);
     }
     if (iter == 0) {
         map(n_, MapBoundViolation{nullptr, vec(V_dx), vec(V_l), vec(V_u), vec(V_x_bar), vec(V_col_norm), b_scale_}, 3);
@@ -1154,14 +1349,14 @@ HPRQPStatus HipBackend::soc_residuals(HPRQPResiduals *res)
 {
     const int vc = (int)num_var_cones_, rc = (int)num_row_cones_;
     spmv(Q_, vec(V_x_bar), EpiResQ{nullptr, vec(V_Qx), vec(V_x_bar), vec(V_c), vec(V_z_bar)}, 0);
-    finish(0, Q_.nblocks, 3, 0, 0);
+    finish(0, red_blocks(Q_), 3, 0, 0);
     zero_red(3, 8);
     spmv(AT_, vec(V_y_bar), EpiResAT{nullptr, vec(V_ATdy), vec(V_Qx), vec(V_c), vec(V_z_bar), vec(V_col_norm), c_scale_}, 1);
-    finish(1, AT_.nblocks, 0, 3, 3);
+    finish(1, red_blocks(AT_), 0, 3, 3);
     if (m_ > 0) {
         spmv(A_, vec(V_x_bar), EpiSocResA{nullptr, vec(V_Ax), vec(V_Rp_soc()), rc ? vec(V_soc_rhs) : nullptr,
                                           vec(V_row_norm), vec(V_y_bar), vec(V_s), b_scale_}, 2);
-        finish(2, A_.nblocks, 2, 1, 6);
+        finish(2, red_blocks(A_), 2, 1, 6);
         soc_project_mixed_kernel<<<grid_for(std::max(lin_m_, (size_t)rc)), kThreads, 0, stream_>>>(
             (int)m_, (int)lin_m_, vec(V_Rp_soc()), vec(V_AL), vec(V_AU), rc, row_off_, vec(V_tr));
         map(m_, MapSocRowResid{nullptr, vec(V_tr), vec(V_Rp_soc()), vec(V_row_norm), b_scale_}, 3);
@@ -1196,12 +1391,12 @@ double HipBackend::compute_m_norm()
 {
     const bool diag = Q_is_diag_ != 0;
     spmv(Q_, vec(V_dw), EpiMnQdw{nullptr, vec(V_dQw), vec(V_dx), vec(V_dw)}, 0);
-    finish(0, Q_.nblocks, 4, 0, 0);
+    finish(0, red_blocks(Q_), 4, 0, 0);
     if (m_ > 0) {
         spmv(AT_, vec(V_dy), EpiMnATdy{nullptr, vec(V_ATdy), vec(V_dx)}, 1);
-        finish(1, AT_.nblocks, 1, 0, 4);
+        finish(1, red_blocks(AT_), 1, 0, 4);
         spmv(Q_, vec(V_ATdy), EpiMnQATdy{nullptr, vec(V_QATdy), vec(V_dQw), vec(V_ATdy), vec(V_fact_M), diag ? 1 : 0}, 2);
-        finish(2, Q_.nblocks, 3, 0, 5);
+        finish(2, red_blocks(Q_), 3, 0, 5);
         map(m_, MapDot{nullptr, vec(V_dy), vec(V_dy)}, 3);
         finish(3, m_grid(), 1, 0, 8);
     }
@@ -1246,10 +1441,10 @@ HPRQPStatus HipBackend::update_sigma(HPRQPRestart *restart, const HPRQPResiduals
         finish(1, m_grid(), 1, 0, 1);
     }
     spmv(Q_, vec(V_dw), EpiSigmaQdw{nullptr, vec(V_dQw), vec(V_ATdy), vec(V_dw)}, 2);
-    finish(2, Q_.nblocks, 3, 0, 2);
+    finish(2, red_blocks(Q_), 3, 0, 2);
     if (m_ > 0) {
         spmv(Q_, vec(V_ATdy), EpiSigmaQATdy{nullptr, vec(V_QATdy), vec(V_ATdy)}, 3);
-        finish(3, Q_.nblocks, 1, 0, 5);
+        finish(3, red_blocks(Q_), 1, 0, 5);
     }
     HPRQP_HIP(hipGetLastError());
     HPRQPStatus st = read_red(6);
@@ -1408,6 +1603,9 @@ double HipBackend::scalar(const char *name) const
     if (!std::strcmp(name, "pobj")) return last_res_.primal_obj_bar;
     if (!std::strcmp(name, "dobj")) return last_res_.dual_obj_bar;
     if (!std::strcmp(name, "restart_times")) return (double)restart_times_;
+    if (!std::strcmp(name, "panels_Q")) return Qp_.P;
+    if (!std::strcmp(name, "panels_A")) return Ap_.P;
+    if (!std::strcmp(name, "panels_AT")) return ATp_.P;
     return NAN;
 }
 
