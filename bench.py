@@ -144,15 +144,16 @@ def main():
     return bench_single(H, Q, args, qp, args.workload)
 
 
-def calibration_copy():
-    """One device-to-device copy of 256 MiB (reads 256 MiB, writes 256 MiB):
-    the known-size kernel against which FETCH_SIZE / WRITE_SIZE are scaled."""
-    import torch
-    src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
-    dst = torch.empty_like(src)
-    dst.copy_(src)
-    torch.cuda.synchronize()
-    log("calibration copy: 268435456 bytes read, 268435456 bytes written")
+def calibration_copy(H):
+    """One device-to-device copy of 256 MiB (reads 256 MiB, writes 256 MiB) by
+    the library's plain copy kernel: the known-size kernel against which
+    FETCH_SIZE / WRITE_SIZE are scaled."""
+    lib = H.lib()
+    lib.hprqp_hip_calibration_copy.argtypes = [C.c_size_t, C.POINTER(C.c_double)]
+    ms = C.c_double(0.0)
+    st = lib.hprqp_hip_calibration_copy(256 << 20, C.byref(ms))
+    log(f"calibration copy: 268435456 bytes read + written in {ms.value:.3f} ms "
+        f"({2 * 268435456 / (ms.value * 1e-3) / 1e9:.0f} GB/s), status {st}")
 
 
 def bench_single(H, Q, args, qp, workload, soc=False):
@@ -169,7 +170,7 @@ def bench_single(H, Q, args, qp, workload, soc=False):
         for p in prof:
             log(f"kernel {p['kernel']}: {p['mean_ms'] * 1e3:.1f} us/launch, {p['bytes'] / 1e6:.1f} MB, "
                 f"{p['gbs']:.0f} GB/s")
-        calibration_copy()
+        calibration_copy(H)
         return
 
     for w in range(args.warmup):

@@ -100,20 +100,34 @@ namespace batch {
 // y = M x over the instance's row blocks, each row's value handed to the
 // epilogue (stream / long-row modes as in spmv_rows_kernel). Per-thread
 // reduction terms accumulate across blocks; the caller reduces them.
+// Rows of at least kWaveRow entries are grouped (up to 8 per block) and each
+// is reduced by one wavefront (lanes stride the row, FMA, shuffle tree); the
+// other rows go through the LDS stream mode with sequential row sums.
+constexpr int kWaveRow = 64;
+
 template <class Epi>
 __device__ void wg_spmv(const int *__restrict__ rp, const int *__restrict__ ci, const double *__restrict__ val,
                         const double *__restrict__ x, const int *__restrict__ blk, int nblk, const Epi &epi,
                         double *sums, double *maxs, double *prod)
 {
     const int t = threadIdx.x;
+    const int lane = t & (kWave - 1), wid = t / kWave;
     for (int b = 0; b < nblk; ++b) {
         const int r0 = blk[b], r1 = blk[b + 1];
         const int k0 = rp[r0], k1 = rp[r1];
-        if (r1 - r0 == 1 && k1 - k0 > kLongRow) {
-            double acc = 0.0;
-            for (int k = k0 + t; k < k1; k += kThreads) acc = fma(val[k], x[ci[k]], acc);
-            acc = block_sum_all(acc);
-            if (t == 0) epi.row(r0, acc, sums, maxs);
+        if (k1 - k0 >= kWaveRow * (r1 - r0)) {
+            for (int r = r0 + wid; r < r1; r += kThreads / kWave) {
+                const int e = rp[r + 1];
+                double a0 = 0.0, a1 = 0.0;
+                int k = rp[r] + lane;
+                for (; k + kWave < e; k += 2 * kWave) {
+                    a0 = fma(val[k], x[ci[k]], a0);
+                    a1 = fma(val[k + kWave], x[ci[k + kWave]], a1);
+                }
+                if (k < e) a0 = fma(val[k], x[ci[k]], a0);
+                const double acc = wave_sum(a0 + a1);
+                if (lane == 0) epi.row(r, acc, sums, maxs);
+            }
         } else {
             const int cnt = k1 - k0;
             for (int k = t; k < cnt; k += kThreads) prod[k] = val[k0 + k] * x[ci[k0 + k]];
@@ -1036,10 +1050,6 @@ __global__ __launch_bounds__(kThreads) void batch_solve_kernel(Arrays ar, Params
 
 // ================================================================ host side
 
-namespace hprqp {
-std::vector<int> plan_row_blocks(const std::vector<int> &rp);
-}
-
 struct HPRQPHipBatch {
     size_t count = 0;
     hprqp::batch::Arrays ar{};
@@ -1066,6 +1076,35 @@ struct HPRQPHipBatch {
 };
 
 namespace {
+
+// Row blocks for the workgroup engine: runs of rows with >= kWaveRow entries
+// (up to 8 per block, one wavefront per row), otherwise stream blocks of
+// <= kThreads rows and <= kStreamCap entries.
+std::vector<int> plan_batch_blocks(const std::vector<int> &rp)
+{
+    using hprqp::batch::kWaveRow;
+    const int nrow = (int)rp.size() - 1;
+    std::vector<int> starts;
+    int r = 0;
+    while (r < nrow) {
+        starts.push_back(r);
+        const int first = r;
+        if (rp[r + 1] - rp[r] >= kWaveRow) {
+            while (r < nrow && r - first < 8 && rp[r + 1] - rp[r] >= kWaveRow) ++r;
+            continue;
+        }
+        int nnz = 0;
+        while (r < nrow && r - first < hprqp::kThreads) {
+            const int len = rp[r + 1] - rp[r];
+            if (len >= kWaveRow || nnz + len > hprqp::kStreamCap) break;
+            nnz += len;
+            ++r;
+        }
+    }
+    starts.push_back(nrow);
+    if (nrow == 0) starts = {0, 0};
+    return starts;
+}
 
 // Host counting-sort transpose permutation (column-major order, rows
 // ascending within a column): perm[k] = source entry of output entry k.
@@ -1169,7 +1208,7 @@ HPRQPStatus hprqp_hip_batch_create(const HPRQPModel *models, size_t count, const
             return off;
         };
         auto push_blocks = [&](const std::vector<int> &r, int &off, int &cnt) {
-            std::vector<int> blk = hprqp::plan_row_blocks(r);
+            std::vector<int> blk = plan_batch_blocks(r);
             off = (int)blocks.size();
             cnt = (int)blk.size() - 1;
             blocks.insert(blocks.end(), blk.begin(), blk.end());

@@ -1806,6 +1806,39 @@ HPRQPStatus hprqp_hip_csr_spmv(int nrow, const int *row_ptr, const int *col_ind,
     return HPRQP_STATUS_OK;
 }
 
+// A plain streaming copy of `bytes` (multiple of 16) between two fresh device
+// buffers: reads and writes exactly `bytes` once. Reference kernel for
+// calibrating the FETCH_SIZE / WRITE_SIZE counters.
+__global__ void hprqp_calibration_copy_kernel(size_t n2, const double2 *__restrict__ src, double2 *__restrict__ dst)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n2; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+HPRQPStatus hprqp_hip_calibration_copy(size_t bytes, double *milliseconds)
+{
+    if (bytes == 0 || bytes % 16 != 0) return HPRQP_STATUS_INVALID_ARGUMENT;
+    double2 *a = nullptr, *b = nullptr;
+    HPRQP_HIP(hipMalloc((void **)&a, bytes));
+    HPRQP_HIP(hipMalloc((void **)&b, bytes));
+    HPRQP_HIP(hipMemset(a, 0, bytes));
+    hipEvent_t e0, e1;
+    HPRQP_HIP(hipEventCreate(&e0));
+    HPRQP_HIP(hipEventCreate(&e1));
+    HPRQP_HIP(hipEventRecord(e0, nullptr));
+    hprqp_calibration_copy_kernel<<<2048, 256>>>(bytes / 16, a, b);
+    HPRQP_HIP(hipEventRecord(e1, nullptr));
+    HPRQP_HIP(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    HPRQP_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (milliseconds) *milliseconds = ms;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    HPRQP_HIP(hipFree(a));
+    HPRQP_HIP(hipFree(b));
+    return HPRQP_STATUS_OK;
+}
+
 HPRQPStatus hprqp_hip_csr_transpose(int nrow, int ncol, int nnz, const int *row_ptr, const int *col_ind,
                                     const double *values, int *t_row_ptr, int *t_col_ind, double *t_values,
                                     int *perm, void *stream)

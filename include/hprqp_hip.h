@@ -90,6 +90,11 @@ void hprqp_hip_batch_destroy(HPRQPHipBatch *batch);
 HPRQPStatus hprqp_hip_batch_adjoint(HPRQPHipBatch *batch, const double *dl_dx, double tol, int max_steps, double *u,
                                     double *v_rows, double *v_bounds, int *steps);
 
+/* Copies `bytes` (a multiple of 16) between two fresh device buffers with a
+ * plain streaming kernel and reports its time: the known-size kernel used to
+ * calibrate HBM traffic counters. No counterpart in the reference. */
+HPRQPStatus hprqp_hip_calibration_copy(size_t bytes, double *milliseconds);
+
 /* Name of the device the library runs on, or "" when none is usable. */
 const char *hprqp_hip_device_name(void);
 
