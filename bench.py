@@ -269,6 +269,7 @@ def bench_layer(H, Q, args):
     """Differentiable layer workload: 1024 dense QPs (n=256, m=320); forward to
     1e-8 and backward (adjoint) times."""
     import torch
+    from hprqp_amd import layer as layer_mod
     from hprqp_amd.layer import QPLayer
     count = min(args.batch_count, 1024)
     Qd, c, A, AL, AU, l, u = Q.layer_instances(count)
@@ -281,13 +282,15 @@ def bench_layer(H, Q, args):
     t = time.time()
     (x * g).sum().backward()
     bwd = time.time() - t
-    log(f"layer: forward {fwd:.3f}s backward {bwd:.3f}s")
+    split = dict(layer_mod.last_timings)
+    log(f"layer: forward {fwd:.3f}s backward {bwd:.3f}s; split {split}")
     line = {"metric": "QP layer fwd+bwd ms (1 GPU)", "value": round((fwd + bwd) * 1e3, 2), "unit": "ms",
             "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": round((fwd + bwd) * 1e3, 2),
             "higher_is_better": False, "scaling": "strong", "vs_baseline": None, "dtype": "f64",
             "data": "synthetic (seeded generator)",
             "config": {"workload": f"layer_{count}", "forward_ms": fwd * 1e3, "backward_ms": bwd * 1e3,
-                       "backward_over_forward": bwd / fwd}}
+                       "backward_over_forward": bwd / fwd,
+                       "split_ms": {k: (round(v * 1e3, 2) if k != "iterations" else v) for k, v in split.items()}}}
     print(json.dumps(line), flush=True)
 
 

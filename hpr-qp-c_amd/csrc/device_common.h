@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <type_traits>
 
 #include "shared/types.h"
 
@@ -14,7 +15,7 @@ namespace hprqp {
 constexpr int kWave = 64;         // CDNA wavefront width
 constexpr int kThreads = 256;     // workgroup size of every row/element kernel
 constexpr int kStreamCap = 2048;  // nnz staged in LDS per stream-mode row block
-constexpr int kLongRow = 256;     // rows longer than this get a workgroup each
+constexpr int kLongRow = 256;     // rows longer than this leave the LDS stream path
 constexpr int kMaxTerms = 8;      // reduction terms one kernel may emit
 
 void report_hip_error(hipError_t err, const char *what, const char *file, int line);
@@ -88,36 +89,107 @@ __device__ inline void block_reduce_store(const double *sums, const double *maxs
 }
 
 // ------------------------------------------------------------------- SpMV
-// Row blocks: block b covers rows [blocks[b], blocks[b+1]). Either one row
-// longer than kLongRow (vector mode: the workgroup strides the row with FMA and
-// reduces), or up to kThreads rows with at most kStreamCap nnz in total
-// (stream mode: products v*x[col] are staged through LDS with coalesced loads,
-// then each row's owner thread sums them left to right without FMA, which
-// reproduces the sequential row sum of the host CSR product bit for bit).
+// Row blocks: block b covers rows [blk[b].x, blk[b+1].x) and entries
+// [blk[b].y, blk[b+1].y), so one round trip gives a block its bounds. A block
+// is one of three kinds, told apart by its mean row length:
+//   * stream (mean <= kLongRow): up to kThreads rows of <= kLongRow entries,
+//     <= kStreamCap entries in total. Products v*x[col] are staged through LDS
+//     with coalesced loads; each row's owner thread then sums them left to
+//     right without FMA, which reproduces the sequential row sum of the host
+//     CSR product bit for bit.
+//   * wave (rows longer than kLongRow, up to kThreads / kWave of them): one
+//     wavefront per row, FMA over a 64-lane stride, then a shuffle tree.
+//   * vector (one row longer than kVecRow): the whole workgroup strides the
+//     row with FMA and reduces.
+// Row bounds, carried partial sums and the epilogue's own row inputs are
+// loaded before the gathers, so they overlap the longest latency chain.
 //
 // The epilogue type supplies:
 //   static constexpr int NS, NM;              // reduction terms
 //   __device__ void row(int r, double v, double *sums, double *maxs) const;
 //   double *partials;                          // [gridDim.x][NS+NM] or null
-
+// and optionally a prefetch split: `struct Pre`, `Pre load(int r) const` and
+// `row_pre(int r, double v, const Pre &, double *sums, double *maxs) const`.
 //
 // Column panels: a matrix may be split into column panels (each its own CSR
 // over the same rows). Panel p > 0 is run with CARRY_IN: a row's sum starts
 // from the partial sum the previous panel left in `carry`, so with columns
 // ascending inside each row the left-to-right order, and hence the bits, are
 // those of the unsplit product; each panel gathers only a slice of x.
+
+constexpr int kVecRow = 4096;  // rows longer than this get a workgroup each
+
+template <class E, class = void>
+struct RowInputs {
+    struct Pre {};
+    __device__ static Pre load(const E &, int) { return Pre{}; }
+    __device__ static void finish(const E &e, int r, double v, const Pre &, double *s, double *m) { e.row(r, v, s, m); }
+};
+template <class E>
+struct RowInputs<E, std::void_t<typename E::Pre>> {
+    using Pre = typename E::Pre;
+    __device__ static Pre load(const E &e, int r) { return e.load(r); }
+    __device__ static void finish(const E &e, int r, double v, const Pre &p, double *s, double *m)
+    {
+        e.row_pre(r, v, p, s, m);
+    }
+};
+
+// Sum of one row's products over a wavefront (lane 0 holds the result).
+__device__ inline double wave_row_sum(const int *__restrict__ ci, const double *__restrict__ val,
+                                      const double *__restrict__ x, int k0, int k1, int lane)
+{
+    constexpr int U = 8;
+    double a[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) a[u] = 0.0;
+    int k = k0 + lane;
+    for (; k + (U - 1) * kWave < k1; k += U * kWave) {
+        int c[U];
+        double v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            c[u] = __builtin_nontemporal_load(ci + k + u * kWave);
+            v[u] = __builtin_nontemporal_load(val + k + u * kWave);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) a[u] = fma(v[u], x[c[u]], a[u]);
+    }
+    // tail: fewer than U*kWave entries left; clamped, masked loads keep them in flight together
+    if (k < k1) {
+        int c[U];
+        double v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int kk = min(k + u * kWave, k1 - 1);
+            c[u] = __builtin_nontemporal_load(ci + kk);
+            v[u] = __builtin_nontemporal_load(val + kk);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const double p = v[u] * x[c[u]];
+            if (k + u * kWave < k1) a[u] += p;
+        }
+    }
+    const double t = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+    return wave_sum(t);
+}
+
 template <class Epi, bool CARRY_IN = false>
 __global__ __launch_bounds__(kThreads) void spmv_rows_kernel(const int *__restrict__ rp, const int *__restrict__ ci,
                                                              const double *__restrict__ val,
                                                              const double *__restrict__ x,
-                                                             const int *__restrict__ blocks, Epi epi,
+                                                             const int2 *__restrict__ blk, Epi epi,
                                                              const double *__restrict__ carry = nullptr)
 {
     constexpr int NS = Epi::NS, NM = Epi::NM;
+    using RI = RowInputs<Epi>;
+    using Pre = typename RI::Pre;
     __shared__ double prod[kStreamCap];
     const int b = blockIdx.x, t = threadIdx.x;
-    const int r0 = blocks[b], r1 = blocks[b + 1];
-    const int k0 = rp[r0], k1 = rp[r1];
+    const int2 lo = blk[b], hi = blk[b + 1];
+    const int r0 = lo.x, r1 = hi.x, k0 = lo.y, k1 = hi.y;
+    const int nr = r1 - r0, cnt = k1 - k0;
     double sums[NS > 0 ? NS : 1];
     double maxs[NM > 0 ? NM : 1];
 #pragma unroll
@@ -125,29 +197,64 @@ __global__ __launch_bounds__(kThreads) void spmv_rows_kernel(const int *__restri
 #pragma unroll
     for (int i = 0; i < (NM > 0 ? NM : 1); ++i) maxs[i] = 0.0;
 
-    if (r1 - r0 == 1 && k1 - k0 > kLongRow) {
-        // vector mode: 4 independent gathers in flight per thread, then a tree
-        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-        int k = k0 + t;
-        for (; k + 3 * kThreads < k1; k += 4 * kThreads) {
-            const int c0 = __builtin_nontemporal_load(ci + k), c1 = __builtin_nontemporal_load(ci + k + kThreads);
-            const int c2 = __builtin_nontemporal_load(ci + k + 2 * kThreads);
-            const int c3 = __builtin_nontemporal_load(ci + k + 3 * kThreads);
-            const double v0 = __builtin_nontemporal_load(val + k), v1 = __builtin_nontemporal_load(val + k + kThreads);
-            const double v2 = __builtin_nontemporal_load(val + k + 2 * kThreads);
-            const double v3 = __builtin_nontemporal_load(val + k + 3 * kThreads);
-            const double x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
-            a0 = fma(v0, x0, a0);
-            a1 = fma(v1, x1, a1);
-            a2 = fma(v2, x2, a2);
-            a3 = fma(v3, x3, a3);
+    if (cnt > kLongRow * nr) {
+        if (nr == 1 && cnt > kVecRow) {
+            // vector mode: 4 independent gathers in flight per thread, then a tree
+            Pre pre{};
+            double cin = 0.0;
+            if (t == 0) {
+                pre = RI::load(epi, r0);
+                if (CARRY_IN) cin = carry[r0];
+            }
+            double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+            int k = k0 + t;
+            for (; k + 3 * kThreads < k1; k += 4 * kThreads) {
+                const int c0 = __builtin_nontemporal_load(ci + k), c1 = __builtin_nontemporal_load(ci + k + kThreads);
+                const int c2 = __builtin_nontemporal_load(ci + k + 2 * kThreads);
+                const int c3 = __builtin_nontemporal_load(ci + k + 3 * kThreads);
+                const double v0 = __builtin_nontemporal_load(val + k);
+                const double v1 = __builtin_nontemporal_load(val + k + kThreads);
+                const double v2 = __builtin_nontemporal_load(val + k + 2 * kThreads);
+                const double v3 = __builtin_nontemporal_load(val + k + 3 * kThreads);
+                const double x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
+                a0 = fma(v0, x0, a0);
+                a1 = fma(v1, x1, a1);
+                a2 = fma(v2, x2, a2);
+                a3 = fma(v3, x3, a3);
+            }
+            for (; k < k1; k += kThreads)
+                a0 = fma(__builtin_nontemporal_load(val + k), x[__builtin_nontemporal_load(ci + k)], a0);
+            double acc = block_sum_all((a0 + a1) + (a2 + a3));
+            if (CARRY_IN) acc = cin + acc;
+            if (t == 0) RI::finish(epi, r0, acc, pre, sums, maxs);
+        } else {
+            // wave mode: wavefront w owns row r0 + w
+            const int lane = t & (kWave - 1), w = t / kWave;
+            const int r = r0 + w;
+            if (r < r1) {
+                Pre pre{};
+                double cin = 0.0;
+                if (lane == 0) {
+                    pre = RI::load(epi, r);
+                    if (CARRY_IN) cin = carry[r];
+                }
+                const int rs = rp[r], re = rp[r + 1];
+                double acc = wave_row_sum(ci, val, x, rs, re, lane);
+                if (CARRY_IN) acc = cin + acc;
+                if (lane == 0) RI::finish(epi, r, acc, pre, sums, maxs);
+            }
         }
-        for (; k < k1; k += kThreads) a0 = fma(__builtin_nontemporal_load(val + k), x[__builtin_nontemporal_load(ci + k)], a0);
-        double acc = block_sum_all((a0 + a1) + (a2 + a3));
-        if (CARRY_IN) acc = carry[r0] + acc;
-        if (t == 0) epi.row(r0, acc, sums, maxs);
     } else {
-        const int cnt = k1 - k0;
+        const int r = r0 + t;
+        Pre pre{};
+        double acc = 0.0;
+        int rs = 0, re = 0;
+        if (r < r1) {
+            pre = RI::load(epi, r);
+            if (CARRY_IN) acc = carry[r];
+            rs = rp[r] - k0;
+            re = rp[r + 1] - k0;
+        }
         constexpr int kPer = kStreamCap / kThreads;
         if (cnt > 0) {
             // Branch-free load phase: indices are clamped into the block so all
@@ -170,12 +277,9 @@ __global__ __launch_bounds__(kThreads) void spmv_rows_kernel(const int *__restri
             }
         }
         __syncthreads();
-        const int r = r0 + t;
         if (r < r1) {
-            double acc = CARRY_IN ? carry[r] : 0.0;
-            const int e = rp[r + 1] - k0;
-            for (int k = rp[r] - k0; k < e; ++k) acc += prod[k];
-            epi.row(r, acc, sums, maxs);
+            for (int k = rs; k < re; ++k) acc += prod[k];
+            RI::finish(epi, r, acc, pre, sums, maxs);
         }
     }
     if constexpr (NS + NM > 0) block_reduce_store<NS, NM>(sums, maxs, epi.partials + (size_t)b * (NS + NM));

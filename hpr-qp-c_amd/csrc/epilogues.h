@@ -41,18 +41,35 @@ struct EpiZxw1 : NoRed {
     double *Qw, *x, *x_bar, *x_hat, *w_bar, *dx, *z_bar;
     const double *w, *ATy, *c, *l, *u, *last_x, *fact1, *fact2;
     double sigma, f1s, f2s, h1, h2;
-    __device__ void row(int i, double qw, double *, double *) const
+    // row inputs, loaded before the product is known
+    struct Pre {
+        double xi, aty, ci, li, ui, lx, wi, f1, f2;
+    };
+    __device__ Pre load(int i) const
+    {
+        Pre p;
+        p.xi = x[i];
+        p.aty = ATy[i];
+        p.ci = c[i];
+        p.li = l[i];
+        p.ui = u[i];
+        p.lx = last_x[i];
+        p.wi = w[i];
+        p.f1 = DIAG ? fact1[i] : f1s;
+        p.f2 = DIAG ? fact2[i] : f2s;
+        return p;
+    }
+    __device__ void row_pre(int i, double qw, const Pre &p, double *, double *) const
     {
         Qw[i] = qw;
-        const double xi = x[i];
-        const double tmp = -qw + ATy[i] - c[i];
-        const double zr = xi + sigma * tmp;
-        const double xb = project(zr, l[i], u[i]);
-        const double xh = 2.0 * xb - xi;
-        const double xn = h2 * xh + h1 * last_x[i];
-        const double wb = DIAG ? fact1[i] * w[i] + fact2[i] * xh : f1s * w[i] + f2s * xh;
+        const double tmp = -qw + p.aty - p.ci;
+        const double zr = p.xi + sigma * tmp;
+        const double xb = project(zr, p.li, p.ui);
+        const double xh = 2.0 * xb - p.xi;
+        const double xn = h2 * xh + h1 * p.lx;
+        const double wb = p.f1 * p.wi + p.f2 * xh;
         if (CHECK) {
-            dx[i] = xb - xi;
+            dx[i] = xb - p.xi;
             z_bar[i] = (xb - zr) / sigma;
         }
         x_bar[i] = xb;
@@ -60,6 +77,7 @@ struct EpiZxw1 : NoRed {
         x[i] = xn;
         w_bar[i] = wb;
     }
+    __device__ void row(int i, double qw, double *s, double *m) const { row_pre(i, qw, load(i), s, m); }
 };
 
 // iteration.c:141-143
@@ -67,7 +85,15 @@ struct EpiTempv : NoRed {
     double *tempv;
     const double *x_hat, *Qw;
     double sigma;
-    __device__ void row(int i, double qwb, double *, double *) const { tempv[i] = x_hat[i] + sigma * (Qw[i] - qwb); }
+    struct Pre {
+        double xh, qw;
+    };
+    __device__ Pre load(int i) const { return Pre{x_hat[i], Qw[i]}; }
+    __device__ void row_pre(int i, double qwb, const Pre &p, double *, double *) const
+    {
+        tempv[i] = p.xh + sigma * (p.qw - qwb);
+    }
+    __device__ void row(int i, double qwb, double *s, double *m) const { row_pre(i, qwb, load(i), s, m); }
 };
 
 // iteration.c:150-171
@@ -76,21 +102,25 @@ struct EpiY : NoRed {
     double *y, *y_bar, *s, *dy;
     const double *AL, *AU, *last_y;
     double f1, f2, h1, h2;
-    __device__ void row(int i, double ax, double *, double *) const
+    struct Pre {
+        double yi, al, au, ly;
+    };
+    __device__ Pre load(int i) const { return Pre{y[i], AL[i], AU[i], last_y[i]}; }
+    __device__ void row_pre(int i, double ax, const Pre &p, double *, double *) const
     {
-        const double yi = y[i];
-        const double sr = ax - f1 * yi;
-        const double sp = project(sr, AL[i], AU[i]);
+        const double sr = ax - f1 * p.yi;
+        const double sp = project(sr, p.al, p.au);
         const double yb = f2 * (sp - sr);
-        const double yh = 2.0 * yb - yi;
-        const double yn = h2 * yh + h1 * last_y[i];
+        const double yh = 2.0 * yb - p.yi;
+        const double yn = h2 * yh + h1 * p.ly;
         if (CHECK) {
             s[i] = sp;
-            dy[i] = yb - yi;
+            dy[i] = yb - p.yi;
         }
         y_bar[i] = yb;
         y[i] = yn;
     }
+    __device__ void row(int i, double ax, double *sm, double *m) const { row_pre(i, ax, load(i), sm, m); }
 };
 
 // iteration.c:218-248
@@ -99,22 +129,28 @@ struct EpiW2 : NoRed {
     double *w_bar, *w, *ATy, *ATy_bar, *dw, *ATdy;
     const double *last_w, *last_ATy, *fact;
     double fs, h1, h2;
-    __device__ void row(int i, double atyb, double *, double *) const
+    struct Pre {
+        double f, aty, wi, wb, lw, laty;
+    };
+    __device__ Pre load(int i) const
     {
-        const double f = DIAG ? fact[i] : fs;
-        const double aty = ATy[i], wi = w[i];
-        const double wbn = w_bar[i] + f * (atyb - aty);
-        const double wn = h1 * last_w[i] + h2 * (2.0 * wbn - wi);
-        const double atyn = h1 * last_ATy[i] + h2 * (2.0 * atyb - aty);
+        return Pre{DIAG ? fact[i] : fs, ATy[i], w[i], w_bar[i], last_w[i], last_ATy[i]};
+    }
+    __device__ void row_pre(int i, double atyb, const Pre &p, double *, double *) const
+    {
+        const double wbn = p.wb + p.f * (atyb - p.aty);
+        const double wn = h1 * p.lw + h2 * (2.0 * wbn - p.wi);
+        const double atyn = h1 * p.laty + h2 * (2.0 * atyb - p.aty);
         if (CHECK) {
-            dw[i] = wbn - wi;
-            ATdy[i] = atyb - aty;
+            dw[i] = wbn - p.wi;
+            ATdy[i] = atyb - p.aty;
             ATy_bar[i] = atyb;
         }
         w_bar[i] = wbn;
         w[i] = wn;
         ATy[i] = atyn;
     }
+    __device__ void row(int i, double atyb, double *s, double *m) const { row_pre(i, atyb, load(i), s, m); }
 };
 
 // iteration.c:19-66 (noC: A^T y first, Halpern on w as well)

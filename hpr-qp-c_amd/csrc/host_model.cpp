@@ -108,41 +108,57 @@ HPRQPStatus symmetrize_q(HPRQPMatrixCSR *out, const HPRQPMatrixCSR *q)
 {
     const double tol = 1e-15;
     const size_t n = q->nrow;
-    std::vector<size_t> count(n + 1, 0);
-    for (size_t r = 0; r < n; ++r)
-        for (size_t k = q->row_ptr[r]; k < q->row_ptr[r + 1]; ++k) {
-            count[r + 1] += 1;
-            count[q->col_ind[k] + 1] += 1;
-        }
-    for (size_t r = 0; r < n; ++r) count[r + 1] += count[r];
-    std::vector<Triple> bucket(count[n]);
-    std::vector<size_t> fill(count.begin(), count.end() - 1);
-    for (size_t r = 0; r < n; ++r)
-        for (size_t k = q->row_ptr[r]; k < q->row_ptr[r + 1]; ++k) {
-            double half = 0.5 * q->values[k];
-            size_t c = q->col_ind[k];
-            bucket[fill[r]++] = Triple{r, c, half};
-            bucket[fill[c]++] = Triple{c, r, half};
-        }
-    // Generation order above is the reference's entry order restricted to one
-    // row, so a stable sort by column inside each bucket reproduces the global
-    // stable (row, col) order.
+    const size_t nnz = q->row_ptr[n];
+    // Transposed contributions grouped by target row, each group in
+    // generation order (source row ascending, then entry order).
+    std::vector<size_t> tptr(n + 1, 0);
+    for (size_t k = 0; k < nnz; ++k) tptr[q->col_ind[k] + 1] += 1;
+    for (size_t r = 0; r < n; ++r) tptr[r + 1] += tptr[r];
+    std::vector<size_t> tsrc(nnz);  // source row of each transposed contribution
+    std::vector<size_t> tent(nnz);  // its entry index in q
+    {
+        std::vector<size_t> fill(tptr.begin(), tptr.end() - 1);
+        for (size_t r = 0; r < n; ++r)
+            for (size_t k = q->row_ptr[r]; k < q->row_ptr[r + 1]; ++k) {
+                const size_t pos = fill[q->col_ind[k]]++;
+                tsrc[pos] = r;
+                tent[pos] = k;
+            }
+    }
+    // Row r's contributions in generation order: transposed ones from source
+    // rows < r; then row r's own entries, each followed by its own transposed
+    // copy when it is diagonal; then transposed ones from source rows > r.
+    // A stable sort by column of that sequence is the reference's stable
+    // (row, col) order restricted to row r.
     std::vector<size_t> ptr(n + 1, 0);
     std::vector<size_t> ind;
     std::vector<double> val;
-    ind.reserve(count[n]);
-    val.reserve(count[n]);
+    ind.reserve(2 * nnz);
+    val.reserve(2 * nnz);
+    std::vector<Triple> row;
     for (size_t r = 0; r < n; ++r) {
-        auto first = bucket.begin() + count[r], last = bucket.begin() + count[r + 1];
-        std::stable_sort(first, last, [](const Triple &a, const Triple &b) { return a.col < b.col; });
+        row.clear();
+        size_t t = tptr[r];
+        const size_t te = tptr[r + 1];
+        for (; t < te && tsrc[t] < r; ++t) row.push_back(Triple{r, tsrc[t], 0.5 * q->values[tent[t]]});
+        for (size_t k = q->row_ptr[r]; k < q->row_ptr[r + 1]; ++k) {
+            row.push_back(Triple{r, q->col_ind[k], 0.5 * q->values[k]});
+            if (q->col_ind[k] == r) {
+                // the transposed copy of this diagonal entry comes right after it
+                row.push_back(Triple{r, r, 0.5 * q->values[tent[t]]});
+                ++t;
+            }
+        }
+        for (; t < te; ++t) row.push_back(Triple{r, tsrc[t], 0.5 * q->values[tent[t]]});
+        std::stable_sort(row.begin(), row.end(), [](const Triple &x, const Triple &y) { return x.col < y.col; });
         size_t row_begin = ind.size();
-        for (auto it = first; it != last; ++it) {
-            if (std::fabs(it->val) <= tol) continue;
-            if (ind.size() > row_begin && ind.back() == it->col) {
-                val.back() += it->val;
+        for (const Triple &e : row) {
+            if (std::fabs(e.val) <= tol) continue;
+            if (ind.size() > row_begin && ind.back() == e.col) {
+                val.back() += e.val;
             } else {
-                ind.push_back(it->col);
-                val.push_back(it->val);
+                ind.push_back(e.col);
+                val.push_back(e.val);
             }
         }
         size_t keep = row_begin;

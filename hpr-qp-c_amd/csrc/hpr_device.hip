@@ -67,37 +67,48 @@ struct DevCsr {
     int *rp = nullptr;
     int *ci = nullptr;
     double *val = nullptr;
-    int *blocks = nullptr;  // row-block starts, nblocks + 1 entries
+    int2 *blocks = nullptr;  // {first row, first entry} per row block, nblocks + 1 entries
     int nblocks = 0;
 };
 
 }  // namespace
 
-// Host planning of row blocks from the row pointer (see device_common.h).
-std::vector<int> plan_row_blocks(const std::vector<int> &rp)
+// Host planning of row blocks from the row pointer (see spmv_rows_kernel):
+// runs of rows of <= kLongRow entries become stream blocks (<= kThreads rows,
+// <= kStreamCap entries), runs of longer rows become wave blocks of up to
+// kThreads / kWave rows, and a row longer than kVecRow a block of its own.
+// Entry b is {first row, first entry} of block b; the last entry closes it.
+std::vector<int2> plan_row_blocks(const std::vector<int> &rp)
 {
     const int nrow = (int)rp.size() - 1;
-    std::vector<int> starts;
-    starts.reserve(nrow / 16 + 2);
+    std::vector<int2> blk;
+    blk.reserve(nrow / 16 + 2);
     int r = 0;
     while (r < nrow) {
-        int len = rp[r + 1] - rp[r];
-        starts.push_back(r);
-        if (len > kLongRow) {
+        const int len = rp[r + 1] - rp[r];
+        blk.push_back(make_int2(r, rp[r]));
+        const int first = r;
+        if (len > kVecRow) {
             ++r;
-            continue;
-        }
-        int nnz = 0, first = r;
-        while (r < nrow && r - first < kThreads) {
-            int l2 = rp[r + 1] - rp[r];
-            if (l2 > kLongRow || nnz + l2 > kStreamCap) break;
-            nnz += l2;
-            ++r;
+        } else if (len > kLongRow) {
+            while (r < nrow && r - first < kThreads / kWave) {
+                const int l2 = rp[r + 1] - rp[r];
+                if (l2 <= kLongRow || l2 > kVecRow) break;
+                ++r;
+            }
+        } else {
+            int nnz = 0;
+            while (r < nrow && r - first < kThreads) {
+                const int l2 = rp[r + 1] - rp[r];
+                if (l2 > kLongRow || nnz + l2 > kStreamCap) break;
+                nnz += l2;
+                ++r;
+            }
         }
     }
-    starts.push_back(nrow);
-    if (nrow == 0) starts = {0, 0};
-    return starts;
+    blk.push_back(make_int2(nrow, nrow >= 0 ? rp[nrow] : 0));
+    if (nrow == 0) blk = {make_int2(0, 0), make_int2(0, 0)};
+    return blk;
 }
 
 namespace {
@@ -107,11 +118,11 @@ HPRQPStatus upload_blocks(DevCsr &m, hipStream_t s)
     std::vector<int> rp(m.nrow + 1);
     HPRQP_HIP(hipMemcpyAsync(rp.data(), m.rp, sizeof(int) * (m.nrow + 1), hipMemcpyDeviceToHost, s));
     HPRQP_HIP(hipStreamSynchronize(s));
-    std::vector<int> blk = plan_row_blocks(rp);
+    std::vector<int2> blk = plan_row_blocks(rp);
     m.nblocks = (int)blk.size() - 1;
     if (m.blocks) HPRQP_HIP(hipFree(m.blocks));
-    HPRQP_HIP(hipMalloc(&m.blocks, sizeof(int) * blk.size()));
-    HPRQP_HIP(hipMemcpyAsync(m.blocks, blk.data(), sizeof(int) * blk.size(), hipMemcpyHostToDevice, s));
+    HPRQP_HIP(hipMalloc(&m.blocks, sizeof(int2) * blk.size()));
+    HPRQP_HIP(hipMemcpyAsync(m.blocks, blk.data(), sizeof(int2) * blk.size(), hipMemcpyHostToDevice, s));
     HPRQP_HIP(hipStreamSynchronize(s));
     return HPRQP_STATUS_OK;
 }

@@ -181,10 +181,10 @@ def _csr_of(matrix, nrow_hint=None):
     d = np.asarray(matrix, dtype=np.float64)
     if d.ndim == 1 and d.size == 0:
         d = d.reshape(0, nrow_hint or 0)
-    rows, cols = np.nonzero(d)
+    mask = d != 0
     rp = np.zeros(d.shape[0] + 1, np.uint64)
-    np.add.at(rp, rows + 1, 1)
-    return d.shape[0], d.shape[1], np.cumsum(rp).astype(np.uint64), cols.astype(np.uint64), d[rows, cols]
+    np.cumsum(np.count_nonzero(mask, axis=1), out=rp[1:])
+    return d.shape[0], d.shape[1], rp, np.nonzero(mask)[1].astype(np.uint64), d[mask]
 
 
 class Model:

@@ -15,6 +15,10 @@ K [u; v] = [dL/dx; 0]:
 """
 from __future__ import annotations
 
+import os
+import time
+from concurrent.futures import ThreadPoolExecutor
+
 import numpy as np
 import torch
 
@@ -25,18 +29,35 @@ def _np(t):
     return t.detach().to("cpu", torch.float64).numpy()
 
 
+# Wall-clock split of the last forward / backward (seconds), for benchmarks.
+last_timings: dict = {}
+
+
+def _build_all(Qn, cn, An, ALn, AUn, ln, un):
+    # build_from_QAbc is a ctypes call, which releases the GIL: build the
+    # per-instance models (host symmetrisation and row checks) on threads.
+    B = cn.shape[0]
+    workers = max(1, min(16, os.cpu_count() or 1, B))
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        return list(ex.map(lambda b: build_from_qabc(Qn[b], cn[b], An[b], ALn[b], AUn[b], ln[b], un[b]), range(B)))
+
+
 class QPLayer(torch.autograd.Function):
     @staticmethod
     def forward(ctx, Q, c, A, AL, AU, l, u, stoptol: float = 1e-8, bodies_per_launch: int = 200):
+        t0 = time.perf_counter()
         Qn, cn, An, ALn, AUn, ln, un = (_np(t) for t in (Q, c, A, AL, AU, l, u))
         B, n = cn.shape
         m = An.shape[1]
-        models = [build_from_qabc(Qn[b], cn[b], An[b], ALn[b], AUn[b], ln[b], un[b]) for b in range(B)]
+        models = _build_all(Qn, cn, An, ALn, AUn, ln, un)
         for md in models:
             if md.num_constraints() != m:
                 raise ValueError("QPLayer: rows with no entries or free bounds are not supported")
+        t1 = time.perf_counter()
         batch = Batch(models, Parameters(verbose=False, stoptol=stoptol))
+        t2 = time.perf_counter()
         batch.solve(bodies_per_launch)
+        t3 = time.perf_counter()
         res = [batch.result(b) for b in range(B)]
         bad = [r.status for r in res if r.status != "OPTIMAL"]
         if bad:
@@ -46,6 +67,9 @@ class QPLayer(torch.autograd.Function):
         z = np.stack([r.z for r in res])
         ctx.batch = batch
         ctx.sol = (x, y, z, ALn, AUn, ln, un)
+        last_timings.clear()
+        last_timings.update(build=t1 - t0, upload=t2 - t1, solve=t3 - t2, device_solve=batch.seconds,
+                            collect=time.perf_counter() - t3, iterations=max(r.iter for r in res))
         ctx.device = Q.device
         return torch.as_tensor(x, dtype=torch.float64, device=Q.device)
 
@@ -55,7 +79,9 @@ class QPLayer(torch.autograd.Function):
         B, n = x.shape
         m = y.shape[1]
         g = _np(gx)
+        t0 = time.perf_counter()
         adj = batch_adjoint(ctx.batch, list(g))
+        last_timings["adjoint"] = time.perf_counter() - t0
         U = np.stack([a[0] for a in adj])
         VR = np.stack([a[1] for a in adj]) if m else np.zeros((B, 0))
         VB = np.stack([a[2] for a in adj])

@@ -646,3 +646,65 @@ def kkt_active_solution(Q, c, A, AL, AU, l, u, y, z, tol=0.0):
     K[n:, :n] = G
     sol = np.linalg.solve(K, np.concatenate([-c, h]))
     return sol[:n], K, rows, vars_
+
+
+# ------------------------------------------------------------ device buffers
+# Plain HIP runtime calls (libamdhip64) for the kernel-level C-ABI entry
+# points, which take device pointers.
+_hip = None
+
+
+def hip_runtime():
+    global _hip
+    if _hip is None:
+        import glob
+        cands = sorted(glob.glob("/opt/rocm/lib/libamdhip64.so*")) or ["libamdhip64.so"]
+        L = C.CDLL(cands[0])
+        L.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        L.hipFree.argtypes = [C.c_void_p]
+        L.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        L.hipDeviceSynchronize.argtypes = []
+        _hip = L
+    return _hip
+
+
+class DeviceArray:
+    """A device copy of a numpy array (freed on close)."""
+
+    H2D, D2H = 1, 2
+
+    def __init__(self, a=None, *, shape=None, dtype=None):
+        self.host = np.ascontiguousarray(a) if a is not None else np.zeros(shape, dtype=dtype)
+        self.ptr = C.c_void_p()
+        nbytes = max(self.host.nbytes, 8)
+        if hip_runtime().hipMalloc(C.byref(self.ptr), nbytes) != 0:
+            raise RuntimeError("hipMalloc failed")
+        if a is not None and self.host.nbytes:
+            if hip_runtime().hipMemcpy(self.ptr, self.host.ctypes.data, self.host.nbytes, self.H2D) != 0:
+                raise RuntimeError("hipMemcpy H2D failed")
+
+    def get(self):
+        out = np.empty_like(self.host)
+        if out.nbytes and hip_runtime().hipMemcpy(out.ctypes.data, self.ptr, out.nbytes, self.D2H) != 0:
+            raise RuntimeError("hipMemcpy D2H failed")
+        return out
+
+    def close(self):
+        if self.ptr:
+            hip_runtime().hipFree(self.ptr)
+            self.ptr = C.c_void_p()
+
+
+def csr_with_row_lengths(lengths, ncol, seed=0):
+    """CSR with the given row lengths, distinct ascending random columns per
+    row, values N(0,1)."""
+    rng = np.random.default_rng(seed)
+    lengths = np.asarray(lengths, dtype=np.int64)
+    rp = np.zeros(len(lengths) + 1, dtype=np.int64)
+    rp[1:] = np.cumsum(lengths)
+    ci = np.empty(rp[-1], dtype=np.int64)
+    for r, L in enumerate(lengths):
+        if L:
+            ci[rp[r]:rp[r + 1]] = np.sort(rng.choice(ncol, size=int(L), replace=False))
+    val = rng.standard_normal(rp[-1])
+    return rp, ci, val

@@ -99,3 +99,57 @@ def test_host_build_symmetrises_and_drops_rows():
     m = H.build_from_qabc(Qm, [0.0, 0.0], Am, [0.0, -inf, 1.0], [3.0, inf, 2.0], [-2.0, -2.0], [2.0, 2.0])
     (qrp, qci, qv), (arp, aci, av), _ = model_arrays(m)
     assert m.num_constraints() == 1 and list(qv) == [2.0, 3.0, 3.0, 4.0]
+
+
+def messy_q_qp(seed=3, n=60, m=20):
+    """A QP whose Q is unsymmetric, with unsorted columns, repeated
+    (row, col) entries (diagonal ones included) and entries at and below the
+    1e-15 drop tolerance, so the symmetrisation order and drop rules matter."""
+    rng = np.random.default_rng(seed)
+    ptr, ind, val = [0], [], []
+    for r in range(n):
+        cols = list(rng.choice(n, size=rng.integers(0, 8), replace=True))
+        if rng.random() < 0.6:
+            cols += [r] * int(rng.integers(1, 3))
+        rng.shuffle(cols)
+        for cidx in cols:
+            ind.append(int(cidx))
+            v = rng.standard_normal() * 10.0 ** rng.integers(-3, 3)
+            if rng.random() < 0.1:
+                v = rng.choice([1e-15, -2e-15, 1.5e-15, 3e-16])
+            val.append(float(v))
+        ptr.append(len(ind))
+    A = rng.standard_normal((m, n)) * (rng.random((m, n)) < 0.2)
+    A[0] = 0.0  # an empty row, dropped by preprocessing
+    base = Q.qp_from_dense(np.eye(n), A, rng.standard_normal(n), -np.ones(m), np.ones(m), -np.ones(n), np.ones(n))
+    base.q_ptr = np.array(ptr, np.uint64)
+    base.q_ind = np.array(ind, np.uint64)
+    base.q_val = np.array(val, np.float64)
+    return base
+
+
+@pytest.mark.parametrize("seed", [3, 4, 5])
+def test_host_build_matches_oracle_on_messy_q(seed):
+    import hprqp_amd as H
+    qp = messy_q_qp(seed)
+    model = H.build_from_qabc((qp.n, qp.n, qp.q_ptr, qp.q_ind, qp.q_val), qp.c,
+                              (qp.m, qp.n, qp.a_ptr, qp.a_ind, qp.a_val), qp.AL, qp.AU, qp.l, qp.u)
+    got = model_arrays(model)
+    o = Q.OracleSession(qp, use_scaling=0, max_iter=0)
+    for g3, name in zip(got, ("Q", "A", "AT")):
+        for g, w_ in zip(g3, o.matrix(name)):
+            assert np.array_equal(g, w_), name
+
+
+@pytest.mark.skipif(not Q.ref_available(), reason="reference not built here (oracle/_ref)")
+@pytest.mark.parametrize("seed", [3, 4, 5])
+def test_oracle_symmetrisation_equals_reference_on_messy_q(seed):
+    # The reference's own preprocessing of the same messy Q, seen through its
+    # solve: identical iterates after 40 iterations need identical matrices.
+    qp = messy_q_qp(seed)
+    r = Q.binding_solve(Q.ref_lib(), qp, use_gpu=0, max_iter=40)
+    o = Q.OracleSession(qp, max_iter=40)
+    res, x, y, z, w = o.solve()
+    assert res.primal_obj == r["objective"]
+    for v, arr in zip("xyzw", (x, y, z, w)):
+        assert np.array_equal(arr, r[v]), v
