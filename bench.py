@@ -45,6 +45,30 @@ WORKLOADS = {
 }
 
 
+def banded_variant(qp):
+    """Same row lengths as `qp` (and the same Q row density), but each row's
+    columns form one contiguous run near the diagonal: the SpMV kernels then
+    gather almost nothing from L2/MALL. Profiling only (--profile-only), to
+    separate the cost of random gathers from the cost of streaming."""
+    n, m = qp.n, qp.m
+    lens = np.diff(qp.a_ptr.astype(np.int64))
+    starts = np.repeat((np.arange(m) * (n // max(m, 1))).astype(np.int64), lens)
+    within = np.arange(len(qp.a_ind), dtype=np.int64) - np.repeat(qp.a_ptr[:-1].astype(np.int64), lens)
+    cols = (starts + within) % n
+    order = np.lexsort((cols, np.repeat(np.arange(m), lens)))
+    qp.a_ind = cols[order].astype(np.uint64)
+    qp.a_val = qp.a_val[order]
+    half = 6
+    rows = np.repeat(np.arange(n), 2 * half + 1)
+    cols = rows + np.tile(np.arange(-half, half + 1), n)
+    ok = (cols >= 0) & (cols < n)
+    rows, cols = rows[ok], cols[ok]
+    qp.q_ptr = np.concatenate([[0], np.cumsum(np.bincount(rows, minlength=n))]).astype(np.uint64)
+    qp.q_ind = cols.astype(np.uint64)
+    qp.q_val = np.where(rows == cols, 20.0, 1.0 / (1.0 + np.abs(rows - cols)))
+    return qp
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -107,7 +131,7 @@ def main():
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="plain_n1M",
-                    choices=sorted(WORKLOADS) + ["soc_n220k", "batched_4096", "layer_1024"])
+                    choices=sorted(WORKLOADS) + ["plain_n1M_banded", "soc_n220k", "batched_4096", "layer_1024"])
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--seq-sample", type=int, default=32)
     ap.add_argument("--stoptol", type=float, default=1e-8)
@@ -136,9 +160,14 @@ def main():
         qp = Q.soc_qp(**SOC_WORKLOAD)
         log(f"generated soc_n220k: n={qp.n} m={qp.m} nnzA={len(qp.a_val)} cones={qp.num_var_cones}")
         return bench_single(H, Q, args, qp, "soc_n220k", soc=True)
-    kw = WORKLOADS[args.workload]
+    banded = args.workload.endswith("_banded")
+    if banded and not args.profile_only:
+        raise SystemExit("the banded variant is a profiling aid: use it with --profile-only")
+    kw = WORKLOADS[args.workload.replace("_banded", "")]
     t = time.time()
     qp = Q.random_qp(**kw)
+    if banded:
+        qp = banded_variant(qp)
     log(f"generated {args.workload}: n={qp.n} m={qp.m} nnzA={len(qp.a_val)} nnzQ={len(qp.q_val)} "
         f"in {time.time() - t:.1f}s")
     return bench_single(H, Q, args, qp, args.workload)
@@ -156,6 +185,45 @@ def calibration_copy(H):
         f"({2 * 268435456 / (ms.value * 1e-3) / 1e9:.0f} GB/s), status {st}")
 
 
+def calibration_gather(H, nrow=1_000_000, windows=(1_000_000, 500_000, 262_144, 131_072, 4_096)):
+    """Random-gather ceiling: an ELL product with 20 coalesced entries per row
+    and columns uniform over a window of the gathered vector (the library's
+    reference gather kernel), for window sizes from the full 8 MB x down to
+    an L1-resident slice."""
+    lib = H.lib()
+    lib.hprqp_hip_calibration_gather.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)]
+    out = []
+    for w in windows:
+        ms = C.c_double(0.0)
+        st = lib.hprqp_hip_calibration_gather(nrow, w, 5, C.byref(ms))
+        gathers = 20 * nrow
+        byts = 12 * gathers + 8 * nrow
+        log(f"gather ceiling: window {w} doubles ({w * 8 / 2**20:.2f} MiB): {ms.value * 1e3:.1f} us for {gathers} "
+            f"gathers -> {gathers / (ms.value * 1e-3) / 1e9:.1f} G gathers/s, {byts / (ms.value * 1e-3) / 1e9:.0f} GB/s "
+            f"streamed, status {st}")
+        out.append(dict(window=w, us=ms.value * 1e3, ggather_s=gathers / (ms.value * 1e-3) / 1e9))
+    return out
+
+
+def profiled_traffic(workload, kernel):
+    """HBM-side bytes per launch of `kernel` from the committed rocprofv3
+    counter summary of this workload (FETCH_SIZE and WRITE_SIZE runs, each
+    scaled by what the counter reads for the library's plain copy kernel), or
+    None when no summary covers it."""
+    path = ROOT / "profiles" / f"spmv_iteration_kernels_{workload}.md"
+    if not path.exists():
+        return None, None
+    text = path.read_text()
+    start = text.rfind("```json")
+    if start < 0:
+        return None, None
+    data = json.loads(text[start + 7:text.index("```", start + 7)])
+    k = data.get("kernels", {}).get(kernel)
+    if not k:
+        return None, None
+    return k["traffic_bytes"], f"profiles/{path.name} (P = {data.get('panels')})"
+
+
 def bench_single(H, Q, args, qp, workload, soc=False):
     t = time.time()
     if soc:
@@ -171,6 +239,7 @@ def bench_single(H, Q, args, qp, workload, soc=False):
             log(f"kernel {p['kernel']}: {p['mean_ms'] * 1e3:.1f} us/launch, {p['bytes'] / 1e6:.1f} MB, "
                 f"{p['gbs']:.0f} GB/s")
         calibration_copy(H)
+        calibration_gather(H)
         return
 
     for w in range(args.warmup):
@@ -196,10 +265,26 @@ def bench_single(H, Q, args, qp, workload, soc=False):
     iter_bytes = sum(p["bytes"] for p in prof)
     roofline = None
     if dom:
+        traffic, source = profiled_traffic(workload, dom["kernel"])
         roofline = {"bound": "hbm", "kernel": dom["kernel"], "achieved": round(dom["gbs"], 1),
                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(dom["gbs"] / HBM_PEAK_GBS, 4),
-                    "traffic": None, "iteration_gbs": round(iter_bytes / (iter_ms * 1e-3) / 1e9, 1),
+                    "traffic": traffic, "traffic_source": source,
+                    "iteration_gbs": round(iter_bytes / (iter_ms * 1e-3) / 1e9, 1),
                     "iteration_us": round(iter_ms * 1e3, 1)}
+        # The SpMV kernels are bound by random gathers, not by streaming: set
+        # the dominant kernel against the gather-only reference kernel at the
+        # same gather count and window (the gathered vector / column panels).
+        s = H.Session(model, H.Parameters(verbose=False))
+        mat = {"zxw1_q": "Q", "tempv_q": "Q", "y_a": "A", "w2_at": "AT"}.get(dom["kernel"])
+        if mat:
+            panels = max(int(s.scalar("panels_" + mat)), 1)
+            nnz = {"Q": model.c.Q.nnz, "A": model.c.A.nnz, "AT": model.c.A.nnz}[mat]
+            window = (qp.m if mat == "AT" else qp.n) // panels
+            g = calibration_gather(H, nrow=max(int(nnz) // 20, 1), windows=(window,))[0]
+            roofline["gathers"] = int(nnz)
+            roofline["gather_ceiling_us"] = round(g["us"], 1)
+            roofline["gather_frac"] = round(g["us"] / (dom["mean_ms"] * 1e3), 3)
+        del s
 
     cpu = None
     if not args.no_cpu:

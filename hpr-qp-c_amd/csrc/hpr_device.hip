@@ -1850,6 +1850,81 @@ HPRQPStatus hprqp_hip_calibration_copy(size_t bytes, double *milliseconds)
     return HPRQP_STATUS_OK;
 }
 
+// Random-gather reference kernel: an ELL product with `kGatherWidth` entries
+// per row stored column-major (so index and value loads are coalesced) and
+// columns spread uniformly over a window of the gathered vector. Its rate is
+// the ceiling a gather-bound SpMV can reach for that window size.
+constexpr int kGatherWidth = 20;
+
+__global__ void hprqp_calibration_index_kernel(size_t count, unsigned window, int *__restrict__ idx,
+                                               double *__restrict__ v)
+{
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < count; k += (size_t)gridDim.x * blockDim.x) {
+        unsigned h = (unsigned)k * 2654435761u;
+        h ^= h >> 15;
+        h *= 2246822519u;
+        h ^= h >> 13;
+        idx[k] = (int)(h % window);
+        v[k] = 1.0;
+    }
+}
+
+__global__ __launch_bounds__(256) void hprqp_calibration_gather_kernel(int nrow, const int *__restrict__ idx,
+                                                                        const double *__restrict__ v,
+                                                                        const double *__restrict__ x,
+                                                                        double *__restrict__ y)
+{
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= nrow) return;
+    int c[kGatherWidth];
+    double a[kGatherWidth];
+#pragma unroll
+    for (int j = 0; j < kGatherWidth; ++j) {
+        c[j] = __builtin_nontemporal_load(idx + (size_t)j * nrow + r);
+        a[j] = __builtin_nontemporal_load(v + (size_t)j * nrow + r);
+    }
+    double acc = 0.0;
+#pragma unroll
+    for (int j = 0; j < kGatherWidth; ++j) acc = fma(a[j], x[c[j]], acc);
+    y[r] = acc;
+}
+
+// Times one random-gather ELL product over `nrow` rows (20 entries each) whose
+// columns fall in [0, window) of a vector of `window` doubles; reports the
+// mean of `reps` launches in milliseconds.
+HPRQPStatus hprqp_hip_calibration_gather(int nrow, int window, int reps, double *milliseconds)
+{
+    if (nrow <= 0 || window <= 0 || reps <= 0) return HPRQP_STATUS_INVALID_ARGUMENT;
+    const size_t count = (size_t)nrow * kGatherWidth;
+    int *idx = nullptr;
+    double *v = nullptr, *x = nullptr, *y = nullptr;
+    HPRQP_HIP(hipMalloc((void **)&idx, count * sizeof(int)));
+    HPRQP_HIP(hipMalloc((void **)&v, count * sizeof(double)));
+    HPRQP_HIP(hipMalloc((void **)&x, (size_t)window * sizeof(double)));
+    HPRQP_HIP(hipMalloc((void **)&y, (size_t)nrow * sizeof(double)));
+    HPRQP_HIP(hipMemset(x, 0, (size_t)window * sizeof(double)));
+    hprqp_calibration_index_kernel<<<4096, 256>>>(count, (unsigned)window, idx, v);
+    const int grid = (nrow + 255) / 256;
+    hprqp_calibration_gather_kernel<<<grid, 256>>>(nrow, idx, v, x, y);  // warm-up
+    hipEvent_t e0, e1;
+    HPRQP_HIP(hipEventCreate(&e0));
+    HPRQP_HIP(hipEventCreate(&e1));
+    HPRQP_HIP(hipEventRecord(e0, nullptr));
+    for (int k = 0; k < reps; ++k) hprqp_calibration_gather_kernel<<<grid, 256>>>(nrow, idx, v, x, y);
+    HPRQP_HIP(hipEventRecord(e1, nullptr));
+    HPRQP_HIP(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    HPRQP_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (milliseconds) *milliseconds = ms / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    HPRQP_HIP(hipFree(idx));
+    HPRQP_HIP(hipFree(v));
+    HPRQP_HIP(hipFree(x));
+    HPRQP_HIP(hipFree(y));
+    return HPRQP_STATUS_OK;
+}
+
 HPRQPStatus hprqp_hip_csr_transpose(int nrow, int ncol, int nnz, const int *row_ptr, const int *col_ind,
                                     const double *values, int *t_row_ptr, int *t_col_ind, double *t_values,
                                     int *perm, void *stream)

@@ -94,6 +94,11 @@ HPRQPStatus hprqp_hip_batch_adjoint(HPRQPHipBatch *batch, const double *dl_dx, d
  * plain streaming kernel and reports its time: the known-size kernel used to
  * calibrate HBM traffic counters. No counterpart in the reference. */
 HPRQPStatus hprqp_hip_calibration_copy(size_t bytes, double *milliseconds);
+/* Random-gather reference: mean time of one ELL product with 20 entries per
+ * row over `nrow` rows, columns uniform over a `window`-element vector
+ * (index and value loads coalesced). No reference counterpart: the ceiling
+ * against which the gather-bound SpMV kernels are judged. */
+HPRQPStatus hprqp_hip_calibration_gather(int nrow, int window, int reps, double *milliseconds);
 
 /* Name of the device the library runs on, or "" when none is usable. */
 const char *hprqp_hip_device_name(void);
