@@ -369,13 +369,27 @@ def bench_layer(H, Q, args):
     bwd = time.time() - t
     split = dict(layer_mod.last_timings)
     log(f"layer: forward {fwd:.3f}s backward {bwd:.3f}s; split {split}")
+    # Dense column-form products: every instance streams Q once and A twice
+    # per iteration (free variables: the noC iteration), 8 bytes per entry;
+    # check iterations, setup and vector traffic come on top (not counted).
+    n, m = Qd.shape[1], A.shape[1]
+    iter_sum = split.get("iterations_sum", 0)
+    dev = split.get("device_solve", 0.0)
+    roofline = None
+    if iter_sum and dev > 0:
+        gbs = 8.0 * (n * n + 2 * m * n) * iter_sum / dev / 1e9
+        roofline = {"bound": "hbm", "kernel": "batch_solve_kernel (dense)", "achieved": round(gbs, 1),
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                    "bytes_per_instance_iteration": 8 * (n * n + 2 * m * n), "instance_iterations": iter_sum}
     line = {"metric": "QP layer fwd+bwd ms (1 GPU)", "value": round((fwd + bwd) * 1e3, 2), "unit": "ms",
             "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": round((fwd + bwd) * 1e3, 2),
             "higher_is_better": False, "scaling": "strong", "vs_baseline": None, "dtype": "f64",
             "data": "synthetic (seeded generator)",
             "config": {"workload": f"layer_{count}", "forward_ms": fwd * 1e3, "backward_ms": bwd * 1e3,
                        "backward_over_forward": bwd / fwd,
-                       "split_ms": {k: (round(v * 1e3, 2) if k != "iterations" else v) for k, v in split.items()}}}
+                       "split_ms": {k: (round(v * 1e3, 2) if not k.startswith("iterations") else v)
+                                    for k, v in split.items()}},
+            "roofline": roofline}
     print(json.dumps(line), flush=True)
 
 

@@ -12,9 +12,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 #include "device_common.h"
@@ -34,6 +36,8 @@ struct Inst {
     long long qt_rp;         // Q^T row_ptr offset (n+1); perm shares q_nz
     long long vn, vm;        // vector offsets
     int q_blk, q_nblk, a_blk, a_nblk, at_blk, at_nblk;  // row-block lists (offset, count)
+    int dense;               // 1: Q, A held as dense row-major arrays in Arrays::dval
+    long long dq, da, dat;   // dense offsets: Q (n x n), A^T (n x m), A (m x n)
     int noC;
     double norm_b_org, norm_c_org, obj_constant;
 };
@@ -77,6 +81,7 @@ struct Arrays {
     long long a_base;      // where A entries start in ci / val
     long long qt_perm_base;
     int *active;           // count of unfinished instances
+    double *dval;          // dense instances' matrices (see Inst::dense)
 };
 
 struct Params {
@@ -105,17 +110,71 @@ namespace batch {
 // other rows go through the LDS stream mode with sequential row sums.
 constexpr int kWaveRow = 64;
 
-template <class Epi>
-__device__ void wg_spmv(const int *__restrict__ rp, const int *__restrict__ ci, const double *__restrict__ val,
-                        const double *__restrict__ x, const int *__restrict__ blk, int nblk, const Epi &epi,
-                        double *sums, double *maxs, double *prod)
+// Matrix view of one instance. Sparse: CSR with row blocks. Dense (dv set):
+// a row-major K x J array S whose product is taken in column form,
+// y_j = sum_{i<K} S[i*ld + j] x_i, i ascending -- S = Q for Q (symmetric),
+// S = A^T for A x, S = A for A^T y -- so every output is one thread's
+// sequential sum, the order of the CSR row sum, with coalesced loads.
+struct Mat {
+    const int *rp, *ci;
+    double *val;
+    const int *blk;
+    int nblk;
+    double *dv;
+    int K, J, ld;
+};
+
+// Largest dense dimension: x is staged whole in the LDS product buffer.
+constexpr int kDenseMax = kStreamCap;
+
+// Column-form dot of a dense view: sum_{i<K} S[i*ld + j] xs[i], i ascending.
+__device__ __forceinline__ double dense_column_dot(const Mat &M, int j, const double *xs)
 {
+    constexpr int U = 8;
+    const double *__restrict__ col = M.dv + j;
+    double acc = 0.0;
+    int i = 0;
+    for (; i + U <= M.K; i += U) {
+        double v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = __builtin_nontemporal_load(col + (long long)(i + u) * M.ld);
+#pragma unroll
+        for (int u = 0; u < U; ++u) acc += v[u] * xs[i + u];
+    }
+    for (; i < M.K; ++i) acc += col[(long long)i * M.ld] * xs[i];
+    return acc;
+}
+
+// Dense views run as "blocks" of kThreads outputs through the same epilogue
+// call as the CSR stream mode, so the epilogue is inlined once per mode.
+template <bool DENSE, class Epi>
+__device__ void wg_spmv(const Mat &M, const double *__restrict__ x, const Epi &epi, double *sums, double *maxs,
+                        double *prod)
+{
+    constexpr bool dense = DENSE;
+    const int *__restrict__ rp = M.rp;
+    const int *__restrict__ ci = M.ci;
+    const double *__restrict__ val = M.val;
+    const int *__restrict__ blk = M.blk;
     const int t = threadIdx.x;
     const int lane = t & (kWave - 1), wid = t / kWave;
+    int nblk = M.nblk;
+    if constexpr (dense) {
+        // x staged whole in LDS (K <= kDenseMax)
+        for (int i = t; i < M.K; i += kThreads) prod[i] = x[i];
+        __syncthreads();
+        nblk = (M.J + kThreads - 1) / kThreads;
+    }
     for (int b = 0; b < nblk; ++b) {
-        const int r0 = blk[b], r1 = blk[b + 1];
-        const int k0 = rp[r0], k1 = rp[r1];
-        if (k1 - k0 >= kWaveRow * (r1 - r0)) {
+        int r0, r1, k0 = 0, k1 = 0;
+        if constexpr (dense) {
+            r0 = b * kThreads;
+            r1 = min(r0 + kThreads, M.J);
+        } else {
+            r0 = blk[b], r1 = blk[b + 1];
+            k0 = rp[r0], k1 = rp[r1];
+        }
+        if (!dense && k1 - k0 >= kWaveRow * (r1 - r0)) {
             for (int r = r0 + wid; r < r1; r += kThreads / kWave) {
                 const int e = rp[r + 1];
                 double a0 = 0.0, a1 = 0.0;
@@ -130,18 +189,25 @@ __device__ void wg_spmv(const int *__restrict__ rp, const int *__restrict__ ci, 
             }
         } else {
             const int cnt = k1 - k0;
-            for (int k = t; k < cnt; k += kThreads) prod[k] = val[k0 + k] * x[ci[k0 + k]];
-            __syncthreads();
+            if (!dense) {
+                for (int k = t; k < cnt; k += kThreads) prod[k] = val[k0 + k] * x[ci[k0 + k]];
+                __syncthreads();
+            }
             const int r = r0 + t;
             if (r < r1) {
                 double acc = 0.0;
-                const int e = rp[r + 1] - k0;
-                for (int k = rp[r] - k0; k < e; ++k) acc += prod[k];
+                if constexpr (dense) {
+                    acc = dense_column_dot(M, r, prod);
+                } else {
+                    const int e = rp[r + 1] - k0;
+                    for (int k = rp[r] - k0; k < e; ++k) acc += prod[k];
+                }
                 epi.row(r, acc, sums, maxs);
             }
         }
-        __syncthreads();
+        if (!dense) __syncthreads();
     }
+    __syncthreads();
 }
 
 template <class F>
@@ -163,18 +229,30 @@ __device__ void wg_reduce(double *sums, double *maxs, double *out)
     for (int i = 0; i < (NM > 0 ? NM : 1); ++i) maxs[i] = 0.0;
 }
 
-__device__ __forceinline__ double dclamp(double v, double lo, double hi) { return v < lo ? lo : (v > hi ? hi : v); }
-
-// Matrix view of one instance.
-struct Mat {
-    const int *rp, *ci;
-    double *val;
-    const int *blk;
-    int nblk;
+// DiagInfo on a dense row-major Q.
+struct DenseDiagInfo {
+    static constexpr int NS = 0, NM = 1;
+    double *partials;
+    double *diag;
+    const double *Qd;
+    int n;
+    __device__ void row(int r, double *, double *mx) const
+    {
+        double off = 0.0;
+        for (int c = 0; c < n; ++c)
+            if (c != r) off = fmax(off, fabs(Qd[(long long)r * n + c]));
+        diag[r] = 0.0 + Qd[(long long)r * n + r];
+        mx[0] = fmax(mx[0], off);
+    }
 };
 
-// One workgroup's view of its instance.
-struct WG {
+__device__ __forceinline__ double dclamp(double v, double lo, double hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+
+// One workgroup's view of its instance; D: the instance's matrices are dense.
+template <bool D>
+struct WGv {
+    static constexpr bool kDense = D;
     const Inst *I;
     State *S;  // in LDS
     Mat Q, A, AT;
@@ -192,8 +270,77 @@ struct WG {
 
 // ---------------------------------------------------------- setup phases
 
+// The same passes on a dense instance (Q: n x n, A: m x n, both row-major;
+// the A^T copy is made after scaling). Zero entries leave every max and sum
+// unchanged, so the factors equal the sparse path's.
+template <class WG>
+__device__ void wg_scale_dense(WG &g)
+{
+    const int n = g.I->n, m = g.I->m, t = threadIdx.x;
+    double *Qd = g.Q.dv, *Ad = g.AT.dv;  // AT's column-form array is A row-major
+    for (int pass = 0; pass < 11; ++pass) {
+        const bool sum = pass == 10;
+        for (int j = t; j < n; j += kThreads) {
+            double acc = 0.0;
+            for (int i = 0; i < m; ++i) {
+                const double v = fabs(Ad[(long long)i * n + j]);
+                if (sum) acc += v;
+                else if (v > acc) acc = v;
+            }
+            for (int i = 0; i < n; ++i) {
+                const double v = fabs(Qd[(long long)i * n + j]);
+                if (sum) acc += v;
+                else if (v > acc) acc = v;
+            }
+            acc = sqrt(acc);
+            g.N(N_tc)[j] = acc == 0.0 ? 1.0 : acc;
+        }
+        for (int i = t; i < m; i += kThreads) {
+            double acc = 0.0;
+            for (int j = 0; j < n; ++j) {
+                const double v = fabs(Ad[(long long)i * n + j]);
+                if (sum) acc += v;
+                else if (v > acc) acc = v;
+            }
+            acc = sqrt(acc);
+            g.M(M_tr)[i] = acc == 0.0 ? 1.0 : acc;
+        }
+        __syncthreads();
+        const double *tc = g.N(N_tc), *tr = g.M(M_tr);
+        for (long long e = t; e < (long long)n * n; e += kThreads) {
+            const int r = (int)(e / n), c = (int)(e % n);
+            double v = Qd[e];
+            v /= tc[r];
+            v /= tc[c];
+            Qd[e] = v;
+        }
+        for (long long e = t; e < (long long)m * n; e += kThreads) {
+            const int r = (int)(e / n), c = (int)(e % n);
+            double v = Ad[e];
+            v /= tr[r];
+            v /= tc[c];
+            Ad[e] = v;
+        }
+        for (int r = t; r < n; r += kThreads) {
+            const double d = tc[r];
+            g.N(N_col)[r] *= d;
+            g.N(N_c)[r] /= d;
+            g.N(N_l)[r] *= d;
+            g.N(N_u)[r] *= d;
+        }
+        for (int r = t; r < m; r += kThreads) {
+            const double d = tr[r];
+            g.M(M_row)[r] *= d;
+            if (!isinf(g.M(M_AL)[r])) g.M(M_AL)[r] /= d;
+            if (!isinf(g.M(M_AU)[r])) g.M(M_AU)[r] /= d;
+        }
+        __syncthreads();
+    }
+}
+
 // scale.c:231-318 on one instance (column j of A through A^T's permutation,
 // column j of Q through Q^T's, so sums run in row order like the host).
+template <class WG>
 __device__ void wg_scale(WG &g)
 {
     const int n = g.I->n, m = g.I->m, t = threadIdx.x;
@@ -201,6 +348,10 @@ __device__ void wg_scale(WG &g)
     for (int i = t; i < m; i += kThreads) g.M(M_row)[i] = 1.0;
     __syncthreads();
     if (!g.P.use_scaling) return;
+    if constexpr (WG::kDense) {
+        wg_scale_dense(g);
+        return;
+    }
     for (int pass = 0; pass < 11; ++pass) {
         const bool sum = pass == 10;
         for (int j = t; j < n; j += kThreads) {
@@ -260,6 +411,7 @@ __device__ void wg_scale(WG &g)
 }
 
 // eigs.c:19-153 on one instance (a_matrix: A A^T on m-vectors; else Q).
+template <class WG>
 __device__ double wg_power(WG &g, bool a_matrix, double *sums, double *maxs)
 {
     const int len = a_matrix ? g.I->m : g.I->n;
@@ -278,10 +430,10 @@ __device__ double wg_power(WG &g, bool a_matrix, double *sums, double *maxs)
         }
         wg_map(len, MapScaleDiv{{}, q, z, qn}, sums, maxs);
         if (a_matrix) {
-            wg_spmv(g.AT.rp, g.AT.ci, g.AT.val, q, g.AT.blk, g.AT.nblk, EpiStore{{}, tmp}, sums, maxs, g.prod);
-            wg_spmv(g.A.rp, g.A.ci, g.A.val, tmp, g.A.blk, g.A.nblk, EpiPowerZ{nullptr, z, q}, sums, maxs, g.prod);
+            wg_spmv<WG::kDense>(g.AT, q, EpiStore{{}, tmp}, sums, maxs, g.prod);
+            wg_spmv<WG::kDense>(g.A, tmp, EpiPowerZ{nullptr, z, q}, sums, maxs, g.prod);
         } else {
-            wg_spmv(g.Q.rp, g.Q.ci, g.Q.val, q, g.Q.blk, g.Q.nblk, EpiPowerZ{nullptr, z, q}, sums, maxs, g.prod);
+            wg_spmv<WG::kDense>(g.Q, q, EpiPowerZ{nullptr, z, q}, sums, maxs, g.prod);
         }
         wg_reduce<2, 0>(sums, maxs, g.red);
         lam = g.red[0];
@@ -298,11 +450,20 @@ __device__ double wg_power(WG &g, bool a_matrix, double *sums, double *maxs)
 }
 
 // workspace.c:191-248
+template <class WG>
 __device__ void wg_setup(WG &g, double *sums, double *maxs)
 {
     const int n = g.I->n, m = g.I->m, t = threadIdx.x;
     wg_scale(g);
-    for (long long k = t; k < (long long)g.AT.rp[n]; k += kThreads) g.AT.val[k] = g.a_val[g.at_perm[k]];
+    if constexpr (WG::kDense) {
+        // A x runs in column form over A^T: copy the scaled A transposed
+        for (long long e = t; e < (long long)m * n; e += kThreads) {
+            const int i = (int)(e / n), j = (int)(e % n);
+            g.A.dv[(long long)j * m + i] = g.AT.dv[e];
+        }
+    } else {
+        for (long long k = t; k < (long long)g.AT.rp[n]; k += kThreads) g.AT.val[k] = g.a_val[g.at_perm[k]];
+    }
     __syncthreads();
     // sigma0 = ||b||/||c|| on the scaled data
     wg_map(n, MapDot{nullptr, g.N(N_c), g.N(N_c)}, sums, maxs);
@@ -319,7 +480,10 @@ __device__ void wg_setup(WG &g, double *sums, double *maxs)
     const double nb = sqrt(g.red[1]);
     __syncthreads();
     double sigma = (nb > 1e-16 && nc > 1e-16 && nb < 1e16 && nc < 1e16) ? nb / nc : 1.0;
-    wg_map(n, DiagInfo{nullptr, g.N(N_diag), g.Q.rp, g.Q.ci, g.Q.val}, sums, maxs);
+    if constexpr (WG::kDense)
+        wg_map(n, DenseDiagInfo{nullptr, g.N(N_diag), g.Q.dv, n}, sums, maxs);
+    else
+        wg_map(n, DiagInfo{nullptr, g.N(N_diag), g.Q.rp, g.Q.ci, g.Q.val}, sums, maxs);
     wg_reduce<0, 1>(sums, maxs, g.red);
     const int diag = n == 0 || g.red[0] <= 1e-15;
     __syncthreads();
@@ -463,24 +627,22 @@ __device__ int d_check_next(const Params &p, const Restart &r, long long next)
 // ----------------------------------------------------------- operators
 
 // residuals.c:88-189 (CPU convention for the dual objective)
+template <class WG>
 __device__ void wg_residuals(WG &g, long long iter, double *sums, double *maxs)
 {
     const int n = g.I->n, m = g.I->m;
     State &S = *g.S;
-    wg_spmv(g.Q.rp, g.Q.ci, g.Q.val, g.N(N_x_bar), g.Q.blk, g.Q.nblk,
-            EpiResQ{nullptr, g.N(N_Qx), g.N(N_x_bar), g.N(N_c), g.N(N_z_bar)}, sums, maxs, g.prod);
+    wg_spmv<WG::kDense>(g.Q, g.N(N_x_bar), EpiResQ{nullptr, g.N(N_Qx), g.N(N_x_bar), g.N(N_c), g.N(N_z_bar)}, sums, maxs, g.prod);
     wg_reduce<3, 0>(sums, maxs, g.red);
     const double cx = g.red[0], xqx = g.red[1], zx = g.red[2];
     __syncthreads();
-    wg_spmv(g.AT.rp, g.AT.ci, g.AT.val, g.M(M_y_bar), g.AT.blk, g.AT.nblk,
-            EpiResAT{nullptr, g.N(N_ATdy), g.N(N_Qx), g.N(N_c), g.N(N_z_bar), g.N(N_col), 1.0}, sums, maxs, g.prod);
+    wg_spmv<WG::kDense>(g.AT, g.M(M_y_bar), EpiResAT{nullptr, g.N(N_ATdy), g.N(N_Qx), g.N(N_c), g.N(N_z_bar), g.N(N_col), 1.0}, sums, maxs, g.prod);
     wg_reduce<0, 3>(sums, maxs, g.red);
     const double rd = g.red[0], qxm = g.red[1], atym = g.red[2];
     __syncthreads();
     double ys = 0.0, rp = 0.0, axm = 0.0;
     if (m > 0) {
-        wg_spmv(g.A.rp, g.A.ci, g.A.val, g.N(N_x_bar), g.A.blk, g.A.nblk,
-                EpiResA{nullptr, g.M(M_pz), g.M(M_AL), g.M(M_AU), g.M(M_row), g.M(M_y_bar), g.M(M_s), 1.0}, sums,
+        wg_spmv<WG::kDense>(g.A, g.N(N_x_bar), EpiResA{nullptr, g.M(M_pz), g.M(M_AL), g.M(M_AU), g.M(M_row), g.M(M_y_bar), g.M(M_s), 1.0}, sums,
                 maxs, g.prod);
         wg_reduce<1, 2>(sums, maxs, g.red);
         ys = g.red[0];
@@ -509,12 +671,13 @@ __device__ void wg_residuals(WG &g, long long iter, double *sums, double *maxs)
 }
 
 // control.c:155-229
+template <class WG>
 __device__ double wg_m_norm(WG &g, double *sums, double *maxs)
 {
     const int m = g.I->m;
     const State &S = *g.S;
     const int diag = S.Q_is_diag;
-    wg_spmv(g.Q.rp, g.Q.ci, g.Q.val, g.N(N_dw), g.Q.blk, g.Q.nblk, EpiMnQdw{nullptr, g.N(N_dQw), g.N(N_dx), g.N(N_dw)},
+    wg_spmv<WG::kDense>(g.Q, g.N(N_dw), EpiMnQdw{nullptr, g.N(N_dQw), g.N(N_dx), g.N(N_dw)},
             sums, maxs, g.prod);
     wg_reduce<4, 0>(sums, maxs, g.red);
     const double dx_sq = g.red[0], dqw_dx = g.red[1], dw_dqw = g.red[2], dqw_sq = g.red[3];
@@ -523,13 +686,12 @@ __device__ double wg_m_norm(WG &g, double *sums, double *maxs)
     double m1 = 0.0, m3 = 0.0, m2 = (1.0 / sg) * dx_sq;
     m2 -= 2.0 * dqw_dx;
     if (m > 0) {
-        wg_spmv(g.AT.rp, g.AT.ci, g.AT.val, g.M(M_dy), g.AT.blk, g.AT.nblk, EpiMnATdy{nullptr, g.N(N_ATdy), g.N(N_dx)},
+        wg_spmv<WG::kDense>(g.AT, g.M(M_dy), EpiMnATdy{nullptr, g.N(N_ATdy), g.N(N_dx)},
                 sums, maxs, g.prod);
         wg_reduce<1, 0>(sums, maxs, g.red);
         const double atdy_dx = g.red[0];
         __syncthreads();
-        wg_spmv(g.Q.rp, g.Q.ci, g.Q.val, g.N(N_ATdy), g.Q.blk, g.Q.nblk,
-                EpiMnQATdy{nullptr, g.N(N_QATdy), g.N(N_dQw), g.N(N_ATdy), g.N(N_fact_M), diag}, sums, maxs, g.prod);
+        wg_spmv<WG::kDense>(g.Q, g.N(N_ATdy), EpiMnQATdy{nullptr, g.N(N_QATdy), g.N(N_dQw), g.N(N_ATdy), g.N(N_fact_M), diag}, sums, maxs, g.prod);
         wg_reduce<3, 0>(sums, maxs, g.red);
         const double dqw_atdy = g.red[0], atdy_qatdy = g.red[1], fm_term = g.red[2];
         __syncthreads();
@@ -555,6 +717,7 @@ __device__ double wg_m_norm(WG &g, double *sums, double *maxs)
 }
 
 // control.c:307-411 (diagonal Q: golden search over the workgroup)
+template <class WG>
 __device__ double wg_golden_diag(WG &g, double a, double b)
 {
     const int n = g.I->n;
@@ -589,6 +752,7 @@ __device__ double wg_golden_diag(WG &g, double a, double b)
     return f1 < f2 ? x1 : x2;
 }
 
+template <class WG>
 __device__ void wg_update_sigma(WG &g, double *sums, double *maxs)
 {
     State &S = *g.S;
@@ -609,15 +773,13 @@ __device__ void wg_update_sigma(WG &g, double *sums, double *maxs)
         dy_sq = g.red[0];
         __syncthreads();
     }
-    wg_spmv(g.Q.rp, g.Q.ci, g.Q.val, g.N(N_dw), g.Q.blk, g.Q.nblk,
-            EpiSigmaQdw{nullptr, g.N(N_dQw), g.N(N_ATdy), g.N(N_dw)}, sums, maxs, g.prod);
+    wg_spmv<WG::kDense>(g.Q, g.N(N_dw), EpiSigmaQdw{nullptr, g.N(N_dQw), g.N(N_ATdy), g.N(N_dw)}, sums, maxs, g.prod);
     wg_reduce<3, 0>(sums, maxs, g.red);
     const double dqw_atdy = g.red[0], dqw_sq = g.red[1], dw_dqw = g.red[2];
     __syncthreads();
     double atdy_qatdy = 0.0;
     if (m > 0) {
-        wg_spmv(g.Q.rp, g.Q.ci, g.Q.val, g.N(N_ATdy), g.Q.blk, g.Q.nblk,
-                EpiSigmaQATdy{nullptr, g.N(N_QATdy), g.N(N_ATdy)}, sums, maxs, g.prod);
+        wg_spmv<WG::kDense>(g.Q, g.N(N_ATdy), EpiSigmaQATdy{nullptr, g.N(N_QATdy), g.N(N_ATdy)}, sums, maxs, g.prod);
         wg_reduce<1, 0>(sums, maxs, g.red);
         atdy_qatdy = g.red[0];
         __syncthreads();
@@ -652,13 +814,13 @@ __device__ void wg_update_sigma(WG &g, double *sums, double *maxs)
 }
 
 // control.c:413-457
+template <class WG>
 __device__ void wg_restart_apply(WG &g, double *sums, double *maxs)
 {
     State &S = *g.S;
     if (S.rs.flag <= 0) return;
     const int m = g.I->m;
-    wg_spmv(g.AT.rp, g.AT.ci, g.AT.val, g.M(M_y_bar), g.AT.blk, g.AT.nblk,
-            EpiRestart{{}, g.N(N_ATy_bar), g.N(N_ATy), g.N(N_last_ATy), g.N(N_x), g.N(N_w), g.N(N_last_x),
+    wg_spmv<WG::kDense>(g.AT, g.M(M_y_bar), EpiRestart{{}, g.N(N_ATy_bar), g.N(N_ATy), g.N(N_last_ATy), g.N(N_x), g.N(N_w), g.N(N_last_x),
                        g.N(N_last_w), g.N(N_Qw), g.N(N_last_Qw), g.N(N_x_bar), g.N(N_w_bar), g.N(N_Qw_bar),
                        m > 0 ? 1 : 0, g.I->noC},
             sums, maxs, g.prod);
@@ -673,59 +835,53 @@ __device__ void wg_restart_apply(WG &g, double *sums, double *maxs)
 }
 
 // iteration.c:250-293 (general variant)
-template <bool DIAG, bool CHECK>
+template <bool DIAG, bool CHECK, class WG>
 __device__ void wg_step_t(WG &g, double h1, double h2, double *sums, double *maxs)
 {
     const State &S = *g.S;
     const int m = g.I->m;
     const double sg = S.sigma;
     const double f2s = 1.0 / (1.0 + sg * S.lam_q), f1s = 1.0 - f2s;
-    wg_spmv(g.Q.rp, g.Q.ci, g.Q.val, g.N(N_w), g.Q.blk, g.Q.nblk,
-            EpiZxw1<DIAG, CHECK>{{}, g.N(N_Qw), g.N(N_x), g.N(N_x_bar), g.N(N_x_hat), g.N(N_w_bar), g.N(N_dx),
+    wg_spmv<WG::kDense>(g.Q, g.N(N_w), EpiZxw1<DIAG, CHECK>{{}, g.N(N_Qw), g.N(N_x), g.N(N_x_bar), g.N(N_x_hat), g.N(N_w_bar), g.N(N_dx),
                                  g.N(N_z_bar), g.N(N_w), g.N(N_ATy), g.N(N_c), g.N(N_l), g.N(N_u), g.N(N_last_x),
                                  g.N(N_fact1), g.N(N_fact2), sg, f1s, f2s, h1, h2},
             sums, maxs, g.prod);
     if (m > 0) {
-        wg_spmv(g.Q.rp, g.Q.ci, g.Q.val, g.N(N_w_bar), g.Q.blk, g.Q.nblk,
-                EpiTempv{{}, g.N(N_tempv), g.N(N_x_hat), g.N(N_Qw), sg}, sums, maxs, g.prod);
+        wg_spmv<WG::kDense>(g.Q, g.N(N_w_bar), EpiTempv{{}, g.N(N_tempv), g.N(N_x_hat), g.N(N_Qw), sg}, sums, maxs, g.prod);
         const double f1 = S.lam_a * sg, f2 = 1.0 / f1;
-        wg_spmv(g.A.rp, g.A.ci, g.A.val, g.N(N_tempv), g.A.blk, g.A.nblk,
-                EpiY<CHECK>{{}, g.M(M_y), g.M(M_y_bar), g.M(M_s), g.M(M_dy), g.M(M_AL), g.M(M_AU), g.M(M_last_y), f1,
+        wg_spmv<WG::kDense>(g.A, g.N(N_tempv), EpiY<CHECK>{{}, g.M(M_y), g.M(M_y_bar), g.M(M_s), g.M(M_dy), g.M(M_AL), g.M(M_AU), g.M(M_last_y), f1,
                             f2, h1, h2},
                 sums, maxs, g.prod);
     }
     const double fs = sg / (1.0 + sg * S.lam_q);
-    wg_spmv(g.AT.rp, g.AT.ci, g.AT.val, g.M(M_y_bar), g.AT.blk, g.AT.nblk,
-            EpiW2<DIAG, CHECK>{{}, g.N(N_w_bar), g.N(N_w), g.N(N_ATy), g.N(N_ATy_bar), g.N(N_dw), g.N(N_ATdy),
+    wg_spmv<WG::kDense>(g.AT, g.M(M_y_bar), EpiW2<DIAG, CHECK>{{}, g.N(N_w_bar), g.N(N_w), g.N(N_ATy), g.N(N_ATy_bar), g.N(N_dw), g.N(N_ATdy),
                                g.N(N_last_w), g.N(N_last_ATy), g.N(N_fact), fs, h1, h2},
             sums, maxs, g.prod);
 }
 
 // iteration.c:19-83 and 176-216 (noC: A^T y first, Halpern on w and Qw)
-template <bool DIAG, bool CHECK>
+template <bool DIAG, bool CHECK, class WG>
 __device__ void wg_step_noc_t(WG &g, double h1, double h2, double *sums, double *maxs)
 {
     const State &S = *g.S;
     const int m = g.I->m;
     const double sg = S.sigma;
     const double f2s = 1.0 / (1.0 + sg * S.lam_q), f1s = 1.0 - f2s;
-    wg_spmv(g.AT.rp, g.AT.ci, g.AT.val, g.M(M_y), g.AT.blk, g.AT.nblk,
-            EpiZxw1NoC<DIAG, CHECK>{{}, g.N(N_ATy), g.N(N_x), g.N(N_x_bar), g.N(N_x_hat), g.N(N_w), g.N(N_w_bar),
+    wg_spmv<WG::kDense>(g.AT, g.M(M_y), EpiZxw1NoC<DIAG, CHECK>{{}, g.N(N_ATy), g.N(N_x), g.N(N_x_bar), g.N(N_x_hat), g.N(N_w), g.N(N_w_bar),
                                     g.N(N_dx), g.N(N_dw), g.N(N_z_bar), g.N(N_Qw), g.N(N_c), g.N(N_l), g.N(N_u),
                                     g.N(N_last_x), g.N(N_last_w), g.N(N_fact1), g.N(N_fact2), sg, f1s, f2s, h1, h2},
             sums, maxs, g.prod);
-    wg_spmv(g.Q.rp, g.Q.ci, g.Q.val, g.N(N_w_bar), g.Q.blk, g.Q.nblk,
-            EpiQwNoC{{}, g.N(N_Qw), g.N(N_Qw_bar), g.N(N_tempv), g.N(N_last_Qw), g.N(N_x_hat), sg, h1, h2}, sums,
+    wg_spmv<WG::kDense>(g.Q, g.N(N_w_bar), EpiQwNoC{{}, g.N(N_Qw), g.N(N_Qw_bar), g.N(N_tempv), g.N(N_last_Qw), g.N(N_x_hat), sg, h1, h2}, sums,
             maxs, g.prod);
     if (m > 0) {
         const double f1 = S.lam_a * sg, f2 = 1.0 / f1;
-        wg_spmv(g.A.rp, g.A.ci, g.A.val, g.N(N_tempv), g.A.blk, g.A.nblk,
-                EpiY<CHECK>{{}, g.M(M_y), g.M(M_y_bar), g.M(M_s), g.M(M_dy), g.M(M_AL), g.M(M_AU), g.M(M_last_y), f1,
+        wg_spmv<WG::kDense>(g.A, g.N(N_tempv), EpiY<CHECK>{{}, g.M(M_y), g.M(M_y_bar), g.M(M_s), g.M(M_dy), g.M(M_AL), g.M(M_AU), g.M(M_last_y), f1,
                             f2, h1, h2},
                 sums, maxs, g.prod);
     }
 }
 
+template <class WG>
 __device__ void wg_step(WG &g, long long inner, double *sums, double *maxs)
 {
     const double h1 = 1.0 / ((double)inner + 2.0), h2 = 1.0 - h1;
@@ -750,6 +906,7 @@ __device__ void wg_step(WG &g, long long inner, double *sums, double *maxs)
 }
 
 // solver_backend.c:432-513, one body; returns true when the instance is done.
+template <class WG>
 __device__ bool wg_body(WG &g, double *sums, double *maxs)
 {
     State &S = *g.S;
@@ -803,6 +960,7 @@ __device__ bool wg_body(WG &g, double *sums, double *maxs)
     return S.finished != 0;
 }
 
+template <class WG>
 __device__ void wg_bind(WG &g, const Arrays &ar, const Params &P, int k, State *S, double *prod, double *red)
 {
     const Inst *I = ar.inst + k;
@@ -811,11 +969,21 @@ __device__ void wg_bind(WG &g, const Arrays &ar, const Params &P, int k, State *
     g.P = P;
     g.prod = prod;
     g.red = red;
-    g.Q = Mat{ar.rp + I->q_rp, ar.ci + I->q_nz, ar.val + I->q_nz, ar.blocks + I->q_blk, I->q_nblk};
+    g.Q = Mat{ar.rp + I->q_rp, ar.ci + I->q_nz, ar.val + I->q_nz, ar.blocks + I->q_blk, I->q_nblk,
+              nullptr, 0, 0, 0};
     g.A = Mat{ar.rp + I->a_rp, ar.ci + ar.a_base + I->a_nz, ar.val + ar.a_base + I->a_nz, ar.blocks + I->a_blk,
-              I->a_nblk};
+              I->a_nblk, nullptr, 0, 0, 0};
     g.AT = Mat{ar.rp + I->at_rp, ar.ci + ar.at_ci_base + I->a_nz, ar.val + ar.at_ci_base + I->a_nz,
-               ar.blocks + I->at_blk, I->at_nblk};
+               ar.blocks + I->at_blk, I->at_nblk, nullptr, 0, 0, 0};
+    if constexpr (WG::kDense) {
+        // column form: Q over Q (symmetric), A x over A^T (n x m), A^T y over A (m x n)
+        g.Q.dv = ar.dval + I->dq;
+        g.Q.K = I->n, g.Q.J = I->n, g.Q.ld = I->n;
+        g.A.dv = ar.dval + I->da;
+        g.A.K = I->n, g.A.J = I->m, g.A.ld = I->m;
+        g.AT.dv = ar.dval + I->dat;
+        g.AT.K = I->m, g.AT.J = I->n, g.AT.ld = I->n;
+    }
     g.a_val = ar.val + ar.a_base + I->a_nz;
     g.at_perm = ar.perm + I->a_nz;
     g.qt_rp = ar.rp + I->qt_rp;
@@ -862,15 +1030,17 @@ struct EpiAdjM : NoRed {  // ym = mask_S (A~ xn)
     __device__ void row(int i, double v, double *, double *) const { ym[i] = mk.active_row(i) ? v : 0.0; }
 };
 
+template <class WG>
 __device__ void adj_op(WG &g, const AdjMask &mk, const double *xn, const double *xm, double *yn, double *ym,
                        double *sums, double *maxs)
 {
     double *tmp = g.N(N_pt);
-    wg_spmv(g.Q.rp, g.Q.ci, g.Q.val, xn, g.Q.blk, g.Q.nblk, EpiStore{{}, tmp}, sums, maxs, g.prod);
-    wg_spmv(g.AT.rp, g.AT.ci, g.AT.val, xm, g.AT.blk, g.AT.nblk, EpiAdjN{{}, yn, tmp, mk}, sums, maxs, g.prod);
-    if (g.I->m > 0) wg_spmv(g.A.rp, g.A.ci, g.A.val, xn, g.A.blk, g.A.nblk, EpiAdjM{{}, ym, mk}, sums, maxs, g.prod);
+    wg_spmv<WG::kDense>(g.Q, xn, EpiStore{{}, tmp}, sums, maxs, g.prod);
+    wg_spmv<WG::kDense>(g.AT, xm, EpiAdjN{{}, yn, tmp, mk}, sums, maxs, g.prod);
+    if (g.I->m > 0) wg_spmv<WG::kDense>(g.A, xn, EpiAdjM{{}, ym, mk}, sums, maxs, g.prod);
 }
 
+template <class WG>
 __device__ double adj_dot(WG &g, const double *an, const double *am, const double *bn, const double *bm,
                           double *sums, double *maxs)
 {
@@ -885,6 +1055,7 @@ __device__ double adj_dot(WG &g, const double *an, const double *am, const doubl
 }
 
 // y += a x over both parts
+template <class WG>
 __device__ void adj_axpy(WG &g, double a, const double *xn, const double *xm, double *yn, double *ym)
 {
     const int n = g.I->n, m = g.I->m;
@@ -893,6 +1064,7 @@ __device__ void adj_axpy(WG &g, double a, const double *xn, const double *xm, do
     __syncthreads();
 }
 
+template <class WG>
 __device__ void adj_copy(WG &g, const double *xn, const double *xm, double *yn, double *ym)
 {
     const int n = g.I->n, m = g.I->m;
@@ -901,6 +1073,7 @@ __device__ void adj_copy(WG &g, const double *xn, const double *xm, double *yn, 
     __syncthreads();
 }
 
+template <bool D>
 __global__ __launch_bounds__(kThreads) void batch_adjoint_kernel(Arrays ar, Params P, int count,
                                                                  const double *__restrict__ gvec,
                                                                  double *__restrict__ u_out,
@@ -912,10 +1085,10 @@ __global__ __launch_bounds__(kThreads) void batch_adjoint_kernel(Arrays ar, Para
     __shared__ double red[kMaxTerms];
     __shared__ State S;
     const int k = blockIdx.x;
-    if (k >= count) return;
+    if (k >= count || ar.inst[k].dense != (int)D) return;
     if (threadIdx.x == 0) S = ar.state[k];
     __syncthreads();
-    WG g;
+    WGv<D> g;
     wg_bind(g, ar, P, k, &S, prod, red);
     const int n = g.I->n, m = g.I->m, t = threadIdx.x;
     double sums[kMaxTerms] = {0}, maxs[kMaxTerms] = {0};
@@ -980,8 +1153,8 @@ __global__ __launch_bounds__(kThreads) void batch_adjoint_kernel(Arrays ar, Para
     }
     // bound multipliers: v~_B = b~_B - (Q~ u~ + A~' v~)_B with b~ = g/d
     double *tn = g.N(N_Qw_bar), *tmp = g.N(N_pt);
-    wg_spmv(g.Q.rp, g.Q.ci, g.Q.val, xn, g.Q.blk, g.Q.nblk, EpiStore{{}, tmp}, sums, maxs, g.prod);
-    wg_spmv(g.AT.rp, g.AT.ci, g.AT.val, xm, g.AT.blk, g.AT.nblk, EpiAddStore{{}, tn, tmp}, sums, maxs, g.prod);
+    wg_spmv<D>(g.Q, xn, EpiStore{{}, tmp}, sums, maxs, g.prod);
+    wg_spmv<D>(g.AT, xm, EpiAddStore{{}, tn, tmp}, sums, maxs, g.prod);
     for (int i = t; i < n; i += kThreads) {
         const bool fr = mk.free_var(i);
         u_out[g.I->vn + i] = xn[i] / d[i];
@@ -991,14 +1164,15 @@ __global__ __launch_bounds__(kThreads) void batch_adjoint_kernel(Arrays ar, Para
     if (t == 0) steps_out[k] = step;
 }
 
+template <bool D>
 __global__ __launch_bounds__(kThreads) void batch_setup_kernel(Arrays ar, Params P, int count)
 {
     __shared__ double prod[kStreamCap];
     __shared__ double red[kMaxTerms];
     __shared__ State S;
     const int k = blockIdx.x;
-    if (k >= count) return;
-    WG g;
+    if (k >= count || ar.inst[k].dense != (int)D) return;
+    WGv<D> g;
     wg_bind(g, ar, P, k, &S, prod, red);
     if (threadIdx.x == 0) {
         S = State{};
@@ -1024,17 +1198,18 @@ __global__ __launch_bounds__(kThreads) void batch_setup_kernel(Arrays ar, Params
 }
 
 // Runs up to `bodies` loop bodies of every unfinished instance.
+template <bool D>
 __global__ __launch_bounds__(kThreads) void batch_solve_kernel(Arrays ar, Params P, int count, int bodies)
 {
     __shared__ double prod[kStreamCap];
     __shared__ double red[kMaxTerms];
     __shared__ State S;
     const int k = blockIdx.x;
-    if (k >= count) return;
+    if (k >= count || ar.inst[k].dense != (int)D) return;
     if (threadIdx.x == 0) S = ar.state[k];
     __syncthreads();
     if (S.finished) return;
-    WG g;
+    WGv<D> g;
     wg_bind(g, ar, P, k, &S, prod, red);
     double sums[kMaxTerms] = {0}, maxs[kMaxTerms] = {0};
     for (int b = 0; b < bodies; ++b)
@@ -1045,6 +1220,30 @@ __global__ __launch_bounds__(kThreads) void batch_solve_kernel(Arrays ar, Params
     }
 }
 
+// build.c:157-252 on dense row-major Q planes (one workgroup per instance
+// row): Q_sym(r, c) = h(r, c) + h(c, r) with h = Q/2 dropped when |h| <=
+// 1e-15, the sum zeroed when |sum| <= 1e-15. With at most two contributions
+// per entry the order of the sum does not matter, so this equals the host
+// build of the same matrix.
+__global__ __launch_bounds__(kThreads) void dense_symmetrize_kernel(double *__restrict__ Qp, int n, int count)
+{
+    const long long row = blockIdx.x;
+    if (row >= (long long)count * n) return;
+    const long long k = row / n;
+    const int r = (int)(row % n);
+    double *Q = Qp + k * n * n;
+    const double tol = 1e-15;
+    for (int c = r + threadIdx.x; c < n; c += kThreads) {
+        double h1 = 0.5 * Q[(long long)r * n + c], h2 = 0.5 * Q[(long long)c * n + r];
+        if (fabs(h1) <= tol) h1 = 0.0;
+        if (fabs(h2) <= tol) h2 = 0.0;
+        double v = h1 + h2;
+        if (fabs(v) <= tol) v = 0.0;
+        Q[(long long)r * n + c] = v;
+        Q[(long long)c * n + r] = v;
+    }
+}
+
 }  // namespace batch
 }  // namespace hprqp
 
@@ -1052,12 +1251,13 @@ __global__ __launch_bounds__(kThreads) void batch_solve_kernel(Arrays ar, Params
 
 struct HPRQPHipBatch {
     size_t count = 0;
+    bool any_dense = false, any_sparse = false;  // which kernel variants have work
     hprqp::batch::Arrays ar{};
     hprqp::batch::Params P{};
     std::vector<hprqp::batch::Inst> inst;
     std::vector<size_t> n, m;
     hipStream_t stream = nullptr;
-    double *d_vn = nullptr, *d_vm = nullptr, *d_val = nullptr;
+    double *d_vn = nullptr, *d_vm = nullptr, *d_val = nullptr, *d_dval = nullptr;
     int *d_rp = nullptr, *d_ci = nullptr, *d_perm = nullptr, *d_blocks = nullptr, *d_active = nullptr;
     hprqp::batch::Inst *d_inst = nullptr;
     hprqp::batch::State *d_state = nullptr;
@@ -1068,7 +1268,7 @@ struct HPRQPHipBatch {
     ~HPRQPHipBatch()
     {
         if (stream) (void)hipStreamSynchronize(stream);
-        for (void *p : {(void *)d_vn, (void *)d_vm, (void *)d_val, (void *)d_rp, (void *)d_ci, (void *)d_perm,
+        for (void *p : {(void *)d_vn, (void *)d_vm, (void *)d_val, (void *)d_dval, (void *)d_rp, (void *)d_ci, (void *)d_perm,
                         (void *)d_blocks, (void *)d_active, (void *)d_inst, (void *)d_state})
             if (p) (void)hipFree(p);
         if (stream) (void)hipStreamDestroy(stream);
@@ -1128,6 +1328,42 @@ void transpose_perm(size_t nrow, size_t ncol, const size_t *rp, const size_t *ci
 
 }  // namespace
 
+namespace {
+
+// Common end of batch creation: instance table upload, state and stream,
+// then the setup kernels (per-instance scaling, power iteration, sigma0,
+// iteration-0 residuals). Deletes the batch on failure.
+HPRQPStatus batch_finish(HPRQPHipBatch *b)
+{
+    using namespace hprqp::batch;
+    const size_t count = b->count;
+    HPRQPStatus st = HPRQP_STATUS_OK;
+    if (hipMalloc((void **)&b->d_inst, sizeof(Inst) * count) != hipSuccess ||
+        hipMemcpy(b->d_inst, b->inst.data(), sizeof(Inst) * count, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMalloc((void **)&b->d_state, sizeof(State) * count) != hipSuccess ||
+        hipMalloc((void **)&b->d_active, sizeof(int)) != hipSuccess ||
+        (!b->stream && hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess))
+        st = HPRQP_STATUS_ALLOCATION_FAILED;
+    if (st != HPRQP_STATUS_OK) {
+        delete b;
+        return st;
+    }
+    Arrays &ar = b->ar;
+    ar.inst = b->d_inst;
+    ar.state = b->d_state;
+    ar.active = b->d_active;
+    for (size_t k = 0; k < count; ++k) (b->inst[k].dense ? b->any_dense : b->any_sparse) = true;
+    if (b->any_sparse) batch_setup_kernel<false><<<(unsigned)count, hprqp::kThreads, 0, b->stream>>>(ar, b->P, (int)count);
+    if (b->any_dense) batch_setup_kernel<true><<<(unsigned)count, hprqp::kThreads, 0, b->stream>>>(ar, b->P, (int)count);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(b->stream) != hipSuccess) {
+        delete b;
+        return HPRQP_STATUS_NOT_IMPLEMENTED;
+    }
+    return HPRQP_STATUS_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 HPRQPStatus hprqp_hip_batch_create(const HPRQPModel *models, size_t count, const HPRQPParams *params,
@@ -1147,7 +1383,7 @@ HPRQPStatus hprqp_hip_batch_create(const HPRQPModel *models, size_t count, const
     b->P = Params{params->stoptol, (long long)std::min(params->max_iter, (size_t)INT64_MAX),
                   (long long)params->check_iter, params->print_frequency, params->use_scaling};
     // sizes and validation
-    long long sum_n = 0, sum_m = 0, sum_q = 0, sum_a = 0, sum_rp = 0;
+    long long sum_n = 0, sum_m = 0;
     for (size_t k = 0; k < count; ++k) {
         const HPRQPModel &md = models[k];
         if (md.num_var_cones || md.num_row_cones || md.Q.ncol == 0 || !matrix_csr_is_valid(&md.Q) ||
@@ -1157,35 +1393,59 @@ HPRQPStatus hprqp_hip_batch_create(const HPRQPModel *models, size_t count, const
         }
         sum_n += (long long)md.Q.ncol;
         sum_m += (long long)md.A.nrow;
-        sum_q += (long long)md.Q.nnz;
-        sum_a += (long long)md.A.nnz;
-        sum_rp += 3 * ((long long)md.Q.ncol + 1) + (long long)md.A.nrow + 1;
+    }
+    // Layout pass: per-instance offsets (dense or CSR), so that packing can
+    // run on worker threads.
+    const char *dense_env = std::getenv("HPRQP_BATCH_DENSE");
+    const int dense_mode = dense_env ? std::atoi(dense_env) : -1;  // -1 auto, 0 never, 1 whenever it fits
+    b->inst.resize(count);
+    long long sum_q = 0, sum_a = 0, sum_rp = 0, sum_d = 0, vno = 0, vmo = 0;
+    for (size_t k = 0; k < count; ++k) {
+        const HPRQPModel &md = models[k];
+        Inst &I = b->inst[k];
+        const long long n = (long long)md.Q.ncol, m = (long long)md.A.nrow;
+        I = Inst{};
+        I.n = (int)n;
+        I.m = (int)m;
+        I.vn = vno;
+        I.vm = vmo;
+        const bool fits = n <= kDenseMax && m <= kDenseMax && m > 0;
+        const double dense_bytes = 8.0 * (double)(n * n + 2 * m * n);
+        const double csr_bytes = 12.0 * (double)(md.Q.nnz + 2 * md.A.nnz);
+        I.dense = fits && (dense_mode == 1 || (dense_mode < 0 && dense_bytes <= csr_bytes)) ? 1 : 0;
+        if (I.dense) {
+            I.dq = sum_d;
+            I.dat = sum_d + n * n;
+            I.da = I.dat + m * n;
+            sum_d += n * n + 2 * m * n;
+        } else {
+            I.q_nz = sum_q;
+            I.a_nz = sum_a;
+            I.q_rp = sum_rp;
+            I.a_rp = I.q_rp + n + 1;
+            I.at_rp = I.a_rp + m + 1;
+            I.qt_rp = I.at_rp + n + 1;
+            sum_rp += 3 * (n + 1) + m + 1;
+            sum_q += (long long)md.Q.nnz;
+            sum_a += (long long)md.A.nnz;
+        }
+        vno += n;
+        vmo += m;
     }
     if (sum_q + 2 * sum_a > INT32_MAX || sum_rp > INT32_MAX) {
         delete b;
         return HPRQP_STATUS_INVALID_ARGUMENT;
     }
-    std::vector<int> rp, ci, perm, blocks;
-    std::vector<double> val;
-    rp.reserve(sum_rp);
-    ci.resize(sum_q + 2 * sum_a);
-    val.resize(sum_q + 2 * sum_a, 0.0);
-    perm.resize(sum_a + sum_q);
-    std::vector<double> vn((size_t)sum_n * N_COUNT, 0.0), vm((size_t)std::max(sum_m, 1LL) * M_COUNT, 0.0);
-    b->inst.resize(count);
-    long long qnz = 0, anz = 0, vno = 0, vmo = 0;
     const long long a_base = sum_q, at_base = sum_q + sum_a;
-    std::vector<int> trp, tci, tperm;
-    for (size_t k = 0; k < count; ++k) {
+    std::vector<int> rp((size_t)std::max(sum_rp, 1LL)), ci((size_t)std::max(sum_q + 2 * sum_a, 1LL));
+    std::vector<int> perm((size_t)std::max(sum_a + sum_q, 1LL));
+    std::vector<double> val((size_t)std::max(sum_q + 2 * sum_a, 1LL)), dval((size_t)std::max(sum_d, 1LL));
+    std::vector<double> vn((size_t)sum_n * N_COUNT), vm((size_t)std::max(sum_m, 1LL) * M_COUNT);
+    std::vector<std::vector<int>> blk_q(count), blk_a(count), blk_at(count);
+    auto pack = [&](size_t k) {
         const HPRQPModel &md = models[k];
         Inst &I = b->inst[k];
         const size_t n = md.Q.ncol, m = md.A.nrow;
-        I.n = (int)n;
-        I.m = (int)m;
-        I.q_nz = qnz;
-        I.a_nz = anz;
-        I.vn = vno;
-        I.vm = vmo;
         {
             size_t free_count = 0;
             for (size_t i = 0; i < n; ++i)
@@ -1202,59 +1462,81 @@ HPRQPStatus hprqp_hip_batch_create(const HPRQPModel *models, size_t count, const
         for (size_t i = 0; i < n; ++i) nc = std::max(nc, std::fabs(md.c[i]));
         I.norm_b_org = nb;
         I.norm_c_org = nc;
-        auto push_rp = [&](const std::vector<int> &v) {
-            long long off = (long long)rp.size();
-            rp.insert(rp.end(), v.begin(), v.end());
-            return off;
-        };
-        auto push_blocks = [&](const std::vector<int> &r, int &off, int &cnt) {
-            std::vector<int> blk = plan_batch_blocks(r);
+        if (I.dense) {
+            // Q and A row-major (zeros where the CSR has no entry); the A^T
+            // copy is written on the device after scaling
+            double *Qd = dval.data() + I.dq, *Ad = dval.data() + I.dat;
+            std::fill(Qd, Qd + n * n, 0.0);
+            std::fill(Ad, Ad + m * n, 0.0);
+            for (size_t r = 0; r < n; ++r)
+                for (size_t e = md.Q.row_ptr[r]; e < md.Q.row_ptr[r + 1]; ++e) Qd[r * n + md.Q.col_ind[e]] += md.Q.values[e];
+            for (size_t r = 0; r < m; ++r)
+                for (size_t e = md.A.row_ptr[r]; e < md.A.row_ptr[r + 1]; ++e) Ad[r * n + md.A.col_ind[e]] += md.A.values[e];
+        } else {
+            std::vector<int> qrp(n + 1), arp(m + 1), trp, tci, tperm;
+            for (size_t r = 0; r <= n; ++r) qrp[r] = (int)md.Q.row_ptr[r];
+            for (size_t r = 0; r <= m; ++r) arp[r] = (int)md.A.row_ptr[r];
+            std::copy(qrp.begin(), qrp.end(), rp.begin() + I.q_rp);
+            std::copy(arp.begin(), arp.end(), rp.begin() + I.a_rp);
+            for (size_t e = 0; e < md.Q.nnz; ++e) {
+                ci[I.q_nz + e] = (int)md.Q.col_ind[e];
+                val[I.q_nz + e] = md.Q.values[e];
+            }
+            for (size_t e = 0; e < md.A.nnz; ++e) {
+                ci[a_base + I.a_nz + e] = (int)md.A.col_ind[e];
+                val[a_base + I.a_nz + e] = md.A.values[e];
+            }
+            transpose_perm(m, n, md.A.row_ptr, md.A.col_ind, trp, tci, tperm);
+            std::copy(trp.begin(), trp.end(), rp.begin() + I.at_rp);
+            for (size_t e = 0; e < md.A.nnz; ++e) {
+                ci[at_base + I.a_nz + e] = tci[e];
+                perm[I.a_nz + e] = tperm[e];
+            }
+            std::vector<int> atrp(trp);
+            transpose_perm(n, n, md.Q.row_ptr, md.Q.col_ind, trp, tci, tperm);
+            std::copy(trp.begin(), trp.end(), rp.begin() + I.qt_rp);
+            for (size_t e = 0; e < md.Q.nnz; ++e) perm[sum_a + I.q_nz + e] = tperm[e];
+            blk_q[k] = plan_batch_blocks(qrp);
+            blk_a[k] = plan_batch_blocks(arp);
+            blk_at[k] = plan_batch_blocks(atrp);
+        }
+        for (size_t i = 0; i < n; ++i) {
+            vn[(size_t)N_c * sum_n + I.vn + i] = md.c[i];
+            vn[(size_t)N_l * sum_n + I.vn + i] = md.l[i];
+            vn[(size_t)N_u * sum_n + I.vn + i] = md.u[i];
+        }
+        for (size_t i = 0; i < m; ++i) {
+            vm[(size_t)M_AL * sum_m + I.vm + i] = md.AL[i];
+            vm[(size_t)M_AU * sum_m + I.vm + i] = md.AU[i];
+        }
+    };
+    {
+        const size_t workers = std::max<size_t>(1, std::min<size_t>({16, count, (size_t)std::max(1u, std::thread::hardware_concurrency())}));
+        std::atomic<size_t> next{0};
+        std::vector<std::thread> pool;
+        for (size_t w = 1; w < workers; ++w)
+            pool.emplace_back([&] {
+                for (size_t k; (k = next.fetch_add(1)) < count;) pack(k);
+            });
+        for (size_t k; (k = next.fetch_add(1)) < count;) pack(k);
+        for (std::thread &th : pool) th.join();
+    }
+    std::vector<int> blocks;
+    for (size_t k = 0; k < count; ++k) {
+        Inst &I = b->inst[k];
+        b->n.push_back(models[k].Q.ncol);
+        b->m.push_back(models[k].A.nrow);
+        if (I.dense) continue;
+        auto push_blocks = [&](const std::vector<int> &blk, int &off, int &cnt) {
             off = (int)blocks.size();
             cnt = (int)blk.size() - 1;
             blocks.insert(blocks.end(), blk.begin(), blk.end());
         };
-        std::vector<int> qrp(n + 1), arp(m + 1);
-        for (size_t r = 0; r <= n; ++r) qrp[r] = (int)md.Q.row_ptr[r];
-        for (size_t r = 0; r <= m; ++r) arp[r] = (int)md.A.row_ptr[r];
-        I.q_rp = push_rp(qrp);
-        I.a_rp = push_rp(arp);
-        for (size_t e = 0; e < md.Q.nnz; ++e) {
-            ci[qnz + e] = (int)md.Q.col_ind[e];
-            val[qnz + e] = md.Q.values[e];
-        }
-        for (size_t e = 0; e < md.A.nnz; ++e) {
-            ci[a_base + anz + e] = (int)md.A.col_ind[e];
-            val[a_base + anz + e] = md.A.values[e];
-        }
-        transpose_perm(m, n, md.A.row_ptr, md.A.col_ind, trp, tci, tperm);
-        I.at_rp = push_rp(trp);
-        for (size_t e = 0; e < md.A.nnz; ++e) {
-            ci[at_base + anz + e] = tci[e];
-            perm[anz + e] = tperm[e];
-        }
-        transpose_perm(n, n, md.Q.row_ptr, md.Q.col_ind, trp, tci, tperm);
-        I.qt_rp = push_rp(trp);
-        for (size_t e = 0; e < md.Q.nnz; ++e) perm[sum_a + qnz + e] = tperm[e];
-        push_blocks(qrp, I.q_blk, I.q_nblk);
-        push_blocks(arp, I.a_blk, I.a_nblk);
-        std::vector<int> atrp(rp.begin() + I.at_rp, rp.begin() + I.at_rp + n + 1);
-        push_blocks(atrp, I.at_blk, I.at_nblk);
-        for (size_t i = 0; i < n; ++i) {
-            vn[(size_t)N_c * sum_n + vno + i] = md.c[i];
-            vn[(size_t)N_l * sum_n + vno + i] = md.l[i];
-            vn[(size_t)N_u * sum_n + vno + i] = md.u[i];
-        }
-        for (size_t i = 0; i < m; ++i) {
-            vm[(size_t)M_AL * sum_m + vmo + i] = md.AL[i];
-            vm[(size_t)M_AU * sum_m + vmo + i] = md.AU[i];
-        }
-        b->n.push_back(n);
-        b->m.push_back(m);
-        qnz += (long long)md.Q.nnz;
-        anz += (long long)md.A.nnz;
-        vno += (long long)n;
-        vmo += (long long)m;
+        push_blocks(blk_q[k], I.q_blk, I.q_nblk);
+        push_blocks(blk_a[k], I.a_blk, I.a_nblk);
+        push_blocks(blk_at[k], I.at_blk, I.at_nblk);
     }
+    if (blocks.empty()) blocks.push_back(0);
     HPRQPStatus st = HPRQP_STATUS_OK;
     auto upload = [&](auto *&dst, const auto &src) -> bool {
         using T = typename std::remove_reference<decltype(src)>::type::value_type;
@@ -1265,41 +1547,28 @@ HPRQPStatus hprqp_hip_batch_create(const HPRQPModel *models, size_t count, const
         return true;
     };
     if (!upload(b->d_rp, rp) || !upload(b->d_ci, ci) || !upload(b->d_val, val) || !upload(b->d_perm, perm) ||
-        !upload(b->d_blocks, blocks) || !upload(b->d_vn, vn) || !upload(b->d_vm, vm) || !upload(b->d_inst, b->inst))
+        !upload(b->d_blocks, blocks) || !upload(b->d_vn, vn) || !upload(b->d_vm, vm) || !upload(b->d_dval, dval))
         st = HPRQP_STATUS_ALLOCATION_FAILED;
-    if (st == HPRQP_STATUS_OK) {
-        if (hipMalloc((void **)&b->d_state, sizeof(State) * count) != hipSuccess ||
-            hipMalloc((void **)&b->d_active, sizeof(int)) != hipSuccess ||
-            hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
-            st = HPRQP_STATUS_ALLOCATION_FAILED;
-    }
     if (st != HPRQP_STATUS_OK) {
         delete b;
         return st;
     }
-    Arrays &ar = b->ar;
-    ar.inst = b->d_inst;
-    ar.state = b->d_state;
-    ar.rp = b->d_rp;
-    ar.ci = b->d_ci;
-    ar.val = b->d_val;
-    ar.perm = b->d_perm;
-    ar.blocks = b->d_blocks;
-    ar.vn = b->d_vn;
-    ar.vm = b->d_vm;
-    ar.sum_n = sum_n;
-    ar.sum_m = std::max(sum_m, 1LL);
-    ar.at_ci_base = at_base;
-    ar.a_base = a_base;
-    ar.qt_perm_base = sum_a;
-    ar.active = b->d_active;
-    batch_setup_kernel<<<(unsigned)count, hprqp::kThreads, 0, b->stream>>>(ar, b->P, (int)count);
-    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(b->stream) != hipSuccess) {
-        delete b;
-        return HPRQP_STATUS_NOT_IMPLEMENTED;
-    }
-    *out = b;
-    return HPRQP_STATUS_OK;
+    b->ar.rp = b->d_rp;
+    b->ar.ci = b->d_ci;
+    b->ar.val = b->d_val;
+    b->ar.perm = b->d_perm;
+    b->ar.blocks = b->d_blocks;
+    b->ar.vn = b->d_vn;
+    b->ar.vm = b->d_vm;
+    b->ar.dval = b->d_dval;
+    b->ar.sum_n = sum_n;
+    b->ar.sum_m = std::max(sum_m, 1LL);
+    b->ar.at_ci_base = at_base;
+    b->ar.a_base = a_base;
+    b->ar.qt_perm_base = sum_a;
+    st = batch_finish(b);
+    if (st == HPRQP_STATUS_OK) *out = b;
+    return st;
 }
 
 // Relaunches the chunk kernel (bodies_per_launch loop bodies per instance)
@@ -1312,8 +1581,12 @@ HPRQPStatus hprqp_hip_batch_solve(HPRQPHipBatch *b, size_t bodies_per_launch, si
     int active = 1;
     while (active > 0) {
         HPRQP_HIP(hipMemsetAsync(b->d_active, 0, sizeof(int), b->stream));
-        batch_solve_kernel<<<(unsigned)b->count, hprqp::kThreads, 0, b->stream>>>(b->ar, b->P, (int)b->count,
-                                                                                  (int)bodies_per_launch);
+        if (b->any_sparse)
+            batch_solve_kernel<false><<<(unsigned)b->count, hprqp::kThreads, 0, b->stream>>>(
+                b->ar, b->P, (int)b->count, (int)bodies_per_launch);
+        if (b->any_dense)
+            batch_solve_kernel<true><<<(unsigned)b->count, hprqp::kThreads, 0, b->stream>>>(
+                b->ar, b->P, (int)b->count, (int)bodies_per_launch);
         HPRQP_HIP(hipGetLastError());
         HPRQP_HIP(hipMemcpyAsync(&active, b->d_active, sizeof(int), hipMemcpyDeviceToHost, b->stream));
         HPRQP_HIP(hipStreamSynchronize(b->stream));
@@ -1370,6 +1643,176 @@ HPRQPStatus hprqp_hip_batch_result(const HPRQPHipBatch *b, size_t k, HPRQPResult
     return HPRQP_STATUS_OK;
 }
 
+// Batch of `count` QPs of one shape given as dense row-major arrays: Q
+// (count x n x n), A (count x m x n), c, l, u (count x n), AL, AU (count x m).
+// Equivalent to build_from_QAbc on each instance (Q symmetrised on the
+// device) and hprqp_hip_batch_create, without building host models; every
+// row of A needs a nonzero entry and a finite bound.
+HPRQPStatus hprqp_hip_batch_create_dense(size_t count, size_t n, size_t m, const double *Q, const double *c,
+                                         const double *A, const double *AL, const double *AU, const double *l,
+                                         const double *u, const HPRQPParams *params, HPRQPHipBatch **out)
+{
+    using namespace hprqp::batch;
+    if (!out || !params || !Q || !c || !l || !u || (m && (!A || !AL || !AU)) || count == 0 || n == 0 || m == 0 ||
+        n > (size_t)kDenseMax || m > (size_t)kDenseMax || count > (size_t)INT32_MAX / (n > m ? n : m))
+        return HPRQP_STATUS_INVALID_ARGUMENT;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return HPRQP_STATUS_NOT_IMPLEMENTED;
+    if (params->device_number < 0 || params->device_number >= ndev) return HPRQP_STATUS_INVALID_ARGUMENT;
+    HPRQP_HIP(hipSetDevice(params->device_number));
+    // validation (build.c:339-400 checks, plus: no row of A to delete)
+    std::atomic<int> bad{0};
+    {
+        const size_t workers = std::max<size_t>(1, std::min<size_t>({16, count, (size_t)std::max(1u, std::thread::hardware_concurrency())}));
+        std::atomic<size_t> next{0};
+        auto check = [&] {
+            for (size_t k; (k = next.fetch_add(1)) < count && !bad.load();) {
+                const double *Qk = Q + k * n * n, *Ak = A + k * m * n;
+                for (size_t e = 0; e < n * n; ++e)
+                    if (!std::isfinite(Qk[e])) bad = 1;
+                for (size_t i = 0; i < m; ++i) {
+                    bool nz = false;
+                    for (size_t j = 0; j < n; ++j) {
+                        const double v = Ak[i * n + j];
+                        if (!std::isfinite(v)) bad = 1;
+                        nz |= v != 0.0;
+                    }
+                    const double al = AL[k * m + i], au = AU[k * m + i];
+                    if (!nz || std::isnan(al) || std::isnan(au) || al > au || (std::isinf(al) && al < 0 && std::isinf(au) && au > 0))
+                        bad = 1;
+                }
+                for (size_t j = 0; j < n; ++j) {
+                    const double cj = c[k * n + j], lj = l[k * n + j], uj = u[k * n + j];
+                    if (!std::isfinite(cj) || std::isnan(lj) || std::isnan(uj) || lj > uj) bad = 1;
+                }
+            }
+        };
+        std::vector<std::thread> pool;
+        for (size_t w = 1; w < workers; ++w) pool.emplace_back(check);
+        check();
+        for (std::thread &th : pool) th.join();
+    }
+    if (bad) return HPRQP_STATUS_INVALID_ARGUMENT;
+    auto *b = new (std::nothrow) HPRQPHipBatch();
+    if (!b) return HPRQP_STATUS_ALLOCATION_FAILED;
+    b->count = count;
+    b->time_limit = params->time_limit;
+    b->P = Params{params->stoptol, (long long)std::min(params->max_iter, (size_t)INT64_MAX),
+                  (long long)params->check_iter, params->print_frequency, params->use_scaling};
+    const long long N = (long long)n, M = (long long)m, C = (long long)count;
+    const long long sum_n = C * N, sum_m = C * M;
+    b->inst.resize(count);
+    for (size_t k = 0; k < count; ++k) {
+        Inst &I = b->inst[k];
+        I = Inst{};
+        I.n = (int)n;
+        I.m = (int)m;
+        I.vn = (long long)k * N;
+        I.vm = (long long)k * M;
+        I.dense = 1;
+        I.dq = (long long)k * N * N;
+        I.dat = C * N * N + (long long)k * M * N;
+        I.da = C * N * N + C * M * N + (long long)k * M * N;
+        size_t free_count = 0;
+        double nb = 0.0, nc = 0.0;
+        for (size_t j = 0; j < n; ++j) {
+            if (std::isinf(l[k * n + j]) && std::isinf(u[k * n + j])) ++free_count;
+            nc = std::max(nc, std::fabs(c[k * n + j]));
+        }
+        for (size_t i = 0; i < m; ++i) {
+            const double a = std::isinf(AL[k * m + i]) ? 0.0 : std::fabs(AL[k * m + i]);
+            const double v = std::isinf(AU[k * m + i]) ? 0.0 : std::fabs(AU[k * m + i]);
+            nb = std::max(nb, std::max(a, v));
+        }
+        I.noC = 5 * free_count > 4 * n ? 1 : 0;  // workspace.c:45-61
+        I.norm_b_org = nb;
+        I.norm_c_org = nc;
+        b->n.push_back(n);
+        b->m.push_back(m);
+    }
+    HPRQPStatus st = HPRQP_STATUS_OK;
+    const size_t dsz = (size_t)(C * N * N + 2 * C * M * N);
+    auto fail = [&]() {
+        delete b;
+        return HPRQP_STATUS_ALLOCATION_FAILED;
+    };
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) return fail();
+    if (hipMalloc((void **)&b->d_dval, sizeof(double) * dsz) != hipSuccess ||
+        hipMalloc((void **)&b->d_vn, sizeof(double) * (size_t)sum_n * N_COUNT) != hipSuccess ||
+        hipMalloc((void **)&b->d_vm, sizeof(double) * (size_t)sum_m * M_COUNT) != hipSuccess ||
+        hipMalloc((void **)&b->d_rp, sizeof(int)) != hipSuccess || hipMalloc((void **)&b->d_ci, sizeof(int)) != hipSuccess ||
+        hipMalloc((void **)&b->d_val, sizeof(double)) != hipSuccess ||
+        hipMalloc((void **)&b->d_perm, sizeof(int)) != hipSuccess ||
+        hipMalloc((void **)&b->d_blocks, sizeof(int)) != hipSuccess)
+        return fail();
+    hipStream_t s = b->stream;
+    bool ok = hipMemcpyAsync(b->d_dval, Q, sizeof(double) * (size_t)(C * N * N), hipMemcpyHostToDevice, s) == hipSuccess &&
+              hipMemcpyAsync(b->d_dval + C * N * N, A, sizeof(double) * (size_t)(C * M * N), hipMemcpyHostToDevice, s) ==
+                  hipSuccess &&
+              hipMemsetAsync(b->d_vn, 0, sizeof(double) * (size_t)sum_n * N_COUNT, s) == hipSuccess &&
+              hipMemsetAsync(b->d_vm, 0, sizeof(double) * (size_t)sum_m * M_COUNT, s) == hipSuccess &&
+              hipMemcpyAsync(b->d_vn + N_c * sum_n, c, sizeof(double) * (size_t)sum_n, hipMemcpyHostToDevice, s) == hipSuccess &&
+              hipMemcpyAsync(b->d_vn + N_l * sum_n, l, sizeof(double) * (size_t)sum_n, hipMemcpyHostToDevice, s) == hipSuccess &&
+              hipMemcpyAsync(b->d_vn + N_u * sum_n, u, sizeof(double) * (size_t)sum_n, hipMemcpyHostToDevice, s) == hipSuccess &&
+              hipMemcpyAsync(b->d_vm + M_AL * sum_m, AL, sizeof(double) * (size_t)sum_m, hipMemcpyHostToDevice, s) ==
+                  hipSuccess &&
+              hipMemcpyAsync(b->d_vm + M_AU * sum_m, AU, sizeof(double) * (size_t)sum_m, hipMemcpyHostToDevice, s) ==
+                  hipSuccess;
+    if (!ok) return fail();
+    dense_symmetrize_kernel<<<(unsigned)(C * N), hprqp::kThreads, 0, s>>>(b->d_dval, (int)n, (int)count);
+    if (hipGetLastError() != hipSuccess) return fail();
+    b->ar.rp = b->d_rp;
+    b->ar.ci = b->d_ci;
+    b->ar.val = b->d_val;
+    b->ar.perm = b->d_perm;
+    b->ar.blocks = b->d_blocks;
+    b->ar.vn = b->d_vn;
+    b->ar.vm = b->d_vm;
+    b->ar.dval = b->d_dval;
+    b->ar.sum_n = sum_n;
+    b->ar.sum_m = sum_m;
+    st = batch_finish(b);
+    if (st == HPRQP_STATUS_OK) *out = b;
+    return st;
+}
+
+// Every instance's solution in original units, instances concatenated:
+// x, z (sum of n), y (sum of m); status, iterations, primal objective and
+// KKT residual per instance. Any output pointer may be null.
+HPRQPStatus hprqp_hip_batch_solution(const HPRQPHipBatch *b, double *x, double *y, double *z, int *status,
+                                     size_t *iters, double *objective, double *kkt)
+{
+    using namespace hprqp::batch;
+    if (!b || b->state.size() != b->count) return HPRQP_STATUS_INVALID_ARGUMENT;
+    const size_t sn = (size_t)b->ar.sum_n, sm = (size_t)b->ar.sum_m;
+    std::vector<double> col(sn), row(sm);
+    HPRQP_HIP(hipMemcpy(col.data(), b->d_vn + (size_t)N_col * sn, sizeof(double) * sn, hipMemcpyDeviceToHost));
+    HPRQP_HIP(hipMemcpy(row.data(), b->d_vm + (size_t)M_row * sm, sizeof(double) * sm, hipMemcpyDeviceToHost));
+    size_t real_m = 0;
+    for (size_t k = 0; k < b->count; ++k) real_m += b->m[k];
+    if (x) {
+        HPRQP_HIP(hipMemcpy(x, b->d_vn + (size_t)N_x_bar * sn, sizeof(double) * sn, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < sn; ++i) x[i] /= col[i];
+    }
+    if (z) {
+        HPRQP_HIP(hipMemcpy(z, b->d_vn + (size_t)N_z_bar * sn, sizeof(double) * sn, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < sn; ++i) z[i] *= col[i];
+    }
+    if (y && real_m) {
+        HPRQP_HIP(hipMemcpy(y, b->d_vm + (size_t)M_y_bar * sm, sizeof(double) * real_m, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < real_m; ++i) y[i] /= row[i];
+    }
+    for (size_t k = 0; k < b->count; ++k) {
+        const State &S = b->state[k];
+        if (status) status[k] = S.finished ? S.status : HPRQP_STATUS_TIME_LIMIT;
+        if (iters) iters[k] = (size_t)S.iter;
+        if (objective) objective[k] = S.pobj;
+        if (kkt) kkt[k] = S.kkt;
+    }
+    return HPRQP_STATUS_OK;
+}
+
 void hprqp_hip_batch_destroy(HPRQPHipBatch *b) { delete b; }
 
 }  // extern "C"
@@ -1396,8 +1839,12 @@ HPRQPStatus hprqp_hip_batch_adjoint(HPRQPHipBatch *b, const double *dl_dx, doubl
     HPRQP_HIP(hipMalloc((void **)&d_steps, sizeof(int) * b->count));
     HPRQP_HIP(hipMemcpy(d_g, dl_dx, sizeof(double) * sn, hipMemcpyHostToDevice));
     HPRQP_HIP(hipMemset(d_vr, 0, sizeof(double) * sm));
-    batch_adjoint_kernel<<<(unsigned)b->count, hprqp::kThreads, 0, b->stream>>>(b->ar, b->P, (int)b->count, d_g, d_u,
-                                                                                d_vr, d_vb, tol, max_steps, d_steps);
+    if (b->any_sparse)
+        batch_adjoint_kernel<false><<<(unsigned)b->count, hprqp::kThreads, 0, b->stream>>>(
+            b->ar, b->P, (int)b->count, d_g, d_u, d_vr, d_vb, tol, max_steps, d_steps);
+    if (b->any_dense)
+        batch_adjoint_kernel<true><<<(unsigned)b->count, hprqp::kThreads, 0, b->stream>>>(
+            b->ar, b->P, (int)b->count, d_g, d_u, d_vr, d_vb, tol, max_steps, d_steps);
     HPRQP_HIP(hipGetLastError());
     HPRQP_HIP(hipStreamSynchronize(b->stream));
     HPRQP_HIP(hipMemcpy(u, d_u, sizeof(double) * sn, hipMemcpyDeviceToHost));

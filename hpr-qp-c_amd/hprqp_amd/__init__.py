@@ -336,12 +336,23 @@ _LIB.hprqp_hip_batch_result.argtypes = [C.c_void_p, C.c_size_t, C.POINTER(CResul
 _LIB.hprqp_hip_batch_destroy.argtypes = [C.c_void_p]
 
 
+_LIB.hprqp_hip_batch_create_dense.argtypes = [C.c_size_t, C.c_size_t, C.c_size_t] + [_db_p] * 7 + [
+    C.POINTER(CParams), C.POINTER(C.c_void_p)]
+_LIB.hprqp_hip_batch_solution.argtypes = [C.c_void_p, _db_p, _db_p, _db_p, C.POINTER(C.c_int),
+                                          C.POINTER(C.c_size_t), _db_p, _db_p]
+
+
 class Batch:
     """hprqp_hip_batch_*: many independent QPs in one persistent launch."""
 
-    def __init__(self, models: Sequence[Model], params: Parameters | None = None) -> None:
-        self.models = list(models)
+    def __init__(self, models: Sequence[Model] | None, params: Parameters | None = None) -> None:
         self.params = params or Parameters(verbose=False)
+        self.h = None
+        if models is None:  # filled in by from_dense
+            return
+        self.models = list(models)
+        self.ns = [m.num_variables() for m in self.models]
+        self.ms = [m.num_constraints() for m in self.models]
         arr = (CModel * len(self.models))(*[m.c for m in self.models])
         self._arr = arr
         h = C.c_void_p()
@@ -349,6 +360,34 @@ class Batch:
         if st != STATUS_OK:
             raise RuntimeError(f"hprqp_hip_batch_create failed: {status_name(st)}")
         self.h = h
+
+    @classmethod
+    def from_dense(cls, Q, c, A, AL, AU, l, u, params: Parameters | None = None) -> "Batch":
+        """hprqp_hip_batch_create_dense: count QPs of one shape from dense
+        arrays Q (count, n, n), A (count, m, n), c, l, u (count, n), AL, AU
+        (count, m); the same result as building every instance with
+        build_from_qabc and batching the models."""
+        b = cls(None, params)
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (Q, c, A, AL, AU, l, u)]
+        Qa, ca, Aa = arrs[0], arrs[1], arrs[2]
+        count, n = ca.shape
+        m = Aa.shape[1]
+        if Qa.shape != (count, n, n) or Aa.shape != (count, m, n) or any(
+                a.shape != (count, m) for a in arrs[3:5]) or any(a.shape != (count, n) for a in arrs[5:]):
+            raise ValueError("Batch.from_dense: inconsistent shapes")
+        b._keep = arrs
+        b.models = None
+        b.ns, b.ms = [n] * count, [m] * count
+        h = C.c_void_p()
+        st = _LIB.hprqp_hip_batch_create_dense(count, n, m, *[a.ctypes.data_as(_db_p) for a in arrs],
+                                               C.byref(b.params.c), C.byref(h))
+        if st != STATUS_OK:
+            raise ValueError(f"hprqp_hip_batch_create_dense failed: {status_name(st)}")
+        b.h = h
+        return b
+
+    def __len__(self) -> int:
+        return len(self.ns)
 
     def __del__(self) -> None:
         if getattr(self, "h", None):
@@ -373,40 +412,64 @@ class Batch:
         try:
             if st != STATUS_OK:
                 raise RuntimeError(f"hprqp_hip_batch_result failed: {status_name(st)}")
-            m = self.models[k]
-            return _result_of(r, r.status, m.num_variables(), m.num_constraints())
+            return _result_of(r, r.status, self.ns[k], self.ms[k])
         finally:
             _LIB.results_free(C.byref(r))
+
+    def solution(self) -> dict:
+        """hprqp_hip_batch_solution: x, z (sum n), y (sum m) concatenated in
+        original units; status names, iterations, objectives, KKT residuals."""
+        sn, sm, cnt = sum(self.ns), sum(self.ms), len(self.ns)
+        x, z, y = np.zeros(sn), np.zeros(sn), np.zeros(max(sm, 1))
+        status = np.zeros(cnt, np.int32)
+        iters = np.zeros(cnt, np.uint64)
+        obj, kkt = np.zeros(cnt), np.zeros(cnt)
+        st = _LIB.hprqp_hip_batch_solution(self.h, x.ctypes.data_as(_db_p), y.ctypes.data_as(_db_p),
+                                           z.ctypes.data_as(_db_p), status.ctypes.data_as(C.POINTER(C.c_int)),
+                                           iters.ctypes.data_as(C.POINTER(C.c_size_t)), obj.ctypes.data_as(_db_p),
+                                           kkt.ctypes.data_as(_db_p))
+        if st != STATUS_OK:
+            raise RuntimeError(f"hprqp_hip_batch_solution failed: {status_name(st)}")
+        return dict(x=x, y=y[:sm], z=z, status=[status_name(int(v)) for v in status], iter=iters.astype(np.int64),
+                    objective=obj, kkt=kkt)
 
 
 def solve_batch(models: Sequence[Model], params: Parameters | None = None, bodies_per_launch: int = 100):
     b = Batch(models, params)
     b.solve(bodies_per_launch)
-    return [b.result(k) for k in range(len(b.models))], b.seconds
+    return [b.result(k) for k in range(len(b))], b.seconds
 
 
 _LIB.hprqp_hip_batch_adjoint.argtypes = [C.c_void_p, _db_p, C.c_double, C.c_int, _db_p, _db_p, _db_p,
                                          C.POINTER(C.c_int)]
 
 
-def batch_adjoint(batch: Batch, dl_dx: Sequence[np.ndarray], tol: float = 1e-12, max_steps: int = 20000):
-    """Per instance (u, v_rows, v_bounds, steps) of the active-set KKT adjoint
-    K [u; v] = [dl_dx; 0] (see include/hprqp_hip.h)."""
-    ns = [m.num_variables() for m in batch.models]
-    ms = [m.num_constraints() for m in batch.models]
-    g = np.ascontiguousarray(np.concatenate([np.asarray(a, np.float64) for a in dl_dx]))
-    assert g.size == sum(ns)
-    u = np.zeros(sum(ns))
-    vb = np.zeros(sum(ns))
-    vr = np.zeros(max(sum(ms), 1))
-    steps = np.zeros(len(ns), np.int32)
+def batch_adjoint_arrays(batch: Batch, dl_dx: np.ndarray, tol: float = 1e-12, max_steps: int = 20000):
+    """Concatenated (u, v_rows, v_bounds, steps) of the active-set KKT adjoint
+    K [u; v] = [dl_dx; 0] of every instance (see include/hprqp_hip.h);
+    dl_dx is the concatenation of the instances' upstream gradients."""
+    sn, sm = sum(batch.ns), sum(batch.ms)
+    g = np.ascontiguousarray(dl_dx, dtype=np.float64).reshape(-1)
+    assert g.size == sn
+    u = np.zeros(sn)
+    vb = np.zeros(sn)
+    vr = np.zeros(max(sm, 1))
+    steps = np.zeros(len(batch.ns), np.int32)
     st = _LIB.hprqp_hip_batch_adjoint(batch.h, g.ctypes.data_as(_db_p), tol, max_steps, u.ctypes.data_as(_db_p),
                                       vr.ctypes.data_as(_db_p), vb.ctypes.data_as(_db_p),
                                       steps.ctypes.data_as(C.POINTER(C.c_int)))
     if st != STATUS_OK:
         raise RuntimeError(f"hprqp_hip_batch_adjoint failed: {status_name(st)}")
+    return u, vr[:sm], vb, steps
+
+
+def batch_adjoint(batch: Batch, dl_dx: Sequence[np.ndarray], tol: float = 1e-12, max_steps: int = 20000):
+    """Per instance (u, v_rows, v_bounds, steps) of the active-set KKT adjoint
+    K [u; v] = [dl_dx; 0] (see include/hprqp_hip.h)."""
+    g = np.concatenate([np.asarray(a, np.float64) for a in dl_dx])
+    u, vr, vb, steps = batch_adjoint_arrays(batch, g, tol, max_steps)
     out, on, om = [], 0, 0
-    for k, (n, m) in enumerate(zip(ns, ms)):
+    for k, (n, m) in enumerate(zip(batch.ns, batch.ms)):
         out.append((u[on:on + n].copy(), vr[om:om + m].copy(), vb[on:on + n].copy(), int(steps[k])))
         on += n
         om += m

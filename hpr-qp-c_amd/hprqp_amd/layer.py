@@ -15,14 +15,12 @@ K [u; v] = [dL/dx; 0]:
 """
 from __future__ import annotations
 
-import os
 import time
-from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 import torch
 
-from . import Batch, Parameters, batch_adjoint, build_from_qabc
+from . import Batch, Parameters, batch_adjoint_arrays
 
 
 def _np(t):
@@ -33,15 +31,6 @@ def _np(t):
 last_timings: dict = {}
 
 
-def _build_all(Qn, cn, An, ALn, AUn, ln, un):
-    # build_from_QAbc is a ctypes call, which releases the GIL: build the
-    # per-instance models (host symmetrisation and row checks) on threads.
-    B = cn.shape[0]
-    workers = max(1, min(16, os.cpu_count() or 1, B))
-    with ThreadPoolExecutor(max_workers=workers) as ex:
-        return list(ex.map(lambda b: build_from_qabc(Qn[b], cn[b], An[b], ALn[b], AUn[b], ln[b], un[b]), range(B)))
-
-
 class QPLayer(torch.autograd.Function):
     @staticmethod
     def forward(ctx, Q, c, A, AL, AU, l, u, stoptol: float = 1e-8, bodies_per_launch: int = 200):
@@ -49,27 +38,30 @@ class QPLayer(torch.autograd.Function):
         Qn, cn, An, ALn, AUn, ln, un = (_np(t) for t in (Q, c, A, AL, AU, l, u))
         B, n = cn.shape
         m = An.shape[1]
-        models = _build_all(Qn, cn, An, ALn, AUn, ln, un)
-        for md in models:
-            if md.num_constraints() != m:
-                raise ValueError("QPLayer: rows with no entries or free bounds are not supported")
         t1 = time.perf_counter()
-        batch = Batch(models, Parameters(verbose=False, stoptol=stoptol))
+        try:
+            # instances go to the device as dense arrays; build_from_QAbc's
+            # preprocessing (Q symmetrised) runs there
+            batch = Batch.from_dense(Qn, cn, An, ALn, AUn, ln, un, Parameters(verbose=False, stoptol=stoptol))
+        except ValueError as e:
+            raise ValueError("QPLayer: rows with no entries or free bounds are not supported "
+                             f"(or invalid data): {e}") from None
         t2 = time.perf_counter()
         batch.solve(bodies_per_launch)
         t3 = time.perf_counter()
-        res = [batch.result(b) for b in range(B)]
-        bad = [r.status for r in res if r.status != "OPTIMAL"]
+        sol = batch.solution()
+        bad = [s_ for s_ in sol["status"] if s_ != "OPTIMAL"]
         if bad:
             raise RuntimeError(f"QPLayer forward: {len(bad)} instances not optimal ({bad[0]})")
-        x = np.stack([r.x for r in res])
-        y = np.stack([r.y for r in res]) if m else np.zeros((B, 0))
-        z = np.stack([r.z for r in res])
+        x = sol["x"].reshape(B, n)
+        y = sol["y"].reshape(B, m)
+        z = sol["z"].reshape(B, n)
         ctx.batch = batch
         ctx.sol = (x, y, z, ALn, AUn, ln, un)
         last_timings.clear()
-        last_timings.update(build=t1 - t0, upload=t2 - t1, solve=t3 - t2, device_solve=batch.seconds,
-                            collect=time.perf_counter() - t3, iterations=max(r.iter for r in res))
+        last_timings.update(to_host=t1 - t0, create=t2 - t1, solve=t3 - t2, device_solve=batch.seconds,
+                            collect=time.perf_counter() - t3, iterations=int(sol["iter"].max()),
+                            iterations_sum=int(sol["iter"].sum()))
         ctx.device = Q.device
         return torch.as_tensor(x, dtype=torch.float64, device=Q.device)
 
@@ -80,22 +72,25 @@ class QPLayer(torch.autograd.Function):
         m = y.shape[1]
         g = _np(gx)
         t0 = time.perf_counter()
-        adj = batch_adjoint(ctx.batch, list(g))
+        u, vr, vb, _ = batch_adjoint_arrays(ctx.batch, g.reshape(-1))
         last_timings["adjoint"] = time.perf_counter() - t0
-        U = np.stack([a[0] for a in adj])
-        VR = np.stack([a[1] for a in adj]) if m else np.zeros((B, 0))
-        VB = np.stack([a[2] for a in adj])
-        dc = -U
-        dQ = -0.5 * (U[:, :, None] * x[:, None, :] + x[:, :, None] * U[:, None, :])
-        dA = y[:, :, None] * U[:, None, :] - VR[:, :, None] * x[:, None, :]
-        eq = ALn == AUn
-        dAL = np.where(eq, 0.5 * VR, np.where(y > 0, VR, 0.0))
-        dAU = np.where(eq, 0.5 * VR, np.where(y < 0, VR, 0.0))
-        feq = ln == un
-        dl = np.where(feq, 0.5 * VB, np.where(z > 0, VB, 0.0))
-        du = np.where(feq, 0.5 * VB, np.where(z < 0, VB, 0.0))
+        # rank-one gradient terms as batched outer products (torch runs them
+        # multi-threaded on the host, or on the device the inputs came from)
         t = lambda a: torch.as_tensor(a, dtype=torch.float64, device=ctx.device)
-        return t(dQ), t(dc), t(dA), t(dAL), t(dAU), t(dl), t(du), None, None
+        U, VR, VB = t(u.reshape(B, n)), t(vr.reshape(B, m)), t(vb.reshape(B, n))
+        X, Y, Z = t(x), t(y), t(z)
+        P = torch.bmm(U[:, :, None], X[:, None, :])
+        dQ = (P + P.transpose(1, 2)).mul_(-0.5)
+        dA = torch.baddbmm(torch.bmm(Y[:, :, None], U[:, None, :]), VR[:, :, None], X[:, None, :], alpha=-1.0)
+        dc = -U
+        eq, feq = t(ALn == AUn).bool(), t(ln == un).bool()
+        zero = torch.zeros((), dtype=torch.float64, device=ctx.device)
+        dAL = torch.where(eq, 0.5 * VR, torch.where(Y > 0, VR, zero))
+        dAU = torch.where(eq, 0.5 * VR, torch.where(Y < 0, VR, zero))
+        dl = torch.where(feq, 0.5 * VB, torch.where(Z > 0, VB, zero))
+        du = torch.where(feq, 0.5 * VB, torch.where(Z < 0, VB, zero))
+        last_timings["backward_host"] = time.perf_counter() - t0
+        return dQ, dc, dA, dAL, dAU, dl, du, None, None
 
 
 def qp_layer(Q, c, A, AL, AU, l, u, stoptol: float = 1e-8):

@@ -79,6 +79,23 @@ HPRQPStatus hprqp_hip_batch_solve(HPRQPHipBatch *batch, size_t bodies_per_launch
 double hprqp_hip_batch_seconds(const HPRQPHipBatch *batch);
 HPRQPStatus hprqp_hip_batch_result(const HPRQPHipBatch *batch, size_t index, HPRQPResults *results);
 void hprqp_hip_batch_destroy(HPRQPHipBatch *batch);
+/* Batch of `count` QPs of one shape from dense row-major host arrays: Q
+ * (count x n x n), A (count x m x n), c, l, u (count x n), AL, AU
+ * (count x m); n, m <= 2048. Same result as build_from_QAbc on every
+ * instance followed by hprqp_hip_batch_create (Q is symmetrised on the
+ * device with the same drop rule), without host models; INVALID_ARGUMENT if
+ * a row of A has no nonzero entry or no finite bound (build_from_QAbc would
+ * delete it). No counterpart in the reference: the entry for
+ * differentiable-layer callers that hold dense batched tensors. */
+HPRQPStatus hprqp_hip_batch_create_dense(size_t count, size_t n, size_t m, const double *Q, const double *c,
+                                         const double *A, const double *AL, const double *AU, const double *l,
+                                         const double *u, const HPRQPParams *params, HPRQPHipBatch **out_batch);
+/* All instances' solutions of a solved batch in original units, instances
+ * concatenated: x and z (sum of n), y (sum of m); per instance status,
+ * iterations, primal objective and KKT residual. Null outputs are skipped.
+ * No counterpart in the reference (bulk form of hprqp_hip_batch_result). */
+HPRQPStatus hprqp_hip_batch_solution(const HPRQPHipBatch *batch, double *x, double *y, double *z, int *status,
+                                     size_t *iterations, double *objective, double *kkt);
 /* Adjoint of a solved batch (implicit differentiation of each converged KKT
  * point, active set from the projections, the active-set KKT system solved
  * matrix-free by MINRES with the batch's SpMV kernels). dl_dx: upstream

@@ -1,5 +1,7 @@
 """Batched solver: every instance of a batch matches a plain solve of the
 same instance (status, objective, iterate) and the oracle."""
+import math
+
 import numpy as np
 import pytest
 
@@ -48,3 +50,94 @@ def test_batch_fixture_traces(H):
         assert Q.STATUS_CODE[r.status] == int(want[0]), (n, r.status)
         scale = max(1.0, float(np.max(np.abs(gold[f"{n}/100/x"]))))
         assert np.max(np.abs(r.x - gold[f"{n}/100/x"])) <= 1e-10 * scale, n
+
+
+def _layer_models(H, count, n, m_eq, m_in, seed, boxed_every=2):
+    Qd, c, A, AL, AU, l, u = Q.layer_instances(count, n=n, m_eq=m_eq, m_in=m_in, seed=seed)
+    models = []
+    for k in range(count):
+        lk, uk = l[k], u[k]
+        if k % boxed_every == 1:  # finite bounds: the general (not noC) iteration
+            lk, uk = np.full(n, -2.0), np.full(n, 2.0)
+        models.append(H.build_from_qabc(Qd[k], c[k], A[k], AL[k], AU[k], lk, uk))
+    return models
+
+
+def _batch_x(H, models, monkeypatch, dense, **params):
+    monkeypatch.setenv("HPRQP_BATCH_DENSE", "1" if dense else "0")
+    res, _ = H.solve_batch(models, H.Parameters(verbose=False, **params), bodies_per_launch=40)
+    return res
+
+
+@pytest.mark.parametrize("n,m_eq,m_in", [(48, 12, 40), (256, 64, 256)])
+def test_batch_dense_mode_matches_sparse_mode(H, monkeypatch, n, m_eq, m_in):
+    """Dense instances (column-form products over row-major Q, A and A^T)
+    against the CSR engine on the same models. At n = 48 every product row
+    is summed in the same sequential order in both modes; the modes differ
+    in which thread owns which row, hence in the order of the workgroup
+    reductions (dot products, norms), so iterates agree to rounding. At
+    n = 256 the CSR rows (>= 64 entries) are also reduced by a wavefront
+    (FMA and a tree)."""
+    models = _layer_models(H, 6, n, m_eq, m_in, seed=3)
+    print(f"dense vs sparse batch n={n}", flush=True)
+    for max_iter in (1, 37, 100):
+        rd = _batch_x(H, models, monkeypatch, True, max_iter=max_iter)
+        rs = _batch_x(H, models, monkeypatch, False, max_iter=max_iter)
+        for k, (a, b) in enumerate(zip(rd, rs)):
+            assert a.iter == b.iter and a.status == b.status, (k, max_iter)
+            for v in ("x", "y", "z"):
+                va, vb = getattr(a, v), getattr(b, v)
+                scale = max(1.0, float(np.max(np.abs(vb))))
+                assert np.max(np.abs(va - vb)) <= 1e-10 * scale, (k, v, max_iter, float(np.max(np.abs(va - vb))))
+
+
+def test_batch_dense_mode_full_solves_match_plain(H, monkeypatch):
+    models = _layer_models(H, 5, 96, 24, 96, seed=4)
+    res = _batch_x(H, models, monkeypatch, True, stoptol=1e-8)
+    for k, (model, r) in enumerate(zip(models, res)):
+        plain = H.optimize(model, H.Parameters(verbose=False, stoptol=1e-8))
+        assert r.status == plain.status == "OPTIMAL", (k, r.status, plain.status)
+        assert abs(r.objective - plain.objective) <= 1e-7 * max(1.0, abs(plain.objective)), k
+        assert r.residual <= 1e-8
+
+
+def test_batch_from_dense_equals_model_batch(H, monkeypatch):
+    """hprqp_hip_batch_create_dense (Q symmetrised on the device) against
+    build_from_QAbc models batched in dense mode: identical iterates. Q is
+    made unsymmetric, with entries at the 1e-15 drop tolerance, so the
+    symmetrisation and its drop rule are exercised."""
+    rng = np.random.default_rng(8)
+    count, n, m_eq, m_in = 5, 40, 10, 30
+    Qd, c, A, AL, AU, l, u = Q.layer_instances(count, n=n, m_eq=m_eq, m_in=m_in, seed=8)
+    Qd = Qd + np.triu(rng.normal(scale=1e-3, size=(count, n, n)), 1)
+    Qd[:, 0, 1] = 1.5e-15
+    Qd[:, 1, 0] = 1e-16
+    Qd[:, 2, 3] = 3e-15
+    Qd[:, 3, 2] = -3e-15 + 1e-30
+    l[1::2], u[1::2] = -2.0, 2.0
+    monkeypatch.setenv("HPRQP_BATCH_DENSE", "1")
+    models = [H.build_from_qabc(Qd[k], c[k], A[k], AL[k], AU[k], l[k], u[k]) for k in range(count)]
+    for max_iter in (1, 60):
+        params = H.Parameters(verbose=False, max_iter=max_iter)
+        bm = H.Batch(models, params)
+        bm.solve(25)
+        bd = H.Batch.from_dense(Qd, c, A, AL, AU, l, u, params)
+        bd.solve(25)
+        sm, sd = bm.solution(), bd.solution()
+        for key in ("x", "y", "z", "objective", "kkt"):
+            np.testing.assert_array_equal(sd[key], sm[key], err_msg=f"{key} {max_iter}")
+        assert sd["status"] == sm["status"] and (sd["iter"] == sm["iter"]).all()
+        for k in range(count):  # bulk getter agrees with the per-instance one
+            r = bm.result(k)
+            np.testing.assert_array_equal(r.x, sm["x"][k * n:(k + 1) * n])
+
+
+def test_batch_from_dense_rejects_rows_it_cannot_keep(H):
+    Qd, c, A, AL, AU, l, u = Q.layer_instances(2, n=8, m_eq=2, m_in=3, seed=1)
+    A[1, 2] = 0.0
+    with pytest.raises(ValueError):
+        H.Batch.from_dense(Qd, c, A, AL, AU, l, u)
+    A, AL, AU = Q.layer_instances(2, n=8, m_eq=2, m_in=3, seed=1)[2:5]
+    AL[0, 4], AU[0, 4] = -math.inf, math.inf
+    with pytest.raises(ValueError):
+        H.Batch.from_dense(Qd, c, A, AL, AU, l, u)
