@@ -1199,7 +1199,7 @@ __global__ __launch_bounds__(kThreads) void batch_setup_kernel(Arrays ar, Params
 
 // Runs up to `bodies` loop bodies of every unfinished instance.
 template <bool D>
-__global__ __launch_bounds__(kThreads) void batch_solve_kernel(Arrays ar, Params P, int count, int bodies)
+__global__ __launch_bounds__(kThreads, 4) void batch_solve_kernel(Arrays ar, Params P, int count, int bodies)
 {
     __shared__ double prod[kStreamCap];
     __shared__ double red[kMaxTerms];
