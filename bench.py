@@ -378,8 +378,17 @@ def bench_layer(H, Q, args):
     roofline = None
     if iter_sum and dev > 0:
         gbs = 8.0 * (n * n + 2 * m * n) * iter_sum / dev / 1e9
+        prof = ROOT / "profiles" / "batch_dense_layer_1024.md"
+        traffic = None
+        if prof.exists():
+            text = prof.read_text()
+            start = text.rfind("```json")
+            if start >= 0:
+                k = json.loads(text[start + 7:text.index("```", start + 7)])["kernels"]["batch_solve_kernel (dense)"]
+                traffic = k["traffic_bytes_per_instance_iteration"]
         roofline = {"bound": "hbm", "kernel": "batch_solve_kernel (dense)", "achieved": round(gbs, 1),
-                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+                    "traffic": traffic, "traffic_unit": "bytes per instance-iteration (profiles/batch_dense_layer_1024.md)",
                     "bytes_per_instance_iteration": 8 * (n * n + 2 * m * n), "instance_iterations": iter_sum}
     line = {"metric": "QP layer fwd+bwd ms (1 GPU)", "value": round((fwd + bwd) * 1e3, 2), "unit": "ms",
             "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": round((fwd + bwd) * 1e3, 2),
