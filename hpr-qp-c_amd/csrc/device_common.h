@@ -97,10 +97,11 @@ __device__ inline void block_reduce_store(const double *sums, const double *maxs
 //     with coalesced loads; each row's owner thread then sums them left to
 //     right without FMA, which reproduces the sequential row sum of the host
 //     CSR product bit for bit.
-//   * wave (rows longer than kLongRow, up to kThreads / kWave of them): one
-//     wavefront per row, FMA over a 64-lane stride, then a shuffle tree.
-//   * vector (one row longer than kVecRow): the whole workgroup strides the
-//     row with FMA and reduces.
+//   * wave (rows longer than kLongRow, up to kThreads / kWave of them):
+//     (kThreads / kWave) / rows wavefronts per row, each summing a 64-aligned
+//     chunk with FMA over a 64-lane stride and a shuffle tree; the chunks'
+//     sums are added in chunk order. A row longer than kVecRow gets a block
+//     (all four wavefronts) of its own.
 // Row bounds, carried partial sums and the epilogue's own row inputs are
 // loaded before the gathers, so they overlap the longest latency chain.
 //
@@ -117,7 +118,7 @@ __device__ inline void block_reduce_store(const double *sums, const double *maxs
 // ascending inside each row the left-to-right order, and hence the bits, are
 // those of the unsplit product; each panel gathers only a slice of x.
 
-constexpr int kVecRow = 4096;  // rows longer than this get a workgroup each
+constexpr int kVecRow = 4096;  // rows longer than this get a block of their own
 
 template <class E, class = void>
 struct RowInputs {
@@ -198,51 +199,36 @@ __global__ __launch_bounds__(kThreads) void spmv_rows_kernel(const int *__restri
     for (int i = 0; i < (NM > 0 ? NM : 1); ++i) maxs[i] = 0.0;
 
     if (cnt > kLongRow * nr) {
-        if (nr == 1 && cnt > kVecRow) {
-            // vector mode: 4 independent gathers in flight per thread, then a tree
-            Pre pre{};
-            double cin = 0.0;
-            if (t == 0) {
-                pre = RI::load(epi, r0);
-                if (CARRY_IN) cin = carry[r0];
+        // wave mode: W = 4 / nr wavefronts per row (rows split into W
+        // 64-aligned chunks whose partial sums are added in chunk order)
+        __shared__ double part_sum[kThreads / kWave];
+        const int lane = t & (kWave - 1), w = t / kWave;
+        const int W = (kThreads / kWave) / nr;
+        const int rl = w / W, part = w - rl * W;
+        const int r = r0 + rl;
+        Pre pre{};
+        double cin = 0.0, acc = 0.0;
+        if (rl < nr) {
+            if (lane == 0 && part == 0) {
+                pre = RI::load(epi, r);
+                if (CARRY_IN) cin = carry[r];
             }
-            double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-            int k = k0 + t;
-            for (; k + 3 * kThreads < k1; k += 4 * kThreads) {
-                const int c0 = __builtin_nontemporal_load(ci + k), c1 = __builtin_nontemporal_load(ci + k + kThreads);
-                const int c2 = __builtin_nontemporal_load(ci + k + 2 * kThreads);
-                const int c3 = __builtin_nontemporal_load(ci + k + 3 * kThreads);
-                const double v0 = __builtin_nontemporal_load(val + k);
-                const double v1 = __builtin_nontemporal_load(val + k + kThreads);
-                const double v2 = __builtin_nontemporal_load(val + k + 2 * kThreads);
-                const double v3 = __builtin_nontemporal_load(val + k + 3 * kThreads);
-                const double x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
-                a0 = fma(v0, x0, a0);
-                a1 = fma(v1, x1, a1);
-                a2 = fma(v2, x2, a2);
-                a3 = fma(v3, x3, a3);
+            const int rs = rp[r], re = rp[r + 1];
+            const int chunk = (((re - rs) + W - 1) / W + kWave - 1) & ~(kWave - 1);
+            const int ps = min(rs + part * chunk, re), pe = min(ps + chunk, re);
+            acc = wave_row_sum(ci, val, x, ps, pe, lane);
+        }
+        if (W > 1) {
+            if (lane == 0) part_sum[w] = acc;
+            __syncthreads();
+            if (lane == 0 && part == 0 && rl < nr) {
+                acc = part_sum[w];
+                for (int q = 1; q < W; ++q) acc += part_sum[w + q];
             }
-            for (; k < k1; k += kThreads)
-                a0 = fma(__builtin_nontemporal_load(val + k), x[__builtin_nontemporal_load(ci + k)], a0);
-            double acc = block_sum_all((a0 + a1) + (a2 + a3));
+        }
+        if (rl < nr && lane == 0 && part == 0) {
             if (CARRY_IN) acc = cin + acc;
-            if (t == 0) RI::finish(epi, r0, acc, pre, sums, maxs);
-        } else {
-            // wave mode: wavefront w owns row r0 + w
-            const int lane = t & (kWave - 1), w = t / kWave;
-            const int r = r0 + w;
-            if (r < r1) {
-                Pre pre{};
-                double cin = 0.0;
-                if (lane == 0) {
-                    pre = RI::load(epi, r);
-                    if (CARRY_IN) cin = carry[r];
-                }
-                const int rs = rp[r], re = rp[r + 1];
-                double acc = wave_row_sum(ci, val, x, rs, re, lane);
-                if (CARRY_IN) acc = cin + acc;
-                if (lane == 0) RI::finish(epi, r, acc, pre, sums, maxs);
-            }
+            RI::finish(epi, r, acc, pre, sums, maxs);
         }
     } else {
         const int r = r0 + t;
