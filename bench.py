@@ -328,6 +328,7 @@ def bench_batched(H, Q, args):
     setup = time.time() - t
     left = b.solve(100)
     secs = b.seconds
+    wave_row = b.wave_row
     res = [b.result(k) for k in range(len(models))]
     ok = sum(r.status == "OPTIMAL" for r in res)
     qps_s = len(models) / secs
@@ -346,7 +347,8 @@ def bench_batched(H, Q, args):
             "scaling": "strong", "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded generator)",
             "config": {"workload": f"batched_{len(models)}", "optimal": ok, "solve_s": secs, "setup_s": setup,
                        "sequential_plain_gpu_qps": round(seq_qps, 3), "speedup_vs_sequential": round(qps_s / seq_qps, 2),
-                       "sequential_sample": len(sample)}}
+                       "sequential_sample": len(sample), "wave_row_autotuned": wave_row,
+                       "chunk": "100 loop bodies per replay of a captured HIP graph"}}
     print(json.dumps(line), flush=True)
 
 

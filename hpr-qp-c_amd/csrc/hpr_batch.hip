@@ -82,6 +82,8 @@ struct Arrays {
     long long qt_perm_base;
     int *active;           // count of unfinished instances
     double *dval;          // dense instances' matrices (see Inst::dense)
+    const int *order;      // workgroup -> instance (largest first)
+    int wave_row;          // CSR rows with at least this many entries get a wavefront
 };
 
 struct Params {
@@ -105,10 +107,11 @@ namespace batch {
 // y = M x over the instance's row blocks, each row's value handed to the
 // epilogue (stream / long-row modes as in spmv_rows_kernel). Per-thread
 // reduction terms accumulate across blocks; the caller reduces them.
-// Rows of at least kWaveRow entries are grouped (up to 8 per block) and each
-// is reduced by one wavefront (lanes stride the row, FMA, shuffle tree); the
-// other rows go through the LDS stream mode with sequential row sums.
-constexpr int kWaveRow = 64;
+// Rows of at least `wave_row` entries (chosen per batch by a timed probe)
+// are grouped (up to 8 per block) and each is reduced by one wavefront (lanes
+// stride the row, FMA, shuffle tree); the other rows go through the LDS
+// stream mode with sequential row sums.
+constexpr int kWaveRowDefault = 64;
 
 // Matrix view of one instance. Sparse: CSR with row blocks. Dense (dv set):
 // a row-major K x J array S whose product is taken in column form,
@@ -122,6 +125,7 @@ struct Mat {
     int nblk;
     double *dv;
     int K, J, ld;
+    int wave_row;  // CSR blocks whose mean row length reaches this are wave blocks
 };
 
 // Largest dense dimension: x is staged whole in the LDS product buffer.
@@ -174,7 +178,7 @@ __device__ void wg_spmv(const Mat &M, const double *__restrict__ x, const Epi &e
             r0 = blk[b], r1 = blk[b + 1];
             k0 = rp[r0], k1 = rp[r1];
         }
-        if (!dense && k1 - k0 >= kWaveRow * (r1 - r0)) {
+        if (!dense && (long long)(k1 - k0) >= (long long)M.wave_row * (r1 - r0)) {
             for (int r = r0 + wid; r < r1; r += kThreads / kWave) {
                 const int e = rp[r + 1];
                 double a0 = 0.0, a1 = 0.0;
@@ -975,6 +979,7 @@ __device__ void wg_bind(WG &g, const Arrays &ar, const Params &P, int k, State *
               I->a_nblk, nullptr, 0, 0, 0};
     g.AT = Mat{ar.rp + I->at_rp, ar.ci + ar.at_ci_base + I->a_nz, ar.val + ar.at_ci_base + I->a_nz,
                ar.blocks + I->at_blk, I->at_nblk, nullptr, 0, 0, 0};
+    g.Q.wave_row = g.A.wave_row = g.AT.wave_row = ar.wave_row;
     if constexpr (WG::kDense) {
         // column form: Q over Q (symmetric), A x over A^T (n x m), A^T y over A (m x n)
         g.Q.dv = ar.dval + I->dq;
@@ -1084,8 +1089,9 @@ __global__ __launch_bounds__(kThreads) void batch_adjoint_kernel(Arrays ar, Para
     __shared__ double prod[kStreamCap];
     __shared__ double red[kMaxTerms];
     __shared__ State S;
-    const int k = blockIdx.x;
-    if (k >= count || ar.inst[k].dense != (int)D) return;
+    if ((int)blockIdx.x >= count) return;
+    const int k = ar.order[blockIdx.x];
+    if (ar.inst[k].dense != (int)D) return;
     if (threadIdx.x == 0) S = ar.state[k];
     __syncthreads();
     WGv<D> g;
@@ -1170,8 +1176,9 @@ __global__ __launch_bounds__(kThreads) void batch_setup_kernel(Arrays ar, Params
     __shared__ double prod[kStreamCap];
     __shared__ double red[kMaxTerms];
     __shared__ State S;
-    const int k = blockIdx.x;
-    if (k >= count || ar.inst[k].dense != (int)D) return;
+    if ((int)blockIdx.x >= count) return;
+    const int k = ar.order[blockIdx.x];
+    if (ar.inst[k].dense != (int)D) return;
     WGv<D> g;
     wg_bind(g, ar, P, k, &S, prod, red);
     if (threadIdx.x == 0) {
@@ -1204,8 +1211,9 @@ __global__ __launch_bounds__(kThreads, 4) void batch_solve_kernel(Arrays ar, Par
     __shared__ double prod[kStreamCap];
     __shared__ double red[kMaxTerms];
     __shared__ State S;
-    const int k = blockIdx.x;
-    if (k >= count || ar.inst[k].dense != (int)D) return;
+    if ((int)blockIdx.x >= count) return;
+    const int k = ar.order[blockIdx.x];
+    if (ar.inst[k].dense != (int)D) return;
     if (threadIdx.x == 0) S = ar.state[k];
     __syncthreads();
     if (S.finished) return;
@@ -1219,6 +1227,30 @@ __global__ __launch_bounds__(kThreads, 4) void batch_solve_kernel(Arrays ar, Par
         if (!S.finished) atomicAdd(ar.active, 1);
     }
 }
+
+// Timing probe for the CSR engine's wave threshold: the three products of an
+// iteration (Q w, A x, A^T y) of every sparse instance, into scratch vectors.
+__global__ __launch_bounds__(kThreads) void batch_probe_kernel(Arrays ar, Params P, int count)
+{
+    __shared__ double prod[kStreamCap];
+    __shared__ double red[kMaxTerms];
+    __shared__ State S;
+    if ((int)blockIdx.x >= count) return;
+    const int k = ar.order[blockIdx.x];
+    if (ar.inst[k].dense) return;
+    WGv<false> g;
+    wg_bind(g, ar, P, k, &S, prod, red);
+    double sums[kMaxTerms] = {0}, maxs[kMaxTerms] = {0};
+    wg_spmv<false>(g.Q, g.N(N_w), EpiStore{{}, g.N(N_pt)}, sums, maxs, g.prod);
+    if (g.I->m > 0) {
+        wg_spmv<false>(g.A, g.N(N_x), EpiStore{{}, g.M(M_pz)}, sums, maxs, g.prod);
+        wg_spmv<false>(g.AT, g.M(M_y), EpiStore{{}, g.N(N_pq)}, sums, maxs, g.prod);
+    }
+}
+
+// Clears the unfinished-instance counter (a kernel, so that it can sit in a
+// captured graph).
+__global__ void batch_zero_counter_kernel(int *counter) { *counter = 0; }
 
 // build.c:157-252 on dense row-major Q planes (one workgroup per instance
 // row): Q_sym(r, c) = h(r, c) + h(c, r) with h = Q/2 dropped when |h| <=
@@ -1252,6 +1284,11 @@ __global__ __launch_bounds__(kThreads) void dense_symmetrize_kernel(double *__re
 struct HPRQPHipBatch {
     size_t count = 0;
     bool any_dense = false, any_sparse = false;  // which kernel variants have work
+    std::vector<double> work;                    // per-instance size estimate (dispatch order)
+    int *d_order = nullptr;
+    size_t blocks_cap = 0;                       // ints allocated at d_blocks
+    hipGraphExec_t graph = nullptr;              // captured chunk (counter clear + solve launches)
+    size_t graph_bodies = 0;
     hprqp::batch::Arrays ar{};
     hprqp::batch::Params P{};
     std::vector<hprqp::batch::Inst> inst;
@@ -1259,6 +1296,7 @@ struct HPRQPHipBatch {
     hipStream_t stream = nullptr;
     double *d_vn = nullptr, *d_vm = nullptr, *d_val = nullptr, *d_dval = nullptr;
     int *d_rp = nullptr, *d_ci = nullptr, *d_perm = nullptr, *d_blocks = nullptr, *d_active = nullptr;
+    std::vector<int> h_rp;  // host copy of all CSR row pointers (block planning)
     hprqp::batch::Inst *d_inst = nullptr;
     hprqp::batch::State *d_state = nullptr;
     std::vector<hprqp::batch::State> state;
@@ -1268,6 +1306,8 @@ struct HPRQPHipBatch {
     ~HPRQPHipBatch()
     {
         if (stream) (void)hipStreamSynchronize(stream);
+        if (graph) (void)hipGraphExecDestroy(graph);
+        if (d_order) (void)hipFree(d_order);
         for (void *p : {(void *)d_vn, (void *)d_vm, (void *)d_val, (void *)d_dval, (void *)d_rp, (void *)d_ci, (void *)d_perm,
                         (void *)d_blocks, (void *)d_active, (void *)d_inst, (void *)d_state})
             if (p) (void)hipFree(p);
@@ -1277,13 +1317,12 @@ struct HPRQPHipBatch {
 
 namespace {
 
-// Row blocks for the workgroup engine: runs of rows with >= kWaveRow entries
+// Row blocks for the workgroup engine: runs of rows with >= wave_row entries
 // (up to 8 per block, one wavefront per row), otherwise stream blocks of
 // <= kThreads rows and <= kStreamCap entries.
-std::vector<int> plan_batch_blocks(const std::vector<int> &rp)
+std::vector<int> plan_batch_blocks(const int *rp, int nrow, int wave_row)
 {
-    using hprqp::batch::kWaveRow;
-    const int nrow = (int)rp.size() - 1;
+    const int kWaveRow = wave_row;
     std::vector<int> starts;
     int r = 0;
     while (r < nrow) {
@@ -1326,6 +1365,43 @@ void transpose_perm(size_t nrow, size_t ncol, const size_t *rp, const size_t *ci
         }
 }
 
+// Row blocks of every CSR instance for wave threshold `wave_row`; sets the
+// instances' block offsets and returns the concatenated block list.
+std::vector<int> plan_all_blocks(HPRQPHipBatch *b, int wave_row)
+{
+    std::vector<int> blocks;
+    auto push = [&](const std::vector<int> &blk, int &off, int &cnt) {
+        off = (int)blocks.size();
+        cnt = (int)blk.size() - 1;
+        blocks.insert(blocks.end(), blk.begin(), blk.end());
+    };
+    for (size_t k = 0; k < b->count; ++k) {
+        hprqp::batch::Inst &I = b->inst[k];
+        if (I.dense) continue;
+        const int *rp = b->h_rp.data();
+        push(plan_batch_blocks(rp + I.q_rp, I.n, wave_row), I.q_blk, I.q_nblk);
+        push(plan_batch_blocks(rp + I.a_rp, I.m, wave_row), I.a_blk, I.a_nblk);
+        push(plan_batch_blocks(rp + I.at_rp, I.n, wave_row), I.at_blk, I.at_nblk);
+    }
+    if (blocks.empty()) blocks.push_back(0);
+    return blocks;
+}
+
+// Uploads a block plan and the instance table that points into it.
+bool upload_plan(HPRQPHipBatch *b, const std::vector<int> &blocks)
+{
+    if (blocks.size() > b->blocks_cap) {
+        if (b->d_blocks) (void)hipFree(b->d_blocks);
+        b->d_blocks = nullptr;
+        if (hipMalloc((void **)&b->d_blocks, sizeof(int) * blocks.size()) != hipSuccess) return false;
+        b->blocks_cap = blocks.size();
+        b->ar.blocks = b->d_blocks;
+    }
+    return hipMemcpy(b->d_blocks, blocks.data(), sizeof(int) * blocks.size(), hipMemcpyHostToDevice) == hipSuccess &&
+           hipMemcpy(b->d_inst, b->inst.data(), sizeof(hprqp::batch::Inst) * b->count, hipMemcpyHostToDevice) ==
+               hipSuccess;
+}
+
 }  // namespace
 
 namespace {
@@ -1337,9 +1413,15 @@ HPRQPStatus batch_finish(HPRQPHipBatch *b)
 {
     using namespace hprqp::batch;
     const size_t count = b->count;
+    // largest instances first, so the short ones fill the tail
+    std::vector<int> order(count);
+    for (size_t k = 0; k < count; ++k) order[k] = (int)k;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return b->work[a] > b->work[c]; });
     HPRQPStatus st = HPRQP_STATUS_OK;
     if (hipMalloc((void **)&b->d_inst, sizeof(Inst) * count) != hipSuccess ||
         hipMemcpy(b->d_inst, b->inst.data(), sizeof(Inst) * count, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMalloc((void **)&b->d_order, sizeof(int) * count) != hipSuccess ||
+        hipMemcpy(b->d_order, order.data(), sizeof(int) * count, hipMemcpyHostToDevice) != hipSuccess ||
         hipMalloc((void **)&b->d_state, sizeof(State) * count) != hipSuccess ||
         hipMalloc((void **)&b->d_active, sizeof(int)) != hipSuccess ||
         (!b->stream && hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess))
@@ -1352,12 +1434,55 @@ HPRQPStatus batch_finish(HPRQPHipBatch *b)
     ar.inst = b->d_inst;
     ar.state = b->d_state;
     ar.active = b->d_active;
+    ar.order = b->d_order;
+    ar.wave_row = kWaveRowDefault;
     for (size_t k = 0; k < count; ++k) (b->inst[k].dense ? b->any_dense : b->any_sparse) = true;
     if (b->any_sparse) batch_setup_kernel<false><<<(unsigned)count, hprqp::kThreads, 0, b->stream>>>(ar, b->P, (int)count);
     if (b->any_dense) batch_setup_kernel<true><<<(unsigned)count, hprqp::kThreads, 0, b->stream>>>(ar, b->P, (int)count);
     if (hipGetLastError() != hipSuccess || hipStreamSynchronize(b->stream) != hipSuccess) {
         delete b;
         return HPRQP_STATUS_NOT_IMPLEMENTED;
+    }
+    if (b->any_sparse) {
+        // Wave threshold for the CSR engine: HPRQP_BATCH_WAVE_ROW, or the
+        // fastest of a few candidates on a timed probe of one iteration's
+        // products (the plan changes only the order of long-row sums).
+        const char *force = std::getenv("HPRQP_BATCH_WAVE_ROW");
+        std::vector<int> cands = {16, 32, 64, 128, 1 << 20};
+        if (force && std::atoi(force) > 0) cands = {std::atoi(force)};
+        int best = kWaveRowDefault;
+        float best_ms = 1e30f;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        bool ok = hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess;
+        for (size_t c = 0; ok && c < cands.size(); ++c) {
+            const int T = cands[c];
+            ok = upload_plan(b, plan_all_blocks(b, T));
+            if (!ok || cands.size() == 1) {
+                best = T;
+                break;
+            }
+            ar.wave_row = T;
+            float ms_min = 1e30f;
+            for (int rep = 0; ok && rep < 4; ++rep) {
+                ok = hipEventRecord(e0, b->stream) == hipSuccess;
+                batch_probe_kernel<<<(unsigned)count, hprqp::kThreads, 0, b->stream>>>(ar, b->P, (int)count);
+                ok = ok && hipEventRecord(e1, b->stream) == hipSuccess && hipEventSynchronize(e1) == hipSuccess;
+                float ms = 0.0f;
+                ok = ok && hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
+                if (rep > 0) ms_min = std::min(ms_min, ms);  // first launch warms up
+            }
+            if (ms_min < best_ms) {
+                best_ms = ms_min;
+                best = T;
+            }
+        }
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        ar.wave_row = best;
+        if (!ok || !upload_plan(b, plan_all_blocks(b, best)) || hipGetLastError() != hipSuccess) {
+            delete b;
+            return HPRQP_STATUS_NOT_IMPLEMENTED;
+        }
     }
     return HPRQP_STATUS_OK;
 }
@@ -1441,7 +1566,6 @@ HPRQPStatus hprqp_hip_batch_create(const HPRQPModel *models, size_t count, const
     std::vector<int> perm((size_t)std::max(sum_a + sum_q, 1LL));
     std::vector<double> val((size_t)std::max(sum_q + 2 * sum_a, 1LL)), dval((size_t)std::max(sum_d, 1LL));
     std::vector<double> vn((size_t)sum_n * N_COUNT), vm((size_t)std::max(sum_m, 1LL) * M_COUNT);
-    std::vector<std::vector<int>> blk_q(count), blk_a(count), blk_at(count);
     auto pack = [&](size_t k) {
         const HPRQPModel &md = models[k];
         Inst &I = b->inst[k];
@@ -1492,13 +1616,9 @@ HPRQPStatus hprqp_hip_batch_create(const HPRQPModel *models, size_t count, const
                 ci[at_base + I.a_nz + e] = tci[e];
                 perm[I.a_nz + e] = tperm[e];
             }
-            std::vector<int> atrp(trp);
             transpose_perm(n, n, md.Q.row_ptr, md.Q.col_ind, trp, tci, tperm);
             std::copy(trp.begin(), trp.end(), rp.begin() + I.qt_rp);
             for (size_t e = 0; e < md.Q.nnz; ++e) perm[sum_a + I.q_nz + e] = tperm[e];
-            blk_q[k] = plan_batch_blocks(qrp);
-            blk_a[k] = plan_batch_blocks(arp);
-            blk_at[k] = plan_batch_blocks(atrp);
         }
         for (size_t i = 0; i < n; ++i) {
             vn[(size_t)N_c * sum_n + I.vn + i] = md.c[i];
@@ -1521,22 +1641,13 @@ HPRQPStatus hprqp_hip_batch_create(const HPRQPModel *models, size_t count, const
         for (size_t k; (k = next.fetch_add(1)) < count;) pack(k);
         for (std::thread &th : pool) th.join();
     }
-    std::vector<int> blocks;
     for (size_t k = 0; k < count; ++k) {
-        Inst &I = b->inst[k];
         b->n.push_back(models[k].Q.ncol);
         b->m.push_back(models[k].A.nrow);
-        if (I.dense) continue;
-        auto push_blocks = [&](const std::vector<int> &blk, int &off, int &cnt) {
-            off = (int)blocks.size();
-            cnt = (int)blk.size() - 1;
-            blocks.insert(blocks.end(), blk.begin(), blk.end());
-        };
-        push_blocks(blk_q[k], I.q_blk, I.q_nblk);
-        push_blocks(blk_a[k], I.a_blk, I.a_nblk);
-        push_blocks(blk_at[k], I.at_blk, I.at_nblk);
+        b->work.push_back((double)(models[k].Q.nnz + 2 * models[k].A.nnz + models[k].Q.ncol + models[k].A.nrow));
     }
-    if (blocks.empty()) blocks.push_back(0);
+    b->h_rp = rp;
+    std::vector<int> blocks = plan_all_blocks(b, kWaveRowDefault);
     HPRQPStatus st = HPRQP_STATUS_OK;
     auto upload = [&](auto *&dst, const auto &src) -> bool {
         using T = typename std::remove_reference<decltype(src)>::type::value_type;
@@ -1549,6 +1660,7 @@ HPRQPStatus hprqp_hip_batch_create(const HPRQPModel *models, size_t count, const
     if (!upload(b->d_rp, rp) || !upload(b->d_ci, ci) || !upload(b->d_val, val) || !upload(b->d_perm, perm) ||
         !upload(b->d_blocks, blocks) || !upload(b->d_vn, vn) || !upload(b->d_vm, vm) || !upload(b->d_dval, dval))
         st = HPRQP_STATUS_ALLOCATION_FAILED;
+    b->blocks_cap = blocks.size();
     if (st != HPRQP_STATUS_OK) {
         delete b;
         return st;
@@ -1578,15 +1690,40 @@ HPRQPStatus hprqp_hip_batch_solve(HPRQPHipBatch *b, size_t bodies_per_launch, si
     using namespace hprqp::batch;
     if (!b || bodies_per_launch == 0) return HPRQP_STATUS_INVALID_ARGUMENT;
     const double t0 = hprqp::wall_seconds();
-    int active = 1;
-    while (active > 0) {
-        HPRQP_HIP(hipMemsetAsync(b->d_active, 0, sizeof(int), b->stream));
+    // One chunk = clear the counter, run `bodies_per_launch` loop bodies of
+    // every unfinished instance (CSR and dense kernels). It is captured once
+    // into a graph and replayed until no instance is left (HPRQP_BATCH_GRAPH=0
+    // launches it eagerly instead).
+    const char *graph_env = std::getenv("HPRQP_BATCH_GRAPH");
+    const bool use_graph = !(graph_env && std::atoi(graph_env) == 0);
+    auto enqueue_chunk = [&]() {
+        batch_zero_counter_kernel<<<1, 1, 0, b->stream>>>(b->d_active);
         if (b->any_sparse)
             batch_solve_kernel<false><<<(unsigned)b->count, hprqp::kThreads, 0, b->stream>>>(
                 b->ar, b->P, (int)b->count, (int)bodies_per_launch);
         if (b->any_dense)
             batch_solve_kernel<true><<<(unsigned)b->count, hprqp::kThreads, 0, b->stream>>>(
                 b->ar, b->P, (int)b->count, (int)bodies_per_launch);
+    };
+    if (use_graph && (!b->graph || b->graph_bodies != bodies_per_launch)) {
+        if (b->graph) HPRQP_HIP(hipGraphExecDestroy(b->graph));
+        b->graph = nullptr;
+        hipGraph_t g = nullptr;
+        HPRQP_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        enqueue_chunk();
+        HPRQP_HIP(hipStreamEndCapture(b->stream, &g));
+        const hipError_t ie = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        HPRQP_HIP(ie);
+        b->graph_bodies = bodies_per_launch;
+    }
+    int active = 1;
+    while (active > 0) {
+        if (use_graph) {
+            HPRQP_HIP(hipGraphLaunch(b->graph, b->stream));
+        } else {
+            enqueue_chunk();
+        }
         HPRQP_HIP(hipGetLastError());
         HPRQP_HIP(hipMemcpyAsync(&active, b->d_active, sizeof(int), hipMemcpyDeviceToHost, b->stream));
         HPRQP_HIP(hipStreamSynchronize(b->stream));
@@ -1601,6 +1738,8 @@ HPRQPStatus hprqp_hip_batch_solve(HPRQPHipBatch *b, size_t bodies_per_launch, si
 }
 
 double hprqp_hip_batch_seconds(const HPRQPHipBatch *b) { return b ? b->solve_seconds : 0.0; }
+
+int hprqp_hip_batch_wave_row(const HPRQPHipBatch *b) { return b ? b->ar.wave_row : 0; }
 
 // Instance k's results in original units (status TIME_LIMIT if it did not
 // finish before the limit).
@@ -1730,7 +1869,9 @@ HPRQPStatus hprqp_hip_batch_create_dense(size_t count, size_t n, size_t m, const
         I.norm_c_org = nc;
         b->n.push_back(n);
         b->m.push_back(m);
+        b->work.push_back((double)(n * n + 2 * m * n));
     }
+    b->blocks_cap = 1;
     HPRQPStatus st = HPRQP_STATUS_OK;
     const size_t dsz = (size_t)(C * N * N + 2 * C * M * N);
     auto fail = [&]() {

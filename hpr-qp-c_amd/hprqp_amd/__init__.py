@@ -332,6 +332,8 @@ _LIB.hprqp_hip_batch_create.argtypes = [C.POINTER(CModel), C.c_size_t, C.POINTER
 _LIB.hprqp_hip_batch_solve.argtypes = [C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]
 _LIB.hprqp_hip_batch_seconds.argtypes = [C.c_void_p]
 _LIB.hprqp_hip_batch_seconds.restype = C.c_double
+_LIB.hprqp_hip_batch_wave_row.argtypes = [C.c_void_p]
+_LIB.hprqp_hip_batch_wave_row.restype = C.c_int
 _LIB.hprqp_hip_batch_result.argtypes = [C.c_void_p, C.c_size_t, C.POINTER(CResults)]
 _LIB.hprqp_hip_batch_destroy.argtypes = [C.c_void_p]
 
@@ -404,6 +406,11 @@ class Batch:
     @property
     def seconds(self) -> float:
         return _LIB.hprqp_hip_batch_seconds(self.h)
+
+    @property
+    def wave_row(self) -> int:
+        """Row length from which the CSR engine uses a wavefront per row."""
+        return _LIB.hprqp_hip_batch_wave_row(self.h)
 
     def result(self, k: int) -> Result:
         r = CResults()

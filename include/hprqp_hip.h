@@ -77,6 +77,14 @@ HPRQPStatus hprqp_hip_batch_create(const HPRQPModel *models, size_t count, const
                                    HPRQPHipBatch **out_batch);
 HPRQPStatus hprqp_hip_batch_solve(HPRQPHipBatch *batch, size_t bodies_per_launch, size_t *remaining);
 double hprqp_hip_batch_seconds(const HPRQPHipBatch *batch);
+/* Row length from which the batch's CSR engine reduces a row with a whole
+ * wavefront (below it: LDS-staged sequential row sums). Chosen at creation by
+ * timing a probe of one iteration's products over 16, 32, 64, 128 and
+ * "never", or fixed by the environment variable HPRQP_BATCH_WAVE_ROW; the
+ * choice changes only the summation order of long rows. No counterpart in
+ * the reference (its SpMV-mode autotune, src/gpu/iteration.cu:1287-1382,
+ * picks between whole SpMV implementations). */
+int hprqp_hip_batch_wave_row(const HPRQPHipBatch *batch);
 HPRQPStatus hprqp_hip_batch_result(const HPRQPHipBatch *batch, size_t index, HPRQPResults *results);
 void hprqp_hip_batch_destroy(HPRQPHipBatch *batch);
 /* Batch of `count` QPs of one shape from dense row-major host arrays: Q

@@ -141,3 +141,52 @@ def test_batch_from_dense_rejects_rows_it_cannot_keep(H):
     AL[0, 4], AU[0, 4] = -math.inf, math.inf
     with pytest.raises(ValueError):
         H.Batch.from_dense(Qd, c, A, AL, AU, l, u)
+
+
+def test_batch_graph_replays_equal_eager_launches(H, monkeypatch):
+    """The captured chunk graph, replayed many times (5 bodies per replay,
+    host synchronisation and counter read-back between replays), gives the
+    same bits as eager launches of the same kernels."""
+    qps = Q.batched_instances(count=12, seed=21, n_lo=30, n_hi=300)
+    models = [build(H, qp) for qp in qps]
+    monkeypatch.setenv("HPRQP_BATCH_WAVE_ROW", "32")
+    out = {}
+    for graph in ("1", "0"):
+        monkeypatch.setenv("HPRQP_BATCH_GRAPH", graph)
+        print("graph", graph, flush=True)
+        b = H.Batch(models, H.Parameters(verbose=False, stoptol=1e-6))
+        assert b.wave_row == 32
+        b.solve(5)
+        out[graph] = b.solution()
+    for key in ("x", "y", "z", "objective", "iter"):
+        np.testing.assert_array_equal(out["1"][key], out["0"][key], err_msg=key)
+    assert out["1"]["status"] == out["0"]["status"] == ["OPTIMAL"] * len(models)
+
+
+@pytest.mark.parametrize("wave_row", ["16", "1048576"])
+def test_batch_wave_threshold_choices_agree(H, monkeypatch, wave_row):
+    """Every wave threshold the autotuner may pick solves the instances to
+    the same optimum (rows of >= 16 entries reduced by a wavefront, or never)."""
+    qps = Q.batched_instances(count=10, seed=5, n_lo=40, n_hi=600)
+    models = [build(H, qp) for qp in qps]
+    monkeypatch.setenv("HPRQP_BATCH_WAVE_ROW", wave_row)
+    b = H.Batch(models, H.Parameters(verbose=False, stoptol=1e-8))
+    assert b.wave_row == int(wave_row)
+    b.solve(50)
+    sol = b.solution()
+    on = 0
+    for k, model in enumerate(models):
+        plain = H.optimize(model, H.Parameters(verbose=False, stoptol=1e-8))
+        n = model.num_variables()
+        assert sol["status"][k] == plain.status == "OPTIMAL"
+        assert abs(sol["objective"][k] - plain.objective) <= 1e-7 * max(1.0, abs(plain.objective)), k
+        assert np.max(np.abs(sol["x"][on:on + n] - plain.x)) <= 1e-5 * max(1.0, np.max(np.abs(plain.x))), k
+        on += n
+
+
+def test_batch_autotuned_wave_threshold_is_a_candidate(H, monkeypatch):
+    monkeypatch.delenv("HPRQP_BATCH_WAVE_ROW", raising=False)
+    qps = Q.batched_instances(count=16, seed=2, n_lo=50, n_hi=800)
+    b = H.Batch([build(H, qp) for qp in qps], H.Parameters(verbose=False, stoptol=1e-6))
+    assert b.wave_row in (16, 32, 64, 128, 1 << 20)
+    assert b.solve(100) == 0
