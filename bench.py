@@ -80,7 +80,7 @@ def build_model(H, qp):
 
 def profile_kernels(H, model, iters):
     s = H.Session(model, H.Parameters(verbose=False))
-    cap = 8
+    cap = 16
     names = C.create_string_buffer(32 * cap)
     ms = np.zeros(cap)
     by = np.zeros(cap)
@@ -311,6 +311,21 @@ def bench_single(H, Q, args, qp, workload, soc=False):
     print(json.dumps(line), flush=True)
 
 
+def cpu_parallel_solves(Q, qps, stoptol, workers):
+    """The oracle (plain-C restatement of the reference CPU path) solving
+    `qps` independently on `workers` host threads (the C calls release the
+    GIL): returns wall seconds and per-instance statuses."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    def one(qp):
+        o = Q.OracleSession(qp, stoptol=stoptol)
+        return Q.STATUS[o.solve()[0].status]
+    t = time.time()
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        status = list(ex.map(one, qps))
+    return time.time() - t, status
+
+
 def bench_batched(H, Q, args):
     """Batched workload: args.batch_count QPs with n ~ U{50..2000} in one
     persistent launch (stoptol 1e-6), against sequential plain GPU solves of a
@@ -342,13 +357,23 @@ def bench_batched(H, Q, args):
     seq = time.time() - t
     seq_qps = len(sample) / seq
     log(f"sequential plain GPU solves: {len(sample)} in {seq:.2f}s -> {seq_qps:.1f} QPs/s")
+    cpu = None
+    if not args.no_cpu:
+        workers = max(1, min(16, os.cpu_count() or 1))
+        csample = list(range(0, len(qps), max(1, len(qps) // (4 * workers))))[:4 * workers]
+        wall, status = cpu_parallel_solves(Q, [qps[k] for k in csample], 1e-6, workers)
+        cpu = dict(value=round(len(csample) / wall, 2), unit="QPs/s", cores=workers, kind="port",
+                   sample=f"oracle solves of {len(csample)} of the 4096 instances (evenly spaced), {workers} "
+                          f"concurrent host threads, {wall:.2f} s wall, {status.count('OPTIMAL')} optimal")
+        log("cpu_baseline:", cpu)
     line = {"metric": "batched QPs/s at 1e-6 (1 GPU)", "value": round(qps_s, 2), "unit": "QPs/s", "n_gpus": 1,
             "steps": 1, "warmup": 1, "ms_per_step": round(secs * 1e3, 2), "higher_is_better": True,
             "scaling": "strong", "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded generator)",
             "config": {"workload": f"batched_{len(models)}", "optimal": ok, "solve_s": secs, "setup_s": setup,
                        "sequential_plain_gpu_qps": round(seq_qps, 3), "speedup_vs_sequential": round(qps_s / seq_qps, 2),
                        "sequential_sample": len(sample), "wave_row_autotuned": wave_row,
-                       "chunk": "100 loop bodies per replay of a captured HIP graph"}}
+                       "chunk": "100 loop bodies per replay of a captured HIP graph"},
+            "cpu_baseline": cpu}
     print(json.dumps(line), flush=True)
 
 
@@ -392,6 +417,18 @@ def bench_layer(H, Q, args):
                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
                     "traffic": traffic, "traffic_unit": "bytes per instance-iteration (profiles/batch_dense_layer_1024.md)",
                     "bytes_per_instance_iteration": 8 * (n * n + 2 * m * n), "instance_iterations": iter_sum}
+    cpu = None
+    if not args.no_cpu:
+        workers = max(1, min(16, os.cpu_count() or 1))
+        csample = list(range(0, count, max(1, count // workers)))[:workers]
+        lqps = [Q.qp_from_dense(Qd[k], A[k], c[k], AL[k], AU[k], l[k], u[k]) for k in csample]
+        wall, status = cpu_parallel_solves(Q, lqps, 1e-8, workers)
+        cpu = dict(value=round(wall * count / len(csample) * 1e3, 1), unit="ms (forward only)",
+                   cores=workers, kind="port",
+                   sample=f"oracle forward solves (to 1e-8) of {len(csample)} of the {count} instances on {workers} "
+                          f"concurrent host threads: {wall:.2f} s wall, {status.count('OPTIMAL')} optimal; value = "
+                          f"wall x {count} / {len(csample)} (the all-cores time for the whole batch; no CPU adjoint)")
+        log("cpu_baseline:", cpu)
     line = {"metric": "QP layer fwd+bwd ms (1 GPU)", "value": round((fwd + bwd) * 1e3, 2), "unit": "ms",
             "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": round((fwd + bwd) * 1e3, 2),
             "higher_is_better": False, "scaling": "strong", "vs_baseline": None, "dtype": "f64",
@@ -400,7 +437,7 @@ def bench_layer(H, Q, args):
                        "backward_over_forward": bwd / fwd,
                        "split_ms": {k: (round(v * 1e3, 2) if not k.startswith("iterations") else v)
                                     for k, v in split.items()}},
-            "roofline": roofline}
+            "roofline": roofline, "cpu_baseline": cpu}
     print(json.dumps(line), flush=True)
 
 

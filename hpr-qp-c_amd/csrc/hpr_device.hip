@@ -8,6 +8,8 @@
 // that row); reductions are two-level and deterministic, and one pinned
 // read-back per residual / M-norm check is the only host synchronisation.
 #include <hip/hip_runtime.h>
+
+#include <cstddef>
 #include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
@@ -380,14 +382,28 @@ struct EpiSocX : NoRed {
     }
 };
 
-__global__ void soc_cone_x_kernel(int count, const int *__restrict__ off, SocFinishX fin, double *__restrict__ x_bar)
+// Cone blocks (one thread per cone; small workgroups so that the few
+// thousand cones spread over every CU): x_bar = projection of the raw point.
+// The elementwise finish of the cone range runs as a coalesced map after it.
+constexpr int kConeThreads = 64;
+
+__global__ __launch_bounds__(kConeThreads) void soc_cone_project_kernel(int count, const int *__restrict__ off,
+                                                                        const double *__restrict__ in,
+                                                                        double *__restrict__ out)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= count) return;
     const int b = off[k], e = off[k + 1];
-    soc_project_block(x_bar + b, fin.tempv + b, e - b);
-    for (int i = b; i < e; ++i) fin.apply(i, x_bar[i]);
+    soc_project_block(out + b, in + b, e - b);
 }
+
+// soc_iteration.cuh:14-26 on the cone range [base, base + len) of x.
+struct MapSocFinishX : NoRed {
+    SocFinishX fin;
+    const double *x_bar;
+    int base;
+    __device__ void row(int i, double *, double *) const { fin.apply(base + i, x_bar[base + i]); }
+};
 
 // soc_iteration.cuh:28-41 for one row
 struct SocFinishY {
@@ -420,14 +436,12 @@ struct EpiSocY : NoRed {
     }
 };
 
-__global__ void soc_cone_y_kernel(int count, const int *__restrict__ off, SocFinishY fin, double *__restrict__ s)
-{
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= count) return;
-    const int b = off[k], e = off[k + 1];
-    soc_project_block(s + b, fin.Rp + b, e - b);
-    for (int i = b; i < e; ++i) fin.apply(i);
-}
+// soc_iteration.cuh:28-41 on the cone range [base, base + len) of y.
+struct MapSocFinishY : NoRed {
+    SocFinishY fin;
+    int base;
+    __device__ void row(int i, double *, double *) const { fin.apply(base + i); }
+};
 
 // Projection of a mixed vector (interval prefix then cones) for residuals.
 __global__ void soc_project_mixed_kernel(int len, int lin, const double *__restrict__ in, const double *__restrict__ lo,
@@ -581,58 +595,104 @@ __global__ void fill_kernel(int n, double *__restrict__ v, double value)
     if (i < n) v[i] = value;
 }
 
-// control.c:79-114 on the device for diagonal Q: the whole golden-section
-// search runs in one 1024-thread workgroup; each probe is a deterministic
-// workgroup reduction of sum_i c_i d_i / (1 + x q_i).
-constexpr int kGoldenThreads = 1024;
+// control.c:79-114 on the device for diagonal Q. Each probe of the
+// golden-section search is a sum over all n variables, so it is spread over
+// the GPU: a partial-sum kernel (grid-stride, one partial per workgroup)
+// followed by a one-workgroup step kernel that adds the partials in index
+// order, evaluates f(x) = a x + b / x + x^2 sum_i c_i d_i / (1 + x q_i) and
+// advances the search state on the device. The host enqueues the largest
+// number of probes the search can take; once it has converged the remaining
+// launches return at once.
+struct GoldenState {
+    double a, b, lo, hi, x1, x2, f1, f2, xnew, result;
+    int phase;  // 0: evaluating x1, 1: evaluating x2, 2: evaluating a search step's new point
+    int pending;  // which of x1 (1) / x2 (2) the new point replaces
+    int it, done;
+};
+constexpr int kGoldenMaxIter = 200;
+constexpr int kGoldenBlocks = 256;
 
-__device__ double golden_diag_eval(double x, double a, double b, const double *__restrict__ dq,
-                                   const double *__restrict__ c, const double *__restrict__ d, int n, double *lds)
+__global__ void golden_init_kernel(GoldenState *st, double a, double b)
 {
+    if (threadIdx.x != 0) return;
+    const double phi = (sqrt(5.0) - 1.0) / 2.0;
+    GoldenState g{};
+    g.a = a;
+    g.b = b;
+    g.lo = 1e-12;
+    g.hi = 1e12;
+    g.x1 = g.hi - phi * (g.hi - g.lo);
+    g.x2 = g.lo + phi * (g.hi - g.lo);
+    g.phase = 0;
+    g.xnew = g.x1;
+    *st = g;
+}
+
+__global__ __launch_bounds__(kThreads) void golden_partials_kernel(const GoldenState *__restrict__ st,
+                                                                   const double *__restrict__ dq,
+                                                                   const double *__restrict__ c,
+                                                                   const double *__restrict__ d, int n,
+                                                                   double *__restrict__ partials)
+{
+    if (st->done) return;
+    const double x = st->xnew;
     double acc = 0.0;
-    for (int i = threadIdx.x; i < n; i += kGoldenThreads) {
+    for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
         const double t = d[i] / (1.0 + x * dq[i]);
         acc += c[i] * t;
     }
-    acc = wave_sum(acc);
-    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
-    __syncthreads();
-    if (lane == 0) lds[wid] = acc;
-    __syncthreads();
-    double total = lds[0];
-    for (int w = 1; w < kGoldenThreads / kWave; ++w) total += lds[w];
-    return a * x + b / x + x * x * total;
+    block_reduce_store<1, 0>(&acc, nullptr, partials + blockIdx.x);
 }
 
-__global__ __launch_bounds__(kGoldenThreads) void golden_diag_kernel(double a, double b, const double *__restrict__ dq,
-                                                                     const double *__restrict__ c,
-                                                                     const double *__restrict__ d, int n,
-                                                                     double *__restrict__ out)
+__global__ __launch_bounds__(kWave) void golden_step_kernel(GoldenState *__restrict__ st,
+                                                             const double *__restrict__ partials, int nblk)
 {
-    __shared__ double lds[kGoldenThreads / kWave];
-    double lo = 1e-12, hi = 1e12;
+    if (st->done) return;
+    // one wavefront adds the partials: lane-strided sums, then a fixed tree
+    double part = 0.0;
+    for (int k = threadIdx.x; k < nblk; k += kWave) part += partials[k];
+    const double total = wave_sum(part);
+    if (threadIdx.x != 0) return;
+    GoldenState g = *st;
+    const double x = g.xnew;
+    const double f = g.a * x + g.b / x + x * x * total;
     const double phi = (sqrt(5.0) - 1.0) / 2.0;
-    double x1 = hi - phi * (hi - lo), x2 = lo + phi * (hi - lo);
-    double f1 = golden_diag_eval(x1, a, b, dq, c, d, n, lds);
-    double f2 = golden_diag_eval(x2, a, b, dq, c, d, n, lds);
-    int it = 0;
-    while (fabs(hi - lo) > 1e-12 * fmax(1.0, fabs(lo)) && it < 200) {
-        if (f1 > f2) {
-            lo = x1;
-            x1 = x2;
-            f1 = f2;
-            x2 = lo + phi * (hi - lo);
-            f2 = golden_diag_eval(x2, a, b, dq, c, d, n, lds);
-        } else {
-            hi = x2;
-            x2 = x1;
-            f2 = f1;
-            x1 = hi - phi * (hi - lo);
-            f1 = golden_diag_eval(x1, a, b, dq, c, d, n, lds);
-        }
-        ++it;
+    if (g.phase == 0) {
+        g.f1 = f;
+        g.phase = 1;
+        g.xnew = g.x2;
+        *st = g;
+        return;
     }
-    if (threadIdx.x == 0) out[0] = f1 < f2 ? x1 : x2;
+    if (g.phase == 1) {
+        g.f2 = f;
+    } else {
+        if (g.pending == 1) g.f1 = f;
+        else g.f2 = f;
+        g.it += 1;
+    }
+    g.phase = 2;
+    if (fabs(g.hi - g.lo) > 1e-12 * fmax(1.0, fabs(g.lo)) && g.it < kGoldenMaxIter) {
+        if (g.f1 > g.f2) {
+            g.lo = g.x1;
+            g.x1 = g.x2;
+            g.f1 = g.f2;
+            g.x2 = g.lo + phi * (g.hi - g.lo);
+            g.xnew = g.x2;
+            g.pending = 2;
+        } else {
+            g.hi = g.x2;
+            g.x2 = g.x1;
+            g.f2 = g.f1;
+            g.x1 = g.hi - phi * (g.hi - g.lo);
+            g.xnew = g.x1;
+            g.pending = 1;
+        }
+    } else {
+        g.done = 1;
+        g.result = g.f1 < g.f2 ? g.x1 : g.x2;
+    }
+    *st = g;
 }
 
 // ------------------------------------------------------------ the backend
@@ -689,12 +749,12 @@ public:
 
     // Per-kernel timing of the iteration kernels with HIP events recorded on
     // this backend's own stream (profile()).
-    enum Tag { T_ZXW1, T_TEMPV, T_Y, T_W2, T_ZXW1_NOC, T_QW_NOC, T_COUNT };
+    enum Tag { T_ZXW1, T_TEMPV, T_Y, T_W2, T_ZXW1_NOC, T_QW_NOC, T_CONE_X, T_CONE_Y, T_COUNT };
     HPRQPStatus profile(size_t iterations, double *mean_ms, double *bytes, size_t *launches);
     double algorithmic_bytes(int tag) const;
     static const char *tag_name(int tag)
     {
-        static const char *names[] = {"zxw1_q", "tempv_q", "y_a", "w2_at", "zxw1_noc_at", "qw_noc_q"};
+        static const char *names[] = {"zxw1_q", "tempv_q", "y_a", "w2_at", "zxw1_noc_at", "qw_noc_q", "cone_x", "cone_y"};
         return names[tag];
     }
 
@@ -712,6 +772,8 @@ private:
     double *partials_ = nullptr;  // [4 regions][max_grid * kMaxTerms]
     size_t region_ = 0;
     double *red_ = nullptr;       // device reduction results
+    struct GoldenState *golden_ = nullptr;  // device state of the diagonal sigma search
+    hipGraphExec_t golden_graph_ = nullptr;  // its captured probe chain
     double *red_host_ = nullptr;  // pinned mirror
     std::vector<double> col_norm_h_, row_norm_h_;
     double sigma_ = 1.0, lambda_A_ = 0.0, lambda_Q_ = 0.0;
@@ -830,6 +892,10 @@ void HipBackend::release()
     if (arena_) (void)hipFree(arena_);
     if (partials_) (void)hipFree(partials_);
     if (red_) (void)hipFree(red_);
+    if (golden_graph_) (void)hipGraphExecDestroy(golden_graph_);
+    golden_graph_ = nullptr;
+    if (golden_) (void)hipFree(golden_);
+    golden_ = nullptr;
     if (red_host_) (void)hipHostFree(red_host_);
     if (stream_) (void)hipStreamDestroy(stream_);
     AT_perm_ = QT_rp_ = QT_perm_ = var_off_ = row_off_ = nullptr;
@@ -1321,9 +1387,13 @@ HPRQPStatus HipBackend::soc_iteration_step(size_t inner)
     ex.u = vec(V_u);
     ex.lin_n = (int)lin_n_;
     spmv(Q_, vec(V_w), ex, 0, T_ZXW1);
-    if (num_var_cones_)
-        soc_cone_x_kernel<<<grid_for(num_var_cones_), kThreads, 0, stream_>>>((int)num_var_cones_, var_off_, fx,
-                                                                              vec(V_x_bar));
+    if (num_var_cones_) {
+        if (profiling_) prof_begin(T_CONE_X);
+        soc_cone_project_kernel<<<(unsigned)((num_var_cones_ + kConeThreads - 1) / kConeThreads), kConeThreads, 0,
+                                  stream_>>>((int)num_var_cones_, var_off_, vec(V_tempv), vec(V_x_bar));
+        map(n_ - lin_n_, MapSocFinishX{{}, fx, vec(V_x_bar), (int)lin_n_});
+        if (profiling_) prof_end();
+    }
     spmv(Q_, vec(V_w_bar), EpiTempv{{}, vec(V_tempv), vec(V_x_hat), vec(V_Qw), sigma_}, 0, T_TEMPV);
     if (m_ > 0) {
         const double tau = sigma_ * lambda_A_;
@@ -1337,9 +1407,13 @@ HPRQPStatus HipBackend::soc_iteration_step(size_t inner)
         ey.AU = vec(V_AU);
         ey.lin_m = (int)lin_m_;
         spmv(A_, vec(V_tempv), ey, 0, T_Y);
-        if (num_row_cones_)
-            soc_cone_y_kernel<<<grid_for(num_row_cones_), kThreads, 0, stream_>>>((int)num_row_cones_, row_off_, fy,
-                                                                                  vec(V_s));
+        if (num_row_cones_) {
+            if (profiling_) prof_begin(T_CONE_Y);
+            soc_cone_project_kernel<<<(unsigned)((num_row_cones_ + kConeThreads - 1) / kConeThreads), kConeThreads,
+                                      0, stream_>>>((int)num_row_cones_, row_off_, vec(V_Rp_soc()), vec(V_s));
+            map(m_ - lin_m_, MapSocFinishY{{}, fy, (int)lin_m_});
+            if (profiling_) prof_end();
+        }
     }
     const double fs = sigma_ / (1.0 + sigma_ * lambda_Q_);
     if (Q_is_diag_)
@@ -1480,10 +1554,29 @@ HPRQPStatus HipBackend::update_sigma(HPRQPRestart *restart, const HPRQPResiduals
     b = b > 1e-12 ? b : 1e-12;
     double est;
     if (diag && m_ > 0) {
-        golden_diag_kernel<<<1, kGoldenThreads, 0, stream_>>>(a, b, vec(V_diag_Q), vec(V_ATdy), vec(V_QATdy), (int)n_,
-                                                              red_ + 10);
+        if (!golden_) HPRQP_HIP(hipMalloc((void **)&golden_, sizeof(GoldenState)));
+        golden_init_kernel<<<1, 64, 0, stream_>>>(golden_, a, b);
+        if (!golden_graph_) {
+            // the probe chain (every argument fixed for the session) is
+            // captured once and replayed for each search
+            const int gblk = std::max(1, std::min(kGoldenBlocks, (int)((n_ + 4 * kThreads - 1) / (4 * kThreads))));
+            double *gpart = region(3);
+            hipGraph_t graph = nullptr;
+            HPRQP_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+            for (int probe = 0; probe < kGoldenMaxIter + 2; ++probe) {
+                golden_partials_kernel<<<gblk, kThreads, 0, stream_>>>(golden_, vec(V_diag_Q), vec(V_ATdy),
+                                                                       vec(V_QATdy), (int)n_, gpart);
+                golden_step_kernel<<<1, kWave, 0, stream_>>>(golden_, gpart, gblk);
+            }
+            HPRQP_HIP(hipStreamEndCapture(stream_, &graph));
+            const hipError_t ie = hipGraphInstantiate(&golden_graph_, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            HPRQP_HIP(ie);
+        }
+        HPRQP_HIP(hipGraphLaunch(golden_graph_, stream_));
         HPRQP_HIP(hipGetLastError());
-        HPRQP_HIP(hipMemcpyAsync(red_host_ + 10, red_ + 10, sizeof(double), hipMemcpyDeviceToHost, stream_));
+        HPRQP_HIP(hipMemcpyAsync(red_host_ + 10, reinterpret_cast<char *>(golden_) + offsetof(GoldenState, result),
+                                 sizeof(double), hipMemcpyDeviceToHost, stream_));
         HPRQP_HIP(hipStreamSynchronize(stream_));
         est = red_host_[10];
     } else if (!diag && m_ > 0) {
@@ -1558,6 +1651,12 @@ double HipBackend::algorithmic_bytes(int tag) const
         return 12.0 * na + 4.0 * (n + 1) + 8.0 * m + 8.0 * n * (8.0 + diag + 6.0);
     case T_QW_NOC:  // reads w_bar(gather), Qw, last_Qw, x_hat; writes Qw_bar, Qw, tempv
         return 12.0 * nq + 4.0 * (n + 1) + 8.0 * n + 8.0 * n * 6.0;
+    case T_CONE_X:  // cone range of x: projection (tempv in, x_bar out), then the finish
+        // (reads x, x_bar, tempv, last_x, w [+fact1, fact2]; writes dx, z_bar, x_hat, x, w_bar)
+        return 8.0 * (n - (double)lin_n_) * (2.0 + 5.0 + diag + 5.0) + 4.0 * (num_var_cones_ + 1);
+    case T_CONE_Y:  // cone range of y: projection (Rp in, s out), then the finish
+        // (reads y, s, Rp, last_y; writes dy, y_bar, y)
+        return 8.0 * (m - (double)lin_m_) * (2.0 + 4.0 + 3.0) + 4.0 * (num_row_cones_ + 1);
     default:
         return 0.0;
     }
@@ -1565,7 +1664,7 @@ double HipBackend::algorithmic_bytes(int tag) const
 
 HPRQPStatus HipBackend::profile(size_t iterations, double *mean_ms, double *bytes, size_t *launches)
 {
-    const size_t need = iterations * 8 + 8;
+    const size_t need = iterations * 16 + 8;
     ev_pool_.resize(need);
     ev_tags_.assign(need, -1);
     for (auto &e : ev_pool_) HPRQP_HIP(hipEventCreate(&e));
