@@ -234,6 +234,16 @@ struct EpiResAT {
     }
 };
 
+// EpiResAT when A^T y_bar is already at hand (the check variant of the
+// iteration's last SpMV wrote it): the same terms from a stored vector.
+struct MapResAT {
+    static constexpr int NS = 0, NM = 3;
+    double *partials;
+    EpiResAT epi;
+    const double *ATy_bar;
+    __device__ void row(int i, double *s, double *mx) const { epi.row(i, ATy_bar[i], s, mx); }
+};
+
 // residuals.c:164-175 and the y_bar.s dot of :53
 struct EpiResA {
     static constexpr int NS = 1, NM = 2;

@@ -773,6 +773,7 @@ private:
     size_t region_ = 0;
     double *red_ = nullptr;       // device reduction results
     struct GoldenState *golden_ = nullptr;  // device state of the diagonal sigma search
+    bool aty_bar_valid_ = false;  // ATy_bar holds A^T y_bar of the current y_bar
     hipGraphExec_t golden_graph_ = nullptr;  // its captured probe chain
     double *red_host_ = nullptr;  // pinned mirror
     std::vector<double> col_norm_h_, row_norm_h_;
@@ -1267,6 +1268,8 @@ HPRQPStatus HipBackend::iteration_step(size_t inner)
     const double f1s = 1.0 - f2s;
     const bool chk = to_check_ != 0, diag = Q_is_diag_ != 0;
     if (num_var_cones_ || num_row_cones_) return soc_iteration_step(inner);
+    // the check variant of the w2 step stores A^T y_bar (reused by the residuals)
+    aty_bar_valid_ = chk && !noC_;
     if (noC_) {
 #define HPRQP_NOC(D, C)                                                                                           \
     spmv(AT_, vec(V_y), EpiZxw1NoC<D, C>{{}, vec(V_ATy), vec(V_x), vec(V_x_bar), vec(V_x_hat), vec(V_w),           \
@@ -1336,8 +1339,17 @@ HPRQPStatus HipBackend::compute_residuals(HPRQPResiduals *res, size_t iter)
     spmv(Q_, vec(V_x_bar), EpiResQ{nullptr, vec(V_Qx), vec(V_x_bar), vec(V_c), vec(V_z_bar)}, 0);
     finish(0, red_blocks(Q_), 3, 0, 0);
     zero_red(3, 8);
-    spmv(AT_, vec(V_y_bar), EpiResAT{nullptr, vec(V_ATdy), vec(V_Qx), vec(V_c), vec(V_z_bar), vec(V_col_norm), c_scale_}, 1);
-    finish(1, red_blocks(AT_), 0, 3, 3);
+    {
+        const EpiResAT rat{nullptr, vec(V_ATdy), vec(V_Qx), vec(V_c), vec(V_z_bar), vec(V_col_norm), c_scale_};
+        if (aty_bar_valid_) {
+            // the check step's A^T y_bar product is reused (same kernel, same bits)
+            map(n_, MapResAT{nullptr, rat, vec(V_ATy_bar)}, 1);
+            finish(1, n_grid(), 0, 3, 3);
+        } else {
+            spmv(AT_, vec(V_y_bar), rat, 1);
+            finish(1, red_blocks(AT_), 0, 3, 3);
+        }
+    }
     if (m_ > 0) {
         spmv(A_, vec(V_x_bar), EpiResA{nullptr, vec(V_Ax), vec(V_AL), vec(V_AU), vec(V_row_norm), vec(V_y_bar), vec(V_s), b_scale_}, 2);
         finish(2, red_blocks(A_), 1, 2, 6);
@@ -1370,6 +1382,7 @@ HPRQPStatus HipBackend::compute_residuals(HPRQPResiduals *res, size_t iter)
 // src/gpu/soc_iteration.cuh:45-82: the generic cone step with full writes.
 HPRQPStatus HipBackend::soc_iteration_step(size_t inner)
 {
+    aty_bar_valid_ = true;  // the cone step always runs the w2 variant that stores A^T y_bar
     const double a = 1.0 / ((double)inner + 2.0), b = 1.0 - a;
     const double f2s = 1.0 / (1.0 + sigma_ * lambda_Q_);
     const double f1s = 1.0 - f2s;
@@ -1436,8 +1449,16 @@ HPRQPStatus HipBackend::soc_residuals(HPRQPResiduals *res)
     spmv(Q_, vec(V_x_bar), EpiResQ{nullptr, vec(V_Qx), vec(V_x_bar), vec(V_c), vec(V_z_bar)}, 0);
     finish(0, red_blocks(Q_), 3, 0, 0);
     zero_red(3, 8);
-    spmv(AT_, vec(V_y_bar), EpiResAT{nullptr, vec(V_ATdy), vec(V_Qx), vec(V_c), vec(V_z_bar), vec(V_col_norm), c_scale_}, 1);
-    finish(1, red_blocks(AT_), 0, 3, 3);
+    {
+        const EpiResAT rat{nullptr, vec(V_ATdy), vec(V_Qx), vec(V_c), vec(V_z_bar), vec(V_col_norm), c_scale_};
+        if (aty_bar_valid_) {
+            map(n_, MapResAT{nullptr, rat, vec(V_ATy_bar)}, 1);
+            finish(1, n_grid(), 0, 3, 3);
+        } else {
+            spmv(AT_, vec(V_y_bar), rat, 1);
+            finish(1, red_blocks(AT_), 0, 3, 3);
+        }
+    }
     if (m_ > 0) {
         spmv(A_, vec(V_x_bar), EpiSocResA{nullptr, vec(V_Ax), vec(V_Rp_soc()), rc ? vec(V_soc_rhs) : nullptr,
                                           vec(V_row_norm), vec(V_y_bar), vec(V_s), b_scale_}, 2);
