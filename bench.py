@@ -104,6 +104,15 @@ def profile_kernels(H, model, iters):
 def cpu_baseline(qp, gpu_iters, lam_a, lam_q, sample_bodies):
     import qp_libs as Q
     if sample_bodies >= 10 ** 9:  # small workload: the whole solve on one core
+        if Q.ref_available():
+            # the reference itself, built from its own sources (oracle/_ref),
+            # through its binding API: solve_qp --use-gpu 0 equivalent
+            t0 = time.time()
+            r = Q.binding_solve(Q.ref_lib(), qp, use_gpu=0)
+            dt = time.time() - t0
+            return dict(value=r["time"], unit="s (solve loop to 1e-8)", cores=1, kind="reference",
+                        sample=f"reference CPU path (oracle/_ref/libhprqp_ref.so), full solve ({r['iter']} "
+                               f"iterations, {r['status']}); setup+solve {dt:.3f} s")
         t0 = time.time()
         o = Q.OracleSession(qp)
         r = o.solve()[0]
