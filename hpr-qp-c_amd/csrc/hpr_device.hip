@@ -597,20 +597,93 @@ __global__ void fill_kernel(int n, double *__restrict__ v, double value)
 
 // control.c:79-114 on the device for diagonal Q. Each probe of the
 // golden-section search is a sum over all n variables, so it is spread over
-// the GPU: a partial-sum kernel (grid-stride, one partial per workgroup)
-// followed by a one-workgroup step kernel that adds the partials in index
-// order, evaluates f(x) = a x + b / x + x^2 sum_i c_i d_i / (1 + x q_i) and
-// advances the search state on the device. The host enqueues the largest
-// number of probes the search can take; once it has converged the remaining
-// launches return at once.
-struct GoldenState {
-    double a, b, lo, hi, x1, x2, f1, f2, xnew, result;
-    int phase;  // 0: evaluating x1, 1: evaluating x2, 2: evaluating a search step's new point
-    int pending;  // which of x1 (1) / x2 (2) the new point replaces
-    int it, done;
-};
+// the GPU, and the search looks ahead: the points a step can visit depend on
+// earlier comparisons only through a branch bit, so the next kGoldenDepth
+// steps have at most 2^kGoldenDepth - 1 candidate points (a binary tree,
+// node (l, j) at index 2^l - 1 + j, child (l + 1, 2j + b) for branch b: b = 0
+// when f1 > f2). One pass evaluates all of them -- a partial-sum kernel over
+// the variables, then a one-wavefront step kernel that adds the partials in
+// a fixed order, walks the tree along the branches the actual values take
+// (applying the search's own updates, so its points and values are those of
+// the one-probe-at-a-time search) and writes the next tree. A pass therefore
+// advances up to kGoldenDepth probes; passes after convergence return at once.
 constexpr int kGoldenMaxIter = 200;
 constexpr int kGoldenBlocks = 256;
+constexpr int kGoldenDepth = 4;
+constexpr int kGoldenPoints = (1 << kGoldenDepth) - 1;
+constexpr int kGoldenPasses = 2 + (kGoldenMaxIter + kGoldenDepth - 1) / kGoldenDepth;
+
+struct GoldenState {
+    double a, b, lo, hi, x1, x2, f1, f2, result;
+    double pts[kGoldenPoints];
+    int npts;  // points to evaluate this pass (2 for the first pass: x1, x2)
+    int it, done;
+};
+
+__device__ __forceinline__ bool golden_continue(const GoldenState &g)
+{
+    return fabs(g.hi - g.lo) > 1e-12 * fmax(1.0, fabs(g.lo)) && g.it < kGoldenMaxIter;
+}
+
+// One search step on branch b (b = 0: f1 > f2); returns the new point. The
+// value moved between f1 / f2 is moved only when the caller knows it.
+__device__ __forceinline__ double golden_branch(GoldenState &g, int b, bool move_values)
+{
+    const double phi = (sqrt(5.0) - 1.0) / 2.0;
+    if (b == 0) {
+        g.lo = g.x1;
+        g.x1 = g.x2;
+        if (move_values) g.f1 = g.f2;
+        g.x2 = g.lo + phi * (g.hi - g.lo);
+        return g.x2;
+    }
+    g.hi = g.x2;
+    g.x2 = g.x1;
+    if (move_values) g.f2 = g.f1;
+    g.x1 = g.hi - phi * (g.hi - g.lo);
+    return g.x1;
+}
+
+// Fills g.pts with the candidate tree of the next kGoldenDepth steps from a
+// state whose f1, f2 are known (or sets done when the search has ended).
+__device__ void golden_plan(GoldenState &g)
+{
+    if (!golden_continue(g)) {
+        g.done = 1;
+        g.result = g.f1 < g.f2 ? g.x1 : g.x2;
+        return;
+    }
+    GoldenState cur[1 << (kGoldenDepth - 1)], nxt[1 << (kGoldenDepth - 1)];
+    bool live[1 << (kGoldenDepth - 1)], nlive[1 << (kGoldenDepth - 1)];
+    // level 0: the branch is known
+    cur[0] = g;
+    g.pts[0] = golden_branch(cur[0], g.f1 > g.f2 ? 0 : 1, false);
+    cur[0].it += 1;
+    live[0] = true;
+    int npts = 1;
+    for (int l = 1; l < kGoldenDepth; ++l) {
+        const int width = 1 << (l - 1);
+        for (int j = 0; j < width; ++j) {
+            for (int bb = 0; bb < 2; ++bb) {
+                const int c = 2 * j + bb;
+                nlive[c] = live[j] && golden_continue(cur[j]);
+                nxt[c] = cur[j];
+                double p = 0.0;
+                if (nlive[c]) {
+                    p = golden_branch(nxt[c], bb, false);
+                    nxt[c].it += 1;
+                }
+                g.pts[(1 << l) - 1 + c] = p;  // unused when the branch cannot be reached
+            }
+        }
+        for (int c = 0; c < 2 * width; ++c) {
+            cur[c] = nxt[c];
+            live[c] = nlive[c];
+        }
+        npts = (1 << (l + 1)) - 1;
+    }
+    g.npts = npts;
+}
 
 __global__ void golden_init_kernel(GoldenState *st, double a, double b)
 {
@@ -623,8 +696,9 @@ __global__ void golden_init_kernel(GoldenState *st, double a, double b)
     g.hi = 1e12;
     g.x1 = g.hi - phi * (g.hi - g.lo);
     g.x2 = g.lo + phi * (g.hi - g.lo);
-    g.phase = 0;
-    g.xnew = g.x1;
+    g.pts[0] = g.x1;
+    g.pts[1] = g.x2;
+    g.npts = 2;
     *st = g;
 }
 
@@ -635,63 +709,64 @@ __global__ __launch_bounds__(kThreads) void golden_partials_kernel(const GoldenS
                                                                    double *__restrict__ partials)
 {
     if (st->done) return;
-    const double x = st->xnew;
-    double acc = 0.0;
-    for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
-        const double t = d[i] / (1.0 + x * dq[i]);
-        acc += c[i] * t;
+    const int np = st->npts;
+    double x[kGoldenPoints], acc[kGoldenPoints];
+#pragma unroll
+    for (int p = 0; p < kGoldenPoints; ++p) {
+        x[p] = p < np ? st->pts[p] : 0.0;
+        acc[p] = 0.0;
     }
-    block_reduce_store<1, 0>(&acc, nullptr, partials + blockIdx.x);
+    for (int i = blockIdx.x * kThreads + threadIdx.x; i < n; i += gridDim.x * kThreads) {
+        const double qi = dq[i], ci = c[i], di = d[i];
+#pragma unroll
+        for (int p = 0; p < kGoldenPoints; ++p) {
+            if (p < np) {
+                const double t = di / (1.0 + x[p] * qi);
+                acc[p] += ci * t;
+            }
+        }
+    }
+    block_reduce_store<kGoldenPoints, 0>(acc, nullptr, partials + (size_t)blockIdx.x * kGoldenPoints);
 }
 
 __global__ __launch_bounds__(kWave) void golden_step_kernel(GoldenState *__restrict__ st,
                                                              const double *__restrict__ partials, int nblk)
 {
     if (st->done) return;
-    // one wavefront adds the partials: lane-strided sums, then a fixed tree
-    double part = 0.0;
-    for (int k = threadIdx.x; k < nblk; k += kWave) part += partials[k];
-    const double total = wave_sum(part);
+    // one wavefront adds each point's partials: lane-strided sums, then a fixed tree
+    __shared__ double total[kGoldenPoints];
+    const int np = st->npts;
+    for (int p = 0; p < np; ++p) {
+        double part = 0.0;
+        for (int k = threadIdx.x; k < nblk; k += kWave) part += partials[(size_t)k * kGoldenPoints + p];
+        part = wave_sum(part);
+        if (threadIdx.x == 0) total[p] = part;
+    }
     if (threadIdx.x != 0) return;
     GoldenState g = *st;
-    const double x = g.xnew;
-    const double f = g.a * x + g.b / x + x * x * total;
-    const double phi = (sqrt(5.0) - 1.0) / 2.0;
-    if (g.phase == 0) {
-        g.f1 = f;
-        g.phase = 1;
-        g.xnew = g.x2;
-        *st = g;
-        return;
-    }
-    if (g.phase == 1) {
-        g.f2 = f;
+    auto f_of = [&](int p) {
+        const double x = g.pts[p];
+        return g.a * x + g.b / x + x * x * total[p];
+    };
+    if (np == 2) {  // first pass: f(x1), f(x2)
+        g.f1 = f_of(0);
+        g.f2 = f_of(1);
     } else {
-        if (g.pending == 1) g.f1 = f;
-        else g.f2 = f;
-        g.it += 1;
-    }
-    g.phase = 2;
-    if (fabs(g.hi - g.lo) > 1e-12 * fmax(1.0, fabs(g.lo)) && g.it < kGoldenMaxIter) {
-        if (g.f1 > g.f2) {
-            g.lo = g.x1;
-            g.x1 = g.x2;
-            g.f1 = g.f2;
-            g.x2 = g.lo + phi * (g.hi - g.lo);
-            g.xnew = g.x2;
-            g.pending = 2;
-        } else {
-            g.hi = g.x2;
-            g.x2 = g.x1;
-            g.f2 = g.f1;
-            g.x1 = g.hi - phi * (g.hi - g.lo);
-            g.xnew = g.x1;
-            g.pending = 1;
+        // walk the tree along the actual branches
+        int j = 0;
+        for (int l = 0; l < kGoldenDepth; ++l) {
+            if (!golden_continue(g)) break;
+            const int bb = g.f1 > g.f2 ? 0 : 1;
+            if (l > 0) j = 2 * j + bb;
+            const int node = (1 << l) - 1 + j;
+            (void)golden_branch(g, bb, true);
+            const double f = f_of(node);
+            if (bb == 0) g.f2 = f;  // the new point replaced x2
+            else g.f1 = f;          // the new point replaced x1
+            g.it += 1;
         }
-    } else {
-        g.done = 1;
-        g.result = g.f1 < g.f2 ? g.x1 : g.x2;
     }
+    golden_plan(g);
     *st = g;
 }
 
@@ -774,7 +849,8 @@ private:
     double *red_ = nullptr;       // device reduction results
     struct GoldenState *golden_ = nullptr;  // device state of the diagonal sigma search
     bool aty_bar_valid_ = false;  // ATy_bar holds A^T y_bar of the current y_bar
-    hipGraphExec_t golden_graph_ = nullptr;  // its captured probe chain
+    double *golden_part_ = nullptr;          // its per-workgroup partial sums
+    hipGraphExec_t golden_graph_ = nullptr;  // its captured pass chain
     double *red_host_ = nullptr;  // pinned mirror
     std::vector<double> col_norm_h_, row_norm_h_;
     double sigma_ = 1.0, lambda_A_ = 0.0, lambda_Q_ = 0.0;
@@ -897,6 +973,8 @@ void HipBackend::release()
     golden_graph_ = nullptr;
     if (golden_) (void)hipFree(golden_);
     golden_ = nullptr;
+    if (golden_part_) (void)hipFree(golden_part_);
+    golden_part_ = nullptr;
     if (red_host_) (void)hipHostFree(red_host_);
     if (stream_) (void)hipStreamDestroy(stream_);
     AT_perm_ = QT_rp_ = QT_perm_ = var_off_ = row_off_ = nullptr;
@@ -1575,19 +1653,21 @@ HPRQPStatus HipBackend::update_sigma(HPRQPRestart *restart, const HPRQPResiduals
     b = b > 1e-12 ? b : 1e-12;
     double est;
     if (diag && m_ > 0) {
-        if (!golden_) HPRQP_HIP(hipMalloc((void **)&golden_, sizeof(GoldenState)));
+        if (!golden_) {
+            HPRQP_HIP(hipMalloc((void **)&golden_, sizeof(GoldenState)));
+            HPRQP_HIP(hipMalloc((void **)&golden_part_, sizeof(double) * kGoldenBlocks * kGoldenPoints));
+        }
         golden_init_kernel<<<1, 64, 0, stream_>>>(golden_, a, b);
         if (!golden_graph_) {
-            // the probe chain (every argument fixed for the session) is
+            // the pass chain (every argument fixed for the session) is
             // captured once and replayed for each search
             const int gblk = std::max(1, std::min(kGoldenBlocks, (int)((n_ + 4 * kThreads - 1) / (4 * kThreads))));
-            double *gpart = region(3);
             hipGraph_t graph = nullptr;
             HPRQP_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-            for (int probe = 0; probe < kGoldenMaxIter + 2; ++probe) {
+            for (int pass = 0; pass < kGoldenPasses; ++pass) {
                 golden_partials_kernel<<<gblk, kThreads, 0, stream_>>>(golden_, vec(V_diag_Q), vec(V_ATdy),
-                                                                       vec(V_QATdy), (int)n_, gpart);
-                golden_step_kernel<<<1, kWave, 0, stream_>>>(golden_, gpart, gblk);
+                                                                       vec(V_QATdy), (int)n_, golden_part_);
+                golden_step_kernel<<<1, kWave, 0, stream_>>>(golden_, golden_part_, gblk);
             }
             HPRQP_HIP(hipStreamEndCapture(stream_, &graph));
             const hipError_t ie = hipGraphInstantiate(&golden_graph_, graph, nullptr, nullptr, 0);
