@@ -474,15 +474,6 @@ HPRQPStatus build_from_QAbc_soc(const HPRQPMatrixCSR *Q, const HPRQPMatrixCSR *A
     return build_impl(Q, A, c, AL, AU, l, u, obj_constant, soc, model);
 }
 
-HPRQPStatus build_from_mps(const char *mps_path, HPRQPModel *model)
-{
-    (void)mps_path;
-    (void)model;
-    std::fprintf(stderr, "hprqp: build_from_mps is not provided by the MI355X library; "
-                         "build the model with build_from_QAbc\n");
-    return HPRQP_STATUS_NOT_IMPLEMENTED;
-}
-
 // solver.c:133-157. use_gpu defaults to 1 (this library is the GPU backend);
 // gram_mode defaults to OFF and polishing to 0 because neither exists here.
 void params_set_defaults(HPRQPParams *params)

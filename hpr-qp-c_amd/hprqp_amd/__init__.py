@@ -255,6 +255,23 @@ def build_from_qabc(Q, c, A, AL, AU, l, u, obj_constant: float = 0.0, soc: dict 
     return model
 
 
+_LIB.build_from_mps.argtypes = [C.c_char_p, C.POINTER(CModel)]
+
+
+def read_from_mps(path) -> Model:
+    """build_from_mps on a free-format MPS / QPS file (same name as the
+    reference wrapper's read_from_mps); raises ValueError on a rejected file."""
+    model = Model()
+    st = _LIB.build_from_mps(os.fsencode(str(path)), C.byref(model.c))
+    if st != STATUS_OK:
+        raise ValueError(f"build_from_mps failed: {status_name(st)}")
+    return model
+
+
+def build_from_mps(path) -> Model:
+    return read_from_mps(path)
+
+
 def _vec(p, n):
     return np.ctypeslib.as_array(p, shape=(n,)).copy() if n and p else np.zeros(0)
 

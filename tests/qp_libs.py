@@ -404,8 +404,9 @@ def ref_lib():
     return _ref
 
 
-def binding_solve(L, qp: QP, **params):
-    """Solve through the hprqp_binding.h API of library L (reference or product)."""
+def binding_solve(L, qp: QP | None, mps_path=None, **params):
+    """Solve through the hprqp_binding.h API of library L (reference or
+    product), the model given as arrays (qp) or as an MPS file (mps_path)."""
     vp = C.c_void_p
     model, prm, res = vp(), vp(), vp()
     L.hprqp_model_create(C.byref(model))
@@ -423,10 +424,14 @@ def binding_solve(L, qp: QP, **params):
             arr, p = _db(a if len(a) else np.zeros(1))
             keep.append(arr)
             return p
-        st = L.hprqp_model_build_from_qabc_csr(model, qp.n, qp.n, len(qp.q_val), sz(qp.q_ptr), sz(qp.q_ind),
-                                               db(qp.q_val), qp.m, qp.n, len(qp.a_val), sz(qp.a_ptr), sz(qp.a_ind),
-                                               db(qp.a_val), db(qp.c), db(qp.AL), db(qp.AU), db(qp.l), db(qp.u),
-                                               qp.obj_constant)
+        if mps_path is not None:
+            L.hprqp_model_read_from_mps.argtypes = [vp, C.c_char_p]
+            st = L.hprqp_model_read_from_mps(model, str(mps_path).encode())
+        else:
+            st = L.hprqp_model_build_from_qabc_csr(model, qp.n, qp.n, len(qp.q_val), sz(qp.q_ptr), sz(qp.q_ind),
+                                                   db(qp.q_val), qp.m, qp.n, len(qp.a_val), sz(qp.a_ptr),
+                                                   sz(qp.a_ind), db(qp.a_val), db(qp.c), db(qp.AL), db(qp.AU),
+                                                   db(qp.l), db(qp.u), qp.obj_constant)
         if st != 0:
             return {"status": STATUS[st], "build_failed": True}
         params.setdefault("verbose", 0)
@@ -708,3 +713,94 @@ def csr_with_row_lengths(lengths, ncol, seed=0):
             ci[rp[r]:rp[r + 1]] = np.sort(rng.choice(ncol, size=int(L), replace=False))
     val = rng.standard_normal(rp[-1])
     return rp, ci, val
+
+
+# --------------------------------------------------------------------------- MPS files
+
+# bound cards per variable kind (MPS semantics in the comment)
+_BOUND_KINDS = {
+    "default": [],                               # [0, inf)
+    "box": [("LO", -2.0), ("UP", 3.0)],          # [-2, 3]
+    "fixed": [("FX", 0.5)],                      # [0.5, 0.5]
+    "free": [("FR", None)],                      # (-inf, inf)
+    "minus": [("MI", None), ("UP", 4.0)],        # (-inf, 4]
+    "up_neg": [("UP", -1.5)],                    # (-inf, -1.5]
+    "up_pos": [("UP", 2.5)],                     # [0, 2.5]
+    "lo_only": [("LO", -3.0), ("PL", None)],     # [-3, inf)
+}
+
+
+def write_mps(qp: QP, path, seed=0, quad="QUADOBJ", extras=True):
+    """Write `qp` (its A, c, row bounds and Q) as a free-format MPS / QPS file
+    that exercises the format: ranges on L, G and E rows, all continuous bound
+    types (variable bounds drawn from _BOUND_KINDS, not qp.l / qp.u), an
+    objective constant through the objective row's RHS, two-pair cards, D
+    exponents, comments, and (extras) an ignored second objective row and
+    ignored second RHS / RANGES / BOUNDS sets. quad: "QUADOBJ" (upper
+    triangle) or "QMATRIX" (full matrix)."""
+    rng = np.random.default_rng(seed)
+    n, m = qp.n, qp.m
+    A = qp.dense_A()
+    Qd = qp.dense_Q()
+    Qd = 0.5 * (Qd + Qd.T)
+    kinds, rhs, ranges = [], {}, {}
+    for i in range(m):
+        lo, hi = qp.AL[i], qp.AU[i]
+        if lo == hi:
+            kinds.append("E")
+            rhs[i] = lo
+            if i % 7 == 3:
+                ranges[i] = float(rng.uniform(-1, 1))
+        elif math.isinf(lo):
+            kinds.append("L")
+            rhs[i] = hi
+        elif math.isinf(hi):
+            kinds.append("G")
+            rhs[i] = lo
+        else:
+            kinds.append("L" if i % 2 else "G")
+            rhs[i] = hi if i % 2 else lo
+            ranges[i] = (hi - lo) * (1 if i % 4 < 2 else -1)
+    num = lambda v: repr(float(v)).replace("e", "D") if abs(v) > 1e3 or (0 < abs(v) < 1e-3) else repr(float(v))
+    L = ["* generated test model", "NAME          GENQP", "ROWS", " N  COST"]
+    if extras:
+        L.append(" N  COST2")
+    L += [f" {k}  R{i}" for i, k in enumerate(kinds)]
+    L.append("COLUMNS")
+    for j in range(n):
+        pairs = [("COST", qp.c[j])] if qp.c[j] != 0.0 else []
+        if extras and j % 4 == 0:
+            pairs.append(("COST2", 1.0))
+        pairs += [(f"R{i}", A[i, j]) for i in np.nonzero(A[:, j])[0]]
+        if not pairs:
+            pairs = [("COST", 0.0)]
+        for k in range(0, len(pairs), 2):
+            L.append(f"    X{j}  " + "  ".join(f"{r}  {num(v)}" for r, v in pairs[k:k + 2]))
+        if extras and j == n // 2:
+            L += ["", "* a comment between cards"]
+    L.append("RHS")
+    items = [(f"R{i}", v) for i, v in rhs.items() if v != 0.0] + [("COST", -1.25)]
+    for k in range(0, len(items), 2):
+        L.append("    RHS  " + "  ".join(f"{r}  {num(v)}" for r, v in items[k:k + 2]))
+    if extras:
+        L.append("    RHS2  R0  123.0")
+    if ranges:
+        L.append("RANGES")
+        L += [f"    RNG  R{i}  {num(r)}" for i, r in ranges.items()]
+        if extras:
+            L.append("    RNG2  R0  5.0")
+    L.append("BOUNDS")
+    names = sorted(_BOUND_KINDS)
+    for j in range(n):
+        for kind, val in _BOUND_KINDS[names[j % len(names)]]:
+            L.append(f" {kind} BND  X{j}" + ("" if val is None else f"  {num(val)}"))
+    if extras:
+        L.append(" UP BND2  X0  -7.0")
+    L.append(quad)
+    for i in range(n):
+        for j in range(n):
+            if Qd[i, j] != 0.0 and (quad == "QMATRIX" or j >= i):
+                L.append(f"    X{i}  X{j}  {num(Qd[i, j])}")
+    L.append("ENDATA")
+    with open(path, "w") as fh:
+        fh.write("\n".join(L) + "\n")

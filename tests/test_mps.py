@@ -1,0 +1,155 @@
+"""build_from_mps (hpr-qp-c_amd/csrc/mps.cpp) against the reference's MPS
+loader (src/shared/mps_reader.cpp, src/shared/mps_qp.cpp): the reference's
+own MPS files (tests/data/simple_qp.mps, model.mps, kept as fixtures under
+tests/golden/mps) with the models written out by hand, and generated files
+exercising every supported section and card, read by both loaders and solved
+on the CPU -- the oracle on this library's model, the reference on its own
+-- with bit-identical iterates required (identical models)."""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+import qp_libs as Q
+
+GOLD = Q.ROOT / "tests" / "golden" / "mps"
+inf = math.inf
+
+
+@pytest.fixture(scope="module")
+def H():
+    import hprqp_amd
+    return hprqp_amd
+
+
+def model_arrays(model):
+    """The preprocessed model as a qp_libs.QP."""
+    m = model.c
+
+    def csr(M):
+        rp = np.ctypeslib.as_array(M.row_ptr, shape=(M.nrow + 1,)).copy()
+        ci = np.ctypeslib.as_array(M.col_ind, shape=(max(M.nnz, 1),))[:M.nnz].copy()
+        v = np.ctypeslib.as_array(M.values, shape=(max(M.nnz, 1),))[:M.nnz].copy()
+        return rp, ci, v
+
+    n, mm = m.Q.ncol, m.A.nrow
+    vec = lambda p, k: np.ctypeslib.as_array(p, shape=(k,)).copy() if k else np.zeros(0)
+    qrp, qci, qv = csr(m.Q)
+    arp, aci, av = csr(m.A)
+    return Q.QP(n, mm, qrp, qci, qv, arp, aci, av, vec(m.c, n), vec(m.AL, mm), vec(m.AU, mm), vec(m.l, n),
+                vec(m.u, n), m.obj_constant)
+
+
+def test_simple_qp_fixture_model(H):
+    qp = model_arrays(H.read_from_mps(GOLD / "simple_qp.mps"))
+    assert (qp.n, qp.m) == (2, 2)
+    np.testing.assert_array_equal(qp.c, [1.0, 2.0])
+    np.testing.assert_array_equal(qp.dense_A(), [[1.0, 1.0], [1.0, -1.0]])
+    np.testing.assert_array_equal(qp.AL, [1.0, -inf])
+    np.testing.assert_array_equal(qp.AU, [inf, 2.0])
+    np.testing.assert_array_equal(qp.l, [0.0, -5.0])
+    np.testing.assert_array_equal(qp.u, [10.0, 5.0])
+    np.testing.assert_array_equal(qp.dense_Q(), [[2.0, 1.0], [1.0, 4.0]])  # one triangle, mirrored
+    assert qp.obj_constant == 0.0
+
+
+def test_model_mps_fixture_model(H):
+    qp = model_arrays(H.read_from_mps(GOLD / "model.mps"))
+    np.testing.assert_array_equal(qp.c, [-3.0, -5.0])
+    np.testing.assert_array_equal(qp.dense_A(), [[1.0, 2.0], [3.0, 1.0]])
+    np.testing.assert_array_equal(qp.AL, [-inf, -inf])
+    np.testing.assert_array_equal(qp.AU, [10.0, 12.0])
+    np.testing.assert_array_equal(qp.l, [0.0, 0.0])
+    np.testing.assert_array_equal(qp.u, [inf, inf])  # PL
+    np.testing.assert_array_equal(qp.dense_Q(), [[2.0, 0.0], [0.0, 2.0]])
+
+
+def test_generated_file_semantics(H, tmp_path):
+    """Ranges, bound kinds and the objective constant as the format defines
+    them, checked against values computed here from the written cards."""
+    qp = Q.random_qp(seed=4, n=16, m_eq=5, m_in=7, a_row_nnz=4, b_rows=4, b_row_nnz=3)
+    path = tmp_path / "gen.mps"
+    Q.write_mps(qp, path, seed=1)
+    got = model_arrays(H.read_from_mps(path))
+    assert got.obj_constant == 1.25  # RHS of -1.25 on the objective row
+    names = sorted(Q._BOUND_KINDS)
+    want = {"default": (0.0, inf), "box": (-2.0, 3.0), "fixed": (0.5, 0.5), "free": (-inf, inf),
+            "minus": (-inf, 4.0), "up_neg": (-inf, -1.5), "up_pos": (0.0, 2.5), "lo_only": (-3.0, inf)}
+    for j in range(qp.n):
+        assert (got.l[j], got.u[j]) == want[names[j % len(names)]], j
+    np.testing.assert_array_equal(got.c, qp.c)  # the second objective row is ignored
+    # rows: equalities keep AL = AU unless ranged; ranged L/G rows reproduce the interval
+    for i in range(qp.m):
+        lo, hi = qp.AL[i], qp.AU[i]
+        if lo == hi:
+            if i % 7 != 3:
+                assert got.AL[i] == got.AU[i] == lo
+        elif not (math.isinf(lo) or math.isinf(hi)):
+            assert got.AL[i] == pytest.approx(lo, abs=1e-12) and got.AU[i] == pytest.approx(hi, abs=1e-12)
+
+
+REF_CASES = [("fixture", "simple_qp.mps", None), ("fixture", "model.mps", None),
+             ("gen", "QUADOBJ", 1), ("gen", "QMATRIX", 2), ("gen", "QUADOBJ", 3)]
+
+
+@pytest.mark.skipif(not Q.ref_available(), reason="reference not built here (oracle/_ref)")
+@pytest.mark.parametrize("kind,arg,seed", REF_CASES)
+def test_model_equals_reference_loader(H, tmp_path, kind, arg, seed):
+    if kind == "fixture":
+        path = GOLD / arg
+    else:
+        qp0 = Q.random_qp(seed=seed, n=40, m_eq=8, m_in=12, a_row_nnz=5, b_rows=10, b_row_nnz=3)
+        path = tmp_path / f"gen{seed}.mps"
+        Q.write_mps(qp0, path, seed=seed, quad=arg)
+    qp = model_arrays(H.read_from_mps(path))
+    for k in (3, 40, 300):
+        extra = {"max_iter": k}
+        r = Q.binding_solve(Q.ref_lib(), None, mps_path=path, use_gpu=0, **extra)
+        assert not r.get("build_failed"), r
+        o = Q.OracleSession(qp, **extra)
+        res, x, y, z, w = o.solve()
+        assert Q.STATUS[res.status] == r["status"] and res.iter == r["iter"], (k, r["status"])
+        assert res.primal_obj == r["objective"]
+        for v, arr in zip("xyzw", (x, y, z, w)):
+            np.testing.assert_array_equal(arr, r[v], err_msg=f"{v} at max_iter {k}")
+
+
+BAD = {
+    "maximise": "NAME X\nOBJSENSE\n MAX\nROWS\n N obj\n L r\nCOLUMNS\n x obj 1 r 1\nRHS\n rhs r 1\nENDATA\n",
+    "integer": ("NAME X\nROWS\n N obj\n L r\nCOLUMNS\n m 'MARKER' 'INTORG'\n x obj 1 r 1\n m 'MARKER' 'INTEND'\n"
+                "RHS\n rhs r 1\nENDATA\n"),
+    "binary_bound": "NAME X\nROWS\n N obj\n L r\nCOLUMNS\n x obj 1 r 1\nBOUNDS\n BV b x\nENDATA\n",
+    "unknown_row": "NAME X\nROWS\n N obj\n L r\nCOLUMNS\n x obj 1 q 1\nENDATA\n",
+    "no_endata": "NAME X\nROWS\n N obj\n L r\nCOLUMNS\n x obj 1 r 1\n",
+    "bad_number": "NAME X\nROWS\n N obj\n L r\nCOLUMNS\n x obj 1 r 1x\nENDATA\n",
+    "unknown_bound": "NAME X\nROWS\n N obj\n L r\nCOLUMNS\n x obj 1 r 1\nBOUNDS\n ZZ b x 1\nENDATA\n",
+    "asymmetric_qmatrix": ("NAME X\nROWS\n N obj\n L r\nCOLUMNS\n x obj 1 r 1\n y obj 1 r 1\nQMATRIX\n x y 1\n"
+                           " y x 2\nENDATA\n"),
+    "duplicate_row": "NAME X\nROWS\n N obj\n L r\n G r\nCOLUMNS\n x obj 1 r 1\nENDATA\n",
+    "long_line": "NAME X\nROWS\n N obj\n L r\nCOLUMNS\n x obj 1 r 1" + " " * 1100 + "\nENDATA\n",
+    "columns_first": "NAME X\nCOLUMNS\n x obj 1\nROWS\n N obj\nENDATA\n",
+}
+
+
+@pytest.mark.parametrize("name", sorted(BAD))
+def test_rejected_files(H, tmp_path, name):
+    path = tmp_path / f"{name}.mps"
+    path.write_text(BAD[name])
+    with pytest.raises(ValueError, match="INVALID_ARGUMENT"):
+        H.read_from_mps(path)
+    if Q.ref_available():
+        r = Q.binding_solve(Q.ref_lib(), None, mps_path=path, use_gpu=0)
+        assert r.get("build_failed"), "the reference loader rejects it too"
+
+
+def test_quadratic_constraints_not_implemented(H, tmp_path):
+    path = tmp_path / "qc.mps"
+    path.write_text("NAME X\nROWS\n N obj\n L r\nCOLUMNS\n x obj 1 r 1\nRHS\n rhs r 1\nQCMATRIX r\n x x 1\nENDATA\n")
+    with pytest.raises(ValueError, match="NOT_IMPLEMENTED"):
+        H.read_from_mps(path)
+
+
+def test_missing_file(H, tmp_path):
+    with pytest.raises(ValueError, match="INVALID_ARGUMENT"):
+        H.read_from_mps(tmp_path / "none.mps")
