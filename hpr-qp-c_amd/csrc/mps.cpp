@@ -14,13 +14,18 @@
 //     alone gives [0, u] or (-inf, u] when u < 0;
 //   * QUADOBJ lists one triangle of Q (mirrored here), QMATRIX the full
 //     symmetric Q (checked); repeated entries are summed;
+//   * QCMATRIX <row> lists x'Hx of a quadratic constraint row; only
+//     diagonal H is accepted, on one-sided rows where the constraint is
+//     convex (H >= 0 on an L row, H <= 0 on a G row). Such rows leave A and
+//     become row cones through build_from_QAbc_qc_diagonal with D = 2H
+//     (src/shared/mps_qp.cpp:119-205); rows whose H is all zero stay linear;
 //   * maximisation, discrete variables and malformed cards are
-//     INVALID_ARGUMENT; quadratic constraints (QCMATRIX) are
-//     NOT_IMPLEMENTED (no QCQP path in this library).
+//     INVALID_ARGUMENT.
 #include <cerrno>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <algorithm>
 #include <fstream>
 #include <string>
 #include <unordered_map>
@@ -28,6 +33,7 @@
 #include <vector>
 
 #include "hprqp.h"
+#include "shared/qcqp.h"
 
 namespace {
 
@@ -74,6 +80,8 @@ void coo_to_csr(size_t nrow, size_t ncol, const Coo &coo, std::vector<size_t> &r
     rp.assign(nrow + 1, 0);
     ci.clear();
     val.clear();
+    ci.reserve(nz + 1);  // never empty: a valid CSR has non-null arrays
+    val.reserve(nz + 1);
     size_t prev_r = (size_t)-1, prev_c = (size_t)-1;
     for (size_t k : order) {
         if (coo.r[k] == prev_r && coo.c[k] == prev_c) {
@@ -98,8 +106,14 @@ struct MpsModel {
     std::vector<double> c, lvar, uvar;
     std::vector<int> discrete;  // 0 continuous, 1 inside an integer marker, 2 integer bound type
     Coo A, Q;
+    struct QcRow {
+        size_t row;
+        std::vector<std::pair<size_t, double>> diag;  // (column, H_jj) as listed
+    };
+    std::vector<QcRow> qc;
+    std::vector<char> qc_seen;
     double c0 = 0.0;
-    bool have_objective = false, maximise = false, full_q = false, quadratic_rows = false;
+    bool have_objective = false, maximise = false, full_q = false;
     std::string rhs_set, range_set, bound_set;
 };
 
@@ -234,7 +248,10 @@ void parse(std::istream &in, MpsModel &M)
                 if (!seen_cols || f.size() != 2) fail(line, "QCMATRIX needs a row name after COLUMNS");
                 const long r = row_of(M, f[1]);
                 if (r <= 0) fail(line, "QCMATRIX on a non-constraint row " + f[1]);
-                M.quadratic_rows = true;
+                if (M.qc_seen.empty()) M.qc_seen.assign(M.row_type.size(), 0);
+                if (M.qc_seen[(size_t)r - 1]) fail(line, "second QCMATRIX block for row " + f[1]);
+                M.qc_seen[(size_t)r - 1] = 1;
+                M.qc.push_back({(size_t)r - 1, {}});
                 sec = Section::QcMatrix;
             } else if (h == "ENDATA") {
                 ended = true;
@@ -347,8 +364,14 @@ void parse(std::istream &in, MpsModel &M)
             M.Q.add((size_t)a->second, (size_t)b->second, v1);
             break;
         }
-        case Section::QcMatrix:
-            break;  // quadratic constraints are rejected after parsing
+        case Section::QcMatrix: {
+            if (nf != 3 || !to_number(f[2], &v1)) fail(line, "invalid QCMATRIX coefficient");
+            auto a = M.vars.find(f[0]), b = M.vars.find(f[1]);
+            if (a == M.vars.end() || b == M.vars.end()) fail(line, "unknown variable in QCMATRIX");
+            if (a->second != b->second && v1 != 0.0) fail(line, "off-diagonal QCMATRIX entry");
+            if (v1 != 0.0) M.qc.back().diag.emplace_back((size_t)a->second, v1);
+            break;
+        }
         default:
             fail(line, "data outside an MPS section");
         }
@@ -366,6 +389,74 @@ void parse(std::istream &in, MpsModel &M)
             M.uvar[j] = INFINITY;
         }
     }
+}
+
+// Quadratic constraint rows leave A and become the diagonal lift's C, D and
+// constants (src/shared/mps_qp.cpp:123-205). A row's H entries are summed
+// per column in ascending (column, value) order; zero sums are dropped, and a
+// row left without entries stays linear. D = 2H times the row's sense sign
+// (+1 for L rows, -1 for G rows), which must come out non-negative.
+HPRQPStatus build_quadratic_rows(MpsModel &M, const HPRQPMatrixCSR &Q, const HPRQPMatrixCSR &A, HPRQPModel *model)
+{
+    const size_t m = A.nrow, n = A.ncol;
+    std::vector<const MpsModel::QcRow *> of_row(m, nullptr);
+    for (auto &q : M.qc) {
+        auto &d = q.diag;
+        std::sort(d.begin(), d.end());
+        size_t w = 0;
+        for (const auto &e : d) {
+            if (w && d[w - 1].first == e.first) d[w - 1].second += e.second;
+            else d[w++] = e;
+        }
+        d.resize(w);
+        d.erase(std::remove_if(d.begin(), d.end(), [](const auto &e) { return e.second == 0.0; }), d.end());
+        if (!d.empty()) of_row[q.row] = &q;
+    }
+    std::vector<size_t> lrp{0}, lci, crp{0}, cci, drp{0}, dci;
+    std::vector<double> lval, cval, dval, lo, hi, constant;
+    for (auto *v : {&lci, &cci, &dci}) v->reserve(A.nnz + 1);  // never empty
+    for (auto *v : {&lval, &cval, &dval}) v->reserve(A.nnz + 1);
+    for (size_t r = 0; r < m; ++r) {
+        if (!of_row[r]) {
+            for (size_t k = A.row_ptr[r]; k < A.row_ptr[r + 1]; ++k) {
+                lci.push_back(A.col_ind[k]);
+                lval.push_back(A.values[k]);
+            }
+            lrp.push_back(lci.size());
+            lo.push_back(M.lcon[r]);
+            hi.push_back(M.ucon[r]);
+            continue;
+        }
+        double sign;
+        if (M.lcon[r] == -INFINITY && std::isfinite(M.ucon[r])) sign = 1.0;
+        else if (M.ucon[r] == INFINITY && std::isfinite(M.lcon[r])) sign = -1.0;
+        else {
+            std::fprintf(stderr, "Error: a QCMATRIX row needs a one-sided inequality (no equality, range or free row).\n");
+            return HPRQP_STATUS_INVALID_ARGUMENT;
+        }
+        for (const auto &e : of_row[r]->diag) {
+            const double dv = e.second * (2.0 * sign);
+            if (!std::isfinite(dv) || dv < 0.0) {
+                std::fprintf(stderr, "Error: a QCMATRIX row is not convex in its inequality sense.\n");
+                return HPRQP_STATUS_INVALID_ARGUMENT;
+            }
+            dci.push_back(e.first);
+            dval.push_back(dv);
+        }
+        drp.push_back(dci.size());
+        for (size_t k = A.row_ptr[r]; k < A.row_ptr[r + 1]; ++k) {
+            cci.push_back(A.col_ind[k]);
+            cval.push_back(sign * A.values[k]);
+        }
+        crp.push_back(cci.size());
+        constant.push_back(sign > 0.0 ? -M.ucon[r] : M.lcon[r]);
+    }
+    const size_t nq = constant.size(), nl = lo.size();
+    const HPRQPMatrixCSR L{nl, n, lci.size(), lrp.data(), lci.data(), lval.data(), 0};
+    const HPRQPMatrixCSR Cm{nq, n, cci.size(), crp.data(), cci.data(), cval.data(), 0};
+    const HPRQPMatrixCSR Dm{nq, n, dci.size(), drp.data(), dci.data(), dval.data(), 0};
+    return build_from_QAbc_qc_diagonal(&Q, &L, M.c.data(), lo.data(), hi.data(), M.lvar.data(), M.uvar.data(), M.c0,
+                                       &Cm, constant.data(), &Dm, model);
 }
 
 }  // namespace
@@ -397,10 +488,6 @@ extern "C" HPRQPStatus build_from_mps(const char *mps_path, HPRQPModel *model)
             std::fprintf(stderr, "Error: discrete MPS variables are unsupported.\n");
             return HPRQP_STATUS_INVALID_ARGUMENT;
         }
-    if (M.quadratic_rows) {
-        std::fprintf(stderr, "Error: quadratic constraints (QCMATRIX) are not supported by this library.\n");
-        return HPRQP_STATUS_NOT_IMPLEMENTED;
-    }
     const size_t n = M.c.size(), m = M.row_type.size();
     try {
         std::vector<size_t> arp, aci, qrp, qci;
@@ -432,6 +519,7 @@ extern "C" HPRQPStatus build_from_mps(const char *mps_path, HPRQPModel *model)
         }
         HPRQPMatrixCSR Qm{n, n, qci.size(), qrp.data(), qci.data(), qval.data(), 0};
         HPRQPMatrixCSR Am{m, n, aci.size(), arp.data(), aci.data(), aval.data(), 0};
+        if (!M.qc.empty()) return build_quadratic_rows(M, Qm, Am, model);
         return build_from_QAbc(&Qm, &Am, M.c.data(), M.lcon.data(), M.ucon.data(), M.lvar.data(), M.uvar.data(),
                                M.c0, model);
     } catch (const std::bad_alloc &) {

@@ -255,6 +255,47 @@ def build_from_qabc(Q, c, A, AL, AU, l, u, obj_constant: float = 0.0, soc: dict 
     return model
 
 
+_LIB.build_from_QAbc_qc_factors.argtypes = ([C.POINTER(MatrixCSR), C.POINTER(MatrixCSR)] + [_db_p] * 5
+                                             + [C.c_double, C.POINTER(MatrixCSR), _db_p, C.POINTER(MatrixCSR),
+                                                _sz_p, C.POINTER(CModel)])
+_LIB.build_from_QAbc_qc_diagonal.argtypes = ([C.POINTER(MatrixCSR), C.POINTER(MatrixCSR)] + [_db_p] * 5
+                                              + [C.c_double, C.POINTER(MatrixCSR), _db_p, C.POINTER(MatrixCSR),
+                                                 C.POINTER(CModel)])
+
+
+def build_from_qabc_qc(Q, c, A, AL, AU, l, u, obj_constant: float, Cq, constant, R=None, offsets=None,
+                       D=None) -> Model:
+    """Quadratic constraints 0.5*||R_i x||^2 + C_i x + constant_i <= 0 lifted
+    to row cones (build_from_QAbc_qc_factors; R's rows offsets[i] ..
+    offsets[i+1]-1 form R_i) or, with D instead of R and offsets,
+    0.5*sum_j D_ij x_j^2 + C_i x + constant_i <= 0
+    (build_from_QAbc_qc_diagonal). Matrices as in build_from_qabc."""
+    c = np.asarray(c, np.float64)
+    n = c.size
+    keep: list = []
+    qn, qm, qrp, qci, qv = _csr_of(Q)
+    an, am, arp, aci, av = _csr_of(A, n)
+    if an == 0:
+        am = n
+    Qm = _mat(qn, qm, qrp, qci, qv, keep)
+    Am = _mat(an, am, arp if an else np.zeros(1, np.uint64), aci, av, keep)
+    Cm = _mat(*_csr_of(Cq, n), keep)
+    args = [C.byref(Qm), C.byref(Am), _dvec(c, keep), _dvec(AL, keep), _dvec(AU, keep), _dvec(l, keep),
+            _dvec(u, keep), float(obj_constant), C.byref(Cm), _dvec(constant, keep)]
+    model = Model()
+    if D is not None:
+        Dm = _mat(*_csr_of(D, n), keep)
+        st = _LIB.build_from_QAbc_qc_diagonal(*args, C.byref(Dm), C.byref(model.c))
+    else:
+        Rm = _mat(*_csr_of(R, n), keep)
+        off = np.ascontiguousarray(offsets, np.uint64)
+        keep.append(off)
+        st = _LIB.build_from_QAbc_qc_factors(*args, C.byref(Rm), off.ctypes.data_as(_sz_p), C.byref(model.c))
+    if st != STATUS_OK:
+        raise ValueError(f"build_from_QAbc_qc failed: {status_name(st)}")
+    return model
+
+
 _LIB.build_from_mps.argtypes = [C.c_char_p, C.POINTER(CModel)]
 
 

@@ -4,8 +4,10 @@
  * the same name:
  *   model_init/free/copy           src/shared/build.c:402-502
  *   build_from_QAbc[_soc]          src/shared/build.c:602-621
- *   build_from_mps                 src/shared/mps_qp.cpp (returns
- *                                  HPRQP_STATUS_NOT_IMPLEMENTED here)
+ *   build_from_mps                 src/shared/mps_qp.cpp:206-326 with
+ *                                  src/shared/mps_reader.cpp
+ *   build_from_QAbc_qc_factors,    src/shared/qcqp.c:16-99 (declared in
+ *   build_from_QAbc_qc_diagonal    shared/qcqp.h, included below)
  *   params_set_defaults            src/shared/solver.c:133-157
  *   results_init/free              src/shared/solver.c:159-177
  *   print_model_info/params_info   src/shared/solver_backend.c:240-271
@@ -22,6 +24,7 @@ extern "C" {
 #endif
 
 #include "gpu/gpu.h"
+#include "shared/qcqp.h"
 #include "shared/sparse.h"
 #include "shared/types.h"
 

@@ -357,6 +357,66 @@ class OracleSession:
 
 # --------------------------------------------------------------------------- reference (compiled from source)
 
+class _CSR(C.Structure):
+    """HPRQPMatrixCSR (the reference's include/shared/types.h)."""
+    _fields_ = [("nrow", C.c_size_t), ("ncol", C.c_size_t), ("nnz", C.c_size_t), ("row_ptr", c_size_p),
+                ("col_ind", c_size_p), ("values", c_double_p), ("owns_data", C.c_int)]
+
+
+class _CModel(C.Structure):
+    """HPRQPModel (the reference's include/shared/types.h:50-66)."""
+    _fields_ = [("Q", _CSR), ("A", _CSR), ("AT", _CSR), ("c", c_double_p), ("AL", c_double_p),
+                ("AU", c_double_p), ("l", c_double_p), ("u", c_double_p), ("obj_constant", C.c_double),
+                ("owns_data", C.c_int), ("num_var_cones", C.c_size_t), ("var_cone_offsets", c_size_p),
+                ("num_row_cones", C.c_size_t), ("row_cone_offsets", c_size_p), ("soc_rhs", c_double_p)]
+
+
+def qp_of_cmodel(m) -> QP:
+    """Copy a built HPRQPModel (any library's) into a QP, cones included."""
+    def arr(p, k, dt):
+        return np.ctypeslib.as_array(p, shape=(k,)).astype(dt, copy=True) if k else np.zeros(0, dt)
+
+    def csr(M):
+        return arr(M.row_ptr, M.nrow + 1, np.uint64), arr(M.col_ind, M.nnz, np.uint64), arr(M.values, M.nnz, float)
+    n, mm = m.Q.ncol, m.A.nrow
+    qrp, qci, qv = csr(m.Q)
+    arp, aci, av = csr(m.A)
+    qp = QP(n, mm, qrp, qci, qv, arp, aci, av, arr(m.c, n, float), arr(m.AL, mm, float), arr(m.AU, mm, float),
+            arr(m.l, n, float), arr(m.u, n, float), m.obj_constant)
+    if m.num_var_cones:
+        qp.var_offsets = arr(m.var_cone_offsets, m.num_var_cones + 1, np.uint64)
+    if m.num_row_cones:
+        qp.row_offsets = arr(m.row_cone_offsets, m.num_row_cones + 1, np.uint64)
+        qp.soc_rhs = arr(m.soc_rhs, mm, float)
+    return qp
+
+
+def ref_model_from_mps(path):
+    """The reference loader's model (build_from_mps of oracle/_ref), or None
+    when it rejects the file."""
+    L = ref_lib()
+    m = _CModel()
+    st = L.build_from_mps(str(path).encode(), C.byref(m))
+    try:
+        return qp_of_cmodel(m) if st == 0 else None
+    finally:
+        L.model_free(C.byref(m))
+
+
+def assert_same_model(a: QP, b: QP):
+    """Bitwise equality of two built models, cones included."""
+    assert (a.n, a.m, a.num_var_cones, a.num_row_cones) == (b.n, b.m, b.num_var_cones, b.num_row_cones)
+    assert a.obj_constant == b.obj_constant or (math.isnan(a.obj_constant) and math.isnan(b.obj_constant))
+    for f in ("q_ptr", "q_ind", "q_val", "a_ptr", "a_ind", "a_val", "c", "AL", "AU", "l", "u", "var_offsets",
+              "row_offsets", "soc_rhs"):
+        x, y = getattr(a, f), getattr(b, f)
+        if x is None or y is None:
+            assert x is None and y is None, f
+            continue
+        np.testing.assert_array_equal(x, y, err_msg=f)
+        assert np.array_equal(np.signbit(x), np.signbit(y)) if x.dtype.kind == "f" else True, f
+
+
 REF_PATH = ROOT / "oracle" / "_ref" / "libhprqp_ref.so"
 _ref = None
 
@@ -388,6 +448,8 @@ def ref_lib():
         L.hprqp_results_create.argtypes = [C.POINTER(vp)]
         L.hprqp_results_destroy.argtypes = [vp]
         L.hprqp_optimize.argtypes = [vp, vp, vp]
+        L.build_from_mps.argtypes = [C.c_char_p, C.POINTER(_CModel)]
+        L.model_free.argtypes = [C.POINTER(_CModel)]
         for nm in ["status", "num_variables", "num_constraints", "iter", "iter_4", "iter_6"]:
             f = getattr(L, "hprqp_results_" + nm)
             f.argtypes = [vp]
@@ -730,14 +792,16 @@ _BOUND_KINDS = {
 }
 
 
-def write_mps(qp: QP, path, seed=0, quad="QUADOBJ", extras=True):
+def write_mps(qp: QP, path, seed=0, quad="QUADOBJ", extras=True, qc=None):
     """Write `qp` (its A, c, row bounds and Q) as a free-format MPS / QPS file
     that exercises the format: ranges on L, G and E rows, all continuous bound
     types (variable bounds drawn from _BOUND_KINDS, not qp.l / qp.u), an
     objective constant through the objective row's RHS, two-pair cards, D
     exponents, comments, and (extras) an ignored second objective row and
     ignored second RHS / RANGES / BOUNDS sets. quad: "QUADOBJ" (upper
-    triangle) or "QMATRIX" (full matrix)."""
+    triangle) or "QMATRIX" (full matrix). qc: {row: [(column, h), ...]}
+    written as QCMATRIX blocks (diagonal cards x_j x_j h, in the given order,
+    repeats allowed)."""
     rng = np.random.default_rng(seed)
     n, m = qp.n, qp.m
     A = qp.dense_A()
@@ -801,6 +865,9 @@ def write_mps(qp: QP, path, seed=0, quad="QUADOBJ", extras=True):
         for j in range(n):
             if Qd[i, j] != 0.0 and (quad == "QMATRIX" or j >= i):
                 L.append(f"    X{i}  X{j}  {num(Qd[i, j])}")
+    for i, cards in (qc or {}).items():
+        L.append(f"QCMATRIX   R{i}")
+        L += [f"    X{j}  X{j}  {num(h)}" for j, h in cards]
     L.append("ENDATA")
     with open(path, "w") as fh:
         fh.write("\n".join(L) + "\n")

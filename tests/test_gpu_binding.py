@@ -62,3 +62,43 @@ def test_generated_mps_model_on_gpu(H, tmp_path):
         assert r.iter == o.iter and r.status == Q.STATUS[o.status]
         scale = max(1.0, float(np.max(np.abs(x))))
         assert np.max(np.abs(r.x - x)) <= 1e-10 * scale, k
+
+
+def test_quadratic_constraint_mps_solved_on_gpu(H, tmp_path):
+    """tests/test_mps_qcqp.cpp:104-117: min -x + 3 with .5x^2 + x <= 4,
+    x in [0, 4], y = 0, read through QCMATRIX and solved as a row cone:
+    x = 2, objective 1."""
+    from test_qcqp import fixture
+    path = tmp_path / "qc.mps"
+    path.write_text(fixture("L", "1", "4", " X X .5\n"))
+    model = H.read_from_mps(path)
+    print("qcqp fixture", flush=True)
+    r = H.optimize(model, H.Parameters(verbose=False, stoptol=1e-8, time_limit=30))
+    assert r.status == "OPTIMAL" and r.residual <= 1e-8
+    assert abs(r.x[0] - 2) < 1e-6 and abs(r.objective - 1) < 1e-6
+    assert .5 * r.x[0] ** 2 + r.x[0] <= 4 + 1e-6
+
+
+@pytest.mark.parametrize("seed", [11, 12])
+def test_generated_quadratic_rows_on_gpu(H, tmp_path, seed):
+    """A generated QCMATRIX model (lifted to row cones) on the GPU against the
+    oracle on the same model, per iterate."""
+    from test_qcqp import qc_mps
+    path = tmp_path / "gqc.mps"
+    qc_mps(path, seed=seed, n=60)
+    model = H.read_from_mps(path)
+    qp = model_arrays(model)
+    assert qp.num_row_cones > 0
+    print("generated qcqp", seed, flush=True)
+    for k in (1, 50, 100):
+        r = H.optimize(model, H.Parameters(verbose=False, max_iter=k))
+        o, x, y, z, w = Q.OracleSession(qp, max_iter=k).solve()
+        assert r.iter == o.iter and r.status == Q.STATUS[o.status]
+        for got, want in ((r.x, x), (r.y, y), (r.z, z)):
+            scale = max(1.0, float(np.max(np.abs(want))))
+            assert np.max(np.abs(got - want)) <= 1e-10 * scale, k
+    r = H.optimize(model, H.Parameters(verbose=False, stoptol=1e-8))
+    o, x, y, z, w = Q.OracleSession(qp, stoptol=1e-8).solve()
+    assert r.status == Q.STATUS[o.status] == "OPTIMAL"
+    assert r.objective == pytest.approx(o.primal_obj, rel=1e-7, abs=1e-9)
+    np.testing.assert_allclose(r.x, x, rtol=1e-5, atol=1e-6)

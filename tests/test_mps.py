@@ -24,21 +24,8 @@ def H():
 
 
 def model_arrays(model):
-    """The preprocessed model as a qp_libs.QP."""
-    m = model.c
-
-    def csr(M):
-        rp = np.ctypeslib.as_array(M.row_ptr, shape=(M.nrow + 1,)).copy()
-        ci = np.ctypeslib.as_array(M.col_ind, shape=(max(M.nnz, 1),))[:M.nnz].copy()
-        v = np.ctypeslib.as_array(M.values, shape=(max(M.nnz, 1),))[:M.nnz].copy()
-        return rp, ci, v
-
-    n, mm = m.Q.ncol, m.A.nrow
-    vec = lambda p, k: np.ctypeslib.as_array(p, shape=(k,)).copy() if k else np.zeros(0)
-    qrp, qci, qv = csr(m.Q)
-    arp, aci, av = csr(m.A)
-    return Q.QP(n, mm, qrp, qci, qv, arp, aci, av, vec(m.c, n), vec(m.AL, mm), vec(m.AU, mm), vec(m.l, n),
-                vec(m.u, n), m.obj_constant)
+    """The preprocessed model as a qp_libs.QP (cones included)."""
+    return Q.qp_of_cmodel(model.c)
 
 
 def test_simple_qp_fixture_model(H):
@@ -103,6 +90,7 @@ def test_model_equals_reference_loader(H, tmp_path, kind, arg, seed):
         path = tmp_path / f"gen{seed}.mps"
         Q.write_mps(qp0, path, seed=seed, quad=arg)
     qp = model_arrays(H.read_from_mps(path))
+    Q.assert_same_model(qp, Q.ref_model_from_mps(path))
     for k in (3, 40, 300):
         extra = {"max_iter": k}
         r = Q.binding_solve(Q.ref_lib(), None, mps_path=path, use_gpu=0, **extra)
@@ -141,13 +129,6 @@ def test_rejected_files(H, tmp_path, name):
     if Q.ref_available():
         r = Q.binding_solve(Q.ref_lib(), None, mps_path=path, use_gpu=0)
         assert r.get("build_failed"), "the reference loader rejects it too"
-
-
-def test_quadratic_constraints_not_implemented(H, tmp_path):
-    path = tmp_path / "qc.mps"
-    path.write_text("NAME X\nROWS\n N obj\n L r\nCOLUMNS\n x obj 1 r 1\nRHS\n rhs r 1\nQCMATRIX r\n x x 1\nENDATA\n")
-    with pytest.raises(ValueError, match="NOT_IMPLEMENTED"):
-        H.read_from_mps(path)
 
 
 def test_missing_file(H, tmp_path):
