@@ -76,6 +76,18 @@ def test_generated_file_semantics(H, tmp_path):
             assert got.AL[i] == pytest.approx(lo, abs=1e-12) and got.AU[i] == pytest.approx(hi, abs=1e-12)
 
 
+def test_linear_program_file(H, tmp_path):
+    """No QUADOBJ section: Q is the empty n x n matrix."""
+    path = tmp_path / "lp.mps"
+    path.write_text("NAME LP\nROWS\n N obj\n L r\n G s\nCOLUMNS\n x obj 1 r 1\n y obj -1 r 1\n y s 2\nRHS\n rhs r 4 s 1\n"
+                    "BOUNDS\n UP b y 3\nENDATA\n")
+    qp = model_arrays(H.read_from_mps(path))
+    assert (qp.n, qp.m, len(qp.q_val)) == (2, 2, 0)
+    np.testing.assert_array_equal(qp.dense_A(), [[1.0, 1.0], [0.0, 2.0]])
+    if Q.ref_available():
+        Q.assert_same_model(qp, Q.ref_model_from_mps(path))
+
+
 REF_CASES = [("fixture", "simple_qp.mps", None), ("fixture", "model.mps", None),
              ("gen", "QUADOBJ", 1), ("gen", "QMATRIX", 2), ("gen", "QUADOBJ", 3)]
 
