@@ -1276,6 +1276,91 @@ __global__ __launch_bounds__(kThreads) void dense_symmetrize_kernel(double *__re
     }
 }
 
+// Per-instance facts of a dense instance that the host needs before setup:
+// whether it fails build_from_QAbc's validation (src/shared/build.c:339-400,
+// plus: no row of A without entries), its number of free variables (noC
+// choice) and the largest finite |AL|, |AU| and |c| (original norms).
+struct DenseStats {
+    int bad;
+    int free_count;
+    double norm_b;
+    double norm_c;
+};
+
+// The same facts computed on the device from device-resident dense arrays
+// (one workgroup per instance; Q and A are read once, coalesced).
+__global__ __launch_bounds__(kThreads) void dense_inspect_kernel(const double *__restrict__ Q, const double *__restrict__ c,
+                                                                 const double *__restrict__ A,
+                                                                 const double *__restrict__ AL,
+                                                                 const double *__restrict__ AU,
+                                                                 const double *__restrict__ l,
+                                                                 const double *__restrict__ u, int n, int m,
+                                                                 DenseStats *__restrict__ out)
+{
+    __shared__ int rownz[kDenseMax];
+    __shared__ int s_bad, s_free;
+    __shared__ double s_nb[kThreads], s_nc[kThreads];
+    const long long k = blockIdx.x;
+    const int t = threadIdx.x;
+    for (int i = t; i < m; i += kThreads) rownz[i] = 0;
+    if (t == 0) s_bad = s_free = 0;
+    __syncthreads();
+    int bad = 0, fr = 0;
+    double nb = 0.0, nc = 0.0;
+    const double *Qk = Q + k * n * n;
+    for (int e = t; e < n * n; e += kThreads)
+        if (!isfinite(Qk[e])) bad = 1;
+    const double *Ak = A + k * m * n;
+    for (int e = t; e < m * n; e += kThreads) {
+        const double v = Ak[e];
+        if (!isfinite(v)) bad = 1;
+        if (v != 0.0) rownz[e / n] = 1;  // benign race: every writer stores 1
+    }
+    for (int i = t; i < m; i += kThreads) {
+        const double al = AL[k * m + i], au = AU[k * m + i];
+        if (isnan(al) || isnan(au) || al > au || (isinf(al) && al < 0 && isinf(au) && au > 0)) bad = 1;
+        nb = fmax(nb, fmax(isinf(al) ? 0.0 : fabs(al), isinf(au) ? 0.0 : fabs(au)));
+    }
+    for (int j = t; j < n; j += kThreads) {
+        const double cj = c[k * n + j], lj = l[k * n + j], uj = u[k * n + j];
+        if (!isfinite(cj) || isnan(lj) || isnan(uj) || lj > uj) bad = 1;
+        if (isinf(lj) && isinf(uj)) ++fr;
+        nc = fmax(nc, fabs(cj));
+    }
+    __syncthreads();
+    for (int i = t; i < m; i += kThreads)
+        if (!rownz[i]) bad = 1;
+    if (bad) atomicOr(&s_bad, 1);
+    if (fr) atomicAdd(&s_free, fr);
+    s_nb[t] = nb;
+    s_nc[t] = nc;
+    __syncthreads();
+    for (int w = kThreads / 2; w > 0; w >>= 1) {
+        if (t < w) {
+            s_nb[t] = fmax(s_nb[t], s_nb[t + w]);
+            s_nc[t] = fmax(s_nc[t], s_nc[t + w]);
+        }
+        __syncthreads();
+    }
+    if (t == 0) out[k] = DenseStats{s_bad, s_free, s_nb[0], s_nc[0]};
+}
+
+// Original-unit solution into caller buffers: x = x_bar / col, z = z_bar *
+// col (sum_n entries), y = y_bar / row (real_m entries).
+__global__ __launch_bounds__(kThreads) void batch_unscale_kernel(const double *__restrict__ vn, const double *__restrict__ vm,
+                                                                 long long sum_n, long long sum_m, long long real_m,
+                                                                 double *__restrict__ x, double *__restrict__ y,
+                                                                 double *__restrict__ z)
+{
+    const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+    if (i < sum_n) {
+        const double col = vn[(long long)N_col * sum_n + i];
+        if (x) x[i] = vn[(long long)N_x_bar * sum_n + i] / col;
+        if (z) z[i] = vn[(long long)N_z_bar * sum_n + i] * col;
+    }
+    if (y && i < real_m) y[i] = vm[(long long)M_y_bar * sum_m + i] / vm[(long long)M_row * sum_m + i];
+}
+
 }  // namespace batch
 }  // namespace hprqp
 
@@ -1296,6 +1381,7 @@ struct HPRQPHipBatch {
     hipStream_t stream = nullptr;
     double *d_vn = nullptr, *d_vm = nullptr, *d_val = nullptr, *d_dval = nullptr;
     int *d_rp = nullptr, *d_ci = nullptr, *d_perm = nullptr, *d_blocks = nullptr, *d_active = nullptr;
+    int *d_steps = nullptr;  // adjoint step counts (device-buffer adjoint)
     std::vector<int> h_rp;  // host copy of all CSR row pointers (block planning)
     hprqp::batch::Inst *d_inst = nullptr;
     hprqp::batch::State *d_state = nullptr;
@@ -1309,7 +1395,7 @@ struct HPRQPHipBatch {
         if (graph) (void)hipGraphExecDestroy(graph);
         if (d_order) (void)hipFree(d_order);
         for (void *p : {(void *)d_vn, (void *)d_vm, (void *)d_val, (void *)d_dval, (void *)d_rp, (void *)d_ci, (void *)d_perm,
-                        (void *)d_blocks, (void *)d_active, (void *)d_inst, (void *)d_state})
+                        (void *)d_blocks, (void *)d_active, (void *)d_inst, (void *)d_state, (void *)d_steps})
             if (p) (void)hipFree(p);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -1782,14 +1868,17 @@ HPRQPStatus hprqp_hip_batch_result(const HPRQPHipBatch *b, size_t k, HPRQPResult
     return HPRQP_STATUS_OK;
 }
 
-// Batch of `count` QPs of one shape given as dense row-major arrays: Q
-// (count x n x n), A (count x m x n), c, l, u (count x n), AL, AU (count x m).
-// Equivalent to build_from_QAbc on each instance (Q symmetrised on the
-// device) and hprqp_hip_batch_create, without building host models; every
-// row of A needs a nonzero entry and a finite bound.
-HPRQPStatus hprqp_hip_batch_create_dense(size_t count, size_t n, size_t m, const double *Q, const double *c,
-                                         const double *A, const double *AL, const double *AU, const double *l,
-                                         const double *u, const HPRQPParams *params, HPRQPHipBatch **out)
+}  // extern "C"
+
+namespace {
+
+// Shared by the host-array and device-array dense entry points. With
+// on_device the arrays live on the batch's device and are read on `caller`
+// (the stream that produced them), which is synchronised before returning;
+// otherwise they are host arrays.
+HPRQPStatus create_dense_impl(size_t count, size_t n, size_t m, const double *Q, const double *c, const double *A,
+                              const double *AL, const double *AU, const double *l, const double *u,
+                              const HPRQPParams *params, bool on_device, hipStream_t caller, HPRQPHipBatch **out)
 {
     using namespace hprqp::batch;
     if (!out || !params || !Q || !c || !l || !u || (m && (!A || !AL || !AU)) || count == 0 || n == 0 || m == 0 ||
@@ -1800,47 +1889,76 @@ HPRQPStatus hprqp_hip_batch_create_dense(size_t count, size_t n, size_t m, const
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return HPRQP_STATUS_NOT_IMPLEMENTED;
     if (params->device_number < 0 || params->device_number >= ndev) return HPRQP_STATUS_INVALID_ARGUMENT;
     HPRQP_HIP(hipSetDevice(params->device_number));
-    // validation (build.c:339-400 checks, plus: no row of A to delete)
-    std::atomic<int> bad{0};
-    {
+    const long long N = (long long)n, M = (long long)m, C = (long long)count;
+    const long long sum_n = C * N, sum_m = C * M;
+    std::vector<DenseStats> stats(count);
+    if (on_device) {
+        // the inputs must live on this device
+        for (const double *p : {Q, c, A, AL, AU, l, u}) {
+            hipPointerAttribute_t at{};
+            if (hipPointerGetAttributes(&at, p) != hipSuccess || at.type != hipMemoryTypeDevice ||
+                at.device != params->device_number) {
+                (void)hipGetLastError();
+                return HPRQP_STATUS_INVALID_ARGUMENT;
+            }
+        }
+        DenseStats *d_stats = nullptr;
+        HPRQP_HIP(hipMalloc((void **)&d_stats, sizeof(DenseStats) * count));
+        dense_inspect_kernel<<<(unsigned)count, hprqp::kThreads, 0, caller>>>(Q, c, A, AL, AU, l, u, (int)n, (int)m,
+                                                                          d_stats);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpyAsync(stats.data(), d_stats, sizeof(DenseStats) * count, hipMemcpyDeviceToHost,
+                                                caller);
+        if (e == hipSuccess) e = hipStreamSynchronize(caller);
+        (void)hipFree(d_stats);
+        HPRQP_HIP(e);
+    } else {
+        // validation (build.c:339-400 checks, plus: no row of A to delete)
         const size_t workers = std::max<size_t>(1, std::min<size_t>({16, count, (size_t)std::max(1u, std::thread::hardware_concurrency())}));
         std::atomic<size_t> next{0};
-        auto check = [&] {
-            for (size_t k; (k = next.fetch_add(1)) < count && !bad.load();) {
+        auto inspect = [&] {
+            for (size_t k; (k = next.fetch_add(1)) < count;) {
+                DenseStats &st = stats[k];
+                st = DenseStats{0, 0, 0.0, 0.0};
                 const double *Qk = Q + k * n * n, *Ak = A + k * m * n;
                 for (size_t e = 0; e < n * n; ++e)
-                    if (!std::isfinite(Qk[e])) bad = 1;
+                    if (!std::isfinite(Qk[e])) st.bad = 1;
                 for (size_t i = 0; i < m; ++i) {
                     bool nz = false;
                     for (size_t j = 0; j < n; ++j) {
                         const double v = Ak[i * n + j];
-                        if (!std::isfinite(v)) bad = 1;
+                        if (!std::isfinite(v)) st.bad = 1;
                         nz |= v != 0.0;
                     }
                     const double al = AL[k * m + i], au = AU[k * m + i];
-                    if (!nz || std::isnan(al) || std::isnan(au) || al > au || (std::isinf(al) && al < 0 && std::isinf(au) && au > 0))
-                        bad = 1;
+                    if (!nz || std::isnan(al) || std::isnan(au) || al > au ||
+                        (std::isinf(al) && al < 0 && std::isinf(au) && au > 0))
+                        st.bad = 1;
+                    st.norm_b = std::max(st.norm_b, std::max(std::isinf(al) ? 0.0 : std::fabs(al),
+                                                             std::isinf(au) ? 0.0 : std::fabs(au)));
                 }
                 for (size_t j = 0; j < n; ++j) {
                     const double cj = c[k * n + j], lj = l[k * n + j], uj = u[k * n + j];
-                    if (!std::isfinite(cj) || std::isnan(lj) || std::isnan(uj) || lj > uj) bad = 1;
+                    if (!std::isfinite(cj) || std::isnan(lj) || std::isnan(uj) || lj > uj) st.bad = 1;
+                    if (std::isinf(lj) && std::isinf(uj)) ++st.free_count;
+                    st.norm_c = std::max(st.norm_c, std::fabs(cj));
                 }
+                if (st.bad) return;  // the instance rejects the batch; others may stop early too
             }
         };
         std::vector<std::thread> pool;
-        for (size_t w = 1; w < workers; ++w) pool.emplace_back(check);
-        check();
+        for (size_t w = 1; w < workers; ++w) pool.emplace_back(inspect);
+        inspect();
         for (std::thread &th : pool) th.join();
     }
-    if (bad) return HPRQP_STATUS_INVALID_ARGUMENT;
+    for (const DenseStats &st : stats)
+        if (st.bad) return HPRQP_STATUS_INVALID_ARGUMENT;
     auto *b = new (std::nothrow) HPRQPHipBatch();
     if (!b) return HPRQP_STATUS_ALLOCATION_FAILED;
     b->count = count;
     b->time_limit = params->time_limit;
     b->P = Params{params->stoptol, (long long)std::min(params->max_iter, (size_t)INT64_MAX),
                   (long long)params->check_iter, params->print_frequency, params->use_scaling};
-    const long long N = (long long)n, M = (long long)m, C = (long long)count;
-    const long long sum_n = C * N, sum_m = C * M;
     b->inst.resize(count);
     for (size_t k = 0; k < count; ++k) {
         Inst &I = b->inst[k];
@@ -1853,26 +1971,14 @@ HPRQPStatus hprqp_hip_batch_create_dense(size_t count, size_t n, size_t m, const
         I.dq = (long long)k * N * N;
         I.dat = C * N * N + (long long)k * M * N;
         I.da = C * N * N + C * M * N + (long long)k * M * N;
-        size_t free_count = 0;
-        double nb = 0.0, nc = 0.0;
-        for (size_t j = 0; j < n; ++j) {
-            if (std::isinf(l[k * n + j]) && std::isinf(u[k * n + j])) ++free_count;
-            nc = std::max(nc, std::fabs(c[k * n + j]));
-        }
-        for (size_t i = 0; i < m; ++i) {
-            const double a = std::isinf(AL[k * m + i]) ? 0.0 : std::fabs(AL[k * m + i]);
-            const double v = std::isinf(AU[k * m + i]) ? 0.0 : std::fabs(AU[k * m + i]);
-            nb = std::max(nb, std::max(a, v));
-        }
-        I.noC = 5 * free_count > 4 * n ? 1 : 0;  // workspace.c:45-61
-        I.norm_b_org = nb;
-        I.norm_c_org = nc;
+        I.noC = 5 * (size_t)stats[k].free_count > 4 * n ? 1 : 0;  // workspace.c:45-61
+        I.norm_b_org = stats[k].norm_b;
+        I.norm_c_org = stats[k].norm_c;
         b->n.push_back(n);
         b->m.push_back(m);
         b->work.push_back((double)(n * n + 2 * m * n));
     }
     b->blocks_cap = 1;
-    HPRQPStatus st = HPRQP_STATUS_OK;
     const size_t dsz = (size_t)(C * N * N + 2 * C * M * N);
     auto fail = [&]() {
         delete b;
@@ -1888,18 +1994,21 @@ HPRQPStatus hprqp_hip_batch_create_dense(size_t count, size_t n, size_t m, const
         hipMalloc((void **)&b->d_blocks, sizeof(int)) != hipSuccess)
         return fail();
     hipStream_t s = b->stream;
-    bool ok = hipMemcpyAsync(b->d_dval, Q, sizeof(double) * (size_t)(C * N * N), hipMemcpyHostToDevice, s) == hipSuccess &&
-              hipMemcpyAsync(b->d_dval + C * N * N, A, sizeof(double) * (size_t)(C * M * N), hipMemcpyHostToDevice, s) ==
-                  hipSuccess &&
-              hipMemsetAsync(b->d_vn, 0, sizeof(double) * (size_t)sum_n * N_COUNT, s) == hipSuccess &&
+    // device inputs are copied on the caller's stream (ordered after their
+    // producer) and that stream is synchronised before the batch stream uses them
+    hipStream_t cs = on_device ? caller : s;
+    const hipMemcpyKind kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    bool ok = hipMemsetAsync(b->d_vn, 0, sizeof(double) * (size_t)sum_n * N_COUNT, s) == hipSuccess &&
               hipMemsetAsync(b->d_vm, 0, sizeof(double) * (size_t)sum_m * M_COUNT, s) == hipSuccess &&
-              hipMemcpyAsync(b->d_vn + N_c * sum_n, c, sizeof(double) * (size_t)sum_n, hipMemcpyHostToDevice, s) == hipSuccess &&
-              hipMemcpyAsync(b->d_vn + N_l * sum_n, l, sizeof(double) * (size_t)sum_n, hipMemcpyHostToDevice, s) == hipSuccess &&
-              hipMemcpyAsync(b->d_vn + N_u * sum_n, u, sizeof(double) * (size_t)sum_n, hipMemcpyHostToDevice, s) == hipSuccess &&
-              hipMemcpyAsync(b->d_vm + M_AL * sum_m, AL, sizeof(double) * (size_t)sum_m, hipMemcpyHostToDevice, s) ==
-                  hipSuccess &&
-              hipMemcpyAsync(b->d_vm + M_AU * sum_m, AU, sizeof(double) * (size_t)sum_m, hipMemcpyHostToDevice, s) ==
-                  hipSuccess;
+              (!on_device || hipStreamSynchronize(s) == hipSuccess) &&
+              hipMemcpyAsync(b->d_dval, Q, sizeof(double) * (size_t)(C * N * N), kind, cs) == hipSuccess &&
+              hipMemcpyAsync(b->d_dval + C * N * N, A, sizeof(double) * (size_t)(C * M * N), kind, cs) == hipSuccess &&
+              hipMemcpyAsync(b->d_vn + N_c * sum_n, c, sizeof(double) * (size_t)sum_n, kind, cs) == hipSuccess &&
+              hipMemcpyAsync(b->d_vn + N_l * sum_n, l, sizeof(double) * (size_t)sum_n, kind, cs) == hipSuccess &&
+              hipMemcpyAsync(b->d_vn + N_u * sum_n, u, sizeof(double) * (size_t)sum_n, kind, cs) == hipSuccess &&
+              hipMemcpyAsync(b->d_vm + M_AL * sum_m, AL, sizeof(double) * (size_t)sum_m, kind, cs) == hipSuccess &&
+              hipMemcpyAsync(b->d_vm + M_AU * sum_m, AU, sizeof(double) * (size_t)sum_m, kind, cs) == hipSuccess &&
+              (!on_device || hipStreamSynchronize(cs) == hipSuccess);
     if (!ok) return fail();
     dense_symmetrize_kernel<<<(unsigned)(C * N), hprqp::kThreads, 0, s>>>(b->d_dval, (int)n, (int)count);
     if (hipGetLastError() != hipSuccess) return fail();
@@ -1913,9 +2022,37 @@ HPRQPStatus hprqp_hip_batch_create_dense(size_t count, size_t n, size_t m, const
     b->ar.dval = b->d_dval;
     b->ar.sum_n = sum_n;
     b->ar.sum_m = sum_m;
-    st = batch_finish(b);
+    HPRQPStatus st = batch_finish(b);
     if (st == HPRQP_STATUS_OK) *out = b;
     return st;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Batch of `count` QPs of one shape given as dense row-major arrays: Q
+// (count x n x n), A (count x m x n), c, l, u (count x n), AL, AU (count x m).
+// Equivalent to build_from_QAbc on each instance (Q symmetrised on the
+// device) and hprqp_hip_batch_create, without building host models; every
+// row of A needs a nonzero entry and a finite bound.
+HPRQPStatus hprqp_hip_batch_create_dense(size_t count, size_t n, size_t m, const double *Q, const double *c,
+                                         const double *A, const double *AL, const double *AU, const double *l,
+                                         const double *u, const HPRQPParams *params, HPRQPHipBatch **out)
+{
+    return create_dense_impl(count, n, m, Q, c, A, AL, AU, l, u, params, false, nullptr, out);
+}
+
+// The same from arrays already in device memory of params->device_number
+// (e.g. tensors of a training step): validated and copied on the device,
+// read on `stream` (a hipStream_t; null = the null stream), which is
+// synchronised before the call returns.
+HPRQPStatus hprqp_hip_batch_create_dense_device(size_t count, size_t n, size_t m, const double *Q, const double *c,
+                                                const double *A, const double *AL, const double *AU, const double *l,
+                                                const double *u, const HPRQPParams *params, void *stream,
+                                                HPRQPHipBatch **out)
+{
+    return create_dense_impl(count, n, m, Q, c, A, AL, AU, l, u, params, true, (hipStream_t)stream, out);
 }
 
 // Every instance's solution in original units, instances concatenated:
@@ -1927,9 +2064,15 @@ HPRQPStatus hprqp_hip_batch_solution(const HPRQPHipBatch *b, double *x, double *
     using namespace hprqp::batch;
     if (!b || b->state.size() != b->count) return HPRQP_STATUS_INVALID_ARGUMENT;
     const size_t sn = (size_t)b->ar.sum_n, sm = (size_t)b->ar.sum_m;
-    std::vector<double> col(sn), row(sm);
-    HPRQP_HIP(hipMemcpy(col.data(), b->d_vn + (size_t)N_col * sn, sizeof(double) * sn, hipMemcpyDeviceToHost));
-    HPRQP_HIP(hipMemcpy(row.data(), b->d_vm + (size_t)M_row * sm, sizeof(double) * sm, hipMemcpyDeviceToHost));
+    std::vector<double> col, row;
+    if (x || z) {
+        col.resize(sn);
+        HPRQP_HIP(hipMemcpy(col.data(), b->d_vn + (size_t)N_col * sn, sizeof(double) * sn, hipMemcpyDeviceToHost));
+    }
+    if (y) {
+        row.resize(sm);
+        HPRQP_HIP(hipMemcpy(row.data(), b->d_vm + (size_t)M_row * sm, sizeof(double) * sm, hipMemcpyDeviceToHost));
+    }
     size_t real_m = 0;
     for (size_t k = 0; k < b->count; ++k) real_m += b->m[k];
     if (x) {
@@ -1951,6 +2094,24 @@ HPRQPStatus hprqp_hip_batch_solution(const HPRQPHipBatch *b, double *x, double *
         if (objective) objective[k] = S.pobj;
         if (kkt) kkt[k] = S.kkt;
     }
+    return HPRQP_STATUS_OK;
+}
+
+// Every instance's x, z (sum of n) and y (sum of m) in original units into
+// device buffers on the batch's device (any may be null), written by a
+// kernel on `stream` (a hipStream_t; null = the null stream) without a host
+// synchronisation. Same values as hprqp_hip_batch_solution.
+HPRQPStatus hprqp_hip_batch_solution_device(const HPRQPHipBatch *b, double *x, double *y, double *z, void *stream)
+{
+    using namespace hprqp::batch;
+    if (!b || b->state.size() != b->count) return HPRQP_STATUS_INVALID_ARGUMENT;
+    if (!x && !y && !z) return HPRQP_STATUS_OK;
+    long long real_m = 0;
+    for (size_t k = 0; k < b->count; ++k) real_m += (long long)b->m[k];
+    const long long len = std::max(b->ar.sum_n, real_m);
+    batch_unscale_kernel<<<(unsigned)((len + hprqp::kThreads - 1) / hprqp::kThreads), hprqp::kThreads, 0,
+                           (hipStream_t)stream>>>(b->d_vn, b->d_vm, b->ar.sum_n, b->ar.sum_m, real_m, x, y, z);
+    HPRQP_HIP(hipGetLastError());
     return HPRQP_STATUS_OK;
 }
 
@@ -1999,6 +2160,37 @@ HPRQPStatus hprqp_hip_batch_adjoint(HPRQPHipBatch *b, const double *dl_dx, doubl
     (void)hipFree(d_vr);
     (void)hipFree(d_vb);
     (void)hipFree(d_steps);
+    return HPRQP_STATUS_OK;
+}
+
+// The adjoint with device buffers: dl_dx, u, v_bounds (sum of n) and v_rows
+// (sum of m) on the batch's device, the kernels launched on `stream` (a
+// hipStream_t; null = the null stream). steps (host, count entries) may be
+// null; when given, the call waits for the kernels to read it back.
+HPRQPStatus hprqp_hip_batch_adjoint_device(HPRQPHipBatch *b, const double *dl_dx, double tol, int max_steps, double *u,
+                                           double *v_rows, double *v_bounds, int *steps, void *stream)
+{
+    using namespace hprqp::batch;
+    if (!b || !dl_dx || !u || !v_bounds || max_steps <= 0 || b->state.size() != b->count)
+        return HPRQP_STATUS_INVALID_ARGUMENT;
+    size_t real_m = 0;
+    for (size_t k = 0; k < b->count; ++k) real_m += b->m[k];
+    if (real_m && !v_rows) return HPRQP_STATUS_INVALID_ARGUMENT;
+    if (!b->d_steps) HPRQP_HIP(hipMalloc((void **)&b->d_steps, sizeof(int) * b->count));
+    hipStream_t s = (hipStream_t)stream;
+    if (b->any_sparse)
+        batch_adjoint_kernel<false><<<(unsigned)b->count, hprqp::kThreads, 0, s>>>(b->ar, b->P, (int)b->count, dl_dx,
+                                                                                 u, v_rows, v_bounds, tol, max_steps,
+                                                                                 b->d_steps);
+    if (b->any_dense)
+        batch_adjoint_kernel<true><<<(unsigned)b->count, hprqp::kThreads, 0, s>>>(b->ar, b->P, (int)b->count, dl_dx,
+                                                                                u, v_rows, v_bounds, tol, max_steps,
+                                                                                b->d_steps);
+    HPRQP_HIP(hipGetLastError());
+    if (steps) {
+        HPRQP_HIP(hipMemcpyAsync(steps, b->d_steps, sizeof(int) * b->count, hipMemcpyDeviceToHost, s));
+        HPRQP_HIP(hipStreamSynchronize(s));
+    }
     return HPRQP_STATUS_OK;
 }
 

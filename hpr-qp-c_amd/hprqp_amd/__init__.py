@@ -446,6 +446,26 @@ class Batch:
         b.h = h
         return b
 
+    @classmethod
+    def from_dense_device(cls, count: int, n: int, m: int, ptrs: Sequence[int], stream: int = 0,
+                          params: Parameters | None = None) -> "Batch":
+        """hprqp_hip_batch_create_dense_device: as from_dense, with Q, c, A,
+        AL, AU, l, u given as device addresses (float64, contiguous, shapes
+        as in from_dense) on params.device_number, read on `stream` (a
+        hipStream_t handle as an integer; 0 = the null stream)."""
+        b = cls(None, params)
+        if len(ptrs) != 7:
+            raise ValueError("Batch.from_dense_device: seven device addresses expected")
+        b.models = None
+        b.ns, b.ms = [n] * count, [m] * count
+        h = C.c_void_p()
+        st = _LIB.hprqp_hip_batch_create_dense_device(count, n, m, *[C.c_void_p(p) for p in ptrs],
+                                                      C.byref(b.params.c), C.c_void_p(stream), C.byref(h))
+        if st != STATUS_OK:
+            raise ValueError(f"hprqp_hip_batch_create_dense_device failed: {status_name(st)}")
+        b.h = h
+        return b
+
     def __len__(self) -> int:
         return len(self.ns)
 
@@ -498,6 +518,29 @@ class Batch:
         return dict(x=x, y=y[:sm], z=z, status=[status_name(int(v)) for v in status], iter=iters.astype(np.int64),
                     objective=obj, kkt=kkt)
 
+    def status(self) -> dict:
+        """Per instance status names, iterations, objectives and KKT
+        residuals (hprqp_hip_batch_solution without the vectors)."""
+        cnt = len(self.ns)
+        status = np.zeros(cnt, np.int32)
+        iters = np.zeros(cnt, np.uint64)
+        obj, kkt = np.zeros(cnt), np.zeros(cnt)
+        st = _LIB.hprqp_hip_batch_solution(self.h, None, None, None, status.ctypes.data_as(C.POINTER(C.c_int)),
+                                           iters.ctypes.data_as(C.POINTER(C.c_size_t)), obj.ctypes.data_as(_db_p),
+                                           kkt.ctypes.data_as(_db_p))
+        if st != STATUS_OK:
+            raise RuntimeError(f"hprqp_hip_batch_solution failed: {status_name(st)}")
+        return dict(status=[status_name(int(v)) for v in status], iter=iters.astype(np.int64), objective=obj, kkt=kkt)
+
+    def solution_device(self, x: int, y: int, z: int, stream: int = 0) -> None:
+        """hprqp_hip_batch_solution_device: x, z (sum n) and y (sum m) in
+        original units written to device addresses (0 skips one) by a kernel
+        on `stream`, without waiting for it."""
+        st = _LIB.hprqp_hip_batch_solution_device(self.h, C.c_void_p(x or None), C.c_void_p(y or None),
+                                                  C.c_void_p(z or None), C.c_void_p(stream))
+        if st != STATUS_OK:
+            raise RuntimeError(f"hprqp_hip_batch_solution_device failed: {status_name(st)}")
+
 
 def solve_batch(models: Sequence[Model], params: Parameters | None = None, bodies_per_launch: int = 100):
     b = Batch(models, params)
@@ -526,6 +569,29 @@ def batch_adjoint_arrays(batch: Batch, dl_dx: np.ndarray, tol: float = 1e-12, ma
     if st != STATUS_OK:
         raise RuntimeError(f"hprqp_hip_batch_adjoint failed: {status_name(st)}")
     return u, vr[:sm], vb, steps
+
+
+_LIB.hprqp_hip_batch_create_dense_device.argtypes = ([C.c_size_t] * 3 + [C.c_void_p] * 7
+                                                     + [C.POINTER(CParams), C.c_void_p, C.POINTER(C.c_void_p)])
+_LIB.hprqp_hip_batch_solution_device.argtypes = [C.c_void_p] * 5
+_LIB.hprqp_hip_batch_adjoint_device.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.c_int, C.c_void_p, C.c_void_p,
+                                                C.c_void_p, C.POINTER(C.c_int), C.c_void_p]
+
+
+def batch_adjoint_device(batch: Batch, dl_dx: int, u: int, v_rows: int, v_bounds: int, stream: int = 0,
+                         tol: float = 1e-12, max_steps: int = 20000, want_steps: bool = False):
+    """hprqp_hip_batch_adjoint_device: the adjoint of batch_adjoint_arrays
+    with device addresses (dl_dx, u, v_bounds: sum n; v_rows: sum m),
+    launched on `stream`; returns the MINRES step counts (waiting for them)
+    when want_steps, else None without waiting."""
+    steps = np.zeros(len(batch.ns), np.int32) if want_steps else None
+    st = _LIB.hprqp_hip_batch_adjoint_device(batch.h, C.c_void_p(dl_dx), tol, max_steps, C.c_void_p(u),
+                                             C.c_void_p(v_rows or None), C.c_void_p(v_bounds),
+                                             steps.ctypes.data_as(C.POINTER(C.c_int)) if want_steps else None,
+                                             C.c_void_p(stream))
+    if st != STATUS_OK:
+        raise RuntimeError(f"hprqp_hip_batch_adjoint_device failed: {status_name(st)}")
+    return steps
 
 
 def batch_adjoint(batch: Batch, dl_dx: Sequence[np.ndarray], tol: float = 1e-12, max_steps: int = 20000):

@@ -12,6 +12,12 @@ K [u; v] = [dL/dx; 0]:
     dL/dAL_i = v_i (row active at its lower bound), dL/dAU_i = v_i (upper),
     equality rows split v_i evenly between AL_i and AU_i (so d/db = v_i),
     dL/dl_j, dL/du_j likewise from the bound multipliers.
+
+Tensors on a ROCm device stay there: the batch is created from their device
+memory (hprqp_hip_batch_create_dense_device, ordered after torch's current
+stream), the solution and the adjoint are written into torch tensors by
+kernels on that stream, and no array crosses PCIe. CPU tensors go through
+host arrays.
 """
 from __future__ import annotations
 
@@ -20,7 +26,7 @@ import time
 import numpy as np
 import torch
 
-from . import Batch, Parameters, batch_adjoint_arrays
+from . import Batch, Parameters, batch_adjoint_arrays, batch_adjoint_device
 
 
 def _np(t):
@@ -34,6 +40,9 @@ last_timings: dict = {}
 class QPLayer(torch.autograd.Function):
     @staticmethod
     def forward(ctx, Q, c, A, AL, AU, l, u, stoptol: float = 1e-8, bodies_per_launch: int = 200):
+        if Q.is_cuda:
+            return _forward_device(ctx, (Q, c, A, AL, AU, l, u), stoptol, bodies_per_launch)
+        ctx.on_device = False
         t0 = time.perf_counter()
         Qn, cn, An, ALn, AUn, ln, un = (_np(t) for t in (Q, c, A, AL, AU, l, u))
         B, n = cn.shape
@@ -67,6 +76,8 @@ class QPLayer(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, gx):
+        if ctx.on_device:
+            return _backward_device(ctx, gx)
         x, y, z, ALn, AUn, ln, un = ctx.sol
         B, n = x.shape
         m = y.shape[1]
@@ -79,18 +90,84 @@ class QPLayer(torch.autograd.Function):
         t = lambda a: torch.as_tensor(a, dtype=torch.float64, device=ctx.device)
         U, VR, VB = t(u.reshape(B, n)), t(vr.reshape(B, m)), t(vb.reshape(B, n))
         X, Y, Z = t(x), t(y), t(z)
-        P = torch.bmm(U[:, :, None], X[:, None, :])
-        dQ = (P + P.transpose(1, 2)).mul_(-0.5)
-        dA = torch.baddbmm(torch.bmm(Y[:, :, None], U[:, None, :]), VR[:, :, None], X[:, None, :], alpha=-1.0)
-        dc = -U
         eq, feq = t(ALn == AUn).bool(), t(ln == un).bool()
-        zero = torch.zeros((), dtype=torch.float64, device=ctx.device)
-        dAL = torch.where(eq, 0.5 * VR, torch.where(Y > 0, VR, zero))
-        dAU = torch.where(eq, 0.5 * VR, torch.where(Y < 0, VR, zero))
-        dl = torch.where(feq, 0.5 * VB, torch.where(Z > 0, VB, zero))
-        du = torch.where(feq, 0.5 * VB, torch.where(Z < 0, VB, zero))
+        grads = _gradients(U, VR, VB, X, Y, Z, eq, feq)
         last_timings["backward_host"] = time.perf_counter() - t0
-        return dQ, dc, dA, dAL, dAU, dl, du, None, None
+        return grads + (None, None)
+
+
+def _gradients(U, VR, VB, X, Y, Z, eq, feq):
+    """dL/d(Q, c, A, AL, AU, l, u) from the adjoint (U, VR, VB) at the
+    solution (X, Y, Z); eq / feq mark equality rows / fixed variables."""
+    P = torch.bmm(U[:, :, None], X[:, None, :])
+    dQ = (P + P.transpose(1, 2)).mul_(-0.5)
+    dA = torch.baddbmm(torch.bmm(Y[:, :, None], U[:, None, :]), VR[:, :, None], X[:, None, :], alpha=-1.0)
+    dc = -U
+    zero = torch.zeros((), dtype=torch.float64, device=U.device)
+    dAL = torch.where(eq, 0.5 * VR, torch.where(Y > 0, VR, zero))
+    dAU = torch.where(eq, 0.5 * VR, torch.where(Y < 0, VR, zero))
+    dl = torch.where(feq, 0.5 * VB, torch.where(Z > 0, VB, zero))
+    du = torch.where(feq, 0.5 * VB, torch.where(Z < 0, VB, zero))
+    return dQ, dc, dA, dAL, dAU, dl, du
+
+
+def _forward_device(ctx, inputs, stoptol, bodies_per_launch):
+    t0 = time.perf_counter()
+    dev = inputs[0].device
+    if any(t.device != dev for t in inputs):
+        raise ValueError("QPLayer: all inputs must be on one device")
+    ts = [t.detach().to(torch.float64).contiguous() for t in inputs]
+    B, n = ts[1].shape
+    m = ts[2].shape[1]
+    if ts[0].shape != (B, n, n) or ts[2].shape != (B, m, n) or any(t.shape != (B, m) for t in ts[3:5]) or any(
+            t.shape != (B, n) for t in ts[5:]):
+        raise ValueError("QPLayer: inconsistent shapes")
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    t1 = time.perf_counter()
+    try:
+        batch = Batch.from_dense_device(B, n, m, [t.data_ptr() for t in ts], stream,
+                                        Parameters(verbose=False, stoptol=stoptol, device_number=dev.index or 0))
+    except ValueError as e:
+        raise ValueError("QPLayer: rows with no entries or free bounds are not supported "
+                         f"(or invalid data): {e}") from None
+    t2 = time.perf_counter()
+    batch.solve(bodies_per_launch)
+    t3 = time.perf_counter()
+    info = batch.status()
+    bad = [s_ for s_ in info["status"] if s_ != "OPTIMAL"]
+    if bad:
+        raise RuntimeError(f"QPLayer forward: {len(bad)} instances not optimal ({bad[0]})")
+    x = torch.empty((B, n), dtype=torch.float64, device=dev)
+    y = torch.empty((B, m), dtype=torch.float64, device=dev)
+    z = torch.empty((B, n), dtype=torch.float64, device=dev)
+    batch.solution_device(x.data_ptr(), y.data_ptr(), z.data_ptr(), stream)
+    ctx.on_device = True
+    ctx.batch = batch
+    ctx.sol = (y, z, ts[3] == ts[4], ts[5] == ts[6])
+    ctx.save_for_backward(x)
+    last_timings.clear()
+    last_timings.update(to_host=t1 - t0, create=t2 - t1, solve=t3 - t2, device_solve=batch.seconds,
+                        collect=time.perf_counter() - t3, iterations=int(info["iter"].max()),
+                        iterations_sum=int(info["iter"].sum()))
+    return x
+
+
+def _backward_device(ctx, gx):
+    (x,) = ctx.saved_tensors
+    y, z, eq, feq = ctx.sol
+    B, n = x.shape
+    m = y.shape[1]
+    t0 = time.perf_counter()
+    g = gx.detach().to(torch.float64).contiguous()
+    U = torch.empty((B, n), dtype=torch.float64, device=x.device)
+    VB = torch.empty_like(U)
+    VR = torch.empty((B, m), dtype=torch.float64, device=x.device)
+    stream = torch.cuda.current_stream(x.device).cuda_stream
+    batch_adjoint_device(ctx.batch, g.data_ptr(), U.data_ptr(), VR.data_ptr(), VB.data_ptr(), stream)
+    last_timings["adjoint"] = time.perf_counter() - t0
+    grads = _gradients(U, VR, VB, x, y, z, eq, feq)
+    last_timings["backward_host"] = time.perf_counter() - t0
+    return grads + (None, None)
 
 
 def qp_layer(Q, c, A, AL, AU, l, u, stoptol: float = 1e-8):

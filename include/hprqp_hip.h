@@ -115,6 +115,27 @@ HPRQPStatus hprqp_hip_batch_solution(const HPRQPHipBatch *batch, double *x, doub
 HPRQPStatus hprqp_hip_batch_adjoint(HPRQPHipBatch *batch, const double *dl_dx, double tol, int max_steps, double *u,
                                     double *v_rows, double *v_bounds, int *steps);
 
+/* Zero-copy forms of the three calls above for callers whose data already
+ * lives in device memory of params->device_number (tensors of a training
+ * step): same arguments and results, with device pointers and the caller's
+ * stream (a hipStream_t, passed as void*; null = the null stream).
+ * _create_dense_device validates and copies the arrays on the device (read
+ * on `stream`, which is synchronised before it returns; INVALID_ARGUMENT
+ * for pointers that are not device memory of that device);
+ * _solution_device writes x, y, z (device, any may be null) with a kernel
+ * on `stream` and returns without waiting; _adjoint_device runs the adjoint
+ * kernels on `stream` with device dl_dx, u, v_rows, v_bounds and waits only
+ * when `steps` (host) is given. Calls on one batch must not overlap. No
+ * counterpart in the reference: the device-resident path of the
+ * differentiable layer. */
+HPRQPStatus hprqp_hip_batch_create_dense_device(size_t count, size_t n, size_t m, const double *Q, const double *c,
+                                                const double *A, const double *AL, const double *AU, const double *l,
+                                                const double *u, const HPRQPParams *params, void *stream,
+                                                HPRQPHipBatch **out_batch);
+HPRQPStatus hprqp_hip_batch_solution_device(const HPRQPHipBatch *batch, double *x, double *y, double *z, void *stream);
+HPRQPStatus hprqp_hip_batch_adjoint_device(HPRQPHipBatch *batch, const double *dl_dx, double tol, int max_steps,
+                                           double *u, double *v_rows, double *v_bounds, int *steps, void *stream);
+
 /* Copies `bytes` (a multiple of 16) between two fresh device buffers with a
  * plain streaming kernel and reports its time: the known-size kernel used to
  * calibrate HBM traffic counters. No counterpart in the reference. */

@@ -114,3 +114,98 @@ def test_layer_solver_fd_on_c(mods):
         xm = QPLayer.apply(args[0], torch.tensor(cm), *args[1:], 1e-11).numpy()
         fd = float(g[0] @ (xp[0] - xm[0]) / (2 * h))
         assert abs(tc.grad[0, j].item() - fd) <= 1e-3 * max(1.0, abs(fd)), (j, tc.grad[0, j].item(), fd)
+
+
+def test_device_batch_equals_host_batch(mods):
+    """hprqp_hip_batch_create_dense_device on device tensors builds the same
+    batch as the host-array entry point: solutions (device writer vs host
+    copy) and adjoints (device buffers vs host arrays) bit-identical."""
+    H, torch, _ = mods
+    B, n, meq, min_ = 6, 40, 8, 30
+    arrs = Q.layer_instances(B, n=n, m_eq=meq, m_in=min_, seed=5)
+    m = meq + min_
+    print("device batch", B, n, m, flush=True)
+    hb = H.Batch.from_dense(*arrs, H.Parameters(verbose=False, stoptol=1e-9))
+    hb.solve()
+    hs = hb.solution()
+    dev = torch.device("cuda:0")
+    ts = [torch.tensor(a, dtype=torch.float64, device=dev) for a in arrs]
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    db = H.Batch.from_dense_device(B, n, m, [t.data_ptr() for t in ts], stream,
+                                   H.Parameters(verbose=False, stoptol=1e-9))
+    db.solve()
+    info = db.status()
+    assert info["status"] == hs["status"] and np.array_equal(info["iter"], hs["iter"])
+    assert np.array_equal(info["objective"], hs["objective"])
+    x, y, z = (torch.full((B * k,), np.nan, dtype=torch.float64, device=dev) for k in (n, m, n))
+    db.solution_device(x.data_ptr(), y.data_ptr(), z.data_ptr(), stream)
+    torch.cuda.synchronize()
+    for name, got in (("x", x), ("y", y), ("z", z)):
+        np.testing.assert_array_equal(got.cpu().numpy(), hs[name], err_msg=name)
+    g = np.random.default_rng(1).normal(size=B * n)
+    u_h, vr_h, vb_h, steps_h = H.batch_adjoint_arrays(hb, g)
+    tg = torch.tensor(g, device=dev)
+    U, VB, VR = (torch.full((B * k,), np.nan, dtype=torch.float64, device=dev) for k in (n, n, m))
+    steps = H.batch_adjoint_device(db, tg.data_ptr(), U.data_ptr(), VR.data_ptr(), VB.data_ptr(), stream,
+                                   want_steps=True)
+    torch.cuda.synchronize()
+    assert np.array_equal(steps, steps_h)
+    for name, got, want in (("u", U, u_h), ("v_rows", VR, vr_h), ("v_bounds", VB, vb_h)):
+        np.testing.assert_array_equal(got.cpu().numpy(), want, err_msg=name)
+
+
+def test_device_create_rejects_bad_input(mods):
+    """Validation on the device: a NaN in Q, a row of A without entries and
+    an inverted bound are rejected like the host entry point rejects them;
+    host memory is refused as input."""
+    H, torch, _ = mods
+    B, n, meq, min_ = 3, 16, 4, 8
+    arrs = [np.array(a) for a in Q.layer_instances(B, n=n, m_eq=meq, m_in=min_, seed=2)]
+    m = meq + min_
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    params = H.Parameters(verbose=False)
+
+    def create(a):
+        ts = [torch.tensor(v, dtype=torch.float64, device=dev) for v in a]
+        return H.Batch.from_dense_device(B, n, m, [t.data_ptr() for t in ts], stream, params)
+    assert len(create(arrs)) == B
+    for k, (idx, val) in enumerate([((0, (1, 2, 3)), np.nan), ((2, (2, 5)), "zero_row"), ((4, (1, 3)), "inverted")]):
+        bad = [a.copy() for a in arrs]
+        which, pos = idx
+        if val == "zero_row":
+            bad[which][pos[0], pos[1], :] = 0.0
+        elif val == "inverted":
+            bad[3][pos] = bad[4][pos] + 1.0
+        else:
+            bad[which][pos] = val
+        print("reject case", k, flush=True)
+        with pytest.raises(ValueError, match="INVALID_ARGUMENT"):
+            create(bad)
+        with pytest.raises(ValueError, match="INVALID_ARGUMENT"):
+            H.Batch.from_dense(*bad, params)
+    host = [np.ascontiguousarray(a, np.float64) for a in arrs]
+    with pytest.raises(ValueError, match="INVALID_ARGUMENT"):
+        H.Batch.from_dense_device(B, n, m, [a.ctypes.data for a in host], stream, params)
+
+
+def test_layer_on_device_tensors_matches_host_tensors(mods):
+    """QPLayer on ROCm tensors (zero-copy path) against the same layer on CPU
+    tensors: forward bit-identical, gradients to 1e-12 (the batched torch
+    products run on different devices)."""
+    H, torch, QPLayer = mods
+    B, n, meq, min_ = 5, 32, 6, 24
+    arrs = Q.layer_instances(B, n=n, m_eq=meq, m_in=min_, seed=9)
+    g = np.random.default_rng(2).normal(size=(B, n))
+    out = {}
+    for where in ("cpu", "cuda:0"):
+        print("layer on", where, flush=True)
+        ts = [torch.tensor(a, dtype=torch.float64, device=where, requires_grad=True) for a in arrs]
+        x = QPLayer.apply(*ts, 1e-9)
+        assert x.device.type == torch.device(where).type
+        (x * torch.tensor(g, device=where)).sum().backward()
+        out[where] = [x.detach().cpu().numpy()] + [t.grad.cpu().numpy() for t in ts]
+    np.testing.assert_array_equal(out["cuda:0"][0], out["cpu"][0])
+    for k, name in enumerate(("Q", "c", "A", "AL", "AU", "l", "u"), start=1):
+        want, got = out["cpu"][k], out["cuda:0"][k]
+        assert np.max(np.abs(got - want)) <= 1e-12 * max(1.0, np.max(np.abs(want))), name
