@@ -80,6 +80,14 @@ class CResults(C.Structure):
 
 
 def _load() -> C.CDLL:
+    # PyTorch-ROCm ships its own HIP and HSA runtimes under the sonames of
+    # ROCm's. Loaded after this library, torch would map a second runtime
+    # into the process and find no GPU; loaded first, this library binds to
+    # torch's runtime and the two share devices, streams and memory.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     if not LIB_PATH.exists():
         raise ImportError(f"{LIB_PATH} is not built: run `python -c 'import __graft_entry__ as g; g.build()'`")
     L = C.CDLL(str(LIB_PATH))

@@ -724,9 +724,11 @@ _hip = None
 def hip_runtime():
     global _hip
     if _hip is None:
-        import glob
-        cands = sorted(glob.glob("/opt/rocm/lib/libamdhip64.so*")) or ["libamdhip64.so"]
-        L = C.CDLL(cands[0])
+        # the HIP runtime the library is bound to (PyTorch's when installed,
+        # see hprqp_amd._load): opened by soname, which resolves to the copy
+        # already mapped instead of mapping a second runtime
+        import hprqp_amd  # noqa: F401
+        L = C.CDLL("libamdhip64.so.7")
         L.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
         L.hipFree.argtypes = [C.c_void_p]
         L.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]

@@ -39,6 +39,7 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_library_loads_and_reports_no_device_here():
+    import hprqp_amd  # noqa: F401  (loads it after PyTorch's HIP runtime)
     lib = C.CDLL(str(LIB))
     lib.hprqp_status_name.restype = C.c_char_p
     assert lib.hprqp_status_name(6) == b"OPTIMAL"
