@@ -387,33 +387,50 @@ def bench_batched(H, Q, args):
 
 
 def bench_layer(H, Q, args):
-    """Differentiable layer workload: 1024 dense QPs (n=256, m=320); forward to
-    1e-8 and backward (adjoint) times."""
+    """Differentiable layer workload: 1024 dense QPs (n=256, m=320) held as
+    FP64 tensors on the GPU (the zero-copy path: no array crosses PCIe);
+    forward to 1e-8 and backward (adjoint) times, averaged over --steps
+    after --warmup untimed passes. One pass with CPU tensors (host arrays in
+    and out) is timed beside it."""
     import torch
     from hprqp_amd import layer as layer_mod
     from hprqp_amd.layer import QPLayer
     count = min(args.batch_count, 1024)
     Qd, c, A, AL, AU, l, u = Q.layer_instances(count)
-    T = lambda a: torch.tensor(a, dtype=torch.float64, requires_grad=True)
-    ts = [T(a) for a in (Qd, c, A, AL, AU, l, u)]
-    t = time.time()
-    x = QPLayer.apply(*ts, 1e-8)
-    fwd = time.time() - t
-    g = torch.tensor(np.random.default_rng(0).normal(size=x.shape))
-    t = time.time()
-    (x * g).sum().backward()
-    bwd = time.time() - t
-    split = dict(layer_mod.last_timings)
-    log(f"layer: forward {fwd:.3f}s backward {bwd:.3f}s; split {split}")
+    dev = torch.device("cuda:0")
+    g_host = np.random.default_rng(0).normal(size=(count, Qd.shape[1]))
+
+    def step(where):
+        ts = [torch.tensor(a, dtype=torch.float64, device=where, requires_grad=True) for a in (Qd, c, A, AL, AU, l, u)]
+        g = torch.tensor(g_host, device=where)
+        torch.cuda.synchronize()
+        t = time.time()
+        x = QPLayer.apply(*ts, 1e-8)
+        torch.cuda.synchronize()
+        f = time.time() - t
+        t = time.time()
+        (x * g).sum().backward()
+        torch.cuda.synchronize()
+        return f, time.time() - t, dict(layer_mod.last_timings)
+
+    for _ in range(args.warmup):
+        step(dev)
+    runs = [step(dev) for _ in range(max(1, args.steps))]
+    fwd = sum(r[0] for r in runs) / len(runs)
+    bwd = sum(r[1] for r in runs) / len(runs)
+    split = runs[-1][2]
+    hf, hb, hsplit = step("cpu")
+    log(f"layer (device tensors): forward {fwd:.3f}s backward {bwd:.3f}s; split {split}")
+    log(f"layer (host tensors): forward {hf:.3f}s backward {hb:.3f}s; split {hsplit}")
     # Dense column-form products: every instance streams Q once and A twice
     # per iteration (free variables: the noC iteration), 8 bytes per entry;
     # check iterations, setup and vector traffic come on top (not counted).
     n, m = Qd.shape[1], A.shape[1]
     iter_sum = split.get("iterations_sum", 0)
-    dev = split.get("device_solve", 0.0)
+    dsolve = split.get("device_solve", 0.0)
     roofline = None
-    if iter_sum and dev > 0:
-        gbs = 8.0 * (n * n + 2 * m * n) * iter_sum / dev / 1e9
+    if iter_sum and dsolve > 0:
+        gbs = 8.0 * (n * n + 2 * m * n) * iter_sum / dsolve / 1e9
         prof = ROOT / "profiles" / "batch_dense_layer_1024.md"
         traffic = None
         if prof.exists():
@@ -438,14 +455,15 @@ def bench_layer(H, Q, args):
                           f"concurrent host threads: {wall:.2f} s wall, {status.count('OPTIMAL')} optimal; value = "
                           f"wall x {count} / {len(csample)} (the all-cores time for the whole batch; no CPU adjoint)")
         log("cpu_baseline:", cpu)
+    ms = lambda d: {k: (round(v * 1e3, 2) if not k.startswith("iterations") else v) for k, v in d.items()}
     line = {"metric": "QP layer fwd+bwd ms (1 GPU)", "value": round((fwd + bwd) * 1e3, 2), "unit": "ms",
-            "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": round((fwd + bwd) * 1e3, 2),
+            "n_gpus": 1, "steps": len(runs), "warmup": args.warmup, "ms_per_step": round((fwd + bwd) * 1e3, 2),
             "higher_is_better": False, "scaling": "strong", "vs_baseline": None, "dtype": "f64",
             "data": "synthetic (seeded generator)",
-            "config": {"workload": f"layer_{count}", "forward_ms": fwd * 1e3, "backward_ms": bwd * 1e3,
-                       "backward_over_forward": bwd / fwd,
-                       "split_ms": {k: (round(v * 1e3, 2) if not k.startswith("iterations") else v)
-                                    for k, v in split.items()}},
+            "config": {"workload": f"layer_{count}", "tensors": "device (zero-copy)", "forward_ms": fwd * 1e3,
+                       "backward_ms": bwd * 1e3, "backward_over_forward": bwd / fwd, "split_ms": ms(split),
+                       "host_tensors": {"forward_ms": round(hf * 1e3, 2), "backward_ms": round(hb * 1e3, 2),
+                                        "split_ms": ms(hsplit)}},
             "roofline": roofline, "cpu_baseline": cpu}
     print(json.dumps(line), flush=True)
 
