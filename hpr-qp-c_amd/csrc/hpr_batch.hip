@@ -132,16 +132,24 @@ struct Mat {
 constexpr int kDenseMax = kStreamCap;
 
 // Column-form dot of a dense view: sum_{i<K} S[i*ld + j] xs[i], i ascending.
+// CACHED: ordinary loads with twice the loads in flight, for products that
+// re-read one instance's matrix many times (power iteration) -- a lone
+// instance's matrices then stay in L2; otherwise streaming (non-temporal)
+// loads for the solve, whose matrices are read once per iteration.
+template <bool CACHED = false>
 __device__ __forceinline__ double dense_column_dot(const Mat &M, int j, const double *xs)
 {
-    constexpr int U = 8;
+    constexpr int U = CACHED ? 16 : 8;
     const double *__restrict__ col = M.dv + j;
     double acc = 0.0;
     int i = 0;
     for (; i + U <= M.K; i += U) {
         double v[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) v[u] = __builtin_nontemporal_load(col + (long long)(i + u) * M.ld);
+        for (int u = 0; u < U; ++u) {
+            const double *p = col + (long long)(i + u) * M.ld;
+            v[u] = CACHED ? *p : __builtin_nontemporal_load(p);
+        }
 #pragma unroll
         for (int u = 0; u < U; ++u) acc += v[u] * xs[i + u];
     }
@@ -151,7 +159,7 @@ __device__ __forceinline__ double dense_column_dot(const Mat &M, int j, const do
 
 // Dense views run as "blocks" of kThreads outputs through the same epilogue
 // call as the CSR stream mode, so the epilogue is inlined once per mode.
-template <bool DENSE, class Epi>
+template <bool DENSE, bool CACHED = false, class Epi>
 __device__ void wg_spmv(const Mat &M, const double *__restrict__ x, const Epi &epi, double *sums, double *maxs,
                         double *prod)
 {
@@ -201,7 +209,7 @@ __device__ void wg_spmv(const Mat &M, const double *__restrict__ x, const Epi &e
             if (r < r1) {
                 double acc = 0.0;
                 if constexpr (dense) {
-                    acc = dense_column_dot(M, r, prod);
+                    acc = dense_column_dot<CACHED>(M, r, prod);
                 } else {
                     const int e = rp[r + 1] - k0;
                     for (int k = rp[r] - k0; k < e; ++k) acc += prod[k];
@@ -434,10 +442,10 @@ __device__ double wg_power(WG &g, bool a_matrix, double *sums, double *maxs)
         }
         wg_map(len, MapScaleDiv{{}, q, z, qn}, sums, maxs);
         if (a_matrix) {
-            wg_spmv<WG::kDense>(g.AT, q, EpiStore{{}, tmp}, sums, maxs, g.prod);
-            wg_spmv<WG::kDense>(g.A, tmp, EpiPowerZ{nullptr, z, q}, sums, maxs, g.prod);
+            wg_spmv<WG::kDense, true>(g.AT, q, EpiStore{{}, tmp}, sums, maxs, g.prod);
+            wg_spmv<WG::kDense, true>(g.A, tmp, EpiPowerZ{nullptr, z, q}, sums, maxs, g.prod);
         } else {
-            wg_spmv<WG::kDense>(g.Q, q, EpiPowerZ{nullptr, z, q}, sums, maxs, g.prod);
+            wg_spmv<WG::kDense, true>(g.Q, q, EpiPowerZ{nullptr, z, q}, sums, maxs, g.prod);
         }
         wg_reduce<2, 0>(sums, maxs, g.red);
         lam = g.red[0];
