@@ -355,8 +355,11 @@ def bench_batched(H, Q, args):
     wave_row = b.wave_row
     res = [b.result(k) for k in range(len(models))]
     ok = sum(r.status == "OPTIMAL" for r in res)
-    qps_s = len(models) / secs
-    log(f"batched: {ok}/{len(models)} optimal, {secs:.3f}s solve ({setup:.2f}s setup), {qps_s:.1f} QPs/s, left={left}")
+    # end to end like the sequential solves it is compared with: batch
+    # creation (packing, upload, scaling, power iterations) plus the solve
+    qps_s = len(models) / (setup + secs)
+    log(f"batched: {ok}/{len(models)} optimal, {secs:.3f}s solve + {setup:.3f}s setup -> {qps_s:.1f} QPs/s "
+        f"({len(models) / secs:.1f} QPs/s solve only), left={left}")
     sample = list(range(0, len(models), max(1, len(models) // args.seq_sample)))[:args.seq_sample]
     t = time.time()
     seq_iters = 0
@@ -376,9 +379,10 @@ def bench_batched(H, Q, args):
                           f"concurrent host threads, {wall:.2f} s wall, {status.count('OPTIMAL')} optimal")
         log("cpu_baseline:", cpu)
     line = {"metric": "batched QPs/s at 1e-6 (1 GPU)", "value": round(qps_s, 2), "unit": "QPs/s", "n_gpus": 1,
-            "steps": 1, "warmup": 1, "ms_per_step": round(secs * 1e3, 2), "higher_is_better": True,
+            "steps": 1, "warmup": 1, "ms_per_step": round((setup + secs) * 1e3, 2), "higher_is_better": True,
             "scaling": "strong", "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded generator)",
             "config": {"workload": f"batched_{len(models)}", "optimal": ok, "solve_s": secs, "setup_s": setup,
+                       "solve_only_qps": round(len(models) / secs, 2),
                        "sequential_plain_gpu_qps": round(seq_qps, 3), "speedup_vs_sequential": round(qps_s / seq_qps, 2),
                        "sequential_sample": len(sample), "wave_row_autotuned": wave_row,
                        "chunk": "100 loop bodies per replay of a captured HIP graph"},

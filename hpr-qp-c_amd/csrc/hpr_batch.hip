@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <thread>
 #include <vector>
 
@@ -1459,23 +1460,45 @@ void transpose_perm(size_t nrow, size_t ncol, const size_t *rp, const size_t *ci
         }
 }
 
-// Row blocks of every CSR instance for wave threshold `wave_row`; sets the
-// instances' block offsets and returns the concatenated block list.
-std::vector<int> plan_all_blocks(HPRQPHipBatch *b, int wave_row)
+// Row blocks of every CSR instance (Q, A, A^T) for wave threshold
+// `wave_row`, planned on worker threads; sets the instances' block offsets
+// and returns the block lists concatenated in instance order.
+std::vector<int> plan_all_blocks(HPRQPHipBatch *b, const int *rp, int wave_row)
 {
+    const size_t count = b->count;
+    std::vector<std::vector<int>> per(3 * count);
+    {
+        const size_t workers = std::max<size_t>(1, std::min<size_t>({16, count, (size_t)std::max(1u, std::thread::hardware_concurrency())}));
+        std::atomic<size_t> next{0};
+        auto work = [&] {
+            for (size_t k; (k = next.fetch_add(1)) < count;) {
+                const hprqp::batch::Inst &I = b->inst[k];
+                if (I.dense) continue;
+                per[3 * k] = plan_batch_blocks(rp + I.q_rp, I.n, wave_row);
+                per[3 * k + 1] = plan_batch_blocks(rp + I.a_rp, I.m, wave_row);
+                per[3 * k + 2] = plan_batch_blocks(rp + I.at_rp, I.n, wave_row);
+            }
+        };
+        std::vector<std::thread> pool;
+        for (size_t w = 1; w < workers; ++w) pool.emplace_back(work);
+        work();
+        for (std::thread &th : pool) th.join();
+    }
     std::vector<int> blocks;
+    size_t total = 0;
+    for (const auto &v : per) total += v.size();
+    blocks.reserve(std::max<size_t>(total, 1));
     auto push = [&](const std::vector<int> &blk, int &off, int &cnt) {
         off = (int)blocks.size();
         cnt = (int)blk.size() - 1;
         blocks.insert(blocks.end(), blk.begin(), blk.end());
     };
-    for (size_t k = 0; k < b->count; ++k) {
+    for (size_t k = 0; k < count; ++k) {
         hprqp::batch::Inst &I = b->inst[k];
         if (I.dense) continue;
-        const int *rp = b->h_rp.data();
-        push(plan_batch_blocks(rp + I.q_rp, I.n, wave_row), I.q_blk, I.q_nblk);
-        push(plan_batch_blocks(rp + I.a_rp, I.m, wave_row), I.a_blk, I.a_nblk);
-        push(plan_batch_blocks(rp + I.at_rp, I.n, wave_row), I.at_blk, I.at_nblk);
+        push(per[3 * k], I.q_blk, I.q_nblk);
+        push(per[3 * k + 1], I.a_blk, I.a_nblk);
+        push(per[3 * k + 2], I.at_blk, I.at_nblk);
     }
     if (blocks.empty()) blocks.push_back(0);
     return blocks;
@@ -1546,13 +1569,18 @@ HPRQPStatus batch_finish(HPRQPHipBatch *b)
         if (force && std::atoi(force) > 0) cands = {std::atoi(force)};
         int best = kWaveRowDefault;
         float best_ms = 1e30f;
+        std::vector<int> best_plan;
+        std::vector<Inst> best_inst;
         hipEvent_t e0 = nullptr, e1 = nullptr;
         bool ok = hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess;
         for (size_t c = 0; ok && c < cands.size(); ++c) {
             const int T = cands[c];
-            ok = upload_plan(b, plan_all_blocks(b, T));
+            std::vector<int> plan = plan_all_blocks(b, b->h_rp.data(), T);
+            ok = upload_plan(b, plan);
             if (!ok || cands.size() == 1) {
                 best = T;
+                best_plan = std::move(plan);
+                best_inst = b->inst;
                 break;
             }
             ar.wave_row = T;
@@ -1568,12 +1596,15 @@ HPRQPStatus batch_finish(HPRQPHipBatch *b)
             if (ms_min < best_ms) {
                 best_ms = ms_min;
                 best = T;
+                best_plan = std::move(plan);
+                best_inst = b->inst;
             }
         }
         if (e0) (void)hipEventDestroy(e0);
         if (e1) (void)hipEventDestroy(e1);
         ar.wave_row = best;
-        if (!ok || !upload_plan(b, plan_all_blocks(b, best)) || hipGetLastError() != hipSuccess) {
+        if (ok) b->inst = std::move(best_inst);
+        if (!ok || !upload_plan(b, best_plan) || hipGetLastError() != hipSuccess) {
             delete b;
             return HPRQP_STATUS_NOT_IMPLEMENTED;
         }
@@ -1656,10 +1687,27 @@ HPRQPStatus hprqp_hip_batch_create(const HPRQPModel *models, size_t count, const
         return HPRQP_STATUS_INVALID_ARGUMENT;
     }
     const long long a_base = sum_q, at_base = sum_q + sum_a;
-    std::vector<int> rp((size_t)std::max(sum_rp, 1LL)), ci((size_t)std::max(sum_q + 2 * sum_a, 1LL));
-    std::vector<int> perm((size_t)std::max(sum_a + sum_q, 1LL));
-    std::vector<double> val((size_t)std::max(sum_q + 2 * sum_a, 1LL)), dval((size_t)std::max(sum_d, 1LL));
-    std::vector<double> vn((size_t)sum_n * N_COUNT), vm((size_t)std::max(sum_m, 1LL) * M_COUNT);
+    // Host staging, written once by the packing threads (no zero fill: every
+    // entry is written by exactly one instance's pack). The work vectors are
+    // cleared on the device; only the five input vectors travel.
+    const size_t n_rp = (size_t)std::max(sum_rp, 1LL), n_nz = (size_t)std::max(sum_q + 2 * sum_a, 1LL);
+    const size_t n_perm = (size_t)std::max(sum_a + sum_q, 1LL), n_d = (size_t)std::max(sum_d, 1LL);
+    const size_t sm1 = (size_t)std::max(sum_m, 1LL);
+    std::unique_ptr<int[]> ci(new (std::nothrow) int[n_nz]), perm(new (std::nothrow) int[n_perm]);
+    std::unique_ptr<double[]> val(new (std::nothrow) double[n_nz]), dval(new (std::nothrow) double[n_d]);
+    std::unique_ptr<double[]> in_n(new (std::nothrow) double[(size_t)sum_n * 3]), in_m(new (std::nothrow) double[sm1 * 2]);
+    std::vector<int> rp;
+    try {
+        rp.resize(n_rp);
+    } catch (const std::bad_alloc &) {
+    }
+    if (!ci || !perm || !val || !dval || !in_n || !in_m || rp.size() != n_rp) {
+        delete b;
+        return HPRQP_STATUS_ALLOCATION_FAILED;
+    }
+    // placeholders of empty arrays (one entry each) are written here
+    ci[0] = perm[0] = 0;
+    val[0] = dval[0] = in_m[0] = in_m[1] = 0.0;
     auto pack = [&](size_t k) {
         const HPRQPModel &md = models[k];
         Inst &I = b->inst[k];
@@ -1683,9 +1731,9 @@ HPRQPStatus hprqp_hip_batch_create(const HPRQPModel *models, size_t count, const
         if (I.dense) {
             // Q and A row-major (zeros where the CSR has no entry); the A^T
             // copy is written on the device after scaling
-            double *Qd = dval.data() + I.dq, *Ad = dval.data() + I.dat;
+            double *Qd = dval.get() + I.dq, *Ad = dval.get() + I.dat;
             std::fill(Qd, Qd + n * n, 0.0);
-            std::fill(Ad, Ad + m * n, 0.0);
+            std::fill(Ad, Ad + 2 * m * n, 0.0);  // A, then its transposed copy (written on the device)
             for (size_t r = 0; r < n; ++r)
                 for (size_t e = md.Q.row_ptr[r]; e < md.Q.row_ptr[r + 1]; ++e) Qd[r * n + md.Q.col_ind[e]] += md.Q.values[e];
             for (size_t r = 0; r < m; ++r)
@@ -1696,6 +1744,7 @@ HPRQPStatus hprqp_hip_batch_create(const HPRQPModel *models, size_t count, const
             for (size_t r = 0; r <= m; ++r) arp[r] = (int)md.A.row_ptr[r];
             std::copy(qrp.begin(), qrp.end(), rp.begin() + I.q_rp);
             std::copy(arp.begin(), arp.end(), rp.begin() + I.a_rp);
+            std::fill(val.get() + at_base + I.a_nz, val.get() + at_base + I.a_nz + md.A.nnz, 0.0);  // A^T: on the device
             for (size_t e = 0; e < md.Q.nnz; ++e) {
                 ci[I.q_nz + e] = (int)md.Q.col_ind[e];
                 val[I.q_nz + e] = md.Q.values[e];
@@ -1715,13 +1764,13 @@ HPRQPStatus hprqp_hip_batch_create(const HPRQPModel *models, size_t count, const
             for (size_t e = 0; e < md.Q.nnz; ++e) perm[sum_a + I.q_nz + e] = tperm[e];
         }
         for (size_t i = 0; i < n; ++i) {
-            vn[(size_t)N_c * sum_n + I.vn + i] = md.c[i];
-            vn[(size_t)N_l * sum_n + I.vn + i] = md.l[i];
-            vn[(size_t)N_u * sum_n + I.vn + i] = md.u[i];
+            in_n[I.vn + i] = md.c[i];
+            in_n[(size_t)sum_n + I.vn + i] = md.l[i];
+            in_n[2 * (size_t)sum_n + I.vn + i] = md.u[i];
         }
         for (size_t i = 0; i < m; ++i) {
-            vm[(size_t)M_AL * sum_m + I.vm + i] = md.AL[i];
-            vm[(size_t)M_AU * sum_m + I.vm + i] = md.AU[i];
+            in_m[I.vm + i] = md.AL[i];
+            in_m[sm1 + I.vm + i] = md.AU[i];
         }
     };
     {
@@ -1740,20 +1789,30 @@ HPRQPStatus hprqp_hip_batch_create(const HPRQPModel *models, size_t count, const
         b->m.push_back(models[k].A.nrow);
         b->work.push_back((double)(models[k].Q.nnz + 2 * models[k].A.nnz + models[k].Q.ncol + models[k].A.nrow));
     }
-    b->h_rp = rp;
-    std::vector<int> blocks = plan_all_blocks(b, kWaveRowDefault);
+    std::vector<int> blocks = plan_all_blocks(b, rp.data(), kWaveRowDefault);
     HPRQPStatus st = HPRQP_STATUS_OK;
-    auto upload = [&](auto *&dst, const auto &src) -> bool {
-        using T = typename std::remove_reference<decltype(src)>::type::value_type;
-        size_t bytes = sizeof(T) * std::max(src.size(), (size_t)1);
-        if (hipMalloc((void **)&dst, bytes) != hipSuccess) return false;
-        if (!src.empty() && hipMemcpy(dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice) != hipSuccess)
+    auto upload = [&](auto *&dst, const auto *src, size_t count) -> bool {
+        using T = typename std::remove_const<typename std::remove_pointer<decltype(src)>::type>::type;
+        if (hipMalloc((void **)&dst, sizeof(T) * count) != hipSuccess) return false;
+        return hipMemcpy(dst, src, sizeof(T) * count, hipMemcpyHostToDevice) == hipSuccess;
+    };
+    auto vectors = [&](double *&dst, size_t len, size_t slots, const double *in, std::initializer_list<int> at) -> bool {
+        if (hipMalloc((void **)&dst, sizeof(double) * len * slots) != hipSuccess ||
+            hipMemset(dst, 0, sizeof(double) * len * slots) != hipSuccess)
             return false;
+        size_t j = 0;
+        for (int slot : at)
+            if (hipMemcpy(dst + (size_t)slot * len, in + len * j++, sizeof(double) * len, hipMemcpyHostToDevice) !=
+                hipSuccess)
+                return false;
         return true;
     };
-    if (!upload(b->d_rp, rp) || !upload(b->d_ci, ci) || !upload(b->d_val, val) || !upload(b->d_perm, perm) ||
-        !upload(b->d_blocks, blocks) || !upload(b->d_vn, vn) || !upload(b->d_vm, vm) || !upload(b->d_dval, dval))
+    if (!upload(b->d_rp, rp.data(), n_rp) || !upload(b->d_ci, ci.get(), n_nz) || !upload(b->d_val, val.get(), n_nz) ||
+        !upload(b->d_perm, perm.get(), n_perm) || !upload(b->d_blocks, blocks.data(), blocks.size()) ||
+        !vectors(b->d_vn, (size_t)sum_n, N_COUNT, in_n.get(), {N_c, N_l, N_u}) ||
+        !vectors(b->d_vm, sm1, M_COUNT, in_m.get(), {M_AL, M_AU}) || !upload(b->d_dval, dval.get(), n_d))
         st = HPRQP_STATUS_ALLOCATION_FAILED;
+    b->h_rp = std::move(rp);
     b->blocks_cap = blocks.size();
     if (st != HPRQP_STATUS_OK) {
         delete b;
