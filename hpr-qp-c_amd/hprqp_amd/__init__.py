@@ -199,12 +199,13 @@ class Model:
     """An owned HPRQPModel built by this library (preprocessed like the reference)."""
 
     def __init__(self) -> None:
+        self._lib = _LIB  # held so that __del__ works during interpreter shutdown
         self._m = CModel()
         _LIB.model_init(C.byref(self._m))
 
     def __del__(self) -> None:
         if getattr(self, "_m", None) is not None:
-            _LIB.model_free(C.byref(self._m))
+            self._lib.model_free(C.byref(self._m))
             self._m = None
 
     @property
@@ -345,6 +346,7 @@ class Session:
     """hprqp_hip_session_*: a device-resident solve advanced loop body by body."""
 
     def __init__(self, model: Model, params: Parameters | None = None) -> None:
+        self._lib = _LIB  # held so that __del__ works during interpreter shutdown
         self.model = model
         self.params = params or Parameters(verbose=False)
         h = C.c_void_p()
@@ -355,7 +357,7 @@ class Session:
 
     def __del__(self) -> None:
         if getattr(self, "h", None):
-            _LIB.hprqp_hip_session_destroy(self.h)
+            self._lib.hprqp_hip_session_destroy(self.h)
             self.h = None
 
     def advance(self, bodies: int = 1) -> tuple[int, bool]:
@@ -414,6 +416,7 @@ class Batch:
     """hprqp_hip_batch_*: many independent QPs in one persistent launch."""
 
     def __init__(self, models: Sequence[Model] | None, params: Parameters | None = None) -> None:
+        self._lib = _LIB  # held so that __del__ works during interpreter shutdown
         self.params = params or Parameters(verbose=False)
         self.h = None
         if models is None:  # filled in by from_dense
@@ -479,7 +482,7 @@ class Batch:
 
     def __del__(self) -> None:
         if getattr(self, "h", None):
-            _LIB.hprqp_hip_batch_destroy(self.h)
+            self._lib.hprqp_hip_batch_destroy(self.h)
             self.h = None
 
     def solve(self, bodies_per_launch: int = 100) -> int:
