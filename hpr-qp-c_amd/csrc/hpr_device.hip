@@ -11,6 +11,7 @@
 
 #include <cstddef>
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -141,6 +142,13 @@ struct Panels {
     int *src[kMaxPanels] = {nullptr, nullptr, nullptr, nullptr};
 };
 
+// Destinations of the device-side panel split (one entry per panel).
+struct PanelOut {
+    const int *rp[kMaxPanels];
+    int *ci[kMaxPanels];
+    int *src[kMaxPanels];
+};
+
 void free_csr(DevCsr &m);
 HPRQPStatus upload_blocks(DevCsr &m, hipStream_t s);
 
@@ -167,7 +175,38 @@ void free_csr(DevCsr &m)
 __global__ void gather_values_kernel(int nnz, const int *__restrict__ perm, const double *__restrict__ src,
                                      double *__restrict__ dst);
 
-// Split M into P column panels (structure planned on the host from M's
+// Column-panel split on the device. Per row (one thread each): the number
+// of its entries in each panel; an inclusive scan of each panel's counts
+// gives the panel's row pointers; then each row copies its entries to their
+// panels in row order, recording every entry's source index.
+__global__ void panel_count_kernel(int nrow, const int *__restrict__ rp, const int *__restrict__ ci, int width,
+                                   int P, int *__restrict__ counts)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrow) return;
+    int c[kMaxPanels] = {0, 0, 0, 0};
+    for (int k = rp[r]; k < rp[r + 1]; ++k) ++c[min(ci[k] / width, P - 1)];
+    for (int p = 0; p < P; ++p) counts[(size_t)p * (nrow + 1) + r + 1] = c[p];
+    if (r == 0)
+        for (int p = 0; p < P; ++p) counts[(size_t)p * (nrow + 1)] = 0;
+}
+
+__global__ void panel_scatter_kernel(int nrow, const int *__restrict__ rp, const int *__restrict__ ci, int width,
+                                     int P, PanelOut out)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrow) return;
+    int pos[kMaxPanels];
+    for (int p = 0; p < P; ++p) pos[p] = out.rp[p][r];
+    for (int k = rp[r]; k < rp[r + 1]; ++k) {
+        const int p = min(ci[k] / width, P - 1);
+        const int d = pos[p]++;
+        out.ci[p][d] = ci[k];
+        out.src[p][d] = k;
+    }
+}
+
+// Split M into P column panels (structure built on the device from M's
 // indices; values filled by fill_panel_values once they are final).
 HPRQPStatus build_panels(const DevCsr &M, int P, Panels &pn, hipStream_t s)
 {
@@ -175,44 +214,44 @@ HPRQPStatus build_panels(const DevCsr &M, int P, Panels &pn, hipStream_t s)
     pn.P = P;
     pn.width = (M.ncol + P - 1) / P;
     if (P <= 1) return HPRQP_STATUS_OK;
-    std::vector<int> rp(M.nrow + 1), ci(M.nnz);
-    HPRQP_HIP(hipMemcpyAsync(rp.data(), M.rp, sizeof(int) * (M.nrow + 1), hipMemcpyDeviceToHost, s));
-    if (M.nnz) HPRQP_HIP(hipMemcpyAsync(ci.data(), M.ci, sizeof(int) * M.nnz, hipMemcpyDeviceToHost, s));
-    HPRQP_HIP(hipStreamSynchronize(s));
-    std::vector<std::vector<int>> prp(P, std::vector<int>(M.nrow + 1, 0)), pci(P), psrc(P);
-    for (int r = 0; r < M.nrow; ++r)
-        for (int k = rp[r]; k < rp[r + 1]; ++k) prp[ci[k] / pn.width][r + 1] += 1;
-    for (int p = 0; p < P; ++p) {
-        for (int r = 0; r < M.nrow; ++r) prp[p][r + 1] += prp[p][r];
-        pci[p].resize(prp[p][M.nrow]);
-        psrc[p].resize(prp[p][M.nrow]);
-    }
-    std::vector<std::vector<int>> fill(P);
-    for (int p = 0; p < P; ++p) fill[p].assign(prp[p].begin(), prp[p].end() - 1);
-    for (int r = 0; r < M.nrow; ++r)
-        for (int k = rp[r]; k < rp[r + 1]; ++k) {
-            const int p = ci[k] / pn.width;
-            const int dst = fill[p][r]++;
-            pci[p][dst] = ci[k];
-            psrc[p][dst] = k;
-        }
+    const size_t rows1 = (size_t)M.nrow + 1;
+    int *counts = nullptr;
+    HPRQP_HIP(hipMalloc(&counts, sizeof(int) * rows1 * P));
+    if (M.nrow > 0)
+        panel_count_kernel<<<grid_for(M.nrow), kThreads, 0, s>>>(M.nrow, M.rp, M.ci, pn.width, P, counts);
+    else
+        HPRQP_HIP(hipMemsetAsync(counts, 0, sizeof(int) * rows1 * P, s));
+    HPRQP_HIP(hipGetLastError());
+    PanelOut out{};
+    std::vector<int> nnz(P, 0);
     for (int p = 0; p < P; ++p) {
         DevCsr &d = pn.part[p];
         d.nrow = M.nrow;
         d.ncol = M.ncol;
-        d.nnz = (int)pci[p].size();
+        HPRQP_HIP(hipMalloc(&d.rp, sizeof(int) * rows1));
+        size_t tmp_bytes = 0;
+        HPRQP_HIP(rocprim::inclusive_scan(nullptr, tmp_bytes, counts + p * rows1, d.rp, rows1, rocprim::plus<int>(), s));
+        void *tmp = nullptr;
+        HPRQP_HIP(hipMalloc(&tmp, std::max(tmp_bytes, (size_t)4)));
+        HPRQP_HIP(rocprim::inclusive_scan(tmp, tmp_bytes, counts + p * rows1, d.rp, rows1, rocprim::plus<int>(), s));
+        HPRQP_HIP(hipMemcpyAsync(&nnz[p], d.rp + M.nrow, sizeof(int), hipMemcpyDeviceToHost, s));
+        HPRQP_HIP(hipStreamSynchronize(s));
+        (void)hipFree(tmp);
+        d.nnz = nnz[p];
         const size_t nz = std::max((size_t)d.nnz, (size_t)1);
-        HPRQP_HIP(hipMalloc(&d.rp, sizeof(int) * (M.nrow + 1)));
         HPRQP_HIP(hipMalloc(&d.ci, sizeof(int) * nz));
         HPRQP_HIP(hipMalloc(&d.val, sizeof(double) * nz));
         HPRQP_HIP(hipMalloc(&pn.src[p], sizeof(int) * nz));
-        HPRQP_HIP(hipMemcpyAsync(d.rp, prp[p].data(), sizeof(int) * (M.nrow + 1), hipMemcpyHostToDevice, s));
-        if (d.nnz) {
-            HPRQP_HIP(hipMemcpyAsync(d.ci, pci[p].data(), sizeof(int) * d.nnz, hipMemcpyHostToDevice, s));
-            HPRQP_HIP(hipMemcpyAsync(pn.src[p], psrc[p].data(), sizeof(int) * d.nnz, hipMemcpyHostToDevice, s));
-        }
-        HPRQP_HIP(hipStreamSynchronize(s));
-        HPRQPStatus st = upload_blocks(d, s);
+        out.rp[p] = d.rp;
+        out.ci[p] = d.ci;
+        out.src[p] = pn.src[p];
+    }
+    if (M.nrow > 0) panel_scatter_kernel<<<grid_for(M.nrow), kThreads, 0, s>>>(M.nrow, M.rp, M.ci, pn.width, P, out);
+    HPRQP_HIP(hipGetLastError());
+    HPRQP_HIP(hipStreamSynchronize(s));
+    (void)hipFree(counts);
+    for (int p = 0; p < P; ++p) {
+        HPRQPStatus st = upload_blocks(pn.part[p], s);
         if (st != HPRQP_STATUS_OK) return st;
     }
     return HPRQP_STATUS_OK;
