@@ -11,11 +11,117 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <cstdint>
+#include <thread>
 #include <vector>
 
 #include "hprqp.h"
 
 namespace {
+
+// ---------------------------------------------------------- host threading
+// Large models are built on worker threads; every parallel step below
+// produces exactly the sequential result (fixed partitions, ordered merges).
+
+size_t host_workers(size_t work, size_t grain)
+{
+    const size_t hw = std::max(1u, std::thread::hardware_concurrency());
+    return std::max<size_t>(1, std::min<size_t>({16, hw, work / grain}));
+}
+
+// f(t) for t in [0, T), t = 0 on the calling thread.
+template <class F>
+void run_workers(size_t T, const F &f)
+{
+    std::vector<std::thread> pool;
+    for (size_t t = 1; t < T; ++t) pool.emplace_back([&f, t] { f(t); });
+    f(0);
+    for (std::thread &th : pool) th.join();
+}
+
+// Row ranges of about equal entry counts: rows [cut[t], cut[t+1]).
+std::vector<size_t> row_cuts(size_t nrow, const size_t *rp, size_t T)
+{
+    std::vector<size_t> cut(T + 1, nrow);
+    cut[0] = 0;
+    for (size_t t = 1; t < T; ++t) {
+        const size_t target = rp[nrow] / T * t;
+        cut[t] = (size_t)(std::lower_bound(rp, rp + nrow + 1, target) - rp);
+        cut[t] = std::max(cut[t], cut[t - 1]);
+    }
+    return cut;
+}
+
+void parallel_copy(void *dst, const void *src, size_t bytes)
+{
+    const size_t T = host_workers(bytes, (size_t)16 << 20);
+    const size_t chunk = (bytes + T - 1) / T;
+    run_workers(T, [&](size_t t) {
+        const size_t b = std::min(bytes, t * chunk), e = std::min(bytes, b + chunk);
+        if (e > b) std::memcpy(static_cast<char *>(dst) + b, static_cast<const char *>(src) + b, e - b);
+    });
+}
+
+// Stable counting-sort transpose of a CSR pattern: tptr (ncol + 1) receives
+// the transposed row pointers and emit(dst, r, k) is called for every entry
+// k of row r with its position dst in the transpose, entries of one column
+// in ascending row order. Worker threads take contiguous row ranges with a
+// column histogram each; within a column, range t's entries are placed
+// before range t + 1's, which is the sequential order.
+template <class Emit>
+void transpose_scatter(size_t nrow, size_t ncol, const size_t *rp, const size_t *ci, size_t *tptr, const Emit &emit)
+{
+    const size_t nnz = rp[nrow];
+    size_t T = host_workers(nnz, (size_t)1 << 20);
+    // per-thread 32-bit histograms: bounded memory and no overflow
+    if (nnz >= ((size_t)1 << 32)) T = 1;
+    T = std::min(T, std::max<size_t>(1, ((size_t)1 << 28) / std::max<size_t>(ncol, 1)));
+    std::fill(tptr, tptr + ncol + 1, 0);
+    if (T == 1) {
+        for (size_t k = 0; k < nnz; ++k) tptr[ci[k] + 1] += 1;
+        for (size_t j = 0; j < ncol; ++j) tptr[j + 1] += tptr[j];
+        std::vector<size_t> next(tptr, tptr + ncol);
+        for (size_t r = 0; r < nrow; ++r)
+            for (size_t k = rp[r]; k < rp[r + 1]; ++k) emit(next[ci[k]]++, r, k);
+        return;
+    }
+    const std::vector<size_t> cut = row_cuts(nrow, rp, T);
+    std::vector<std::vector<uint32_t>> hist(T);
+    run_workers(T, [&](size_t t) {
+        hist[t].assign(ncol, 0);
+        for (size_t k = rp[cut[t]]; k < rp[cut[t + 1]]; ++k) hist[t][ci[k]] += 1;
+    });
+    // column totals, then per-range start offsets within each column
+    const size_t cchunk = (ncol + T - 1) / T;
+    run_workers(T, [&](size_t t) {
+        const size_t b = std::min(ncol, t * cchunk), e = std::min(ncol, b + cchunk);
+        for (size_t j = b; j < e; ++j) {
+            size_t sum = 0;
+            for (size_t u = 0; u < T; ++u) sum += hist[u][j];
+            tptr[j + 1] = sum;
+        }
+    });
+    for (size_t j = 0; j < ncol; ++j) tptr[j + 1] += tptr[j];
+    run_workers(T, [&](size_t t) {
+        const size_t b = std::min(ncol, t * cchunk), e = std::min(ncol, b + cchunk);
+        for (size_t j = b; j < e; ++j) {
+            size_t run = tptr[j];
+            for (size_t u = 0; u < T; ++u) {
+                const size_t c = hist[u][j];
+                hist[u][j] = (uint32_t)(run - tptr[j]);  // offset within the column
+                run += c;
+            }
+        }
+    });
+    run_workers(T, [&](size_t t) {
+        std::vector<uint32_t> &h = hist[t];
+        for (size_t r = cut[t]; r < cut[t + 1]; ++r)
+            for (size_t k = rp[r]; k < rp[r + 1]; ++k) {
+                const size_t j = ci[k];
+                emit(tptr[j] + h[j]++, r, k);
+            }
+    });
+}
 
 void csr_reset(HPRQPMatrixCSR *m)
 {
@@ -112,75 +218,78 @@ HPRQPStatus symmetrize_q(HPRQPMatrixCSR *out, const HPRQPMatrixCSR *q)
     // Transposed contributions grouped by target row, each group in
     // generation order (source row ascending, then entry order).
     std::vector<size_t> tptr(n + 1, 0);
-    for (size_t k = 0; k < nnz; ++k) tptr[q->col_ind[k] + 1] += 1;
-    for (size_t r = 0; r < n; ++r) tptr[r + 1] += tptr[r];
     std::vector<size_t> tsrc(nnz);  // source row of each transposed contribution
     std::vector<size_t> tent(nnz);  // its entry index in q
-    {
-        std::vector<size_t> fill(tptr.begin(), tptr.end() - 1);
-        for (size_t r = 0; r < n; ++r)
-            for (size_t k = q->row_ptr[r]; k < q->row_ptr[r + 1]; ++k) {
-                const size_t pos = fill[q->col_ind[k]]++;
-                tsrc[pos] = r;
-                tent[pos] = k;
-            }
-    }
+    transpose_scatter(n, n, q->row_ptr, q->col_ind, tptr.data(), [&](size_t dst, size_t r, size_t k) {
+        tsrc[dst] = r;
+        tent[dst] = k;
+    });
     // Row r's contributions in generation order: transposed ones from source
     // rows < r; then row r's own entries, each followed by its own transposed
     // copy when it is diagonal; then transposed ones from source rows > r.
     // A stable sort by column of that sequence is the reference's stable
-    // (row, col) order restricted to row r.
-    std::vector<size_t> ptr(n + 1, 0);
-    std::vector<size_t> ind;
-    std::vector<double> val;
-    ind.reserve(2 * nnz);
-    val.reserve(2 * nnz);
-    std::vector<Triple> row;
-    for (size_t r = 0; r < n; ++r) {
-        row.clear();
-        size_t t = tptr[r];
-        const size_t te = tptr[r + 1];
-        for (; t < te && tsrc[t] < r; ++t) row.push_back(Triple{r, tsrc[t], 0.5 * q->values[tent[t]]});
-        for (size_t k = q->row_ptr[r]; k < q->row_ptr[r + 1]; ++k) {
-            row.push_back(Triple{r, q->col_ind[k], 0.5 * q->values[k]});
-            if (q->col_ind[k] == r) {
-                // the transposed copy of this diagonal entry comes right after it
-                row.push_back(Triple{r, r, 0.5 * q->values[tent[t]]});
-                ++t;
+    // (row, col) order restricted to row r. Row ranges run on worker threads
+    // into their own buffers, concatenated in row order.
+    const size_t T = host_workers(2 * nnz, (size_t)1 << 19);
+    const std::vector<size_t> cut = row_cuts(n, q->row_ptr, T);
+    std::vector<size_t> len(n, 0);
+    std::vector<std::vector<size_t>> ind_t(T);
+    std::vector<std::vector<double>> val_t(T);
+    run_workers(T, [&](size_t th) {
+        std::vector<size_t> &ind = ind_t[th];
+        std::vector<double> &val = val_t[th];
+        ind.reserve(2 * (q->row_ptr[cut[th + 1]] - q->row_ptr[cut[th]]));
+        val.reserve(ind.capacity());
+        std::vector<Triple> row;
+        for (size_t r = cut[th]; r < cut[th + 1]; ++r) {
+            row.clear();
+            size_t t = tptr[r];
+            const size_t te = tptr[r + 1];
+            for (; t < te && tsrc[t] < r; ++t) row.push_back(Triple{r, tsrc[t], 0.5 * q->values[tent[t]]});
+            for (size_t k = q->row_ptr[r]; k < q->row_ptr[r + 1]; ++k) {
+                row.push_back(Triple{r, q->col_ind[k], 0.5 * q->values[k]});
+                if (q->col_ind[k] == r) {
+                    // the transposed copy of this diagonal entry comes right after it
+                    row.push_back(Triple{r, r, 0.5 * q->values[tent[t]]});
+                    ++t;
+                }
             }
-        }
-        for (; t < te; ++t) row.push_back(Triple{r, tsrc[t], 0.5 * q->values[tent[t]]});
-        std::stable_sort(row.begin(), row.end(), [](const Triple &x, const Triple &y) { return x.col < y.col; });
-        size_t row_begin = ind.size();
-        for (const Triple &e : row) {
-            if (std::fabs(e.val) <= tol) continue;
-            if (ind.size() > row_begin && ind.back() == e.col) {
-                val.back() += e.val;
-            } else {
-                ind.push_back(e.col);
-                val.push_back(e.val);
+            for (; t < te; ++t) row.push_back(Triple{r, tsrc[t], 0.5 * q->values[tent[t]]});
+            std::stable_sort(row.begin(), row.end(), [](const Triple &x, const Triple &y) { return x.col < y.col; });
+            const size_t row_begin = ind.size();
+            for (const Triple &e : row) {
+                if (std::fabs(e.val) <= tol) continue;
+                if (ind.size() > row_begin && ind.back() == e.col) {
+                    val.back() += e.val;
+                } else {
+                    ind.push_back(e.col);
+                    val.push_back(e.val);
+                }
             }
-        }
-        size_t keep = row_begin;
-        for (size_t k = row_begin; k < ind.size(); ++k) {
-            double v = std::fabs(val[k]) <= tol ? 0.0 : val[k];
-            if (v != 0.0) {
-                ind[keep] = ind[k];
-                val[keep] = v;
-                ++keep;
+            size_t keep = row_begin;
+            for (size_t k = row_begin; k < ind.size(); ++k) {
+                double v = std::fabs(val[k]) <= tol ? 0.0 : val[k];
+                if (v != 0.0) {
+                    ind[keep] = ind[k];
+                    val[keep] = v;
+                    ++keep;
+                }
             }
+            ind.resize(keep);
+            val.resize(keep);
+            len[r] = keep - row_begin;
         }
-        ind.resize(keep);
-        val.resize(keep);
-        ptr[r + 1] = keep;
-    }
-    HPRQPStatus st = matrix_csr_alloc(out, n, q->ncol, ind.size());
+    });
+    std::vector<size_t> base(T + 1, 0);
+    for (size_t th = 0; th < T; ++th) base[th + 1] = base[th] + ind_t[th].size();
+    HPRQPStatus st = matrix_csr_alloc(out, n, q->ncol, base[T]);
     if (st != HPRQP_STATUS_OK) return st;
-    std::memcpy(out->row_ptr, ptr.data(), (n + 1) * sizeof(size_t));
-    if (!ind.empty()) {
-        std::memcpy(out->col_ind, ind.data(), ind.size() * sizeof(size_t));
-        std::memcpy(out->values, val.data(), val.size() * sizeof(double));
-    }
+    for (size_t r = 0; r < n; ++r) out->row_ptr[r + 1] = out->row_ptr[r] + len[r];
+    run_workers(T, [&](size_t th) {
+        if (ind_t[th].empty()) return;
+        std::memcpy(out->col_ind + base[th], ind_t[th].data(), ind_t[th].size() * sizeof(size_t));
+        std::memcpy(out->values + base[th], val_t[th].data(), val_t[th].size() * sizeof(double));
+    });
     return HPRQP_STATUS_OK;
 }
 
@@ -354,9 +463,9 @@ HPRQPStatus matrix_csr_copy(HPRQPMatrixCSR *dest, const HPRQPMatrixCSR *src)
     if (dest == src) return HPRQP_STATUS_OK;
     HPRQPStatus st = matrix_csr_alloc(dest, src->nrow, src->ncol, src->nnz);
     if (st != HPRQP_STATUS_OK) return st;
-    std::memcpy(dest->row_ptr, src->row_ptr, (src->nrow + 1) * sizeof(size_t));
-    std::memcpy(dest->col_ind, src->col_ind, src->nnz * sizeof(size_t));
-    std::memcpy(dest->values, src->values, src->nnz * sizeof(double));
+    parallel_copy(dest->row_ptr, src->row_ptr, (src->nrow + 1) * sizeof(size_t));
+    parallel_copy(dest->col_ind, src->col_ind, src->nnz * sizeof(size_t));
+    parallel_copy(dest->values, src->values, src->nnz * sizeof(double));
     return HPRQP_STATUS_OK;
 }
 
@@ -366,15 +475,11 @@ HPRQPStatus matrix_csr_transpose(HPRQPMatrixCSR *dest, const HPRQPMatrixCSR *src
     if (!dest || !src || !matrix_csr_is_valid(src)) return HPRQP_STATUS_INVALID_ARGUMENT;
     HPRQPStatus st = matrix_csr_alloc(dest, src->ncol, src->nrow, src->nnz);
     if (st != HPRQP_STATUS_OK) return st;
-    for (size_t k = 0; k < src->nnz; ++k) dest->row_ptr[src->col_ind[k] + 1] += 1;
-    for (size_t j = 0; j < dest->nrow; ++j) dest->row_ptr[j + 1] += dest->row_ptr[j];
-    std::vector<size_t> next(dest->row_ptr, dest->row_ptr + dest->nrow);
-    for (size_t r = 0; r < src->nrow; ++r)
-        for (size_t k = src->row_ptr[r]; k < src->row_ptr[r + 1]; ++k) {
-            size_t dst = next[src->col_ind[k]]++;
-            dest->col_ind[dst] = r;
-            dest->values[dst] = src->values[k];
-        }
+    transpose_scatter(src->nrow, src->ncol, src->row_ptr, src->col_ind, dest->row_ptr,
+                      [&](size_t dst, size_t r, size_t k) {
+                          dest->col_ind[dst] = r;
+                          dest->values[dst] = src->values[k];
+                      });
     return HPRQP_STATUS_OK;
 }
 

@@ -199,13 +199,13 @@ class Model:
     """An owned HPRQPModel built by this library (preprocessed like the reference)."""
 
     def __init__(self) -> None:
-        self._lib = _LIB  # held so that __del__ works during interpreter shutdown
+        self._lib, self._byref = _LIB, C.byref  # held so that __del__ works during interpreter shutdown
         self._m = CModel()
         _LIB.model_init(C.byref(self._m))
 
     def __del__(self) -> None:
         if getattr(self, "_m", None) is not None:
-            self._lib.model_free(C.byref(self._m))
+            self._lib.model_free(self._byref(self._m))
             self._m = None
 
     @property

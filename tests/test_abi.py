@@ -154,3 +154,49 @@ def test_oracle_symmetrisation_equals_reference_on_messy_q(seed):
     assert res.primal_obj == r["objective"]
     for v, arr in zip("xyzw", (x, y, z, w)):
         assert np.array_equal(arr, r[v]), v
+
+
+def big_messy_qp(seed=7, n=200_000, m=60_000, q_row=14, a_row=40):
+    """messy_q_qp's pattern at a size where the host build runs its
+    transposes and the symmetrisation on worker threads (several million
+    entries): unsorted and repeated Q entries, entries around the 1e-15 drop
+    tolerance, a few empty A rows."""
+    rng = np.random.default_rng(seed)
+    lens = rng.integers(0, 2 * q_row, size=n)
+    q_ptr = np.zeros(n + 1, np.uint64)
+    q_ptr[1:] = np.cumsum(lens)
+    nq = int(q_ptr[-1])
+    q_ind = rng.integers(0, n, size=nq).astype(np.uint64)
+    rows = np.repeat(np.arange(n), lens)
+    diag = rng.random(nq) < 0.08
+    q_ind[diag] = rows[diag]  # repeated diagonal entries
+    q_val = rng.standard_normal(nq) * 10.0 ** rng.integers(-3, 3, size=nq)
+    tiny = rng.random(nq) < 0.05
+    q_val[tiny] = rng.choice([1e-15, -2e-15, 1.5e-15, 3e-16], size=int(tiny.sum()))
+    alen = rng.integers(0, 2 * a_row, size=m)
+    alen[rng.random(m) < 0.01] = 0
+    a_ptr = np.zeros(m + 1, np.uint64)
+    a_ptr[1:] = np.cumsum(alen)
+    na = int(a_ptr[-1])
+    a_ind = np.empty(na, np.uint64)
+    for r in range(m):  # sorted, distinct columns per row
+        b, e = int(a_ptr[r]), int(a_ptr[r + 1])
+        a_ind[b:e] = np.sort(rng.choice(n, size=e - b, replace=False))
+    a_val = rng.standard_normal(na)
+    return Q.QP(n, m, q_ptr, q_ind, q_val, a_ptr, a_ind, a_val, rng.standard_normal(n), -np.ones(m), np.ones(m),
+                -np.ones(n), np.ones(n))
+
+
+def test_threaded_host_build_matches_oracle():
+    """The worker-thread transposes and symmetrisation give the oracle's
+    (the reference's sequential) model bit for bit."""
+    import hprqp_amd as H
+    qp = big_messy_qp()
+    assert len(qp.q_val) > (1 << 21) and len(qp.a_val) > (1 << 21)
+    model = H.build_from_qabc((qp.n, qp.n, qp.q_ptr, qp.q_ind, qp.q_val), qp.c,
+                              (qp.m, qp.n, qp.a_ptr, qp.a_ind, qp.a_val), qp.AL, qp.AU, qp.l, qp.u)
+    got = model_arrays(model)
+    o = Q.OracleSession(qp, use_scaling=0, max_iter=0)
+    for g3, name in zip(got, ("Q", "A", "AT")):
+        for g, w_ in zip(g3, o.matrix(name)):
+            assert np.array_equal(g, w_), name
