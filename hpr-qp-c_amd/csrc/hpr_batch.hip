@@ -1391,6 +1391,7 @@ struct HPRQPHipBatch {
     double *d_vn = nullptr, *d_vm = nullptr, *d_val = nullptr, *d_dval = nullptr;
     int *d_rp = nullptr, *d_ci = nullptr, *d_perm = nullptr, *d_blocks = nullptr, *d_active = nullptr;
     int *d_steps = nullptr;  // adjoint step counts (device-buffer adjoint)
+    mutable hipEvent_t caller_done = nullptr;  // last work queued on a caller's stream that reads this batch
     std::vector<int> h_rp;  // host copy of all CSR row pointers (block planning)
     hprqp::batch::Inst *d_inst = nullptr;
     hprqp::batch::State *d_state = nullptr;
@@ -1401,6 +1402,10 @@ struct HPRQPHipBatch {
     ~HPRQPHipBatch()
     {
         if (stream) (void)hipStreamSynchronize(stream);
+        if (caller_done) {
+            (void)hipEventSynchronize(caller_done);
+            (void)hipEventDestroy(caller_done);
+        }
         if (graph) (void)hipGraphExecDestroy(graph);
         if (d_order) (void)hipFree(d_order);
         for (void *p : {(void *)d_vn, (void *)d_vm, (void *)d_val, (void *)d_dval, (void *)d_rp, (void *)d_ci, (void *)d_perm,
@@ -2164,6 +2169,25 @@ HPRQPStatus hprqp_hip_batch_solution(const HPRQPHipBatch *b, double *x, double *
     return HPRQP_STATUS_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Marks the end of work queued on a caller's stream that reads the batch, so
+// that destroying the batch waits for it.
+hipError_t mark_caller_use(const HPRQPHipBatch *b, hipStream_t s)
+{
+    if (!b->caller_done) {
+        const hipError_t e = hipEventCreateWithFlags(&b->caller_done, hipEventDisableTiming);
+        if (e != hipSuccess) return e;
+    }
+    return hipEventRecord(b->caller_done, s);
+}
+
+}  // namespace
+
+extern "C" {
+
 // Every instance's x, z (sum of n) and y (sum of m) in original units into
 // device buffers on the batch's device (any may be null), written by a
 // kernel on `stream` (a hipStream_t; null = the null stream) without a host
@@ -2179,6 +2203,7 @@ HPRQPStatus hprqp_hip_batch_solution_device(const HPRQPHipBatch *b, double *x, d
     batch_unscale_kernel<<<(unsigned)((len + hprqp::kThreads - 1) / hprqp::kThreads), hprqp::kThreads, 0,
                            (hipStream_t)stream>>>(b->d_vn, b->d_vm, b->ar.sum_n, b->ar.sum_m, real_m, x, y, z);
     HPRQP_HIP(hipGetLastError());
+    HPRQP_HIP(mark_caller_use(b, (hipStream_t)stream));
     return HPRQP_STATUS_OK;
 }
 
@@ -2254,6 +2279,7 @@ HPRQPStatus hprqp_hip_batch_adjoint_device(HPRQPHipBatch *b, const double *dl_dx
                                                                                 u, v_rows, v_bounds, tol, max_steps,
                                                                                 b->d_steps);
     HPRQP_HIP(hipGetLastError());
+    HPRQP_HIP(mark_caller_use(b, s));
     if (steps) {
         HPRQP_HIP(hipMemcpyAsync(steps, b->d_steps, sizeof(int) * b->count, hipMemcpyDeviceToHost, s));
         HPRQP_HIP(hipStreamSynchronize(s));

@@ -209,3 +209,33 @@ def test_layer_on_device_tensors_matches_host_tensors(mods):
     for k, name in enumerate(("Q", "c", "A", "AL", "AU", "l", "u"), start=1):
         want, got = out["cpu"][k], out["cuda:0"][k]
         assert np.max(np.abs(got - want)) <= 1e-12 * max(1.0, np.max(np.abs(want))), name
+
+
+def test_device_outputs_complete_when_batch_destroyed(mods):
+    """The device-buffer solution and adjoint are queued on the caller's
+    stream; destroying the batch right after waits for them, so the outputs
+    are whole."""
+    H, torch, _ = mods
+    B, n, meq, min_ = 4, 24, 4, 16
+    arrs = Q.layer_instances(B, n=n, m_eq=meq, m_in=min_, seed=11)
+    m = meq + min_
+    hb = H.Batch.from_dense(*arrs, H.Parameters(verbose=False, stoptol=1e-9))
+    hb.solve()
+    want = hb.solution()
+    g = np.random.default_rng(3).normal(size=B * n)
+    u_h = H.batch_adjoint_arrays(hb, g)[0]
+    dev = torch.device("cuda:0")
+    ts = [torch.tensor(a, dtype=torch.float64, device=dev) for a in arrs]
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    db = H.Batch.from_dense_device(B, n, m, [t.data_ptr() for t in ts], stream, H.Parameters(verbose=False, stoptol=1e-9))
+    db.solve()
+    x = torch.empty(B * n, dtype=torch.float64, device=dev)
+    tg = torch.tensor(g, device=dev)
+    U, VB = torch.empty_like(x), torch.empty_like(x)
+    VR = torch.empty(B * m, dtype=torch.float64, device=dev)
+    db.solution_device(x.data_ptr(), 0, 0, stream)
+    H.batch_adjoint_device(db, tg.data_ptr(), U.data_ptr(), VR.data_ptr(), VB.data_ptr(), stream)
+    del db  # no synchronisation by the caller before the batch goes
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(x.cpu().numpy(), want["x"])
+    np.testing.assert_array_equal(U.cpu().numpy(), u_h)
