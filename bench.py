@@ -219,8 +219,9 @@ def profiled_traffic(workload, kernel):
     counter summary of this workload (FETCH_SIZE and WRITE_SIZE runs, each
     scaled by what the counter reads for the library's plain copy kernel), or
     None when no summary covers it."""
-    path = ROOT / "profiles" / f"spmv_iteration_kernels_{workload}.md"
-    if not path.exists():
+    path = next((p for p in (ROOT / "profiles" / f"{kind}_iteration_kernels_{workload}.md" for kind in ("spmv", "soc"))
+                 if p.exists()), None)
+    if path is None:
         return None, None
     text = path.read_text()
     start = text.rfind("```json")
