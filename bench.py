@@ -356,6 +356,29 @@ def bench_batched(H, Q, args):
     wave_row = b.wave_row
     res = [b.result(k) for k in range(len(models))]
     ok = sum(r.status == "OPTIMAL" for r in res)
+    # Algorithmic bytes of one HPR iteration of an instance (general Q,
+    # interval rows, between checks): the CSR streams at 12 B per entry (Q
+    # twice, A and A^T once each), the row pointers and 23 n + 7 m vector
+    # passes; summed over every instance's iterations.
+    def iter_bytes(md):
+        n, m, nq, na = md.num_variables(), md.num_constraints(), md.c.Q.nnz, md.c.A.nnz
+        return 12 * (2 * nq + 2 * na) + 4 * (2 * (n + 1) + (m + 1) + (n + 1)) + 8 * (23 * n + 7 * m)
+    inst_iters = sum(int(r.iter) for r in res)
+    alg_bytes = sum(int(r.iter) * iter_bytes(md) for r, md in zip(res, models))
+    gbs = alg_bytes / secs / 1e9
+    prof = ROOT / "profiles" / "batch_solve_batched_4096.md"
+    traffic = None
+    if prof.exists() and "```json" in prof.read_text():
+        text = prof.read_text()
+        start = text.rfind("```json")
+        traffic = json.loads(text[start + 7:text.index("```", start + 7)])["kernels"]["batch_solve_kernel (CSR)"][
+            "traffic_bytes_per_instance_iteration"]
+    roofline = {"bound": "hbm", "kernel": "batch_solve_kernel (CSR)", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "traffic_unit": "bytes per instance-iteration (profiles/batch_solve_batched_4096.md)",
+                "algorithmic_bytes_per_instance_iteration": round(alg_bytes / max(inst_iters, 1)),
+                "instance_iterations": inst_iters}
+    log(f"roofline: {roofline}")
     # end to end like the sequential solves it is compared with: batch
     # creation (packing, upload, scaling, power iterations) plus the solve
     qps_s = len(models) / (setup + secs)
@@ -387,7 +410,7 @@ def bench_batched(H, Q, args):
                        "sequential_plain_gpu_qps": round(seq_qps, 3), "speedup_vs_sequential": round(qps_s / seq_qps, 2),
                        "sequential_sample": len(sample), "wave_row_autotuned": wave_row,
                        "chunk": "100 loop bodies per replay of a captured HIP graph"},
-            "cpu_baseline": cpu}
+            "roofline": roofline, "cpu_baseline": cpu}
     print(json.dumps(line), flush=True)
 
 
