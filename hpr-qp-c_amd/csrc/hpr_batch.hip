@@ -133,14 +133,15 @@ struct Mat {
 constexpr int kDenseMax = kStreamCap;
 
 // Column-form dot of a dense view: sum_{i<K} S[i*ld + j] xs[i], i ascending.
-// CACHED: ordinary loads with twice the loads in flight, for products that
-// re-read one instance's matrix many times (power iteration) -- a lone
-// instance's matrices then stay in L2; otherwise streaming (non-temporal)
-// loads for the solve, whose matrices are read once per iteration.
+// CACHED: ordinary loads, 16 in flight per thread, for products that re-read
+// one instance's matrix many times (power iteration) -- a lone instance's
+// matrices then stay in L2; otherwise streaming (non-temporal) loads, 32 in
+// flight per thread, for the solve, whose matrices are read once per
+// iteration (its dense kernel runs two waves per SIMD for the registers).
 template <bool CACHED = false>
 __device__ __forceinline__ double dense_column_dot(const Mat &M, int j, const double *xs)
 {
-    constexpr int U = CACHED ? 16 : 8;
+    constexpr int U = CACHED ? 16 : 32;
     const double *__restrict__ col = M.dv + j;
     double acc = 0.0;
     int i = 0;
@@ -1214,8 +1215,11 @@ __global__ __launch_bounds__(kThreads) void batch_setup_kernel(Arrays ar, Params
 }
 
 // Runs up to `bodies` loop bodies of every unfinished instance.
+// Dense instances stream their matrices with many loads in flight, which
+// needs more registers than four waves per SIMD leave; CSR instances keep
+// four waves per SIMD to hide their gathers.
 template <bool D>
-__global__ __launch_bounds__(kThreads, 4) void batch_solve_kernel(Arrays ar, Params P, int count, int bodies)
+__global__ __launch_bounds__(kThreads, D ? 2 : 4) void batch_solve_kernel(Arrays ar, Params P, int count, int bodies)
 {
     __shared__ double prod[kStreamCap];
     __shared__ double red[kMaxTerms];
