@@ -39,6 +39,7 @@ struct Inst {
     int q_blk, q_nblk, a_blk, a_nblk, at_blk, at_nblk;  // row-block lists (offset, count)
     int dense;               // 1: Q, A held as dense row-major arrays in Arrays::dval
     long long dq, da, dat;   // dense offsets: Q (n x n), A^T (n x m), A (m x n)
+    long long dg;            // dense: offset of G = A A^T (m x m) in Arrays::gram
     int noC;
     double norm_b_org, norm_c_org, obj_constant;
 };
@@ -83,6 +84,7 @@ struct Arrays {
     long long qt_perm_base;
     int *active;           // count of unfinished instances
     double *dval;          // dense instances' matrices (see Inst::dense)
+    double *gram;          // dense instances' G = A A^T of the scaled A (setup only)
     const int *order;      // workgroup -> instance (largest first)
     int wave_row;          // CSR rows with at least this many entries get a wavefront
 };
@@ -270,6 +272,7 @@ struct WGv {
     const Inst *I;
     State *S;  // in LDS
     Mat Q, A, AT;
+    Mat G;  // dense: A A^T (m x m) of the scaled A, for the lambda_max(A A^T) estimate
     const int *at_perm, *qt_rp, *qt_perm;
     double *a_val;
     double *vn[N_COUNT];
@@ -444,8 +447,15 @@ __device__ double wg_power(WG &g, bool a_matrix, double *sums, double *maxs)
         }
         wg_map(len, MapScaleDiv{{}, q, z, qn}, sums, maxs);
         if (a_matrix) {
-            wg_spmv<WG::kDense, true>(g.AT, q, EpiStore{{}, tmp}, sums, maxs, g.prod);
-            wg_spmv<WG::kDense, true>(g.A, tmp, EpiPowerZ{nullptr, z, q}, sums, maxs, g.prod);
+            if constexpr (WG::kDense) {
+                // z = G q with G = A A^T formed once: the same iteration as
+                // A (A^T q) up to rounding, one matrix pass instead of two
+                (void)tmp;
+                wg_spmv<true, true>(g.G, q, EpiPowerZ{nullptr, z, q}, sums, maxs, g.prod);
+            } else {
+                wg_spmv<false, true>(g.AT, q, EpiStore{{}, tmp}, sums, maxs, g.prod);
+                wg_spmv<false, true>(g.A, tmp, EpiPowerZ{nullptr, z, q}, sums, maxs, g.prod);
+            }
         } else {
             wg_spmv<WG::kDense, true>(g.Q, q, EpiPowerZ{nullptr, z, q}, sums, maxs, g.prod);
         }
@@ -463,9 +473,11 @@ __device__ double wg_power(WG &g, bool a_matrix, double *sums, double *maxs)
     return lam < 0.0 ? 0.0 : lam;
 }
 
-// workspace.c:191-248
+// workspace.c:191-248, in two parts: scaling, the A^T (or dense
+// transposed) copy, sigma0 and the diagonal-Q test; then the power iterations.
+// A dense batch forms G = A A^T between the two (dense_gram_kernel).
 template <class WG>
-__device__ void wg_setup(WG &g, double *sums, double *maxs)
+__device__ void wg_setup_pre(WG &g, double *sums, double *maxs)
 {
     const int n = g.I->n, m = g.I->m, t = threadIdx.x;
     wg_scale(g);
@@ -504,6 +516,18 @@ __device__ void wg_setup(WG &g, double *sums, double *maxs)
     if (diag)
         wg_map(n, MapDiagFactors{{}, g.N(N_fact1), g.N(N_fact2), g.N(N_fact), g.N(N_fact_M), g.N(N_diag), sigma},
                sums, maxs);
+    if (t == 0) {
+        g.S->sigma = sigma;
+        g.S->Q_is_diag = diag;
+    }
+    __syncthreads();
+}
+
+template <class WG>
+__device__ void wg_setup_post(WG &g, double *sums, double *maxs)
+{
+    const int n = g.I->n, t = threadIdx.x;
+    const int diag = g.S->Q_is_diag;
     double lam_a = wg_power(g, true, sums, maxs) * 1.05;
     double lam_q = 0.0;
     if (diag) {
@@ -515,7 +539,6 @@ __device__ void wg_setup(WG &g, double *sums, double *maxs)
         lam_q = wg_power(g, false, sums, maxs) * 1.05;
     }
     if (t == 0) {
-        g.S->sigma = sigma;
         g.S->lam_a = lam_a;
         g.S->lam_q = lam_q;
         g.S->Q_is_diag = diag;
@@ -998,6 +1021,7 @@ __device__ void wg_bind(WG &g, const Arrays &ar, const Params &P, int k, State *
         g.A.K = I->n, g.A.J = I->m, g.A.ld = I->m;
         g.AT.dv = ar.dval + I->dat;
         g.AT.K = I->m, g.AT.J = I->n, g.AT.ld = I->n;
+        g.G = Mat{nullptr, nullptr, nullptr, nullptr, 0, ar.gram ? ar.gram + I->dg : nullptr, I->m, I->m, I->m, 0};
     }
     g.a_val = ar.val + ar.a_base + I->a_nz;
     g.at_perm = ar.perm + I->a_nz;
@@ -1181,7 +1205,10 @@ __global__ __launch_bounds__(kThreads) void batch_adjoint_kernel(Arrays ar, Para
 }
 
 template <bool D>
-__global__ __launch_bounds__(kThreads) void batch_setup_kernel(Arrays ar, Params P, int count)
+// phase 0: the whole setup; 1: up to the power iterations (the state is
+// stored); 2: from the power iterations on (the state is loaded). Dense
+// batches run 1, dense_gram_kernel, 2.
+__global__ __launch_bounds__(kThreads) void batch_setup_kernel(Arrays ar, Params P, int count, int phase)
 {
     __shared__ double prod[kStreamCap];
     __shared__ double red[kMaxTerms];
@@ -1192,18 +1219,27 @@ __global__ __launch_bounds__(kThreads) void batch_setup_kernel(Arrays ar, Params
     WGv<D> g;
     wg_bind(g, ar, P, k, &S, prod, red);
     if (threadIdx.x == 0) {
-        S = State{};
-        S.first4 = S.first6 = 1;
-        S.kkt = INFINITY;
-        S.rs.first = 1;
-        S.rs.save_gap = S.rs.current_gap = S.rs.last_gap = S.rs.best_gap = INFINITY;
-        S.rs.weighted_norm = INFINITY;
-        S.rs.best_sigma = 1.0;
-        S.status = HPRQP_STATUS_CONTINUE;
+        if (phase == 2) {
+            S = ar.state[k];
+        } else {
+            S = State{};
+            S.first4 = S.first6 = 1;
+            S.kkt = INFINITY;
+            S.rs.first = 1;
+            S.rs.save_gap = S.rs.current_gap = S.rs.last_gap = S.rs.best_gap = INFINITY;
+            S.rs.weighted_norm = INFINITY;
+            S.rs.best_sigma = 1.0;
+            S.status = HPRQP_STATUS_CONTINUE;
+        }
     }
     __syncthreads();
     double sums[kMaxTerms] = {0}, maxs[kMaxTerms] = {0};
-    wg_setup(g, sums, maxs);
+    if (phase != 2) wg_setup_pre(g, sums, maxs);
+    if (phase == 1) {
+        if (threadIdx.x == 0) ar.state[k] = S;
+        return;
+    }
+    wg_setup_post(g, sums, maxs);
     // iteration-0 residuals (solver_backend.c:396)
     wg_residuals(g, 0, sums, maxs);
     if (threadIdx.x == 0) {
@@ -1287,6 +1323,53 @@ __global__ __launch_bounds__(kThreads) void dense_symmetrize_kernel(double *__re
         Q[(long long)r * n + c] = v;
         Q[(long long)c * n + r] = v;
     }
+}
+
+// G = A A^T of every dense instance (after scaling, for its power
+// iteration): one workgroup per instance, 32 x 32 output tiles of the upper
+// triangle with both halves written, the inner dimension in LDS chunks of
+// 32; each entry is summed over the columns in ascending order, so G is
+// exactly symmetric.
+__global__ __launch_bounds__(kThreads) void dense_gram_kernel(Arrays ar, int count)
+{
+    constexpr int T = 32;
+    __shared__ double ta[T][T + 1], tb[T][T + 1];
+    if ((int)blockIdx.x >= count) return;
+    const Inst &I = ar.inst[ar.order[blockIdx.x]];
+    if (!I.dense) return;
+    const int n = I.n, m = I.m, t = threadIdx.x;
+    const double *A = ar.dval + I.dat;  // row-major m x n
+    double *G = ar.gram + I.dg;
+    const int tr = t / (T / 4), tc = (t % (T / 4)) * 4;  // this thread's row and first of four columns
+    const int tiles = (m + T - 1) / T;
+    for (int bi = 0; bi < tiles; ++bi)
+        for (int bj = bi; bj < tiles; ++bj) {
+            double acc[4] = {0.0, 0.0, 0.0, 0.0};
+            for (int j0 = 0; j0 < n; j0 += T) {
+                for (int e = t; e < T * T; e += kThreads) {
+                    const int r = e / T, c = e % T, j = j0 + c;
+                    const int ra = bi * T + r, rb = bj * T + r;
+                    ta[r][c] = (ra < m && j < n) ? A[(long long)ra * n + j] : 0.0;
+                    tb[r][c] = (rb < m && j < n) ? A[(long long)rb * n + j] : 0.0;
+                }
+                __syncthreads();
+                for (int c = 0; c < T; ++c) {
+                    const double a = ta[tr][c];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) acc[u] += a * tb[tc + u][c];
+                }
+                __syncthreads();
+            }
+            const int i = bi * T + tr;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int i2 = bj * T + tc + u;
+                if (i < m && i2 < m) {
+                    G[(long long)i * m + i2] = acc[u];
+                    G[(long long)i2 * m + i] = acc[u];
+                }
+            }
+        }
 }
 
 // Per-instance facts of a dense instance that the host needs before setup:
@@ -1395,6 +1478,7 @@ struct HPRQPHipBatch {
     double *d_vn = nullptr, *d_vm = nullptr, *d_val = nullptr, *d_dval = nullptr;
     int *d_rp = nullptr, *d_ci = nullptr, *d_perm = nullptr, *d_blocks = nullptr, *d_active = nullptr;
     int *d_steps = nullptr;  // adjoint step counts (device-buffer adjoint)
+    double *d_gram = nullptr;  // G = A A^T of the dense instances (setup)
     mutable hipEvent_t caller_done = nullptr;  // last work queued on a caller's stream that reads this batch
     std::vector<int> h_rp;  // host copy of all CSR row pointers (block planning)
     hprqp::batch::Inst *d_inst = nullptr;
@@ -1413,7 +1497,8 @@ struct HPRQPHipBatch {
         if (graph) (void)hipGraphExecDestroy(graph);
         if (d_order) (void)hipFree(d_order);
         for (void *p : {(void *)d_vn, (void *)d_vm, (void *)d_val, (void *)d_dval, (void *)d_rp, (void *)d_ci, (void *)d_perm,
-                        (void *)d_blocks, (void *)d_active, (void *)d_inst, (void *)d_state, (void *)d_steps})
+                        (void *)d_blocks, (void *)d_active, (void *)d_inst, (void *)d_state, (void *)d_steps,
+                        (void *)d_gram})
             if (p) (void)hipFree(p);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -1563,8 +1648,28 @@ HPRQPStatus batch_finish(HPRQPHipBatch *b)
     ar.order = b->d_order;
     ar.wave_row = kWaveRowDefault;
     for (size_t k = 0; k < count; ++k) (b->inst[k].dense ? b->any_dense : b->any_sparse) = true;
-    if (b->any_sparse) batch_setup_kernel<false><<<(unsigned)count, hprqp::kThreads, 0, b->stream>>>(ar, b->P, (int)count);
-    if (b->any_dense) batch_setup_kernel<true><<<(unsigned)count, hprqp::kThreads, 0, b->stream>>>(ar, b->P, (int)count);
+    if (b->any_dense) {
+        // G = A A^T per dense instance (m x m each)
+        long long gsum = 0;
+        for (size_t k = 0; k < count; ++k)
+            if (b->inst[k].dense) {
+                b->inst[k].dg = gsum;
+                gsum += (long long)b->inst[k].m * b->inst[k].m;
+            }
+        if (hipMalloc((void **)&b->d_gram, sizeof(double) * (size_t)std::max(gsum, 1LL)) != hipSuccess ||
+            hipMemcpy(b->d_inst, b->inst.data(), sizeof(Inst) * count, hipMemcpyHostToDevice) != hipSuccess) {
+            delete b;
+            return HPRQP_STATUS_ALLOCATION_FAILED;
+        }
+        ar.gram = b->d_gram;
+    }
+    if (b->any_sparse)
+        batch_setup_kernel<false><<<(unsigned)count, hprqp::kThreads, 0, b->stream>>>(ar, b->P, (int)count, 0);
+    if (b->any_dense) {
+        batch_setup_kernel<true><<<(unsigned)count, hprqp::kThreads, 0, b->stream>>>(ar, b->P, (int)count, 1);
+        dense_gram_kernel<<<(unsigned)count, hprqp::kThreads, 0, b->stream>>>(ar, (int)count);
+        batch_setup_kernel<true><<<(unsigned)count, hprqp::kThreads, 0, b->stream>>>(ar, b->P, (int)count, 2);
+    }
     if (hipGetLastError() != hipSuccess || hipStreamSynchronize(b->stream) != hipSuccess) {
         delete b;
         return HPRQP_STATUS_NOT_IMPLEMENTED;
