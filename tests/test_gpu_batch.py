@@ -91,6 +91,31 @@ def test_batch_dense_mode_matches_sparse_mode(H, monkeypatch, n, m_eq, m_in):
                 assert np.max(np.abs(va - vb)) <= 1e-10 * scale, (k, v, max_iter, float(np.max(np.abs(va - vb))))
 
 
+def test_batch_dense_mixed_shapes_match_sparse_mode(H, monkeypatch):
+    """Dense instances of different shapes in one batch (m below, at and
+    above the 32-row tiles of the setup's G = A A^T, m < n and m > n) against
+    the CSR engine on the same models, per iterate and to optimality."""
+    shapes = [(10, 3, 4), (33, 8, 24), (70, 30, 35), (40, 16, 48), (20, 1, 2)]
+    models = []
+    for k, (n, meq, min_) in enumerate(shapes):
+        models.append(_layer_models(H, 2, n, meq, min_, seed=20 + k)[k % 2])  # free and boxed variables in turn
+    print("dense mixed shapes", flush=True)
+    for max_iter in (1, 60):
+        rd = _batch_x(H, models, monkeypatch, True, max_iter=max_iter)
+        rs = _batch_x(H, models, monkeypatch, False, max_iter=max_iter)
+        for k, (a, b) in enumerate(zip(rd, rs)):
+            assert a.iter == b.iter and a.status == b.status, (k, max_iter)
+            for v in ("x", "y", "z"):
+                va, vb = getattr(a, v), getattr(b, v)
+                scale = max(1.0, float(np.max(np.abs(vb))))
+                assert np.max(np.abs(va - vb)) <= 1e-10 * scale, (k, v, max_iter)
+    rd = _batch_x(H, models, monkeypatch, True, stoptol=1e-8)
+    for k, (model, r) in enumerate(zip(models, rd)):
+        plain = H.optimize(model, H.Parameters(verbose=False, stoptol=1e-8))
+        assert r.status == plain.status == "OPTIMAL", (k, r.status, plain.status)
+        assert abs(r.objective - plain.objective) <= 1e-7 * max(1.0, abs(plain.objective)), k
+
+
 def test_batch_dense_mode_full_solves_match_plain(H, monkeypatch):
     models = _layer_models(H, 5, 96, 24, 96, seed=4)
     res = _batch_x(H, models, monkeypatch, True, stoptol=1e-8)
