@@ -53,6 +53,15 @@ bool to_number(const std::string &tok, double *out)
     return true;
 }
 
+// A vector's data as a non-null pointer (the reference loader allocates its
+// arrays even for a model without variables or rows, and build_from_QAbc
+// rejects null ones).
+const double *data_of(const std::vector<double> &v)
+{
+    static const double none = 0.0;
+    return v.empty() ? &none : v.data();
+}
+
 struct Coo {
     std::vector<size_t> r, c;
     std::vector<double> v;
@@ -455,8 +464,8 @@ HPRQPStatus build_quadratic_rows(MpsModel &M, const HPRQPMatrixCSR &Q, const HPR
     const HPRQPMatrixCSR L{nl, n, lci.size(), lrp.data(), lci.data(), lval.data(), 0};
     const HPRQPMatrixCSR Cm{nq, n, cci.size(), crp.data(), cci.data(), cval.data(), 0};
     const HPRQPMatrixCSR Dm{nq, n, dci.size(), drp.data(), dci.data(), dval.data(), 0};
-    return build_from_QAbc_qc_diagonal(&Q, &L, M.c.data(), lo.data(), hi.data(), M.lvar.data(), M.uvar.data(), M.c0,
-                                       &Cm, constant.data(), &Dm, model);
+    return build_from_QAbc_qc_diagonal(&Q, &L, data_of(M.c), data_of(lo), data_of(hi), data_of(M.lvar),
+                                       data_of(M.uvar), M.c0, &Cm, data_of(constant), &Dm, model);
 }
 
 }  // namespace
@@ -520,7 +529,7 @@ extern "C" HPRQPStatus build_from_mps(const char *mps_path, HPRQPModel *model)
         HPRQPMatrixCSR Qm{n, n, qci.size(), qrp.data(), qci.data(), qval.data(), 0};
         HPRQPMatrixCSR Am{m, n, aci.size(), arp.data(), aci.data(), aval.data(), 0};
         if (!M.qc.empty()) return build_quadratic_rows(M, Qm, Am, model);
-        return build_from_QAbc(&Qm, &Am, M.c.data(), M.lcon.data(), M.ucon.data(), M.lvar.data(), M.uvar.data(),
+        return build_from_QAbc(&Qm, &Am, data_of(M.c), data_of(M.lcon), data_of(M.ucon), data_of(M.lvar), data_of(M.uvar),
                                M.c0, model);
     } catch (const std::bad_alloc &) {
         return HPRQP_STATUS_ALLOCATION_FAILED;

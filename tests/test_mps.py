@@ -88,6 +88,17 @@ def test_linear_program_file(H, tmp_path):
         Q.assert_same_model(qp, Q.ref_model_from_mps(path))
 
 
+def test_empty_model_file(H, tmp_path):
+    """NAME, ROWS and ENDATA alone: a model without variables or rows, as the
+    reference loader builds it."""
+    path = tmp_path / "empty.mps"
+    path.write_text("NAME E\nROWS\nENDATA\n")
+    qp = model_arrays(H.read_from_mps(path))
+    assert (qp.n, qp.m) == (0, 0)
+    if Q.ref_available():
+        Q.assert_same_model(qp, Q.ref_model_from_mps(path))
+
+
 REF_CASES = [("fixture", "simple_qp.mps", None), ("fixture", "model.mps", None),
              ("gen", "QUADOBJ", 1), ("gen", "QMATRIX", 2), ("gen", "QUADOBJ", 3)]
 
@@ -146,3 +157,72 @@ def test_rejected_files(H, tmp_path, name):
 def test_missing_file(H, tmp_path):
     with pytest.raises(ValueError, match="INVALID_ARGUMENT"):
         H.read_from_mps(tmp_path / "none.mps")
+
+
+MUTANT_TOKENS = ["1e999", "nan", "x", "1.5", "-0", "1d2", "+3", ".5e1", "inf", "1e-400"]
+MUTANT_LINES = ["RANGES", "BOUNDS", "RHS", "OBJSENSE", " MAX", " MIN", "OBJSENSE MIN", "ENDATA", "*c", "", "NAME B",
+                "QCMATRIX R1", " X1 X1 0.5", "QCMATRIX R6", " X2 X3 1", " X2 X2 -1", "QUADOBJ", " RNG R7 1"]
+
+
+def mutate(lines, rng):
+    """One to three random edits of an MPS file's lines: deleted, repeated,
+    inserted or re-indented lines, replaced, dropped or extra fields, tabs."""
+    lines = list(lines)
+    for _ in range(rng.integers(1, 4)):
+        i = int(rng.integers(len(lines)))
+        toks, lead = lines[i].split(), (" " if lines[i].startswith((" ", "\t")) else "")
+        op = int(rng.integers(9))
+        if op == 0 and len(lines) > 3:
+            del lines[i]
+        elif op == 1:
+            lines.insert(i, lines[int(rng.integers(len(lines)))])
+        elif op == 2 and toks:
+            toks[int(rng.integers(len(toks)))] = str(rng.choice(MUTANT_TOKENS))
+            lines[i] = lead + " ".join(toks)
+        elif op == 3 and len(toks) > 1:
+            del toks[int(rng.integers(len(toks)))]
+            lines[i] = lead + " ".join(toks)
+        elif op == 4:
+            lines[i] = lines[i].strip()
+        elif op == 5:
+            lines[i] = "   " + lines[i]
+        elif op == 6:
+            lines.insert(i, str(rng.choice(MUTANT_LINES)))
+        elif op == 7 and toks:
+            lines[i] = lead + " ".join(toks + [str(rng.choice(["R0", "X1", "1.0", "BND"]))])
+        elif op == 8:
+            lines[i] = lines[i].replace(" ", "\t", 1)
+    return lines
+
+
+@pytest.mark.skipif(not Q.ref_available(), reason="reference not built here (oracle/_ref)")
+def test_mutated_files_agree_with_reference_loader(H, tmp_path):
+    """Randomly edited MPS files (generated QUADOBJ, QMATRIX and QCMATRIX
+    models): this library and the reference loader accept and reject the
+    same files, and build bitwise-equal models from the accepted ones."""
+    from test_qcqp import qc_mps
+    bases = []
+    for seed in range(4):
+        qp0 = Q.random_qp(seed=seed, n=12, m_eq=3, m_in=4, a_row_nnz=3, b_rows=3, b_row_nnz=2)
+        p = tmp_path / f"base{seed}.mps"
+        Q.write_mps(qp0, p, seed=seed, quad=("QUADOBJ", "QMATRIX")[seed % 2])
+        bases.append(p.read_text().splitlines())
+    for seed in range(2):
+        p = tmp_path / f"qc{seed}.mps"
+        qc_mps(p, seed=30 + seed, n=10)
+        bases.append(p.read_text().splitlines())
+    rng = np.random.default_rng(2024)
+    accepted = 0
+    for it in range(240):
+        p = tmp_path / f"m{it}.mps"
+        p.write_text("\n".join(mutate(bases[int(rng.integers(len(bases)))], rng)) + "\n")
+        try:
+            mine = model_arrays(H.read_from_mps(p))
+        except ValueError:
+            mine = None
+        ref = Q.ref_model_from_mps(p)
+        assert (mine is None) == (ref is None), (it, "accepted by " + ("the reference" if mine is None else "this library"))
+        if mine is not None:
+            Q.assert_same_model(mine, ref)
+            accepted += 1
+    assert accepted > 30  # the edits leave a fair share of files valid
