@@ -200,3 +200,102 @@ def test_threaded_host_build_matches_oracle():
     for g3, name in zip(got, ("Q", "A", "AT")):
         for g, w_ in zip(g3, o.matrix(name)):
             assert np.array_equal(g, w_), name
+
+
+def _ref_build(qp):
+    """build_from_QAbc of the reference library (oracle/_ref): (status, QP)."""
+    L = Q.ref_lib()
+    keep = []
+
+    def mat(nrow, ncol, rp, ci, v):
+        rp, ci, v = (np.ascontiguousarray(a if len(a) else np.zeros(1, a.dtype)) for a in
+                     (np.asarray(rp, np.uint64), np.asarray(ci, np.uint64), np.asarray(v, np.float64)))
+        keep.extend([rp, ci, v])
+        return Q._CSR(nrow, ncol, int(rp[-1]) if nrow else 0, rp.ctypes.data_as(Q.c_size_p),
+                      ci.ctypes.data_as(Q.c_size_p), v.ctypes.data_as(Q.c_double_p), 0)
+
+    def dv(a):
+        a = np.ascontiguousarray(a if len(a) else np.zeros(1), np.float64)
+        keep.append(a)
+        return a.ctypes.data_as(Q.c_double_p)
+    L.build_from_QAbc.argtypes = [C.POINTER(Q._CSR), C.POINTER(Q._CSR)] + [Q.c_double_p] * 5 + [C.c_double,
+                                                                                                C.POINTER(Q._CModel)]
+    m = Q._CModel()
+    Qm = mat(qp.n, qp.n, qp.q_ptr, qp.q_ind, qp.q_val)
+    Am = mat(qp.m, qp.n, qp.a_ptr if qp.m else np.zeros(1, np.uint64), qp.a_ind, qp.a_val)
+    st = L.build_from_QAbc(C.byref(Qm), C.byref(Am), dv(qp.c), dv(qp.AL), dv(qp.AU), dv(qp.l), dv(qp.u),
+                           C.c_double(qp.obj_constant), C.byref(m))
+    try:
+        return st, (Q.qp_of_cmodel(m) if st == 0 else None)
+    finally:
+        L.model_free(C.byref(m))
+
+
+def _random_messy(rng, n, m):
+    """A small model with some of: NaN or inf entries, crossed bounds, free
+    and empty rows, repeated and unsorted entries, sub-tolerance values."""
+    def csr(nrow, ncol, per_row):
+        ptr, ind, val = [0], [], []
+        for _ in range(nrow):
+            k = int(rng.integers(0, per_row + 1))
+            cols = list(rng.integers(0, ncol, size=k))
+            rng.shuffle(cols)
+            ind += [int(c) for c in cols]
+            val += [float(v) for v in rng.standard_normal(k) * 10.0 ** rng.integers(-2, 2, size=k)]
+            ptr.append(len(ind))
+        return np.array(ptr, np.uint64), np.array(ind, np.uint64), np.array(val)
+    qptr, qind, qval = csr(n, n, 4)
+    aptr, aind, aval = csr(m, n, 4)
+    c = rng.standard_normal(n)
+    lo = rng.standard_normal(m) - 1.0
+    AL, AU = lo, lo + rng.uniform(0, 2, size=m)
+    l, u = -np.ones(n) * 3, np.ones(n) * 3
+    for _ in range(int(rng.integers(0, 3))):  # damage
+        kind = int(rng.integers(8))
+        if kind == 0 and len(qval):
+            qval[int(rng.integers(len(qval)))] = rng.choice([np.nan, np.inf, 1e-16, -3e-16])
+        elif kind == 1 and len(aval):
+            aval[int(rng.integers(len(aval)))] = rng.choice([np.nan, -np.inf, 0.0])
+        elif kind == 2 and m:
+            i = int(rng.integers(m))
+            AL[i], AU[i] = AU[i] + 1.0, AL[i]  # crossed
+        elif kind == 3 and m:
+            i = int(rng.integers(m))
+            AL[i], AU[i] = -np.inf, np.inf  # free row: deleted
+        elif kind == 4:
+            j = int(rng.integers(n))
+            l[j], u[j] = 2.0, 1.0  # crossed
+        elif kind == 5:
+            c[int(rng.integers(n))] = rng.choice([np.nan, np.inf])
+        elif kind == 6 and m:
+            AL[int(rng.integers(m))] = np.nan
+        elif kind == 7:
+            j = int(rng.integers(n))
+            l[j], u[j] = -np.inf, np.inf
+    return Q.QP(n, m, qptr, qind, qval, aptr, aind, aval, c, AL, AU, l, u, float(rng.standard_normal()))
+
+
+@pytest.mark.skipif(not Q.ref_available(), reason="reference not built here (oracle/_ref)")
+def test_random_builds_agree_with_reference():
+    """build_from_QAbc of this library and of the reference on 300 small
+    random models, a share of them invalid: the same status, and bitwise
+    equal models when both accept."""
+    import hprqp_amd as H
+    rng = np.random.default_rng(11)
+    ok = 0
+    for it in range(300):
+        n, m = int(rng.integers(1, 9)), int(rng.integers(0, 7))
+        qp = _random_messy(rng, n, m)
+        st_ref, ref = _ref_build(qp)
+        try:
+            model = H.build_from_qabc((qp.n, qp.n, qp.q_ptr, qp.q_ind, qp.q_val), qp.c,
+                                      (qp.m, qp.n, qp.a_ptr, qp.a_ind, qp.a_val) if qp.m else np.zeros((0, qp.n)),
+                                      qp.AL, qp.AU, qp.l, qp.u, qp.obj_constant)
+            mine = Q.qp_of_cmodel(model.c)
+        except ValueError:
+            mine = None
+        assert (mine is None) == (st_ref != 0), (it, st_ref)
+        if mine is not None:
+            Q.assert_same_model(mine, ref)
+            ok += 1
+    assert 60 < ok < 300
