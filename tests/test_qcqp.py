@@ -291,3 +291,77 @@ def test_factor_lift_equals_reference(H):
         Q.assert_same_model(mine, Q.qp_of_cmodel(m))
     finally:
         L.model_free(C.byref(m))
+
+
+@pytest.mark.skipif(not Q.ref_available(), reason="reference not built here (oracle/_ref)")
+def test_random_factor_lifts_agree_with_reference(H):
+    """build_from_QAbc_qc_factors of this library and of the reference on
+    random small inputs (factor blocks of any rank, a share of them invalid:
+    NaN factors, bad offsets, non-finite constants): the same status, and
+    bitwise-equal models when both accept."""
+    import ctypes as C
+    L = Q.ref_lib()
+    rng = np.random.default_rng(5)
+
+    def rand_csr(nrow, ncol, per_row):
+        ptr, ind, val = [0], [], []
+        for _ in range(nrow):
+            cols = np.sort(rng.choice(ncol, size=int(rng.integers(0, min(per_row, ncol) + 1)), replace=False))
+            ind += [int(c) for c in cols]
+            val += [float(v) for v in rng.standard_normal(len(cols))]
+            ptr.append(len(ind))
+        return csr(nrow, ncol, ptr, ind, val)
+
+    ok = 0
+    for it in range(200):
+        n, nq = int(rng.integers(1, 6)), int(rng.integers(0, 4))
+        offs = np.sort(rng.integers(0, 4, size=nq + 1)) if nq else np.zeros(1, np.int64)
+        offs = np.cumsum(np.concatenate([[0], np.diff(offs)])).astype(np.uint64) if nq else np.zeros(1, np.uint64)
+        R = rand_csr(int(offs[-1]), n, 3)
+        Cm = rand_csr(nq, n, 3)
+        const = rng.standard_normal(nq)
+        damage = int(rng.integers(6))
+        if damage == 1 and len(R[4]):
+            R[4][0] = np.nan
+        elif damage == 2 and nq:
+            offs = offs.copy()
+            offs[-1] += 1  # does not end at R's row count
+        elif damage == 3 and nq:
+            const[0] = np.inf
+        A = rand_csr(int(rng.integers(0, 3)), n, 2)
+        AL = -np.abs(rng.standard_normal(A[0])) - 0.1
+        AU = np.abs(rng.standard_normal(A[0])) + 0.1
+        lb, ub = -np.ones(n), np.ones(n)
+        try:
+            mine = model_arrays(H.build_from_qabc_qc(np.eye(n), np.ones(n), A, AL, AU, lb, ub, 0.5, Cm, const, R=R,
+                                                     offsets=offs))
+        except ValueError:
+            mine = None
+        keep = []
+
+        def mat(M):
+            nrow, ncol, rp, ci, v = M
+            rp, ci, v = (np.ascontiguousarray(a if len(a) else np.zeros(1, a.dtype)) for a in (rp, ci, v))
+            keep.extend([rp, ci, v])
+            return Q._CSR(nrow, ncol, int(rp[-1]) if nrow else 0, rp.ctypes.data_as(Q.c_size_p),
+                          ci.ctypes.data_as(Q.c_size_p), v.ctypes.data_as(Q.c_double_p), 0)
+
+        def dv(a):
+            a = np.ascontiguousarray(a if len(a) else np.zeros(1), np.float64)
+            keep.append(a)
+            return a.ctypes.data_as(Q.c_double_p)
+        eye = csr(n, n, list(range(n + 1)), list(range(n)), [1.0] * n)
+        off = np.ascontiguousarray(offs, np.uint64)
+        m = Q._CModel()
+        st = L.build_from_QAbc_qc_factors(C.byref(mat(eye)), C.byref(mat(A)), dv(np.ones(n)), dv(AL), dv(AU), dv(lb),
+                                          dv(ub), C.c_double(0.5), C.byref(mat(Cm)), dv(const), C.byref(mat(R)),
+                                          off.ctypes.data_as(Q.c_size_p), C.byref(m))
+        try:
+            ref = Q.qp_of_cmodel(m) if st == 0 else None
+        finally:
+            L.model_free(C.byref(m))
+        assert (mine is None) == (ref is None), (it, st, damage)
+        if mine is not None:
+            Q.assert_same_model(mine, ref)
+            ok += 1
+    assert 40 < ok < 200
