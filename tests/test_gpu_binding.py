@@ -102,3 +102,38 @@ def test_generated_quadratic_rows_on_gpu(H, tmp_path, seed):
     assert r.status == Q.STATUS[o.status] == "OPTIMAL"
     assert r.objective == pytest.approx(o.primal_obj, rel=1e-7, abs=1e-9)
     np.testing.assert_allclose(r.x, x, rtol=1e-5, atol=1e-6)
+
+
+@pytest.mark.parametrize("params,status", [({"stoptol": 1.0}, "OPTIMAL"), ({"time_limit": 0.0}, "TIME_LIMIT"),
+                                           ({"max_iter": 0}, "MAX_ITER")])
+def test_termination_at_iteration_zero(H, params, status):
+    """The loop's stopping rules at iteration 0 (solver_backend.c: a KKT below
+    stoptol = 1, a time limit of zero, max_iter = 0), as the oracle stops."""
+    qp = Q.reference_fixtures()["STANDARD"]
+    model = H.build_from_qabc((qp.n, qp.n, qp.q_ptr, qp.q_ind, qp.q_val), qp.c, (qp.m, qp.n, qp.a_ptr, qp.a_ind, qp.a_val),
+                              qp.AL, qp.AU, qp.l, qp.u)
+    print("termination", params, flush=True)
+    r = H.optimize(model, H.Parameters(verbose=False, **params))
+    o, x, y, z, w = Q.OracleSession(qp, **params).solve()
+    assert r.status == Q.STATUS[o.status] == status
+    assert r.iter == o.iter == 0
+    np.testing.assert_allclose(r.x, x, rtol=1e-12, atol=1e-14)
+
+
+def test_batch_time_limit_stops_after_a_chunk(H):
+    """A batch checks its time limit between chunks: with a limit of zero it
+    stops after the first chunk, instances not yet converged report
+    TIME_LIMIT with the iterations they ran, converged ones their result."""
+    models = [H.build_from_qabc((qp.n, qp.n, qp.q_ptr, qp.q_ind, qp.q_val), qp.c,
+                                (qp.m, qp.n, qp.a_ptr, qp.a_ind, qp.a_val), qp.AL, qp.AU, qp.l, qp.u)
+              for qp in (Q.random_qp(seed=s, n=300, m_eq=60, m_in=90, a_row_nnz=6, b_rows=75, b_row_nnz=4)
+                         for s in range(3))]
+    b = H.Batch(models, H.Parameters(verbose=False, stoptol=1e-10, time_limit=0.0))
+    print("batch time limit", flush=True)
+    left = b.solve(bodies_per_launch=20)
+    res = [b.result(k) for k in range(len(models))]
+    assert left == sum(r.status == "TIME_LIMIT" for r in res) > 0
+    for r in res:
+        assert r.status in ("TIME_LIMIT", "OPTIMAL")
+        if r.status == "TIME_LIMIT":
+            assert 0 < r.iter <= 20
