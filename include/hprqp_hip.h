@@ -143,7 +143,7 @@ HPRQPStatus hprqp_hip_calibration_copy(size_t bytes, double *milliseconds);
 /* Random-gather reference: mean time of one ELL product with 20 entries per
  * row over `nrow` rows, columns uniform over a `window`-element vector
  * (index and value loads coalesced). No reference counterpart: the ceiling
- * against which the gather-bound SpMV kernels are judged. */
+ * against which the gather-bound SpMV kernels are measured. */
 HPRQPStatus hprqp_hip_calibration_gather(int nrow, int window, int reps, double *milliseconds);
 
 /* Name of the device the library runs on, or "" when none is usable. */

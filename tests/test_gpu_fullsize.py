@@ -7,7 +7,7 @@ properties and the first loop bodies against the oracle.
 * Q's and A's scaled values agree with the oracle's scaling bit for bit;
 * after 10 loop bodies x_bar, y_bar, w_bar agree with the oracle to 1e-10
   relative (the oracle reuses the device's power-iteration lambdas to bound
-  its setup time; lambda parity itself is covered at C1 size);
+  its setup time; lambda parity itself is covered at n = 2000);
 * the column-panel carry leaves each row's sum sequential: the panelled
   products equal the unsplit ones bit for bit (HPRQP_PANELS forces counts).
 """

@@ -96,7 +96,7 @@ def test_traces_match_reference(H, name):
 
 
 def test_per_iterate_first_100_c1(H):
-    """BASELINE config 1 (n=2000, m=1000): x_bar, y_bar, w_bar agree to 1e-10
+    """The n = 2000, m = 1000 generated workload: x_bar, y_bar, w_bar agree to 1e-10
     relative with the oracle after each of the first 100 loop bodies."""
     qp = Q.random_qp(seed=0)
     model = build(H, qp)
