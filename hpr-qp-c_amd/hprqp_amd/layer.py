@@ -125,8 +125,9 @@ def _forward_device(ctx, inputs, stoptol, bodies_per_launch):
     stream = torch.cuda.current_stream(dev).cuda_stream
     t1 = time.perf_counter()
     try:
+        index = dev.index if dev.index is not None else torch.cuda.current_device()
         batch = Batch.from_dense_device(B, n, m, [t.data_ptr() for t in ts], stream,
-                                        Parameters(verbose=False, stoptol=stoptol, device_number=dev.index or 0))
+                                        Parameters(verbose=False, stoptol=stoptol, device_number=index))
     except ValueError as e:
         raise ValueError("QPLayer: rows with no entries or free bounds are not supported "
                          f"(or invalid data): {e}") from None
