@@ -1252,10 +1252,11 @@ __global__ __launch_bounds__(kThreads) void batch_setup_kernel(Arrays ar, Params
 
 // Runs up to `bodies` loop bodies of every unfinished instance.
 // Dense instances stream their matrices with many loads in flight, which
-// needs more registers than four waves per SIMD leave; CSR instances keep
-// four waves per SIMD to hide their gathers.
+// needs more registers than four waves per SIMD leave; CSR instances are
+// latency-bound (gathers, barriers) and run six waves per SIMD, which beats
+// the spills that the smaller register budget costs.
 template <bool D>
-__global__ __launch_bounds__(kThreads, D ? 2 : 4) void batch_solve_kernel(Arrays ar, Params P, int count, int bodies)
+__global__ __launch_bounds__(kThreads, D ? 2 : 6) void batch_solve_kernel(Arrays ar, Params P, int count, int bodies)
 {
     __shared__ double prod[kStreamCap];
     __shared__ double red[kMaxTerms];
