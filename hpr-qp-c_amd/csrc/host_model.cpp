@@ -580,7 +580,7 @@ HPRQPStatus build_from_QAbc_soc(const HPRQPMatrixCSR *Q, const HPRQPMatrixCSR *A
 }
 
 // solver.c:133-157. use_gpu defaults to 1 (this library is the GPU backend);
-// gram_mode defaults to OFF and polishing to 0 because neither exists here.
+// gram_mode defaults to OFF because the Gram path does not exist here.
 void params_set_defaults(HPRQPParams *params)
 {
     if (!params) return;
@@ -597,7 +597,7 @@ void params_set_defaults(HPRQPParams *params)
     params->spmv_mode_q = HPRQP_SPMV_MODE_AUTO;
     params->spmv_mode_a = HPRQP_SPMV_MODE_AUTO;
     params->gram_mode = HPRQP_GRAM_OFF;
-    params->polishing = 0;
+    params->polishing = 1;
     params->polish_max_attempts = 12;
     params->polish_episode_seconds = 20.0;
     params->polish_total_seconds = 0.0;

@@ -1916,7 +1916,7 @@ HPRQPStatus hprqp_hip_session_create(const HPRQPModel *model, const HPRQPParams 
         // The cone step is provided by the SOC kernels; refuse until then.
     }
     s->setup_seconds = hprqp::wall_seconds() - t0;
-    s->loop.reset(new hprqp::SolveLoop(&s->backend, &s->params, "GPU mode"));
+    s->loop.reset(new hprqp::SolveLoop(&s->backend, &s->params, "GPU mode", model));
     *out = s;
     return HPRQP_STATUS_OK;
 }

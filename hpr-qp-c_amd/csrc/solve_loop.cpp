@@ -3,9 +3,11 @@
 #include "solve_loop.h"
 
 #include "hprqp.h"
+#include "polish_owner.h"
 
 #include <sys/time.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -161,11 +163,33 @@ static int check_next(const HPRQPParams *p, const HPRQPRestart *r, size_t next)
     return chk;
 }
 
-SolveLoop::SolveLoop(Backend *backend, const HPRQPParams *params, const char *label)
-    : b_(backend), p_(params), label_(label)
+SolveLoop::SolveLoop(Backend *backend, const HPRQPParams *params, const char *label, const HPRQPModel *model)
+    : b_(backend), p_(params), label_(label), model_(model)
 {
     std::memset(&res_, 0, sizeof(res_));
     restart_init(&restart_);
+}
+
+SolveLoop::~SolveLoop() = default;
+
+// solver_backend.c:193-211 (check_termination_status)
+HPRQPStatus SolveLoop::termination(size_t iter, double t) const
+{
+    if (res_.KKTx_and_gap_org_bar < p_->stoptol) return HPRQP_STATUS_OPTIMAL;
+    if (iter == p_->max_iter) return HPRQP_STATUS_MAX_ITER;
+    if (t > p_->time_limit) return HPRQP_STATUS_TIME_LIMIT;
+    return HPRQP_STATUS_CONTINUE;
+}
+
+// solver_backend.c:547, 567 (hprqp_polish_destroy at the end of the solve):
+// a worker still running is stopped and its files removed; the counts stay
+// for the results.
+void SolveLoop::end_polishing()
+{
+    if (!polisher_) return;
+    polish_attempts_ = polisher_->attempts();
+    polish_accepted_ = polisher_->accepted();
+    polisher_.reset();
 }
 
 double SolveLoop::elapsed() const
@@ -207,6 +231,8 @@ HPRQPStatus SolveLoop::begin()
         std::printf(" iter     errRp        errRd         p_obj           d_obj          gap        sigma       time\n");
         log_line(0, elapsed());
     }
+    // solver_backend.c:429
+    if (model_) polisher_.reset(polish::Owner::create(model_, p_));
     restart_.best_sigma = b_->sigma();
     iter_ = 0;
     started_ = true;
@@ -235,10 +261,31 @@ HPRQPStatus SolveLoop::body()
             res_.is_updated = 0;
         }
     }
-    final_status_ = HPRQP_STATUS_CONTINUE;
-    if (res_.KKTx_and_gap_org_bar < p_->stoptol) final_status_ = HPRQP_STATUS_OPTIMAL;
-    else if (iter == p_->max_iter) final_status_ = HPRQP_STATUS_MAX_ITER;
-    else if (t > p_->time_limit) final_status_ = HPRQP_STATUS_TIME_LIMIT;
+    final_status_ = termination(iter, t);
+    // solver_backend.c:467-479: a certified candidate from the polishing
+    // worker ends the solve at this check point
+    if (polisher_ && final_status_ == HPRQP_STATUS_CONTINUE && res_.is_updated) {
+        const size_t n = b_->n(), m = b_->m();
+        polished_x_.assign(n, 0.0);
+        polished_w_.assign(n, 0.0);
+        polished_z_.assign(n, 0.0);
+        polished_y_.assign(m, 0.0);
+        HPRQPResiduals polished;
+        if (polisher_->poll(b_, res_, iter, t, polished_x_.data(), polished_w_.data(), polished_z_.data(),
+                            polished_y_.data(), &polished)) {
+            res_ = polished;
+            final_status_ = HPRQP_STATUS_OPTIMAL;
+            finished_ = true;
+            end_polishing();
+            return HPRQP_STATUS_OK;
+        }
+        polished_x_.clear();
+        polished_w_.clear();
+        polished_z_.clear();
+        polished_y_.clear();
+        t = elapsed();
+        final_status_ = termination(iter, t);
+    }
 
     restart_check(&restart_, iter, p_->check_iter, b_->sigma());
     st = b_->update_sigma(&restart_, &res_);
@@ -248,6 +295,7 @@ HPRQPStatus SolveLoop::body()
     if (p_->verbose && (print_yes || final_status_ != HPRQP_STATUS_CONTINUE)) log_line(iter, elapsed());
     if (final_status_ != HPRQP_STATUS_CONTINUE) {
         finished_ = true;
+        end_polishing();
         return HPRQP_STATUS_OK;
     }
     b_->set_to_check(check_next(p_, &restart_, iter + 1));
@@ -277,8 +325,16 @@ HPRQPStatus SolveLoop::collect(HPRQPResults *results)
         results_free(results);
         return HPRQP_STATUS_ALLOCATION_FAILED;
     }
-    HPRQPStatus st = b_->copy_output_iterates(results->x, results->w, results->z, results->y);
-    if (st != HPRQP_STATUS_OK) return st;
+    if (polish_accepted_ && polished_x_.size() == n) {
+        // solver_backend.c:472-474: the accepted candidate is the output
+        std::copy(polished_x_.begin(), polished_x_.end(), results->x);
+        std::copy(polished_w_.begin(), polished_w_.end(), results->w);
+        std::copy(polished_z_.begin(), polished_z_.end(), results->z);
+        std::copy(polished_y_.begin(), polished_y_.end(), results->y);
+    } else {
+        HPRQPStatus st = b_->copy_output_iterates(results->x, results->w, results->z, results->y);
+        if (st != HPRQP_STATUS_OK) return st;
+    }
     results->iter = iter_;
     results->time = t;
     results->iter_4 = iter4_ == 0 ? iter_ : iter4_;
@@ -289,6 +345,8 @@ HPRQPStatus SolveLoop::collect(HPRQPResults *results)
     results->residuals = res_.KKTx_and_gap_org_bar;
     results->gap = res_.rel_gap_bar;
     results->status = finished_ ? final_status_ : HPRQP_STATUS_CONTINUE;
+    results->polish_attempts = polisher_ ? polisher_->attempts() : polish_attempts_;
+    results->polish_accepted = polisher_ ? (int)polisher_->accepted() : (int)polish_accepted_;
     return HPRQP_STATUS_OK;
 }
 

@@ -6,10 +6,16 @@
 #pragma once
 
 #include <cstddef>
+#include <memory>
+#include <vector>
 
 #include "shared/types.h"
 
 namespace hprqp {
+
+namespace polish {
+class Owner;
+}
 
 class Backend {
 public:
@@ -38,9 +44,13 @@ double golden_search_sigma(double a, double b, double c, double d);
 double sigma_from_estimate(double estimate, double sigma_old, const HPRQPRestart *r, const HPRQPResiduals *res);
 
 // The loop of src/shared/solver_backend.c:338-569, runnable one body at a time.
+// With `model` given and params->polishing set, check points also feed the
+// host polishing worker (solver_backend.c:429, 467-479); the model must then
+// outlive the loop.
 class SolveLoop {
 public:
-    SolveLoop(Backend *backend, const HPRQPParams *params, const char *label);
+    SolveLoop(Backend *backend, const HPRQPParams *params, const char *label, const HPRQPModel *model = nullptr);
+    ~SolveLoop();
     HPRQPStatus begin();                    // iteration-0 residuals, header
     HPRQPStatus body();                     // one loop body
     bool finished() const { return finished_; }
@@ -66,6 +76,13 @@ private:
     int first4_ = 1, first6_ = 1;
     size_t iter4_ = 0, iter6_ = 0;
     double time4_ = 0.0, time6_ = 0.0;
+    const HPRQPModel *model_;
+    std::unique_ptr<polish::Owner> polisher_;
+    size_t polish_attempts_ = 0;
+    bool polish_accepted_ = false;
+    std::vector<double> polished_x_, polished_w_, polished_z_, polished_y_;  // an accepted candidate
+    HPRQPStatus termination(size_t iter, double t) const;
+    void end_polishing();
     void milestones(size_t iter, double t);
     void log_line(size_t iter, double t) const;
 };

@@ -157,6 +157,8 @@ class Result:
     z: np.ndarray
     w: np.ndarray
     total_time: float = 0.0
+    polish_attempts: int = 0
+    polish_accepted: bool = False
 
 
 class Parameters:
@@ -168,7 +170,11 @@ class Parameters:
         for key, value in kwargs.items():
             if not hasattr(self._p, key):
                 raise TypeError(f"unknown parameter {key!r}")
-            setattr(self._p, key, int(value) if isinstance(value, bool) else value)
+            if isinstance(value, bool):
+                value = int(value)
+            elif isinstance(value, str):  # the path fields are char pointers
+                value = value.encode()
+            setattr(self._p, key, value)
 
     @property
     def c(self) -> CParams:
@@ -328,7 +334,8 @@ def _vec(p, n):
 
 def _result_of(r: CResults, st: int, n: int, m: int) -> Result:
     return Result(st, status_name(st), r.iter, r.iter_4, r.iter_6, r.time, r.time_4, r.time_6, r.primal_obj,
-                  r.residuals, r.gap, _vec(r.x, n), _vec(r.y, m), _vec(r.z, n), _vec(r.w, n), r.total_time)
+                  r.residuals, r.gap, _vec(r.x, n), _vec(r.y, m), _vec(r.z, n), _vec(r.w, n), r.total_time,
+                  int(r.polish_attempts), bool(r.polish_accepted))
 
 
 def optimize(model: Model, params: Parameters | None = None) -> Result:

@@ -19,7 +19,10 @@ extern "C" {
  * setup happen in create; advance runs bodies of the solve loop of
  * src/shared/solver_backend.c:432-513 (one body = residual check if due,
  * termination test, restart / sigma update, one HPR iteration, M-norm if due).
- * Vectors read back are in the scaled space unless noted. */
+ * Vectors read back are in the scaled space unless noted. With
+ * params->polishing set (the default, as in the reference) check points also
+ * feed the host polishing worker when one is installed, so the model and the
+ * strings params points to must outlive the session's last advance. */
 typedef struct HPRQPHipSession HPRQPHipSession;
 
 HPRQPStatus hprqp_hip_session_create(const HPRQPModel *model, const HPRQPParams *params,
