@@ -266,10 +266,12 @@ HPRQPStatus SolveLoop::body()
     // worker ends the solve at this check point
     if (polisher_ && final_status_ == HPRQP_STATUS_CONTINUE && res_.is_updated) {
         const size_t n = b_->n(), m = b_->m();
-        polished_x_.assign(n, 0.0);
-        polished_w_.assign(n, 0.0);
-        polished_z_.assign(n, 0.0);
-        polished_y_.assign(m, 0.0);
+        if (polished_x_.size() != n || polished_y_.size() != m) {  // written only when a candidate is accepted
+            polished_x_.resize(n);
+            polished_w_.resize(n);
+            polished_z_.resize(n);
+            polished_y_.resize(m);
+        }
         HPRQPResiduals polished;
         if (polisher_->poll(b_, res_, iter, t, polished_x_.data(), polished_w_.data(), polished_z_.data(),
                             polished_y_.data(), &polished)) {
@@ -279,10 +281,6 @@ HPRQPStatus SolveLoop::body()
             end_polishing();
             return HPRQP_STATUS_OK;
         }
-        polished_x_.clear();
-        polished_w_.clear();
-        polished_z_.clear();
-        polished_y_.clear();
         t = elapsed();
         final_status_ = termination(iter, t);
     }
