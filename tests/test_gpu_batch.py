@@ -34,6 +34,31 @@ def test_batch_matches_plain_and_oracle(H):
         assert np.max(np.abs(r.x - x)) <= 1e-5 * max(1.0, np.max(np.abs(x))), k
 
 
+def test_batch_ragged_edge_instances(H):
+    """One batch holding a one-variable QP, an LP (Q = 0), an unconstrained
+    QP (m = 0) and a generated QP: each instance as the plain solve."""
+    inf = math.inf
+    qp = Q.random_qp(seed=5, n=50, m_eq=10, m_in=8, a_row_nnz=4, b_rows=12, b_row_nnz=3)
+    models = [
+        H.build_from_qabc([[2.0]], [1.0], [[1.0]], [0.0], [1.0], [-10.0], [10.0]),
+        H.build_from_qabc([[0.0, 0.0], [0.0, 0.0]], [1.0, 2.0], [[1.0, 1.0]], [1.0], [inf], [0.0, 0.0],
+                          [10.0, 10.0]),
+        H.build_from_qabc([[2.0, 0.0], [0.0, 2.0]], [-4.0, -2.0], [], [], [], [-inf, -inf], [inf, inf]),
+        build(H, qp),
+    ]
+    params = H.Parameters(verbose=False, stoptol=1e-8)
+    res, _ = H.solve_batch(models, params, bodies_per_launch=25)
+    expect = [0.0, 1.0, -5.0, None]
+    for k, (model, r) in enumerate(zip(models, res)):
+        plain = H.optimize(model, params)
+        assert r.status == plain.status == "OPTIMAL", (k, r.status, plain.status)
+        assert abs(r.objective - plain.objective) <= 1e-7 * max(1.0, abs(plain.objective)), k
+        assert np.max(np.abs(r.x - plain.x)) <= 1e-5 * max(1.0, np.max(np.abs(plain.x))), k
+        assert len(r.y) == len(plain.y), k
+        if expect[k] is not None:
+            assert r.objective == pytest.approx(expect[k], abs=1e-6), (k, r.objective)
+
+
 def test_batch_fixture_traces(H):
     """Reference fixtures (tests/test_support.c) as one batch at max_iter
     100: each instance's iterate equals the reference's golden trace."""

@@ -117,8 +117,10 @@ def test_termination_at_iteration_zero(H, params, status):
     o, x, y, z, w = Q.OracleSession(qp, **params).solve()
     assert r.status == Q.STATUS[o.status] == status
     if "time_limit" in params:
-        # elapsed > 0 is first true at iteration 0 or 1, by the clock
-        assert r.iter <= 1 and o.iter <= 1
+        # elapsed > 0 is first true when the microsecond clock ticks: at
+        # iteration 0 or 1 on the GPU (an iteration takes microseconds), after
+        # a few iterations of the CPU oracle on this three-variable model
+        assert r.iter <= 1 and o.iter <= 10
         return
     assert r.iter == o.iter == 0
     np.testing.assert_allclose(r.x, x, rtol=1e-12, atol=1e-14)
