@@ -437,9 +437,12 @@ std::string find_worker(const char *override_path, const std::vector<std::string
     }
     for (const auto &c : candidates) {
         std::error_code ec;
-        if (fs::is_regular_file(c, ec) && (python || access(c.c_str(), X_OK) == 0)) return fs::absolute(c).string();
+        if (fs::is_regular_file(c, ec) && (python || access(c.c_str(), X_OK) == 0)) {
+            const fs::path abs = fs::absolute(c, ec);
+            if (!ec) return abs.string();
+        }
     }
-    throw std::runtime_error("CPU worker unavailable; build/install hprqp_polish_worker");
+    return std::string();
 }
 
 // worker_path.hpp:42-52: this library's file and the running executable
@@ -472,6 +475,7 @@ Owner::Owner(const HPRQPModel *model, const HPRQPParams *params) : params_(*para
     python_ = params->polish_python ? params->polish_python : "";
     directory_ = params->polish_directory ? params->polish_directory : "/tmp";
     worker_ = find_worker(params->polish_worker, default_anchors(), !python_.empty());
+    if (worker_.empty()) throw std::runtime_error("CPU worker unavailable; build/install hprqp_polish_worker");
     if (params->polish_episode_seconds < 0 || params->polish_total_seconds < 0 ||
         !std::isfinite(params->polish_episode_seconds) || !std::isfinite(params->polish_total_seconds))
         throw std::runtime_error("invalid worker budgets");
@@ -483,6 +487,13 @@ Owner::Owner(const HPRQPModel *model, const HPRQPParams *params) : params_(*para
 Owner *Owner::create(const HPRQPModel *model, const HPRQPParams *params)
 {
     if (!model || !params || !params->polishing || model->num_var_cones) return nullptr;
+    // The usual case, no worker installed, is settled without an exception:
+    // a first throw in a process with many large libraries costs tens of ms.
+    if (find_worker(params->polish_worker, default_anchors(), params->polish_python != nullptr).empty()) {
+        if (params->verbose && diagnostics_on())
+            std::fprintf(stderr, "POLISH_DISABLED CPU worker unavailable; build/install hprqp_polish_worker\n");
+        return nullptr;
+    }
     try {
         return new Owner(model, params);
     } catch (const std::exception &e) {

@@ -99,7 +99,7 @@ void write_snapshot(const std::string &path, size_t n, size_t m, const double *x
 // for a Python worker) beside each anchor (this library, the executable),
 // then `libexec/hprqp/hprqp_polish_worker` (`share/hprqp/polishing/
 // worker.py`) up to three directories above it. A native worker must be
-// executable. Throws when none is found.
+// executable. Returns an empty string when none is found.
 std::string find_worker(const char *override_path, const std::vector<std::string> &anchors, bool python);
 std::vector<std::string> default_anchors();
 
