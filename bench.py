@@ -285,6 +285,10 @@ def bench_single(H, Q, args, qp, workload, soc=False):
         # the dominant kernel against the gather-only reference kernel at the
         # same gather count and window (the gathered vector / column panels).
         s = H.Session(model, H.Parameters(verbose=False))
+        # the setup autotuner's choices: column panels and the residency cap
+        # (dynamic LDS bytes per workgroup; 0 = uncapped) per matrix
+        roofline["panels"] = {m_: int(s.scalar("panels_" + m_)) for m_ in ("Q", "A", "AT")}
+        roofline["lds_pad"] = {m_: int(s.scalar("lds_pad_" + m_)) for m_ in ("Q", "A", "AT")}
         mat = {"zxw1_q": "Q", "tempv_q": "Q", "y_a": "A", "w2_at": "AT"}.get(dom["kernel"])
         if mat:
             panels = max(int(s.scalar("panels_" + mat)), 1)

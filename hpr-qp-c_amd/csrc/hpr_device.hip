@@ -138,6 +138,7 @@ constexpr int kMaxPanels = 4;
 struct Panels {
     int P = 1;
     int width = 0;
+    unsigned lds_pad = 0;  // unused dynamic LDS per workgroup: caps residency (autotune_panels)
     DevCsr part[kMaxPanels];
     int *src[kMaxPanels] = {nullptr, nullptr, nullptr, nullptr};
 };
@@ -941,21 +942,22 @@ private:
         const bool timed = profiling_ && tag >= 0;
         if (timed) prof_begin(tag);
         const Panels &pn = panels_of(M);
+        const size_t pad = pn.lds_pad;
         if (pn.P <= 1) {
-            spmv_rows_kernel<Epi><<<M.nblocks, kThreads, 0, stream_>>>(M.rp, M.ci, M.val, x, M.blocks, epi);
+            spmv_rows_kernel<Epi><<<M.nblocks, kThreads, pad, stream_>>>(M.rp, M.ci, M.val, x, M.blocks, epi);
         } else {
             double *carry = M.nrow == (int)n_ ? vec(V_carry_n) : vec(V_carry_m);
             for (int p = 0; p + 1 < pn.P; ++p) {
                 const DevCsr &d = pn.part[p];
                 if (p == 0)
-                    spmv_rows_kernel<EpiCarry, false><<<d.nblocks, kThreads, 0, stream_>>>(
+                    spmv_rows_kernel<EpiCarry, false><<<d.nblocks, kThreads, pad, stream_>>>(
                         d.rp, d.ci, d.val, x, d.blocks, EpiCarry{{}, carry}, carry);
                 else
-                    spmv_rows_kernel<EpiCarry, true><<<d.nblocks, kThreads, 0, stream_>>>(
+                    spmv_rows_kernel<EpiCarry, true><<<d.nblocks, kThreads, pad, stream_>>>(
                         d.rp, d.ci, d.val, x, d.blocks, EpiCarry{{}, carry}, carry);
             }
             const DevCsr &d = pn.part[pn.P - 1];
-            spmv_rows_kernel<Epi, true><<<d.nblocks, kThreads, 0, stream_>>>(d.rp, d.ci, d.val, x, d.blocks, epi,
+            spmv_rows_kernel<Epi, true><<<d.nblocks, kThreads, pad, stream_>>>(d.rp, d.ci, d.val, x, d.blocks, epi,
                                                                               carry);
         }
         if (timed) prof_end();
@@ -1171,6 +1173,21 @@ HPRQPStatus HipBackend::autotune_panels()
 {
     const char *force = std::getenv("HPRQP_PANELS");
     const int forced = force ? std::atoi(force) : 0;
+    // Residency cap for the products (see below): a dynamic LDS request of
+    // a third of the CU's LDS minus the kernel's static LDS (the 16 KiB
+    // product stage and a few reduction slots) holds three workgroups per
+    // CU. HPRQP_OCCUPANCY_CAP=0 never applies it, =1 always where eligible.
+    const char *cap_env = std::getenv("HPRQP_OCCUPANCY_CAP");
+    const int cap_forced = cap_env ? std::atoi(cap_env) : -1;
+    int lds_cu = 0, lds_block = 0, cus = 0;
+    HPRQP_HIP(hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device_));
+    HPRQP_HIP(hipDeviceGetAttribute(&lds_block, hipDeviceAttributeMaxSharedMemoryPerBlock, device_));
+    HPRQP_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device_));
+    constexpr int kStaticLds = 17 * 1024;  // above every spmv_rows_kernel's static LDS
+    int pad3 = (lds_cu / 3 - kStaticLds) & ~255;
+    if (pad3 <= 0 || kStaticLds + pad3 > lds_block || lds_cu / (kStaticLds + pad3) != 3 ||
+        lds_cu / (kStaticLds - 1024 + pad3) != 3)
+        pad3 = 0;
     struct Item {
         const DevCsr *M;
         Panels *pn;
@@ -1190,6 +1207,20 @@ HPRQPStatus HipBackend::autotune_panels()
             if (M.ncol >= 2 * 32768) cands.push_back(2);
             if (M.ncol >= 4 * 32768) cands.push_back(4);
         }
+        auto probe_ms = [&](float *out) -> HPRQPStatus {
+            float ms_min = 1e30f;
+            for (int rep = 0; rep < 5; ++rep) {
+                HPRQP_HIP(hipEventRecord(e0, stream_));
+                spmv(M, it.x, EpiStore{{}, it.y});
+                HPRQP_HIP(hipEventRecord(e1, stream_));
+                HPRQP_HIP(hipEventSynchronize(e1));
+                float ms = 0.0f;
+                HPRQP_HIP(hipEventElapsedTime(&ms, e0, e1));
+                ms_min = std::min(ms_min, ms);
+            }
+            *out = ms_min;
+            return HPRQP_STATUS_OK;
+        };
         int best = 1;
         float best_ms = 1e30f;
         for (int P : cands) {
@@ -1201,15 +1232,8 @@ HPRQPStatus HipBackend::autotune_panels()
                 break;
             }
             float ms_min = 1e30f;
-            for (int rep = 0; rep < 5; ++rep) {
-                HPRQP_HIP(hipEventRecord(e0, stream_));
-                spmv(M, it.x, EpiStore{{}, it.y});
-                HPRQP_HIP(hipEventRecord(e1, stream_));
-                HPRQP_HIP(hipEventSynchronize(e1));
-                float ms = 0.0f;
-                HPRQP_HIP(hipEventElapsedTime(&ms, e0, e1));
-                ms_min = std::min(ms_min, ms);
-            }
+            st = probe_ms(&ms_min);
+            if (st != HPRQP_STATUS_OK) return st;
             // more panels must win by 3 %: the loop's products carry heavier
             // epilogues than this probe, and every panel adds a launch and a
             // carry pass
@@ -1223,6 +1247,29 @@ HPRQPStatus HipBackend::autotune_panels()
             HPRQPStatus st = build_panels(M, best, *it.pn, stream_);
             if (st == HPRQP_STATUS_OK) st = fill_panel_values(M, *it.pn, stream_);
             if (st != HPRQP_STATUS_OK) return st;
+        }
+        // Occupancy: the products are bound by the L2 request rate of their
+        // random gathers; past about three resident workgroups per CU the
+        // extra ones mostly evict each other's lines of x. For a matrix with
+        // many more row blocks than the capped residency holds, the cap is
+        // kept when it beats the uncapped product by 1 % on this probe (it
+        // changes no result: the same blocks run in the same order).
+        it.pn->lds_pad = 0;
+        int launch_blocks = M.nblocks;
+        if (it.pn->P > 1) {
+            launch_blocks = 0;
+            for (int p = 0; p < it.pn->P; ++p) launch_blocks = std::max(launch_blocks, it.pn->part[p].nblocks);
+        }
+        if (pad3 > 0 && cap_forced != 0 && M.nnz > 0 && launch_blocks >= 6 * cus) {
+            float plain = 0.0f, capped = 0.0f;
+            HPRQPStatus st = cap_forced == 1 ? HPRQP_STATUS_OK : probe_ms(&plain);
+            if (st != HPRQP_STATUS_OK) return st;
+            it.pn->lds_pad = (unsigned)pad3;
+            if (cap_forced != 1) {
+                st = probe_ms(&capped);
+                if (st != HPRQP_STATUS_OK) return st;
+                if (!(capped < 0.99f * plain)) it.pn->lds_pad = 0;
+            }
         }
     }
     (void)hipEventDestroy(e0);
@@ -1861,6 +1908,9 @@ double HipBackend::scalar(const char *name) const
     if (!std::strcmp(name, "panels_Q")) return Qp_.P;
     if (!std::strcmp(name, "panels_A")) return Ap_.P;
     if (!std::strcmp(name, "panels_AT")) return ATp_.P;
+    if (!std::strcmp(name, "lds_pad_Q")) return Qp_.lds_pad;
+    if (!std::strcmp(name, "lds_pad_A")) return Ap_.lds_pad;
+    if (!std::strcmp(name, "lds_pad_AT")) return ATp_.lds_pad;
     return NAN;
 }
 

@@ -77,3 +77,26 @@ def test_fullsize_panel_products_bitwise(setup):
             del os.environ["HPRQP_PANELS"]
     for a, b in zip(outs["1"], outs["4"]):
         assert close(a, b, 1e-12)
+
+
+def test_fullsize_occupancy_cap_changes_no_bit(setup):
+    """The residency cap (unused dynamic LDS per workgroup) runs the same
+    blocks in the same order: iterates with it forced on and forced off are
+    bitwise equal, at two column panels (the value the autotuner picks
+    here) and without panels."""
+    H, qp, model = setup
+    for P in ("2", "1"):
+        outs = {}
+        for cap in ("1", "0"):
+            os.environ["HPRQP_PANELS"], os.environ["HPRQP_OCCUPANCY_CAP"] = P, cap
+            try:
+                s = H.Session(model, H.Parameters(verbose=False, max_iter=3))
+                pads = [s.scalar("lds_pad_" + m) for m in ("Q", "A", "AT")]
+                assert (min(pads) > 0) if cap == "1" else (max(pads) == 0), (P, cap, pads)
+                s.advance(4)
+                outs[cap] = [s.read(v) for v in ("x_bar", "y_bar", "w_bar", "x", "w", "y")]
+                del s
+            finally:
+                del os.environ["HPRQP_PANELS"], os.environ["HPRQP_OCCUPANCY_CAP"]
+        for a, b in zip(outs["1"], outs["0"]):
+            assert np.array_equal(a, b), P
