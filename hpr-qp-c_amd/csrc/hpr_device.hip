@@ -1179,13 +1179,16 @@ HPRQPStatus HipBackend::autotune_panels()
     // CU. HPRQP_OCCUPANCY_CAP=0 never applies it, =1 always where eligible.
     const char *cap_env = std::getenv("HPRQP_OCCUPANCY_CAP");
     const int cap_forced = cap_env ? std::atoi(cap_env) : -1;
-    int lds_cu = 0, lds_block = 0, cus = 0;
-    HPRQP_HIP(hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device_));
-    HPRQP_HIP(hipDeviceGetAttribute(&lds_block, hipDeviceAttributeMaxSharedMemoryPerBlock, device_));
-    HPRQP_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device_));
+    int lds_cu = 0, lds_block = 0, cus = 0;  // a failed query leaves the cap off
+    if (hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device_) != hipSuccess ||
+        hipDeviceGetAttribute(&lds_block, hipDeviceAttributeMaxSharedMemoryPerBlock, device_) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device_) != hipSuccess) {
+        lds_cu = lds_block = cus = 0;
+        (void)hipGetLastError();  // not an error of this solve: clear it
+    }
     constexpr int kStaticLds = 17 * 1024;  // above every spmv_rows_kernel's static LDS
     int pad3 = (lds_cu / 3 - kStaticLds) & ~255;
-    if (pad3 <= 0 || kStaticLds + pad3 > lds_block || lds_cu / (kStaticLds + pad3) != 3 ||
+    if (cus <= 0 || pad3 <= 0 || kStaticLds + pad3 > lds_block || lds_cu / (kStaticLds + pad3) != 3 ||
         lds_cu / (kStaticLds - 1024 + pad3) != 3)
         pad3 = 0;
     struct Item {
