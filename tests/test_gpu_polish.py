@@ -62,7 +62,7 @@ def test_accepted_candidate_is_the_result(H, model, tmp_path):
     assert events == ["LAUNCHED", "CPU_CERTIFICATE_ACCEPT"]
 
 
-@pytest.mark.parametrize("mode,iters", [("crash", 3000), ("sleep", 20_000)])
+@pytest.mark.parametrize("mode,iters", [("crash", 3000), ("sleep", 50_000)])  # the sleeper must start before the end
 def test_failed_or_slow_worker_leaves_the_trajectory(H, model, tmp_path, mode, iters):
     base = H.optimize(model, H.Parameters(verbose=False, polishing=0, stoptol=1e-300, max_iter=iters))
     r = H.optimize(model, _params(H, tmp_path, mode, stoptol=1e-300, max_iter=iters))
