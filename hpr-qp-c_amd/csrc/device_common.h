@@ -20,14 +20,56 @@ constexpr int kMaxTerms = 8;      // reduction terms one kernel may emit
 
 void report_hip_error(hipError_t err, const char *what, const char *file, int line);
 
+// Status of a failed HIP call: a device allocation that ran out of memory is
+// ALLOCATION_FAILED (as the reference reports a failed cudaMalloc,
+// src/gpu/context.cu:89,163); every other device error NOT_IMPLEMENTED
+// (src/gpu/cuda_common.h:52-55).
+inline HPRQPStatus hip_status(hipError_t err)
+{
+    if (err == hipSuccess) return HPRQP_STATUS_OK;
+    return err == hipErrorOutOfMemory ? HPRQP_STATUS_ALLOCATION_FAILED : HPRQP_STATUS_NOT_IMPLEMENTED;
+}
+
 #define HPRQP_HIP(call)                                                   \
     do {                                                                  \
         hipError_t hprqp_err_ = (call);                                   \
         if (hprqp_err_ != hipSuccess) {                                   \
             ::hprqp::report_hip_error(hprqp_err_, #call, __FILE__, __LINE__); \
-            return HPRQP_STATUS_NOT_IMPLEMENTED;                          \
+            return ::hprqp::hip_status(hprqp_err_);                       \
         }                                                                 \
     } while (0)
+
+// Device allocations and events owned by one scope: released on every return
+// path of the host function that made them.
+template <class T>
+struct DeviceBuffer {
+    T *p = nullptr;
+    DeviceBuffer() = default;
+    DeviceBuffer(const DeviceBuffer &) = delete;
+    DeviceBuffer &operator=(const DeviceBuffer &) = delete;
+    ~DeviceBuffer()
+    {
+        if (p) (void)hipFree(p);
+    }
+    hipError_t alloc(size_t count) { return hipMalloc((void **)&p, sizeof(T) * (count ? count : 1)); }
+};
+
+struct EventPair {
+    hipEvent_t a = nullptr, b = nullptr;
+    EventPair() = default;
+    EventPair(const EventPair &) = delete;
+    EventPair &operator=(const EventPair &) = delete;
+    ~EventPair()
+    {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+    hipError_t create()
+    {
+        hipError_t e = hipEventCreate(&a);
+        return e != hipSuccess ? e : hipEventCreate(&b);
+    }
+};
 
 // ---------------------------------------------------------------- reductions
 // Fixed-shape trees, so results are run-to-run deterministic.

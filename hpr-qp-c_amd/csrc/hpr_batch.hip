@@ -1118,7 +1118,8 @@ __global__ __launch_bounds__(kThreads) void batch_adjoint_kernel(Arrays ar, Para
                                                                  double *__restrict__ u_out,
                                                                  double *__restrict__ vrow_out,
                                                                  double *__restrict__ vbnd_out, double tol,
-                                                                 int max_steps, int *__restrict__ steps_out)
+                                                                 int max_steps, int *__restrict__ steps_out,
+                                                                 double *__restrict__ resid_out)
 {
     __shared__ double prod[kStreamCap];
     __shared__ double red[kMaxTerms];
@@ -1162,6 +1163,7 @@ __global__ __launch_bounds__(kThreads) void batch_adjoint_kernel(Arrays ar, Para
     // ring indices: i0 = newest (p0,s0), i1 = p1,s1, i2 = p2,s2
     int i0 = 0, i1 = 1, i2 = 2;
     int step = 0;
+    double rel = bnorm > 0.0 ? 1.0 : 0.0;  // ||r|| / ||b|| of the current iterate
     if (bnorm > 0.0) {
         for (step = 1; step <= max_steps; ++step) {
             // rotate: p2 <- p1, p1 <- p0; the next p0 is written into p2's old buffer
@@ -1175,6 +1177,7 @@ __global__ __launch_bounds__(kThreads) void batch_adjoint_kernel(Arrays ar, Para
             adj_axpy(g, alpha, pn[i1], pm[i1], xn, xm);
             adj_axpy(g, -alpha, sn[i1], sm[i1], rn, rm);
             const double rnorm = sqrt(adj_dot(g, rn, rm, rn, rm, sums, maxs));
+            rel = rnorm / bnorm;
             if (rnorm <= tol * bnorm) break;
             adj_copy(g, sn[i1], sm[i1], pn[i0], pm[i0]);
             adj_op(g, mk, sn[i1], sm[i1], sn[i0], sm[i0], sums, maxs);
@@ -1201,7 +1204,10 @@ __global__ __launch_bounds__(kThreads) void batch_adjoint_kernel(Arrays ar, Para
         vbnd_out[g.I->vn + i] = fr ? 0.0 : d[i] * (gk[i] / d[i] - tn[i]);
     }
     for (int i = t; i < m; i += kThreads) vrow_out[g.I->vm + i] = xm[i] / rr[i];
-    if (t == 0) steps_out[k] = step;
+    if (t == 0) {
+        steps_out[k] = step;
+        resid_out[k] = rel;
+    }
 }
 
 template <bool D>
@@ -1478,7 +1484,8 @@ struct HPRQPHipBatch {
     hipStream_t stream = nullptr;
     double *d_vn = nullptr, *d_vm = nullptr, *d_val = nullptr, *d_dval = nullptr;
     int *d_rp = nullptr, *d_ci = nullptr, *d_perm = nullptr, *d_blocks = nullptr, *d_active = nullptr;
-    int *d_steps = nullptr;  // adjoint step counts (device-buffer adjoint)
+    int *d_steps = nullptr;        // adjoint step counts of the last adjoint
+    double *d_adj_resid = nullptr;  // adjoint final relative residuals of the last adjoint
     double *d_gram = nullptr;  // G = A A^T of the dense instances (setup)
     mutable hipEvent_t caller_done = nullptr;  // last work queued on a caller's stream that reads this batch
     std::vector<int> h_rp;  // host copy of all CSR row pointers (block planning)
@@ -1499,7 +1506,7 @@ struct HPRQPHipBatch {
         if (d_order) (void)hipFree(d_order);
         for (void *p : {(void *)d_vn, (void *)d_vm, (void *)d_val, (void *)d_dval, (void *)d_rp, (void *)d_ci, (void *)d_perm,
                         (void *)d_blocks, (void *)d_active, (void *)d_inst, (void *)d_state, (void *)d_steps,
-                        (void *)d_gram})
+                        (void *)d_adj_resid, (void *)d_gram})
             if (p) (void)hipFree(p);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -2283,6 +2290,18 @@ HPRQPStatus hprqp_hip_batch_solution(const HPRQPHipBatch *b, double *x, double *
 
 namespace {
 
+// The batch-owned outputs of the adjoint besides the caller's vectors: step
+// counts and final relative residuals, one per instance.
+hipError_t ensure_adjoint_outputs(HPRQPHipBatch *b)
+{
+    if (!b->d_steps) {
+        const hipError_t e = hipMalloc((void **)&b->d_steps, sizeof(int) * b->count);
+        if (e != hipSuccess) return e;
+    }
+    if (!b->d_adj_resid) return hipMalloc((void **)&b->d_adj_resid, sizeof(double) * b->count);
+    return hipSuccess;
+}
+
 // Marks the end of work queued on a caller's stream that reads the batch, so
 // that destroying the batch waits for it.
 hipError_t mark_caller_use(const HPRQPHipBatch *b, hipStream_t s)
@@ -2327,41 +2346,35 @@ extern "C" {
 // (dL/dx in original units, instances concatenated) returns u (Sum n), the
 // row multipliers v_rows (Sum m) and the bound multipliers v_bounds (Sum n) of
 // the active-set KKT system K [u; v] = [g; 0]; steps[k] = MINRES steps used.
-// Host arrays in and out.
+// Host arrays in and out; the scratch copies are freed on every return.
 HPRQPStatus hprqp_hip_batch_adjoint(HPRQPHipBatch *b, const double *dl_dx, double tol, int max_steps, double *u,
                                     double *v_rows, double *v_bounds, int *steps)
 {
     using namespace hprqp::batch;
     if (!b || !dl_dx || !u || !v_rows || !v_bounds || max_steps <= 0) return HPRQP_STATUS_INVALID_ARGUMENT;
     const size_t sn = (size_t)b->ar.sum_n, sm = (size_t)b->ar.sum_m;
-    double *d_g = nullptr, *d_u = nullptr, *d_vr = nullptr, *d_vb = nullptr;
-    int *d_steps = nullptr;
-    HPRQP_HIP(hipMalloc((void **)&d_g, sizeof(double) * sn));
-    HPRQP_HIP(hipMalloc((void **)&d_u, sizeof(double) * sn));
-    HPRQP_HIP(hipMalloc((void **)&d_vb, sizeof(double) * sn));
-    HPRQP_HIP(hipMalloc((void **)&d_vr, sizeof(double) * sm));
-    HPRQP_HIP(hipMalloc((void **)&d_steps, sizeof(int) * b->count));
-    HPRQP_HIP(hipMemcpy(d_g, dl_dx, sizeof(double) * sn, hipMemcpyHostToDevice));
-    HPRQP_HIP(hipMemset(d_vr, 0, sizeof(double) * sm));
+    HPRQP_HIP(ensure_adjoint_outputs(b));
+    hprqp::DeviceBuffer<double> d_g, d_u, d_vr, d_vb;
+    HPRQP_HIP(d_g.alloc(sn));
+    HPRQP_HIP(d_u.alloc(sn));
+    HPRQP_HIP(d_vb.alloc(sn));
+    HPRQP_HIP(d_vr.alloc(sm));
+    HPRQP_HIP(hipMemcpy(d_g.p, dl_dx, sizeof(double) * sn, hipMemcpyHostToDevice));
+    HPRQP_HIP(hipMemsetAsync(d_vr.p, 0, sizeof(double) * std::max<size_t>(sm, 1), b->stream));
     if (b->any_sparse)
         batch_adjoint_kernel<false><<<(unsigned)b->count, hprqp::kThreads, 0, b->stream>>>(
-            b->ar, b->P, (int)b->count, d_g, d_u, d_vr, d_vb, tol, max_steps, d_steps);
+            b->ar, b->P, (int)b->count, d_g.p, d_u.p, d_vr.p, d_vb.p, tol, max_steps, b->d_steps, b->d_adj_resid);
     if (b->any_dense)
         batch_adjoint_kernel<true><<<(unsigned)b->count, hprqp::kThreads, 0, b->stream>>>(
-            b->ar, b->P, (int)b->count, d_g, d_u, d_vr, d_vb, tol, max_steps, d_steps);
+            b->ar, b->P, (int)b->count, d_g.p, d_u.p, d_vr.p, d_vb.p, tol, max_steps, b->d_steps, b->d_adj_resid);
     HPRQP_HIP(hipGetLastError());
     HPRQP_HIP(hipStreamSynchronize(b->stream));
-    HPRQP_HIP(hipMemcpy(u, d_u, sizeof(double) * sn, hipMemcpyDeviceToHost));
-    HPRQP_HIP(hipMemcpy(v_bounds, d_vb, sizeof(double) * sn, hipMemcpyDeviceToHost));
+    HPRQP_HIP(hipMemcpy(u, d_u.p, sizeof(double) * sn, hipMemcpyDeviceToHost));
+    HPRQP_HIP(hipMemcpy(v_bounds, d_vb.p, sizeof(double) * sn, hipMemcpyDeviceToHost));
     size_t real_m = 0;
     for (size_t k = 0; k < b->count; ++k) real_m += b->m[k];
-    HPRQP_HIP(hipMemcpy(v_rows, d_vr, sizeof(double) * real_m, hipMemcpyDeviceToHost));
-    if (steps) HPRQP_HIP(hipMemcpy(steps, d_steps, sizeof(int) * b->count, hipMemcpyDeviceToHost));
-    (void)hipFree(d_g);
-    (void)hipFree(d_u);
-    (void)hipFree(d_vr);
-    (void)hipFree(d_vb);
-    (void)hipFree(d_steps);
+    HPRQP_HIP(hipMemcpy(v_rows, d_vr.p, sizeof(double) * real_m, hipMemcpyDeviceToHost));
+    if (steps) HPRQP_HIP(hipMemcpy(steps, b->d_steps, sizeof(int) * b->count, hipMemcpyDeviceToHost));
     return HPRQP_STATUS_OK;
 }
 
@@ -2378,22 +2391,36 @@ HPRQPStatus hprqp_hip_batch_adjoint_device(HPRQPHipBatch *b, const double *dl_dx
     size_t real_m = 0;
     for (size_t k = 0; k < b->count; ++k) real_m += b->m[k];
     if (real_m && !v_rows) return HPRQP_STATUS_INVALID_ARGUMENT;
-    if (!b->d_steps) HPRQP_HIP(hipMalloc((void **)&b->d_steps, sizeof(int) * b->count));
+    HPRQP_HIP(ensure_adjoint_outputs(b));
     hipStream_t s = (hipStream_t)stream;
     if (b->any_sparse)
         batch_adjoint_kernel<false><<<(unsigned)b->count, hprqp::kThreads, 0, s>>>(b->ar, b->P, (int)b->count, dl_dx,
                                                                                  u, v_rows, v_bounds, tol, max_steps,
-                                                                                 b->d_steps);
+                                                                                 b->d_steps, b->d_adj_resid);
     if (b->any_dense)
         batch_adjoint_kernel<true><<<(unsigned)b->count, hprqp::kThreads, 0, s>>>(b->ar, b->P, (int)b->count, dl_dx,
                                                                                 u, v_rows, v_bounds, tol, max_steps,
-                                                                                b->d_steps);
+                                                                                b->d_steps, b->d_adj_resid);
     HPRQP_HIP(hipGetLastError());
     HPRQP_HIP(mark_caller_use(b, s));
     if (steps) {
         HPRQP_HIP(hipMemcpyAsync(steps, b->d_steps, sizeof(int) * b->count, hipMemcpyDeviceToHost, s));
         HPRQP_HIP(hipStreamSynchronize(s));
     }
+    return HPRQP_STATUS_OK;
+}
+
+// Per-instance final relative residual ||K [u; v] - [g; 0]|| / ||[g; 0]|| of
+// the batch's last adjoint (0 where g vanished on the free variables), read
+// on `stream` (the stream the device-buffer adjoint ran on; any stream after
+// the host-array adjoint) and waited for.
+HPRQPStatus hprqp_hip_batch_adjoint_residuals(const HPRQPHipBatch *b, double *residuals, void *stream)
+{
+    if (!b || !residuals) return HPRQP_STATUS_INVALID_ARGUMENT;
+    if (!b->d_adj_resid) return HPRQP_STATUS_INVALID_ARGUMENT;  // no adjoint has run
+    hipStream_t s = (hipStream_t)stream;
+    HPRQP_HIP(hipMemcpyAsync(residuals, b->d_adj_resid, sizeof(double) * b->count, hipMemcpyDeviceToHost, s));
+    HPRQP_HIP(hipStreamSynchronize(s));
     return HPRQP_STATUS_OK;
 }
 

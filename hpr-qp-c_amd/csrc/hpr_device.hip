@@ -81,36 +81,49 @@ struct DevCsr {
 // <= kStreamCap entries), runs of longer rows become wave blocks of up to
 // kThreads / kWave rows, and a row longer than kVecRow a block of its own.
 // Entry b is {first row, first entry} of block b; the last entry closes it.
-std::vector<int2> plan_row_blocks(const std::vector<int> &rp)
+// `rp(i)` returns row_ptr[i] and is called with nondecreasing i, except that
+// after rp(i + 1) the next call may be rp(i); `emit` receives the entries in
+// order.
+template <class RowPtr, class Emit>
+void plan_row_blocks_with(int nrow, RowPtr &&rp, Emit &&emit)
 {
-    const int nrow = (int)rp.size() - 1;
-    std::vector<int2> blk;
-    blk.reserve(nrow / 16 + 2);
     int r = 0;
     while (r < nrow) {
-        const int len = rp[r + 1] - rp[r];
-        blk.push_back(make_int2(r, rp[r]));
+        const int start = rp(r);
+        const int len = rp(r + 1) - start;
+        emit(make_int2(r, start));
         const int first = r;
         if (len > kVecRow) {
             ++r;
         } else if (len > kLongRow) {
             while (r < nrow && r - first < kThreads / kWave) {
-                const int l2 = rp[r + 1] - rp[r];
+                const int b = rp(r);
+                const int l2 = rp(r + 1) - b;
                 if (l2 <= kLongRow || l2 > kVecRow) break;
                 ++r;
             }
         } else {
             int nnz = 0;
             while (r < nrow && r - first < kThreads) {
-                const int l2 = rp[r + 1] - rp[r];
+                const int b = rp(r);
+                const int l2 = rp(r + 1) - b;
                 if (l2 > kLongRow || nnz + l2 > kStreamCap) break;
                 nnz += l2;
                 ++r;
             }
         }
     }
-    blk.push_back(make_int2(nrow, nrow >= 0 ? rp[nrow] : 0));
-    if (nrow == 0) blk = {make_int2(0, 0), make_int2(0, 0)};
+    emit(make_int2(nrow, nrow >= 0 ? rp(nrow) : 0));
+}
+
+std::vector<int2> plan_row_blocks(const std::vector<int> &rp)
+{
+    const int nrow = (int)rp.size() - 1;
+    if (nrow == 0) return {make_int2(0, 0), make_int2(0, 0)};
+    std::vector<int2> blk;
+    blk.reserve(nrow / 16 + 2);
+    plan_row_blocks_with(
+        nrow, [&](int i) { return rp[i]; }, [&](int2 e) { blk.push_back(e); });
     return blk;
 }
 
@@ -894,6 +907,7 @@ private:
     double *red_host_ = nullptr;  // pinned mirror
     std::vector<double> col_norm_h_, row_norm_h_;
     double sigma_ = 1.0, lambda_A_ = 0.0, lambda_Q_ = 0.0;
+    int power_steps_A_ = 0, power_steps_Q_ = 0;  // products taken by the power iterations
     int Q_is_diag_ = 0, noC_ = 0, to_check_ = 1;
     double norm_b_org_ = 0.0, norm_c_org_ = 0.0, obj_constant_ = 0.0;
     double b_scale_ = 1.0, c_scale_ = 1.0;
@@ -920,7 +934,7 @@ private:
     HPRQPStatus upload(const HPRQPModel *model);
     HPRQPStatus scale(const HPRQPParams *params);
     HPRQPStatus workspace_setup(const HPRQPModel *model);
-    HPRQPStatus power_iteration(bool q_matrix, double *lambda);
+    HPRQPStatus power_iteration(bool q_matrix, double *lambda, int *steps);
     HPRQPStatus soc_iteration_step(size_t inner);
     HPRQPStatus soc_residuals(HPRQPResiduals *res);
     size_t lin_n_ = 0, lin_m_ = 0;
@@ -1198,9 +1212,9 @@ HPRQPStatus HipBackend::autotune_panels()
     } items[3] = {{&Q_, &Qp_, vec(V_pn0), vec(V_pn1)},
                   {&A_, &Ap_, vec(V_pn0), vec(V_pm0)},
                   {&AT_, &ATp_, vec(V_pm1), vec(V_pn2)}};
-    hipEvent_t e0, e1;
-    HPRQP_HIP(hipEventCreate(&e0));
-    HPRQP_HIP(hipEventCreate(&e1));
+    EventPair ev;
+    HPRQP_HIP(ev.create());
+    hipEvent_t e0 = ev.a, e1 = ev.b;
     for (Item &it : items) {
         const DevCsr &M = *it.M;
         std::vector<int> cands = {1};
@@ -1275,8 +1289,6 @@ HPRQPStatus HipBackend::autotune_panels()
             }
         }
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     // the reduction partials must cover the largest panel launch
     size_t need = region_ / kMaxTerms;
     for (const Panels *pn : {&Qp_, &Ap_, &ATp_})
@@ -1292,9 +1304,10 @@ HPRQPStatus HipBackend::autotune_panels()
 }
 
 // eigs.c:19-153: power iteration on A A^T (q_matrix = false) or Q.
-HPRQPStatus HipBackend::power_iteration(bool q_matrix, double *lambda)
+HPRQPStatus HipBackend::power_iteration(bool q_matrix, double *lambda, int *steps)
 {
     const size_t len = q_matrix ? n_ : m_;
+    *steps = 0;
     if (len == 0) {
         *lambda = 0.0;
         return HPRQP_STATUS_OK;
@@ -1315,6 +1328,7 @@ HPRQPStatus HipBackend::power_iteration(bool q_matrix, double *lambda)
             break;
         }
         map(len, MapScaleDiv{{}, q, z, qn});
+        *steps = it + 1;
         const DevCsr &M = q_matrix ? Q_ : A_;
         if (!q_matrix) spmv(AT_, q, EpiStore{{}, tmp});
         spmv(M, q_matrix ? q : tmp, EpiPowerZ{nullptr, z, q}, 0);
@@ -1375,7 +1389,7 @@ HPRQPStatus HipBackend::workspace_setup(const HPRQPModel *md)
     Q_is_diag_ = n_ == 0 || red_host_[0] <= 1e-15;
     if (Q_is_diag_) map(n_, MapDiagFactors{{}, vec(V_fact1), vec(V_fact2), vec(V_fact), vec(V_fact_M), vec(V_diag_Q), sigma_});
     const bool cones = num_var_cones_ || num_row_cones_;
-    st = power_iteration(false, &lambda_A_);
+    st = power_iteration(false, &lambda_A_, &power_steps_A_);
     if (st != HPRQP_STATUS_OK) return st;
     lambda_A_ *= 1.05;
     if (Q_is_diag_) {
@@ -1388,7 +1402,7 @@ HPRQPStatus HipBackend::workspace_setup(const HPRQPModel *md)
             if (v > lambda_Q_) lambda_Q_ = v;
         }
     } else {
-        st = power_iteration(true, &lambda_Q_);
+        st = power_iteration(true, &lambda_Q_, &power_steps_Q_);
         if (st != HPRQP_STATUS_OK) return st;
         lambda_Q_ *= 1.05;
     }
@@ -1897,6 +1911,8 @@ double HipBackend::scalar(const char *name) const
     if (!std::strcmp(name, "sigma")) return sigma_;
     if (!std::strcmp(name, "lambda_max_A")) return lambda_A_;
     if (!std::strcmp(name, "lambda_max_Q")) return lambda_Q_;
+    if (!std::strcmp(name, "power_steps_A")) return power_steps_A_;
+    if (!std::strcmp(name, "power_steps_Q")) return power_steps_Q_;
     if (!std::strcmp(name, "Q_is_diag")) return Q_is_diag_;
     if (!std::strcmp(name, "noC")) return noC_;
     if (!std::strcmp(name, "norm_b_org")) return norm_b_org_;
@@ -2104,15 +2120,17 @@ HPRQPStatus hprqp_hip_csr_spmv(int nrow, const int *row_ptr, const int *col_ind,
     M.ci = const_cast<int *>(col_ind);
     M.val = const_cast<double *>(values);
     HPRQPStatus st = hprqp::upload_blocks(M, s);
+    hprqp::DeviceBuffer<int2> blocks;  // freed on every return below
+    blocks.p = M.blocks;
     if (st != HPRQP_STATUS_OK) return st;
     hprqp::EpiStore epi;
     epi.out = y;
     hprqp::spmv_rows_kernel<hprqp::EpiStore><<<M.nblocks, hprqp::kThreads, 0, s>>>(M.rp, M.ci, M.val, x, M.blocks, epi);
     HPRQP_HIP(hipGetLastError());
     HPRQP_HIP(hipStreamSynchronize(s));
-    HPRQP_HIP(hipFree(M.blocks));
     return HPRQP_STATUS_OK;
 }
+
 
 // A plain streaming copy of `bytes` (multiple of 16) between two fresh device
 // buffers: reads and writes exactly `bytes` once. Reference kernel for
@@ -2126,24 +2144,19 @@ __global__ void hprqp_calibration_copy_kernel(size_t n2, const double2 *__restri
 HPRQPStatus hprqp_hip_calibration_copy(size_t bytes, double *milliseconds)
 {
     if (bytes == 0 || bytes % 16 != 0) return HPRQP_STATUS_INVALID_ARGUMENT;
-    double2 *a = nullptr, *b = nullptr;
-    HPRQP_HIP(hipMalloc((void **)&a, bytes));
-    HPRQP_HIP(hipMalloc((void **)&b, bytes));
-    HPRQP_HIP(hipMemset(a, 0, bytes));
-    hipEvent_t e0, e1;
-    HPRQP_HIP(hipEventCreate(&e0));
-    HPRQP_HIP(hipEventCreate(&e1));
-    HPRQP_HIP(hipEventRecord(e0, nullptr));
-    hprqp_calibration_copy_kernel<<<2048, 256>>>(bytes / 16, a, b);
-    HPRQP_HIP(hipEventRecord(e1, nullptr));
-    HPRQP_HIP(hipEventSynchronize(e1));
+    hprqp::DeviceBuffer<double2> a, b;
+    HPRQP_HIP(a.alloc(bytes / 16));
+    HPRQP_HIP(b.alloc(bytes / 16));
+    HPRQP_HIP(hipMemset(a.p, 0, bytes));
+    hprqp::EventPair ev;
+    HPRQP_HIP(ev.create());
+    HPRQP_HIP(hipEventRecord(ev.a, nullptr));
+    hprqp_calibration_copy_kernel<<<2048, 256>>>(bytes / 16, a.p, b.p);
+    HPRQP_HIP(hipEventRecord(ev.b, nullptr));
+    HPRQP_HIP(hipEventSynchronize(ev.b));
     float ms = 0.0f;
-    HPRQP_HIP(hipEventElapsedTime(&ms, e0, e1));
+    HPRQP_HIP(hipEventElapsedTime(&ms, ev.a, ev.b));
     if (milliseconds) *milliseconds = ms;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    HPRQP_HIP(hipFree(a));
-    HPRQP_HIP(hipFree(b));
     return HPRQP_STATUS_OK;
 }
 
@@ -2193,32 +2206,25 @@ HPRQPStatus hprqp_hip_calibration_gather(int nrow, int window, int reps, double 
 {
     if (nrow <= 0 || window <= 0 || reps <= 0) return HPRQP_STATUS_INVALID_ARGUMENT;
     const size_t count = (size_t)nrow * kGatherWidth;
-    int *idx = nullptr;
-    double *v = nullptr, *x = nullptr, *y = nullptr;
-    HPRQP_HIP(hipMalloc((void **)&idx, count * sizeof(int)));
-    HPRQP_HIP(hipMalloc((void **)&v, count * sizeof(double)));
-    HPRQP_HIP(hipMalloc((void **)&x, (size_t)window * sizeof(double)));
-    HPRQP_HIP(hipMalloc((void **)&y, (size_t)nrow * sizeof(double)));
-    HPRQP_HIP(hipMemset(x, 0, (size_t)window * sizeof(double)));
-    hprqp_calibration_index_kernel<<<4096, 256>>>(count, (unsigned)window, idx, v);
+    hprqp::DeviceBuffer<int> idx;
+    hprqp::DeviceBuffer<double> v, x, y;
+    HPRQP_HIP(idx.alloc(count));
+    HPRQP_HIP(v.alloc(count));
+    HPRQP_HIP(x.alloc((size_t)window));
+    HPRQP_HIP(y.alloc((size_t)nrow));
+    HPRQP_HIP(hipMemset(x.p, 0, (size_t)window * sizeof(double)));
+    hprqp_calibration_index_kernel<<<4096, 256>>>(count, (unsigned)window, idx.p, v.p);
     const int grid = (nrow + 255) / 256;
-    hprqp_calibration_gather_kernel<<<grid, 256>>>(nrow, idx, v, x, y);  // warm-up
-    hipEvent_t e0, e1;
-    HPRQP_HIP(hipEventCreate(&e0));
-    HPRQP_HIP(hipEventCreate(&e1));
-    HPRQP_HIP(hipEventRecord(e0, nullptr));
-    for (int k = 0; k < reps; ++k) hprqp_calibration_gather_kernel<<<grid, 256>>>(nrow, idx, v, x, y);
-    HPRQP_HIP(hipEventRecord(e1, nullptr));
-    HPRQP_HIP(hipEventSynchronize(e1));
+    hprqp_calibration_gather_kernel<<<grid, 256>>>(nrow, idx.p, v.p, x.p, y.p);  // warm-up
+    hprqp::EventPair ev;
+    HPRQP_HIP(ev.create());
+    HPRQP_HIP(hipEventRecord(ev.a, nullptr));
+    for (int k = 0; k < reps; ++k) hprqp_calibration_gather_kernel<<<grid, 256>>>(nrow, idx.p, v.p, x.p, y.p);
+    HPRQP_HIP(hipEventRecord(ev.b, nullptr));
+    HPRQP_HIP(hipEventSynchronize(ev.b));
     float ms = 0.0f;
-    HPRQP_HIP(hipEventElapsedTime(&ms, e0, e1));
+    HPRQP_HIP(hipEventElapsedTime(&ms, ev.a, ev.b));
     if (milliseconds) *milliseconds = ms / reps;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    HPRQP_HIP(hipFree(idx));
-    HPRQP_HIP(hipFree(v));
-    HPRQP_HIP(hipFree(x));
-    HPRQP_HIP(hipFree(y));
     return HPRQP_STATUS_OK;
 }
 

@@ -612,6 +612,19 @@ def batch_adjoint_device(batch: Batch, dl_dx: int, u: int, v_rows: int, v_bounds
     return steps
 
 
+_LIB.hprqp_hip_batch_adjoint_residuals.argtypes = [C.c_void_p, _db_p, C.c_void_p]
+
+
+def batch_adjoint_residuals(batch: Batch, stream: int = 0) -> np.ndarray:
+    """Per-instance final relative residual of the batch's last adjoint
+    (hprqp_hip_batch_adjoint_residuals), read on `stream` and waited for."""
+    out = np.zeros(len(batch.ns))
+    st = _LIB.hprqp_hip_batch_adjoint_residuals(batch.h, out.ctypes.data_as(_db_p), C.c_void_p(stream))
+    if st != STATUS_OK:
+        raise RuntimeError(f"hprqp_hip_batch_adjoint_residuals failed: {status_name(st)}")
+    return out
+
+
 def batch_adjoint(batch: Batch, dl_dx: Sequence[np.ndarray], tol: float = 1e-12, max_steps: int = 20000):
     """Per instance (u, v_rows, v_bounds, steps) of the active-set KKT adjoint
     K [u; v] = [dl_dx; 0] (see include/hprqp_hip.h)."""

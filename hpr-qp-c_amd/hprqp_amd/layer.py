@@ -22,11 +22,12 @@ host arrays.
 from __future__ import annotations
 
 import time
+import warnings
 
 import numpy as np
 import torch
 
-from . import Batch, Parameters, batch_adjoint_arrays, batch_adjoint_device
+from . import Batch, Parameters, batch_adjoint_arrays, batch_adjoint_device, batch_adjoint_residuals
 
 
 def _np(t):
@@ -35,6 +36,27 @@ def _np(t):
 
 # Wall-clock split of the last forward / backward (seconds), for benchmarks.
 last_timings: dict = {}
+
+# The adjoint's MINRES stops at a relative residual of ADJOINT_TOL or after
+# ADJOINT_MAX_STEPS steps. A backward whose worst instance ends above
+# ADJOINT_TOL warns (the gradients are as accurate as that residual allows);
+# above ADJOINT_FAIL it raises, naming the instance.
+ADJOINT_TOL = 1e-12
+ADJOINT_MAX_STEPS = 20000
+ADJOINT_FAIL = 1e-6
+
+
+def _check_adjoint(residuals: np.ndarray) -> None:
+    worst = int(np.argmax(residuals)) if len(residuals) else 0
+    r = float(residuals[worst]) if len(residuals) else 0.0
+    last_timings["adjoint_max_residual"] = r
+    if r > ADJOINT_FAIL:
+        raise RuntimeError(f"QPLayer backward: the adjoint of instance {worst} did not converge (relative residual "
+                           f"{r:.3e} after at most {ADJOINT_MAX_STEPS} steps); its gradients are not reliable")
+    if r > ADJOINT_TOL:
+        bad = int(np.count_nonzero(residuals > ADJOINT_TOL))
+        warnings.warn(f"QPLayer backward: {bad} adjoint solve(s) stopped above the tolerance {ADJOINT_TOL:g} "
+                      f"(worst: instance {worst}, relative residual {r:.3e})", RuntimeWarning, stacklevel=3)
 
 
 class QPLayer(torch.autograd.Function):
@@ -83,8 +105,9 @@ class QPLayer(torch.autograd.Function):
         m = y.shape[1]
         g = _np(gx)
         t0 = time.perf_counter()
-        u, vr, vb, _ = batch_adjoint_arrays(ctx.batch, g.reshape(-1))
+        u, vr, vb, _ = batch_adjoint_arrays(ctx.batch, g.reshape(-1), ADJOINT_TOL, ADJOINT_MAX_STEPS)
         last_timings["adjoint"] = time.perf_counter() - t0
+        _check_adjoint(batch_adjoint_residuals(ctx.batch))
         # rank-one gradient terms as batched outer products (torch runs them
         # multi-threaded on the host, or on the device the inputs came from)
         t = lambda a: torch.as_tensor(a, dtype=torch.float64, device=ctx.device)
@@ -164,9 +187,13 @@ def _backward_device(ctx, gx):
     VB = torch.empty_like(U)
     VR = torch.empty((B, m), dtype=torch.float64, device=x.device)
     stream = torch.cuda.current_stream(x.device).cuda_stream
-    batch_adjoint_device(ctx.batch, g.data_ptr(), U.data_ptr(), VR.data_ptr(), VB.data_ptr(), stream)
-    last_timings["adjoint"] = time.perf_counter() - t0
+    batch_adjoint_device(ctx.batch, g.data_ptr(), U.data_ptr(), VR.data_ptr(), VB.data_ptr(), stream,
+                         ADJOINT_TOL, ADJOINT_MAX_STEPS)
+    # the gradient products are queued behind the adjoint before its
+    # convergence is read back (one wait for `count` residuals)
     grads = _gradients(U, VR, VB, x, y, z, eq, feq)
+    last_timings["adjoint"] = time.perf_counter() - t0
+    _check_adjoint(batch_adjoint_residuals(ctx.batch, stream))
     last_timings["backward_host"] = time.perf_counter() - t0
     return grads + (None, None)
 

@@ -34,7 +34,8 @@ HPRQPStatus hprqp_hip_session_advance(HPRQPHipSession *session, size_t bodies, i
  * Qw, x_hat, last_x, last_w, last_y, last_ATy, dx, dy, dw, c, l, u, AL, AU,
  * row_norm, col_norm) to host; returns its length (0: unknown name). */
 size_t hprqp_hip_session_read(const HPRQPHipSession *session, const char *name, double *out, size_t capacity);
-/* Named scalars (sigma, lambda_max_A, lambda_max_Q, Q_is_diag, noC, iter,
+/* Named scalars (sigma, lambda_max_A, lambda_max_Q, power_steps_A and
+ * power_steps_Q (products taken by the two power iterations), Q_is_diag, noC, iter,
  * inner, restart_times, kkt, err_rp, err_rd, gap, pobj, dobj, norm_b_org,
  * norm_c_org); NaN if unknown. */
 double hprqp_hip_session_scalar(const HPRQPHipSession *session, const char *name);
@@ -55,8 +56,9 @@ int hprqp_hip_session_profile(HPRQPHipSession *session, size_t iterations, int c
 
 /* Kernel-level entry points on caller-provided device buffers (int32 CSR,
  * FP64 values, device pointers). stream may be NULL for the default stream.
- * y = M x with the row-blocked kernel; row blocks are computed internally and
- * cached per matrix pointer triple. */
+ * y = M x with the row-blocked kernel; the row blocks are planned from the
+ * row pointer on each call (device memory allocated and freed by the call),
+ * which waits for the product before it returns. */
 HPRQPStatus hprqp_hip_csr_spmv(int nrow, const int *row_ptr, const int *col_ind, const double *values,
                                const double *x, double *y, void *stream);
 /* Device transpose with the stable (column, row) ordering of the host
@@ -138,6 +140,13 @@ HPRQPStatus hprqp_hip_batch_create_dense_device(size_t count, size_t n, size_t m
 HPRQPStatus hprqp_hip_batch_solution_device(const HPRQPHipBatch *batch, double *x, double *y, double *z, void *stream);
 HPRQPStatus hprqp_hip_batch_adjoint_device(HPRQPHipBatch *batch, const double *dl_dx, double tol, int max_steps,
                                            double *u, double *v_rows, double *v_bounds, int *steps, void *stream);
+/* Convergence of the batch's last adjoint (either form): per instance the
+ * final relative residual ||K [u; v] - [g; 0]|| / ||[g; 0]|| of the MINRES
+ * iterate (0 when g vanishes on the free variables); a value above the tol
+ * passed to the adjoint means it stopped at max_steps or broke down. Read on
+ * `stream` (the stream the device-buffer adjoint ran on) and waited for;
+ * INVALID_ARGUMENT before any adjoint. No counterpart in the reference. */
+HPRQPStatus hprqp_hip_batch_adjoint_residuals(const HPRQPHipBatch *batch, double *residuals, void *stream);
 
 /* Copies `bytes` (a multiple of 16) between two fresh device buffers with a
  * plain streaming kernel and reports its time: the known-size kernel used to

@@ -554,6 +554,7 @@ static int scale_model(Model *md, int use_scaling, Scaling *sc)
 typedef struct Work {
     size_t n, m;
     double sigma, lambda_max_A, lambda_max_Q;
+    size_t power_steps_A, power_steps_Q; /* power-iteration products taken (0: not run) */
     int Q_is_diag, noC, to_check, soc;
     double *x, *x_bar, *x_hat, *w, *w_bar, *y, *y_bar, *y_hat;
     double *Ax, *ATy, *ATy_bar, *ATdy, *Qw, *Qw_bar, *Qx, *dQw, *QATdy;
@@ -616,12 +617,14 @@ static void diag_factors(Work *w)
     }
 }
 
-/* eigs.c:11-153: power iteration on A A^T (which=0) or Q (which=1). */
-static double power_iteration(const Model *md, int which)
+/* eigs.c:11-153: power iteration on A A^T (which=0) or Q (which=1); *steps
+ * receives the number of products taken. */
+static double power_iteration(const Model *md, int which, size_t *steps)
 {
     size_t len = which ? md->n : md->m;
     double *z, *q, *tmp = NULL;
     double lam = 1.0;
+    *steps = 0;
     if (len == 0) return 0.0;
     z = (double *)malloc(len * sizeof(double));
     q = (double *)malloc(len * sizeof(double));
@@ -640,6 +643,7 @@ static double power_iteration(const Model *md, int which)
             spmv(&md->AT, q, tmp);
             spmv(&md->A, tmp, z);
         }
+        *steps = it + 1;
         lam = dot(q, z, len);
         zn = nrm2(z, len);
         for (size_t i = 0; i < len; ++i) q[i] = z[i] - lam * q[i];
@@ -674,7 +678,7 @@ static int work_setup(Work *w, const Model *md, const OrParams *p)
     w->Q_is_diag = q_diag_check(&md->Q, w->diag_Q);
     if (w->Q_is_diag) diag_factors(w);
     /* eigs.c:155-185 */
-    w->lambda_max_A = p->lambda_A_given > 0.0 ? p->lambda_A_given : power_iteration(md, 0) * 1.05;
+    w->lambda_max_A = p->lambda_A_given > 0.0 ? p->lambda_A_given : power_iteration(md, 0, &w->power_steps_A) * 1.05;
     if (w->Q_is_diag) {
         w->lambda_max_Q = 0.0;
         for (size_t i = 0; i < md->n; ++i) {
@@ -682,7 +686,7 @@ static int work_setup(Work *w, const Model *md, const OrParams *p)
             if (d > w->lambda_max_Q) w->lambda_max_Q = d;
         }
     } else {
-        w->lambda_max_Q = p->lambda_Q_given > 0.0 ? p->lambda_Q_given : power_iteration(md, 1) * 1.05;
+        w->lambda_max_Q = p->lambda_Q_given > 0.0 ? p->lambda_Q_given : power_iteration(md, 1, &w->power_steps_Q) * 1.05;
     }
     if (w->soc) {
         w->noC = 0;
@@ -1505,6 +1509,8 @@ double or_session_scalar(const OrSession *s, const char *name)
     if (!strcmp(name, "sigma")) return s->w.sigma;
     if (!strcmp(name, "lambda_max_A")) return s->w.lambda_max_A;
     if (!strcmp(name, "lambda_max_Q")) return s->w.lambda_max_Q;
+    if (!strcmp(name, "power_steps_A")) return (double)s->w.power_steps_A;
+    if (!strcmp(name, "power_steps_Q")) return (double)s->w.power_steps_Q;
     if (!strcmp(name, "Q_is_diag")) return s->w.Q_is_diag;
     if (!strcmp(name, "noC")) return s->w.noC;
     if (!strcmp(name, "iter")) return (double)s->iter;
