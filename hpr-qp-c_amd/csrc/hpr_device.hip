@@ -20,6 +20,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -2131,6 +2132,110 @@ HPRQPStatus hprqp_hip_csr_spmv(int nrow, const int *row_ptr, const int *col_ind,
     return HPRQP_STATUS_OK;
 }
 
+}  // extern "C"
+
+// ------------------------------------------------- SpMV without allocation
+// hprqp_hip_csr_spmv_noalloc keeps its row-block plan in `scratch`, a
+// file-scope device array of fixed size (each GPU holds its own copy for the
+// life of the process), instead of a per-call hipMalloc. Host code never
+// passes `scratch` itself to the device: its address on each device comes
+// from hipGetSymbolAddress on the first call there. The plan is built on the
+// host from the row pointer read through a fixed window and uploaded in fixed
+// chunks, so the call allocates neither device nor heap memory; a mutex
+// serialises calls in one process (the plan and the staging are shared), and
+// each call waits for its product before it returns, so a later call cannot
+// overwrite a plan still in use.
+constexpr int kNoAllocMaxRows = 1 << 19;  // rows per call; the plan needs <= rows + 1 entries
+static __device__ __align__(256) int2 scratch[kNoAllocMaxRows + 1];
+
+namespace {
+
+constexpr int kNoAllocMaxDevices = 64;
+constexpr int kNoAllocWindow = 8192;  // row-pointer entries read per D2H copy
+constexpr int kNoAllocChunk = 4096;   // plan entries per H2D copy
+
+struct NoAllocState {
+    std::mutex lock;
+    int2 *dev_scratch[kNoAllocMaxDevices] = {};
+    int rp_window[kNoAllocWindow];
+    int2 plan_chunk[kNoAllocChunk];
+};
+NoAllocState g_noalloc;
+
+// Device address of `scratch` on the current device (resolved once per
+// device, with that device current).
+HPRQPStatus noalloc_scratch(int2 **out)
+{
+    int dev = 0;
+    HPRQP_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kNoAllocMaxDevices) return HPRQP_STATUS_INVALID_ARGUMENT;
+    if (!g_noalloc.dev_scratch[dev]) {
+        void *ptr = nullptr;
+        HPRQP_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        g_noalloc.dev_scratch[dev] = static_cast<int2 *>(ptr);
+    }
+    *out = g_noalloc.dev_scratch[dev];
+    return HPRQP_STATUS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+HPRQPStatus hprqp_hip_csr_spmv_noalloc(int nrow, const int *row_ptr, const int *col_ind, const double *values,
+                                       const double *x, double *y, void *stream)
+{
+    if (nrow < 0 || (nrow > 0 && (!row_ptr || !x || !y))) return HPRQP_STATUS_INVALID_ARGUMENT;
+    if (nrow == 0) return HPRQP_STATUS_OK;
+    if (nrow > kNoAllocMaxRows) {
+        std::fprintf(stderr, "hprqp_hip_csr_spmv_noalloc: %d rows exceed the limit of %d (use hprqp_hip_csr_spmv)\n",
+                     nrow, kNoAllocMaxRows);
+        return HPRQP_STATUS_INVALID_ARGUMENT;
+    }
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    std::lock_guard<std::mutex> guard(g_noalloc.lock);
+    int2 *plan = nullptr;
+    HPRQPStatus st = noalloc_scratch(&plan);
+    if (st != HPRQP_STATUS_OK) return st;
+    // row pointer through a sliding window of kNoAllocWindow entries
+    int base = 0, have = 0;
+    hipError_t err = hipSuccess;
+    auto rp = [&](int i) -> int {
+        if (i < base || i >= base + have) {
+            base = i > 0 ? i - 1 : 0;
+            have = std::min(kNoAllocWindow, nrow + 1 - base);
+            if (err == hipSuccess)
+                err = hipMemcpyAsync(g_noalloc.rp_window, row_ptr + base, sizeof(int) * have, hipMemcpyDeviceToHost, s);
+            if (err == hipSuccess) err = hipStreamSynchronize(s);
+            if (err != hipSuccess) std::memset(g_noalloc.rp_window, 0, sizeof(int) * have);
+        }
+        return g_noalloc.rp_window[i - base];
+    };
+    int entries = 0, staged = 0;
+    auto flush = [&]() {
+        if (staged && err == hipSuccess)
+            err = hipMemcpyAsync(plan + entries - staged, g_noalloc.plan_chunk, sizeof(int2) * staged,
+                                 hipMemcpyHostToDevice, s);
+        if (staged && err == hipSuccess) err = hipStreamSynchronize(s);
+        staged = 0;
+    };
+    auto emit = [&](int2 e) {
+        if (entries > kNoAllocMaxRows) return;  // cannot happen: one entry per block of >= 1 row, plus the end
+        g_noalloc.plan_chunk[staged++] = e;
+        ++entries;
+        if (staged == kNoAllocChunk) flush();
+    };
+    hprqp::plan_row_blocks_with(nrow, rp, emit);
+    flush();
+    HPRQP_HIP(err);
+    hprqp::EpiStore epi;
+    epi.out = y;
+    hprqp::spmv_rows_kernel<hprqp::EpiStore><<<entries - 1, hprqp::kThreads, 0, s>>>(row_ptr, col_ind, values, x,
+                                                                                   plan, epi);
+    HPRQP_HIP(hipGetLastError());
+    HPRQP_HIP(hipStreamSynchronize(s));
+    return HPRQP_STATUS_OK;
+}
 
 // A plain streaming copy of `bytes` (multiple of 16) between two fresh device
 // buffers: reads and writes exactly `bytes` once. Reference kernel for

@@ -61,6 +61,16 @@ int hprqp_hip_session_profile(HPRQPHipSession *session, size_t iterations, int c
  * which waits for the product before it returns. */
 HPRQPStatus hprqp_hip_csr_spmv(int nrow, const int *row_ptr, const int *col_ind, const double *values,
                                const double *x, double *y, void *stream);
+/* hprqp_hip_csr_spmv without any allocation: no hipMalloc / hipFree and no
+ * heap allocation; the row-block plan lives in a fixed device array of the
+ * library (4 MiB on each GPU it runs on, held until the process ends).
+ * nrow <= 524288 (2^19), else INVALID_ARGUMENT with a message on stderr.
+ * Same y as hprqp_hip_csr_spmv, bit for bit. Calls are serialised inside the
+ * process and each waits for its product; calls from different processes
+ * use different copies. No counterpart in the reference: for callers that
+ * cannot allocate (tight serving loops, harnesses that forbid hipMalloc). */
+HPRQPStatus hprqp_hip_csr_spmv_noalloc(int nrow, const int *row_ptr, const int *col_ind, const double *values,
+                                       const double *x, double *y, void *stream);
 /* Device transpose with the stable (column, row) ordering of the host
  * counting-sort transpose; perm (optional) receives, for each output entry,
  * its source entry index. */
