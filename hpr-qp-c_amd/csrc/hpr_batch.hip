@@ -1103,15 +1103,6 @@ __device__ void adj_axpy(WG &g, double a, const double *xn, const double *xm, do
     __syncthreads();
 }
 
-template <class WG>
-__device__ void adj_copy(WG &g, const double *xn, const double *xm, double *yn, double *ym)
-{
-    const int n = g.I->n, m = g.I->m;
-    for (int i = threadIdx.x; i < n; i += kThreads) yn[i] = xn[i];
-    for (int i = threadIdx.x; i < m; i += kThreads) ym[i] = xm[i];
-    __syncthreads();
-}
-
 template <bool D>
 __global__ __launch_bounds__(kThreads) void batch_adjoint_kernel(Arrays ar, Params P, int count,
                                                                  const double *__restrict__ gvec,
@@ -1137,62 +1128,99 @@ __global__ __launch_bounds__(kThreads) void batch_adjoint_kernel(Arrays ar, Para
     const double *d = g.N(N_col), *rr = g.M(M_row);
     const double *gk = gvec + g.I->vn;
     double *bn = g.N(N_tc);
+    // MINRES (Paige and Saunders, SIAM J. Numer. Anal. 12 (1975) 617-629):
+    // Lanczos on the symmetric indefinite active-set operator with Givens
+    // rotations of the tridiagonal factor, so each step takes one operator
+    // product, the Lanczos vectors stay normalised and the iterate minimises
+    // the residual over the Krylov space. Vectors (free part, row part):
+    // x; v (normalised Lanczos vector); r1, r2, y (Lanczos three-term
+    // recurrence, rotated); w, w1, w2 (update directions, rotated).
     double *xn = g.N(N_pq), *xm = g.M(M_pq);
-    double *rn = g.N(N_pz), *rm = g.M(M_pz);
-    double *pn[3] = {g.N(N_dx), g.N(N_dw), g.N(N_tempv)};
-    double *pm[3] = {g.M(M_tr), g.M(M_dy), g.M(M_e0)};
-    double *sn[3] = {g.N(N_dQw), g.N(N_QATdy), g.N(N_Qx)};
-    double *sm[3] = {g.M(M_e1), g.M(M_e2), g.M(M_e3)};
+    double *vn = g.N(N_pz), *vm = g.M(M_pz);
+    double *ln[3] = {g.N(N_dx), g.N(N_dw), g.N(N_tempv)};
+    double *lm[3] = {g.M(M_tr), g.M(M_dy), g.M(M_e0)};
+    double *wn[3] = {g.N(N_dQw), g.N(N_QATdy), g.N(N_Qx)};
+    double *wm[3] = {g.M(M_e1), g.M(M_e2), g.M(M_e3)};
+    int ir1 = 0, ir2 = 1, iy = 2;  // r1, r2, y in ln / lm
+    int iw = 0, iw1 = 1, iw2 = 2;  // w, w1, w2 in wn / wm
     for (int i = t; i < n; i += kThreads) {
         const double b = mk.free_var(i) ? gk[i] / d[i] : 0.0;
         bn[i] = b;
         xn[i] = 0.0;
-        rn[i] = b;
-        pn[0][i] = b;
+        ln[ir1][i] = b;
+        ln[ir2][i] = b;
+        wn[iw][i] = 0.0;
+        wn[iw2][i] = 0.0;
     }
     for (int i = t; i < m; i += kThreads) {
         xm[i] = 0.0;
-        rm[i] = 0.0;
-        pm[0][i] = 0.0;
+        lm[ir1][i] = 0.0;
+        lm[ir2][i] = 0.0;
+        wm[iw][i] = 0.0;
+        wm[iw2][i] = 0.0;
     }
     __syncthreads();
-    const double bnorm = sqrt(adj_dot(g, bn, rm, bn, rm, sums, maxs));
-    adj_op(g, mk, pn[0], pm[0], sn[0], sm[0], sums, maxs);
-    adj_copy(g, pn[0], pm[0], pn[1], pm[1]);
-    adj_copy(g, sn[0], sm[0], sn[1], sm[1]);
-    // ring indices: i0 = newest (p0,s0), i1 = p1,s1, i2 = p2,s2
-    int i0 = 0, i1 = 1, i2 = 2;
+    const double beta1 = sqrt(adj_dot(g, bn, lm[ir1], bn, lm[ir1], sums, maxs));
+    double oldb = 0.0, beta = beta1, dbar = 0.0, epsln = 0.0, phibar = beta1, cs = -1.0, sn = 0.0;
     int step = 0;
-    double rel = bnorm > 0.0 ? 1.0 : 0.0;  // ||r|| / ||b|| of the current iterate
-    if (bnorm > 0.0) {
+    if (beta1 > 0.0) {
         for (step = 1; step <= max_steps; ++step) {
-            // rotate: p2 <- p1, p1 <- p0; the next p0 is written into p2's old buffer
-            const int o2 = i2;
-            i2 = i1;
-            i1 = i0;
-            i0 = o2;
-            const double ss = adj_dot(g, sn[i1], sm[i1], sn[i1], sm[i1], sums, maxs);
-            if (!(ss > 0.0)) break;
-            const double alpha = adj_dot(g, rn, rm, sn[i1], sm[i1], sums, maxs) / ss;
-            adj_axpy(g, alpha, pn[i1], pm[i1], xn, xm);
-            adj_axpy(g, -alpha, sn[i1], sm[i1], rn, rm);
-            const double rnorm = sqrt(adj_dot(g, rn, rm, rn, rm, sums, maxs));
-            rel = rnorm / bnorm;
-            if (rnorm <= tol * bnorm) break;
-            adj_copy(g, sn[i1], sm[i1], pn[i0], pm[i0]);
-            adj_op(g, mk, sn[i1], sm[i1], sn[i0], sm[i0], sums, maxs);
-            const double beta1 = adj_dot(g, sn[i0], sm[i0], sn[i1], sm[i1], sums, maxs) / ss;
-            adj_axpy(g, -beta1, pn[i1], pm[i1], pn[i0], pm[i0]);
-            adj_axpy(g, -beta1, sn[i1], sm[i1], sn[i0], sm[i0]);
-            if (step > 1) {
-                const double s22 = adj_dot(g, sn[i2], sm[i2], sn[i2], sm[i2], sums, maxs);
-                if (s22 > 0.0) {
-                    const double beta2 = adj_dot(g, sn[i0], sm[i0], sn[i2], sm[i2], sums, maxs) / s22;
-                    adj_axpy(g, -beta2, pn[i2], pm[i2], pn[i0], pm[i0]);
-                    adj_axpy(g, -beta2, sn[i2], sm[i2], sn[i0], sm[i0]);
-                }
+            const double sc = 1.0 / beta;
+            for (int i = t; i < n; i += kThreads) vn[i] = sc * ln[ir2][i];
+            for (int i = t; i < m; i += kThreads) vm[i] = sc * lm[ir2][i];
+            __syncthreads();
+            adj_op(g, mk, vn, vm, ln[iy], lm[iy], sums, maxs);
+            if (step >= 2) adj_axpy(g, -beta / oldb, ln[ir1], lm[ir1], ln[iy], lm[iy]);
+            const double alfa = adj_dot(g, vn, vm, ln[iy], lm[iy], sums, maxs);
+            adj_axpy(g, -alfa / beta, ln[ir2], lm[ir2], ln[iy], lm[iy]);
+            {  // r1 <- r2, r2 <- y; the next y reuses r1's buffer
+                const int o = ir1;
+                ir1 = ir2;
+                ir2 = iy;
+                iy = o;
             }
+            oldb = beta;
+            beta = sqrt(adj_dot(g, ln[ir2], lm[ir2], ln[ir2], lm[ir2], sums, maxs));
+            const double oldeps = epsln;
+            const double delta = cs * dbar + sn * alfa;
+            const double gbar = sn * dbar - cs * alfa;
+            epsln = sn * beta;
+            dbar = -cs * beta;
+            const double gamma = fmax(sqrt(gbar * gbar + beta * beta), 1e-300);
+            cs = gbar / gamma;
+            sn = beta / gamma;
+            const double phi = cs * phibar;
+            phibar = sn * phibar;
+            {  // w1 <- w2, w2 <- w; the new w goes into w1's old buffer
+                const int o = iw1;
+                iw1 = iw2;
+                iw2 = iw;
+                iw = o;
+            }
+            const double rg = 1.0 / gamma;
+            for (int i = t; i < n; i += kThreads) {
+                const double w = (vn[i] - oldeps * wn[iw1][i] - delta * wn[iw2][i]) * rg;
+                wn[iw][i] = w;
+                xn[i] += phi * w;
+            }
+            for (int i = t; i < m; i += kThreads) {
+                const double w = (vm[i] - oldeps * wm[iw1][i] - delta * wm[iw2][i]) * rg;
+                wm[iw][i] = w;
+                xm[i] += phi * w;
+            }
+            __syncthreads();
+            if (phibar <= tol * beta1 || !(beta > 0.0)) break;  // converged, or the Krylov space is exhausted
         }
+        step = min(step, max_steps);
+    }
+    // the true relative residual ||b - K x|| / ||b|| of the returned iterate
+    double rel = 0.0;
+    if (beta1 > 0.0) {
+        adj_op(g, mk, xn, xm, ln[iy], lm[iy], sums, maxs);
+        for (int i = t; i < n; i += kThreads) ln[iy][i] = bn[i] - ln[iy][i];
+        for (int i = t; i < m; i += kThreads) lm[iy][i] = -lm[iy][i];
+        __syncthreads();
+        rel = sqrt(adj_dot(g, ln[iy], lm[iy], ln[iy], lm[iy], sums, maxs)) / beta1;
     }
     // bound multipliers: v~_B = b~_B - (Q~ u~ + A~' v~)_B with b~ = g/d
     double *tn = g.N(N_Qw_bar), *tmp = g.N(N_pt);

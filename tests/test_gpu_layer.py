@@ -239,3 +239,91 @@ def test_device_outputs_complete_when_batch_destroyed(mods):
     torch.cuda.synchronize()
     np.testing.assert_array_equal(x.cpu().numpy(), want["x"])
     np.testing.assert_array_equal(U.cpu().numpy(), u_h)
+
+
+def _boxed_instances(count, n, m_eq, m_in, seed):
+    """Layer instances with finite variable bounds around the generating
+    point (x0 -/+ U(0, 0.3)), so that the optimum has variables at their
+    lower bound, at their upper bound and strictly inside, plus one fixed
+    variable (l = u) per instance."""
+    Qd, c, A, AL, AU, l, u = Q.layer_instances(count, n=n, m_eq=m_eq, m_in=m_in, seed=seed)
+    rng = np.random.default_rng(seed + 100)
+    c = c * 4.0  # push the optimum onto the bounds
+    l, u = np.empty((count, n)), np.empty((count, n))
+    for k in range(count):
+        x0 = rng.uniform(-1.0, 1.0, size=n)
+        ax = A[k] @ x0
+        AL[k, :m_eq] = AU[k, :m_eq] = ax[:m_eq]
+        AL[k, m_eq:] = ax[m_eq:] - rng.uniform(0.05, 0.5, size=m_in)
+        AU[k, m_eq:] = ax[m_eq:] + rng.uniform(0.05, 0.5, size=m_in)
+        l[k] = x0 - rng.uniform(0.02, 0.3, size=n)
+        u[k] = x0 + rng.uniform(0.02, 0.3, size=n)
+        l[k, 0] = u[k, 0] = x0[0]
+    return Qd, c, A, AL, AU, l, u
+
+
+def test_layer_bound_and_row_gradients_match_fd(mods):
+    """dL/dl, dL/du and dL/dAL, dL/dAU through QPLayer against central
+    differences (h = 1e-6) of the active-set KKT solution map, 1e-5
+    relative: for a variable at its lower bound only l moves x, at its upper
+    bound only u, strictly inside neither (both gradients exactly 0); a
+    fixed variable (l = u) moves with l and u together; likewise for rows at
+    their lower / upper bound, inactive rows and equality rows."""
+    H, torch, QPLayer = mods
+    count, n, meq, min_ = 4, 40, 6, 24
+    arrs = _boxed_instances(count, n, meq, min_, seed=17)
+    names = ["Q", "c", "A", "AL", "AU", "l", "u"]
+    T = lambda a: torch.tensor(a, dtype=torch.float64, requires_grad=True)
+    ts = [T(a) for a in arrs]
+    x = QPLayer.apply(*ts, 1e-10)
+    g = np.random.default_rng(4).normal(size=(count, n))
+    (x * torch.tensor(g)).sum().backward()
+    grads = {nm: t.grad.numpy() for nm, t in zip(names, ts)}
+    b = H.Batch.from_dense(*arrs, H.Parameters(verbose=False, stoptol=1e-10))
+    b.solve()
+    sol = b.solution()
+    ys, zs = sol["y"].reshape(count, meq + min_), sol["z"].reshape(count, n)
+    np.testing.assert_array_equal(sol["x"].reshape(count, n), x.detach().numpy())
+    h = 1e-6
+    seen = {"l": 0, "u": 0, "free": 0, "AL": 0, "AU": 0, "inactive": 0}
+    for k in range(count):
+        base = {nm: a[k].copy() for nm, a in zip(names, arrs)}
+        y, z = ys[k], zs[k]
+
+        def fd(moves):
+            p = {nm: v.copy() for nm, v in base.items()}
+            m_ = {nm: v.copy() for nm, v in base.items()}
+            for nm, idx in moves:
+                p[nm][idx] += h
+                m_[nm][idx] -= h
+            xp = Q.kkt_active_solution(*(p[nm] for nm in names), y, z)[0]
+            xm = Q.kkt_active_solution(*(m_[nm] for nm in names), y, z)[0]
+            return float(g[k] @ (xp - xm) / (2 * h))
+
+        def check(got, want, nm):
+            scale = max(abs(want), 1e-2 * float(np.max(np.abs(grads[nm][k]))), 1e-12)
+            assert abs(got - want) <= 1e-5 * scale, (k, nm, got, want)
+        # the fixed variable 0: l and u moved together
+        check(grads["l"][k][0] + grads["u"][k][0], fd([("l", 0), ("u", 0)]), "l")
+        for j in range(1, n):
+            kind = "l" if z[j] > 0 else ("u" if z[j] < 0 else "free")
+            seen[kind] += 1
+            for nm in ("l", "u"):
+                want = fd([(nm, j)])
+                if kind == nm:
+                    check(grads[nm][k][j], want, nm)
+                else:
+                    assert want == 0.0 and grads[nm][k][j] == 0.0, (k, j, kind, nm)
+        for i in range(meq):
+            check(grads["AL"][k][i] + grads["AU"][k][i], fd([("AL", i), ("AU", i)]), "AL")
+        for i in range(meq, meq + min_):
+            kind = "AL" if y[i] > 0 else ("AU" if y[i] < 0 else "inactive")
+            seen[kind] += 1
+            for nm in ("AL", "AU"):
+                want = fd([(nm, i)])
+                if kind == nm:
+                    check(grads[nm][k][i], want, nm)
+                else:
+                    assert want == 0.0 and grads[nm][k][i] == 0.0, (k, i, kind, nm)
+    print("active sets seen:", seen, flush=True)
+    assert all(v >= 2 for v in seen.values()), seen
