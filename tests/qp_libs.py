@@ -428,42 +428,56 @@ def ref_available():
 def ref_lib():
     global _ref
     if _ref is None:
-        L = C.CDLL(str(REF_PATH))
-        vp = C.c_void_p
-        L.hprqp_model_create.argtypes = [C.POINTER(vp)]
-        L.hprqp_model_destroy.argtypes = [vp]
-        L.hprqp_model_build_from_qabc_csr.argtypes = ([vp] + [C.c_size_t] * 3 + [c_size_p, c_size_p, c_double_p]
-                                                      + [C.c_size_t] * 3 + [c_size_p, c_size_p, c_double_p]
-                                                      + [c_double_p] * 5 + [C.c_double])
-        L.hprqp_model_num_variables.argtypes = [vp]
-        L.hprqp_model_num_variables.restype = C.c_size_t
-        L.hprqp_model_num_constraints.argtypes = [vp]
-        L.hprqp_model_num_constraints.restype = C.c_size_t
-        L.hprqp_params_create.argtypes = [C.POINTER(vp)]
-        L.hprqp_params_destroy.argtypes = [vp]
-        for nm, t in [("stoptol", C.c_double), ("max_iter", C.c_size_t), ("use_gpu", C.c_int),
-                      ("time_limit", C.c_double), ("check_iter", C.c_size_t), ("print_frequency", C.c_int),
-                      ("use_scaling", C.c_int), ("verbose", C.c_int)]:
-            getattr(L, "hprqp_params_set_" + nm).argtypes = [vp, t]
-        L.hprqp_results_create.argtypes = [C.POINTER(vp)]
-        L.hprqp_results_destroy.argtypes = [vp]
-        L.hprqp_optimize.argtypes = [vp, vp, vp]
-        L.build_from_mps.argtypes = [C.c_char_p, C.POINTER(_CModel)]
-        L.model_free.argtypes = [C.POINTER(_CModel)]
-        for nm in ["status", "num_variables", "num_constraints", "iter", "iter_4", "iter_6"]:
-            f = getattr(L, "hprqp_results_" + nm)
-            f.argtypes = [vp]
-            f.restype = C.c_size_t if nm != "status" else C.c_int
-        for nm in ["time", "time_4", "time_6", "objective", "residual", "gap"]:
-            f = getattr(L, "hprqp_results_" + nm)
-            f.argtypes = [vp]
-            f.restype = C.c_double
-        for nm in "xyzw":
-            f = getattr(L, f"hprqp_results_{nm}_data")
-            f.argtypes = [vp]
-            f.restype = c_double_p
-        _ref = L
+        _ref = binding_lib(REF_PATH)
+        _ref.build_from_mps.argtypes = [C.c_char_p, C.POINTER(_CModel)]
+        _ref.model_free.argtypes = [C.POINTER(_CModel)]
     return _ref
+
+
+def binding_lib(path):
+    """A fresh ctypes handle on `path` (the reference library or this
+    package's libhprqp_hip.so) with the argument types of the
+    include/hprqp_binding.h functions set; independent of other handles on
+    the same library."""
+    L = C.CDLL(str(path))
+    vp = C.c_void_p
+    L.hprqp_model_create.argtypes = [C.POINTER(vp)]
+    L.hprqp_model_destroy.argtypes = [vp]
+    L.hprqp_model_build_from_qabc_csr.argtypes = ([vp] + [C.c_size_t] * 3 + [c_size_p, c_size_p, c_double_p]
+                                                  + [C.c_size_t] * 3 + [c_size_p, c_size_p, c_double_p]
+                                                  + [c_double_p] * 5 + [C.c_double])
+    L.hprqp_model_num_variables.argtypes = [vp]
+    L.hprqp_model_num_variables.restype = C.c_size_t
+    L.hprqp_model_num_constraints.argtypes = [vp]
+    L.hprqp_model_num_constraints.restype = C.c_size_t
+    L.hprqp_params_create.argtypes = [C.POINTER(vp)]
+    L.hprqp_params_destroy.argtypes = [vp]
+    for nm, t in [("stoptol", C.c_double), ("max_iter", C.c_size_t), ("use_gpu", C.c_int),
+                  ("time_limit", C.c_double), ("check_iter", C.c_size_t), ("print_frequency", C.c_int),
+                  ("use_scaling", C.c_int), ("verbose", C.c_int)]:
+        getattr(L, "hprqp_params_set_" + nm).argtypes = [vp, t]
+    L.hprqp_results_create.argtypes = [C.POINTER(vp)]
+    L.hprqp_results_destroy.argtypes = [vp]
+    L.hprqp_optimize.argtypes = [vp, vp, vp]
+    L.hprqp_model_read_from_mps.argtypes = [vp, C.c_char_p]
+    L.hprqp_status_name.argtypes = [C.c_int]
+    L.hprqp_status_name.restype = C.c_char_p
+    L.hprqp_params_set_defaults.argtypes = [vp]
+    for nm in ("device_number", "spmv_mode_q", "spmv_mode_a"):
+        getattr(L, "hprqp_params_set_" + nm).argtypes = [vp, C.c_int]
+    for nm in ["status", "num_variables", "num_constraints", "iter", "iter_4", "iter_6"]:
+        f = getattr(L, "hprqp_results_" + nm)
+        f.argtypes = [vp]
+        f.restype = C.c_size_t if nm != "status" else C.c_int
+    for nm in ["time", "time_4", "time_6", "objective", "residual", "gap"]:
+        f = getattr(L, "hprqp_results_" + nm)
+        f.argtypes = [vp]
+        f.restype = C.c_double
+    for nm in "xyzw":
+        f = getattr(L, f"hprqp_results_{nm}_data")
+        f.argtypes = [vp]
+        f.restype = c_double_p
+    return L
 
 
 def binding_solve(L, qp: QP | None, mps_path=None, **params):
@@ -611,30 +625,30 @@ def soc_certificate(qp: QP, x, y, z):
     """Independent original-units optimality certificate
     (tests/gpu/test_gpu_soc.cu:152-196): primal/dual cone residuals,
     stationarity and the relative gap using support values."""
-    Qd, Ad = qp.dense_Q(), qp.dense_A()
-    Qs = 0.5 * (Qd + Qd.T)
+    import scipy.sparse as sp
     n, m = qp.n, qp.m
+    Qm = sp.csr_matrix((qp.q_val, qp.q_ind.astype(np.int64), qp.q_ptr.astype(np.int64)), shape=(n, n))
+    Am = sp.csr_matrix((qp.a_val, qp.a_ind.astype(np.int64), qp.a_ptr.astype(np.int64)), shape=(m, n))
     lin_n = int(qp.var_offsets[0]) if qp.num_var_cones else n
     lin_m = int(qp.row_offsets[0]) if qp.num_row_cones else m
-    qx, ax, aty = Qs @ x, Ad @ x, Ad.T @ y
+    qx, ax, aty = 0.5 * (Qm @ x + Qm.T @ x), Am @ x, Am.T @ y
     rd = qx + qp.c - aty - z
     primal = dual = support = 0.0
-    for j in range(lin_n):
-        primal = max(primal, qp.l[j] - x[j], x[j] - qp.u[j])
-        if z[j] > 0 and math.isfinite(qp.l[j]):
-            support += z[j] * qp.l[j]
-        elif z[j] < 0 and math.isfinite(qp.u[j]):
-            support += z[j] * qp.u[j]
-        else:
-            dual = max(dual, abs(z[j]))
-    for i in range(lin_m):
-        primal = max(primal, qp.AL[i] - ax[i], ax[i] - qp.AU[i])
-        if y[i] > 0 and math.isfinite(qp.AL[i]):
-            support += y[i] * qp.AL[i]
-        elif y[i] < 0 and math.isfinite(qp.AU[i]):
-            support += y[i] * qp.AU[i]
-        else:
-            dual = max(dual, abs(y[i]))
+
+    def bounds_part(v, lo, hi, dual_v):
+        # primal bound violation, support-function value and the dual
+        # residual of the multipliers that no finite bound supports
+        pos = (dual_v > 0) & np.isfinite(lo)
+        neg = ~pos & (dual_v < 0) & np.isfinite(hi)
+        other = ~pos & ~neg
+        viol = float(np.max(np.maximum(lo - v, v - hi))) if len(v) else 0.0
+        sup = float(np.sum(dual_v[pos] * lo[pos]) + np.sum(dual_v[neg] * hi[neg]))
+        res = float(np.max(np.abs(dual_v[other]))) if other.any() else 0.0
+        return viol, sup, res
+    for vv, lo, hi, dv in ((x[:lin_n], qp.l[:lin_n], qp.u[:lin_n], z[:lin_n]),
+                           (ax[:lin_m], qp.AL[:lin_m], qp.AU[:lin_m], y[:lin_m])):
+        viol, sup, res = bounds_part(vv, lo, hi, dv)
+        primal, dual, support = max(primal, viol), max(dual, res), support + sup
     for k in range(qp.num_var_cones):
         b, e = int(qp.var_offsets[k]), int(qp.var_offsets[k + 1])
         primal = max(primal, np.max(np.abs(cone_project(x[b:e]) - x[b:e])))
@@ -653,6 +667,22 @@ def soc_certificate(qp: QP, x, y, z):
     inf_ = lambda a: float(np.max(np.abs(a))) if len(a) else 0.0
     stat = inf_(rd) / (1 + max(inf_(qp.c), inf_(qx), inf_(aty)))
     return dict(primal=primal, dual=dual, stationarity=stat, gap=gap)
+
+
+# The benchmark workloads (generator arguments), shared by bench.py and the
+# full-size tests.
+# plain_n2k: n = 2,000, m = 1,000 (600 equality + 400 two-sided rows), A at
+# 10 nnz/row, Q = B'DB + 1e-2 I with B 500 x n at 5 nnz/row, box [-10, 10].
+PLAIN_N2K = dict(seed=0, n=2_000, m_eq=600, m_in=400, a_row_nnz=10, heavy_rows=0, heavy_nnz=0, b_rows=500,
+                 b_row_nnz=5, box=10.0)
+# plain_n1M: n = 1e6, m = 5e5 (300k + 200k), A at 20 nnz/row with 5,000 rows
+# of 2,000 nnz, B 200,000 x n at 8 nnz/row.
+PLAIN_N1M = dict(seed=0, n=1_000_000, m_eq=300_000, m_in=200_000, a_row_nnz=20, heavy_rows=5_000, heavy_nnz=2_000,
+                 b_rows=200_000, b_row_nnz=8, box=10.0)
+# soc_n220k: 20,000 cones of dimension 10 after a 20,000-variable box
+# prefix [-5, 5] (n = 220,000), 100,000 equality rows at 40 nnz/row, Q
+# diagonal U(0, 1).
+SOC_N220K = dict(seed=0, n_box=20_000, cones=20_000, cone_dim=10, m=100_000, a_row_nnz=40, box=5.0)
 
 
 def batched_instances(count=4096, seed=0, n_lo=50, n_hi=2000):

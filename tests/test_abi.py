@@ -299,3 +299,47 @@ def test_random_builds_agree_with_reference():
             Q.assert_same_model(mine, ref)
             ok += 1
     assert 60 < ok < 300
+
+
+CLIENT = ROOT / "tests" / "c_client"
+REF_INCLUDE = Path("/root/reference/include")
+
+
+def _compile(src, out, include, libs=()):
+    subprocess.run(["gcc", "-std=c11", "-O1", "-Wall", "-Wextra", "-Werror", f"-I{include}", str(src), "-o", str(out),
+                    *libs], check=True, capture_output=True, text=True)
+
+
+def test_abi_struct_layouts_match_reference_headers(tmp_path):
+    """sizeof / alignof / offsetof of every field of the ABI structs of
+    include/shared/types.h and the enum values, compiled against this
+    repository's headers and against the reference's
+    (include/shared/types.h:10-214): identical, so a client compiled against
+    either runs with this library."""
+    if not REF_INCLUDE.is_dir():
+        pytest.skip("the reference's headers are not present here")
+    outs = []
+    for k, inc in enumerate((ROOT / "include", REF_INCLUDE)):
+        exe = tmp_path / f"layout{k}"
+        _compile(CLIENT / "abi_layout.c", exe, inc)
+        outs.append(subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout)
+    assert outs[0].count("\n") > 140
+    assert outs[0] == outs[1]
+
+
+def test_c_client_compiles_against_include_and_links():
+    """A plain C11 client (tests/c_client/client.c) compiles with -Wall
+    -Werror against include/ alone and links to libhprqp_hip.so; run here
+    without a device, each solve reports NOT_IMPLEMENTED (no CPU fallback)
+    and the CPU path is refused."""
+    subprocess.run(["make", "-s", "-C", str(CLIENT)], check=True, capture_output=True, text=True)
+    out = subprocess.run([str(CLIENT / "_build" / "client"), str(ROOT / "tests" / "golden" / "mps" / "simple_qp.mps")],
+                         check=True, capture_output=True, text=True).stdout
+    lines = {ln.split()[0]: dict(kv.split("=", 1) for kv in ln.split()[1:]) for ln in out.splitlines()}
+    assert lines["done"]["rc"] == "0"
+    assert lines["binding_mps_read"] == {"rc": "OK", "n": "2", "m": "2"}
+    assert lines["model_api_cpu"]["rc"] == "NOT_IMPLEMENTED"
+    import hprqp_amd as H
+    if not H.gpu_backend_available():
+        for case in ("binding_unconstrained", "binding_mps_max_iter", "model_api_equality"):
+            assert lines[case]["rc"] == "NOT_IMPLEMENTED", case
