@@ -251,6 +251,21 @@ def bench_single(H, Q, args, qp, workload, steps, warmup, cpu_bodies, gather=Tru
                 f"{p['gbs']:.0f} GB/s")
         calibration_copy(H)
         calibration_gather(H)
+        # each iteration kernel against the gather ceiling at its own gather
+        # count and window
+        s = H.Session(model, H.Parameters(verbose=False))
+        mats = {"zxw1_q": "Q", "tempv_q": "Q", "y_a": "A", "w2_at": "AT"}
+        for p in prof:
+            mat = mats.get(p["kernel"])
+            if not mat:
+                continue
+            panels = max(int(s.scalar("panels_" + mat)), 1)
+            nnz = {"Q": model.c.Q.nnz, "A": model.c.A.nnz, "AT": model.c.A.nnz}[mat]
+            window = (qp.m if mat == "AT" else qp.n) // panels
+            g = calibration_gather(H, nrow=max(int(nnz) // 20, 1), windows=(window,))[0]
+            log(f"kernel {p['kernel']}: {nnz} gathers over a {window}-double window (P = {panels}, "
+                f"lds_pad {int(s.scalar('lds_pad_' + mat))}): ceiling {g['us']:.1f} us, kernel "
+                f"{p['mean_ms'] * 1e3:.1f} us -> gather_frac {g['us'] / (p['mean_ms'] * 1e3):.3f}")
         return None
 
     for w in range(warmup):
@@ -485,8 +500,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="all",
-                    choices=["all", "plain_n1M", "plain_n2k", "plain_n1M_banded", "soc_n220k", "batched_4096",
-                             "layer_1024"])
+                    choices=["all", "plain_n1M", "plain_n2k", "plain_n1M_banded", "plain_n1M_no_heavy",
+                             "plain_n1M_only_heavy", "soc_n220k", "batched_4096", "layer_1024"])
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--seq-sample", type=int, default=32)
     ap.add_argument("--stoptol", type=float, default=1e-8)
@@ -533,13 +548,20 @@ def main():
         if res:
             print(json.dumps(emit("time-to-1e-8 KKT (cone HPR, 1 GPU)", res, False)), flush=True)
         return
-    if w in ("plain_n1M", "plain_n1M_banded", "plain_n2k"):
-        banded = w.endswith("_banded")
-        if banded and not args.profile_only:
-            raise SystemExit("the banded variant is a profiling aid: use it with --profile-only")
+    if w.startswith("plain_"):
+        variant = w[len("plain_n1M"):] if w.startswith("plain_n1M_") else ""
+        if variant and not args.profile_only:
+            raise SystemExit(f"{w} is a profiling aid: use it with --profile-only")
+        kw = dict(_workloads()["plain_n2k" if w == "plain_n2k" else "plain_n1M"])
+        if variant == "_no_heavy":
+            # only the 20-nnz rows of A (stream blocks): 500,000 rows, 10M nnz
+            kw.update(heavy_rows=0)
+        elif variant == "_only_heavy":
+            # only 2,000-nnz rows (wave blocks): 5,000 rows, 10M nnz
+            kw.update(m_eq=3_000, m_in=2_000, a_row_nnz=2_000, heavy_rows=0)
         t = time.time()
-        qp = Q.random_qp(**_workloads()[w.replace("_banded", "")])
-        if banded:
+        qp = Q.random_qp(**kw)
+        if variant == "_banded":
             qp = banded_variant(qp)
         log(f"generated {w}: n={qp.n} m={qp.m} nnzA={len(qp.a_val)} in {time.time() - t:.1f}s")
         small = w == "plain_n2k"
