@@ -1079,6 +1079,96 @@ __device__ void adj_op(WG &g, const AdjMask &mk, const double *xn, const double 
     if (g.I->m > 0) wg_spmv<WG::kDense>(g.A, xn, EpiAdjM{{}, ym, mk}, sums, maxs, g.prod);
 }
 
+// Sums of kAdjRows vectors over a wavefront in one pass (a reduce-scatter
+// by halves, then one exchange): v[u] holds lane's term of row u on entry;
+// on exit lane L holds the wavefront's sum of row L >> 1 in v[0]. Fixed
+// pairing, so deterministic.
+constexpr int kAdjRows = 32;  // rows of A per chunk: loads in flight per thread
+
+__device__ inline void wave_reduce_scatter(double (&v)[kAdjRows], int lane)
+{
+#pragma unroll
+    for (int st = 0; st < 5; ++st) {
+        const int o = 32 >> st;
+        const int h = kAdjRows >> (st + 1);
+        const bool up = (lane & o) != 0;
+#pragma unroll
+        for (int k = 0; k < h; ++k) {
+            const double mine = up ? v[h + k] : v[k];
+            const double send = up ? v[k] : v[h + k];
+            v[k] = mine + __shfl_xor(send, o, kWave);
+        }
+    }
+    v[0] += __shfl_xor(v[0], 1, kWave);
+}
+
+// The same operator on a dense instance in one pass over A instead of two
+// (Q: n x n, A: m x n, row-major; g.AT's value plane is A itself): thread j
+// of a 256-column tile adds A[i][j] xm_i to its column sum, i ascending (the
+// order of the column-form A' product), and the row products A[i][j] xn_j
+// are summed over the workgroup -- each wavefront's 64 columns by
+// wave_reduce_scatter into wrow[wavefront][i], then the four wavefronts in
+// order, then the column tiles in order -- into ym_i. Q xn stays a
+// column-form product. The next chunk's rows of A are loaded before the
+// current chunk is reduced, and no barrier interrupts the pass. x_n and x_m
+// are staged in LDS (xs_n, xs_m: kDenseMax each); wrow (4 x mld doubles,
+// dynamic LDS) holds the wavefronts' partial row sums.
+template <class WG>
+__device__ void adj_op_dense(WG &g, const AdjMask &mk, const double *xn, const double *xm, double *yn, double *ym,
+                             double *xs_n, double *xs_m, double *wrow, int mld)
+{
+    const int n = g.I->n, m = g.I->m, t = threadIdx.x;
+    const int lane = t & (kWave - 1), wid = t / kWave;
+    for (int i = t; i < n; i += kThreads) xs_n[i] = xn[i];
+    for (int i = t; i < m; i += kThreads) xs_m[i] = xm[i];
+    __syncthreads();
+    const double *__restrict__ Ad = g.AT.dv;  // A[i][j] at i * n + j
+    double *__restrict__ wr = wrow + wid * mld;
+    for (int j0 = 0; j0 < n; j0 += kThreads) {
+        const int j = j0 + t;
+        const bool col = j < n;
+        const int jc = col ? j : n - 1;
+        const double xj = col ? xs_n[j] : 0.0;
+        const double qx = dense_column_dot<false>(g.Q, jc, xs_n);
+        double acc = 0.0;
+        double a[kAdjRows];
+        if (m > 0) {
+#pragma unroll
+            for (int u = 0; u < kAdjRows; ++u)
+                a[u] = __builtin_nontemporal_load(Ad + (long long)min(u, m - 1) * n + jc);
+        }
+        for (int i0 = 0; i0 < m; i0 += kAdjRows) {
+            double p[kAdjRows];
+#pragma unroll
+            for (int u = 0; u < kAdjRows; ++u) {
+                const bool ok = i0 + u < m;
+                if (ok) acc += a[u] * xs_m[i0 + u];
+                p[u] = col && ok ? a[u] * xj : 0.0;
+            }
+            if (i0 + kAdjRows < m) {  // next chunk in flight during the reduction
+#pragma unroll
+                for (int u = 0; u < kAdjRows; ++u)
+                    a[u] = __builtin_nontemporal_load(Ad + (long long)min(i0 + kAdjRows + u, m - 1) * n + jc);
+            }
+            wave_reduce_scatter(p, lane);
+            const int r = i0 + (lane >> 1);
+            if ((lane & 1) == 0 && r < m) wr[r] = p[0];
+        }
+        if (col) yn[j] = mk.free_var(j) ? qx + acc : 0.0;
+        __syncthreads();
+        for (int i = t; i < m; i += kThreads) {
+            double r = wrow[i];
+#pragma unroll
+            for (int w = 1; w < kThreads / kWave; ++w) r += wrow[w * mld + i];
+            ym[i] = j0 == 0 ? r : ym[i] + r;
+        }
+        __syncthreads();
+    }
+    for (int i = t; i < m; i += kThreads)
+        if (!mk.active_row(i)) ym[i] = 0.0;
+    __syncthreads();
+}
+
 template <class WG>
 __device__ double adj_dot(WG &g, const double *an, const double *am, const double *bn, const double *bm,
                           double *sums, double *maxs)
@@ -1110,11 +1200,14 @@ __global__ __launch_bounds__(kThreads) void batch_adjoint_kernel(Arrays ar, Para
                                                                  double *__restrict__ vrow_out,
                                                                  double *__restrict__ vbnd_out, double tol,
                                                                  int max_steps, int *__restrict__ steps_out,
-                                                                 double *__restrict__ resid_out)
+                                                                 double *__restrict__ resid_out, int mld)
 {
     __shared__ double prod[kStreamCap];
     __shared__ double red[kMaxTerms];
     __shared__ State S;
+    // dense instances: x_m staging of the one-pass operator (adj_op_dense)
+    __shared__ double dense_lds[D ? kDenseMax : 1];
+    extern __shared__ double adj_dyn[];  // dense: 4 x mld partial row sums
     if ((int)blockIdx.x >= count) return;
     const int k = ar.order[blockIdx.x];
     if (ar.inst[k].dense != (int)D) return;
@@ -1124,6 +1217,12 @@ __global__ __launch_bounds__(kThreads) void batch_adjoint_kernel(Arrays ar, Para
     wg_bind(g, ar, P, k, &S, prod, red);
     const int n = g.I->n, m = g.I->m, t = threadIdx.x;
     double sums[kMaxTerms] = {0}, maxs[kMaxTerms] = {0};
+    auto apply = [&](const AdjMask &mk_, const double *xn_, const double *xm_, double *yn_, double *ym_) {
+        if constexpr (D)
+            adj_op_dense(g, mk_, xn_, xm_, yn_, ym_, prod, dense_lds, adj_dyn, mld);
+        else
+            adj_op(g, mk_, xn_, xm_, yn_, ym_, sums, maxs);
+    };
     const AdjMask mk{g.N(N_z_bar), g.N(N_l), g.N(N_u), g.M(M_y_bar), g.M(M_AL), g.M(M_AU)};
     const double *d = g.N(N_col), *rr = g.M(M_row);
     const double *gk = gvec + g.I->vn;
@@ -1169,7 +1268,7 @@ __global__ __launch_bounds__(kThreads) void batch_adjoint_kernel(Arrays ar, Para
             for (int i = t; i < n; i += kThreads) vn[i] = sc * ln[ir2][i];
             for (int i = t; i < m; i += kThreads) vm[i] = sc * lm[ir2][i];
             __syncthreads();
-            adj_op(g, mk, vn, vm, ln[iy], lm[iy], sums, maxs);
+            apply(mk, vn, vm, ln[iy], lm[iy]);
             if (step >= 2) adj_axpy(g, -beta / oldb, ln[ir1], lm[ir1], ln[iy], lm[iy]);
             const double alfa = adj_dot(g, vn, vm, ln[iy], lm[iy], sums, maxs);
             adj_axpy(g, -alfa / beta, ln[ir2], lm[ir2], ln[iy], lm[iy]);
@@ -1216,7 +1315,7 @@ __global__ __launch_bounds__(kThreads) void batch_adjoint_kernel(Arrays ar, Para
     // the true relative residual ||b - K x|| / ||b|| of the returned iterate
     double rel = 0.0;
     if (beta1 > 0.0) {
-        adj_op(g, mk, xn, xm, ln[iy], lm[iy], sums, maxs);
+        apply(mk, xn, xm, ln[iy], lm[iy]);
         for (int i = t; i < n; i += kThreads) ln[iy][i] = bn[i] - ln[iy][i];
         for (int i = t; i < m; i += kThreads) lm[iy][i] = -lm[iy][i];
         __syncthreads();
@@ -2318,6 +2417,18 @@ HPRQPStatus hprqp_hip_batch_solution(const HPRQPHipBatch *b, double *x, double *
 
 namespace {
 
+// Row stride (doubles, a multiple of 32) of the dense adjoint's per-wavefront
+// partial row sums, and the dynamic LDS bytes they take.
+int adj_dense_mld(const HPRQPHipBatch *b)
+{
+    size_t mx = 1;
+    for (size_t k = 0; k < b->count; ++k)
+        if (b->inst[k].dense) mx = std::max(mx, b->m[k]);
+    return (int)((mx + 31) / 32 * 32);
+}
+
+size_t adj_dense_lds(const HPRQPHipBatch *b) { return sizeof(double) * (hprqp::kThreads / hprqp::kWave) * adj_dense_mld(b); }
+
 // The batch-owned outputs of the adjoint besides the caller's vectors: step
 // counts and final relative residuals, one per instance.
 hipError_t ensure_adjoint_outputs(HPRQPHipBatch *b)
@@ -2391,10 +2502,11 @@ HPRQPStatus hprqp_hip_batch_adjoint(HPRQPHipBatch *b, const double *dl_dx, doubl
     HPRQP_HIP(hipMemsetAsync(d_vr.p, 0, sizeof(double) * std::max<size_t>(sm, 1), b->stream));
     if (b->any_sparse)
         batch_adjoint_kernel<false><<<(unsigned)b->count, hprqp::kThreads, 0, b->stream>>>(
-            b->ar, b->P, (int)b->count, d_g.p, d_u.p, d_vr.p, d_vb.p, tol, max_steps, b->d_steps, b->d_adj_resid);
+            b->ar, b->P, (int)b->count, d_g.p, d_u.p, d_vr.p, d_vb.p, tol, max_steps, b->d_steps, b->d_adj_resid, 0);
     if (b->any_dense)
-        batch_adjoint_kernel<true><<<(unsigned)b->count, hprqp::kThreads, 0, b->stream>>>(
-            b->ar, b->P, (int)b->count, d_g.p, d_u.p, d_vr.p, d_vb.p, tol, max_steps, b->d_steps, b->d_adj_resid);
+        batch_adjoint_kernel<true><<<(unsigned)b->count, hprqp::kThreads, adj_dense_lds(b), b->stream>>>(
+            b->ar, b->P, (int)b->count, d_g.p, d_u.p, d_vr.p, d_vb.p, tol, max_steps, b->d_steps, b->d_adj_resid,
+            adj_dense_mld(b));
     HPRQP_HIP(hipGetLastError());
     HPRQP_HIP(hipStreamSynchronize(b->stream));
     HPRQP_HIP(hipMemcpy(u, d_u.p, sizeof(double) * sn, hipMemcpyDeviceToHost));
@@ -2424,11 +2536,11 @@ HPRQPStatus hprqp_hip_batch_adjoint_device(HPRQPHipBatch *b, const double *dl_dx
     if (b->any_sparse)
         batch_adjoint_kernel<false><<<(unsigned)b->count, hprqp::kThreads, 0, s>>>(b->ar, b->P, (int)b->count, dl_dx,
                                                                                  u, v_rows, v_bounds, tol, max_steps,
-                                                                                 b->d_steps, b->d_adj_resid);
+                                                                                 b->d_steps, b->d_adj_resid, 0);
     if (b->any_dense)
-        batch_adjoint_kernel<true><<<(unsigned)b->count, hprqp::kThreads, 0, s>>>(b->ar, b->P, (int)b->count, dl_dx,
-                                                                                u, v_rows, v_bounds, tol, max_steps,
-                                                                                b->d_steps, b->d_adj_resid);
+        batch_adjoint_kernel<true><<<(unsigned)b->count, hprqp::kThreads, adj_dense_lds(b), s>>>(
+            b->ar, b->P, (int)b->count, dl_dx, u, v_rows, v_bounds, tol, max_steps, b->d_steps, b->d_adj_resid,
+            adj_dense_mld(b));
     HPRQP_HIP(hipGetLastError());
     HPRQP_HIP(mark_caller_use(b, s));
     if (steps) {
