@@ -243,7 +243,7 @@ def cpu_parallel_solves(Q, qps, stoptol, workers):
 def bench_single(H, Q, args, qp, workload, steps, warmup, cpu_bodies, gather=True):
     t = time.time()
     model = build_model(H, qp)
-    log(f"{workload}: host build/preprocess {time.time() - t:.1f}s (nnzQ after symmetrisation {model.c.Q.nnz})")
+    log(f"{workload}: model build (build_from_QAbc) {time.time() - t:.2f}s (nnzQ after symmetrisation {model.c.Q.nnz})")
     if args.profile_only:
         prof = profile_kernels(H, model, args.profile_iters)
         for p in prof:

@@ -15,6 +15,7 @@
 #include <thread>
 #include <vector>
 
+#include "device_build.h"
 #include "hprqp.h"
 
 namespace {
@@ -293,6 +294,23 @@ HPRQPStatus symmetrize_q(HPRQPMatrixCSR *out, const HPRQPMatrixCSR *q)
     return HPRQP_STATUS_OK;
 }
 
+// The transposes and the symmetrisation of a build run on the GPU for large
+// models (device_build.h) and on host threads otherwise; both give the same
+// bits.
+HPRQPStatus build_transpose(HPRQPMatrixCSR *dest, const HPRQPMatrixCSR *src)
+{
+    if (hprqp::device_transpose_csr(src, dest) == HPRQP_STATUS_OK) return HPRQP_STATUS_OK;
+    csr_reset(dest);
+    return matrix_csr_transpose(dest, src);
+}
+
+HPRQPStatus build_symmetrize(HPRQPMatrixCSR *dest, const HPRQPMatrixCSR *q)
+{
+    if (hprqp::device_symmetrize_q(q, dest) == HPRQP_STATUS_OK) return HPRQP_STATUS_OK;
+    csr_reset(dest);
+    return symmetrize_q(dest, q);
+}
+
 // build.c:254-337
 HPRQPStatus drop_empty_and_free_rows(HPRQPModel *md)
 {
@@ -340,7 +358,7 @@ HPRQPStatus drop_empty_and_free_rows(HPRQPModel *md)
     md->A = na;
     md->AL = nal;
     md->AU = nau;
-    return matrix_csr_transpose(&md->AT, &md->A);
+    return build_transpose(&md->AT, &md->A);
 }
 
 bool csr_all_finite(const HPRQPMatrixCSR *m)
@@ -374,7 +392,7 @@ HPRQPStatus preprocess(HPRQPModel *md)
     if (st != HPRQP_STATUS_OK) return st;
     HPRQPMatrixCSR sym;
     csr_reset(&sym);
-    st = symmetrize_q(&sym, &md->Q);
+    st = build_symmetrize(&sym, &md->Q);
     if (st != HPRQP_STATUS_OK) return st;
     matrix_csr_free(&md->Q);
     md->Q = sym;
@@ -396,7 +414,7 @@ HPRQPStatus build_impl(const HPRQPMatrixCSR *Q, const HPRQPMatrixCSR *A, const d
     md->owns_data = 1;
     HPRQPStatus st = matrix_csr_copy(&md->Q, Q);
     if (st == HPRQP_STATUS_OK) st = matrix_csr_copy(&md->A, A);
-    if (st == HPRQP_STATUS_OK) st = matrix_csr_transpose(&md->AT, A);
+    if (st == HPRQP_STATUS_OK) st = build_transpose(&md->AT, A);
     if (st == HPRQP_STATUS_OK) {
         md->c = copy_doubles(c, n);
         md->l = copy_doubles(l, n);
