@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -523,14 +524,18 @@ __device__ void wg_setup_pre(WG &g, double *sums, double *maxs)
     __syncthreads();
 }
 
+// lambdas_given: batch_power_kernel has stored lambda_max(A A') and, for a
+// non-diagonal Q, lambda_max(Q) (both x 1.05) in the state.
 template <class WG>
-__device__ void wg_setup_post(WG &g, double *sums, double *maxs)
+__device__ void wg_setup_post(WG &g, double *sums, double *maxs, bool lambdas_given = false)
 {
     const int n = g.I->n, t = threadIdx.x;
     const int diag = g.S->Q_is_diag;
-    double lam_a = wg_power(g, true, sums, maxs) * 1.05;
+    double lam_a = lambdas_given ? g.S->lam_a : wg_power(g, true, sums, maxs) * 1.05;
     double lam_q = 0.0;
-    if (diag) {
+    if (lambdas_given && !diag) {
+        lam_q = g.S->lam_q;
+    } else if (diag) {
         for (int i = t; i < n; i += kThreads) maxs[0] = fmax(maxs[0], g.N(N_diag)[i]);
         wg_reduce<0, 1>(sums, maxs, g.red);
         lam_q = g.red[0];
@@ -1337,10 +1342,235 @@ __global__ __launch_bounds__(kThreads) void batch_adjoint_kernel(Arrays ar, Para
     }
 }
 
+// ------------------------------------------------ power iterations (CSR)
+// eigs.c:19-153 for CSR instances with n, m <= kPowMax, one 1024-thread
+// workgroup per instance, vectors in LDS. When a matrix has at most
+// kPowCache entries per thread and its products fit in LDS, each thread
+// holds entries e = t + 1024 u in registers for the whole iteration, so a
+// power step touches no memory: the products val * x go to an LDS array in
+// the order of the target CSR rows (A' order for A' q, A order for A tmp)
+// and each row is then one thread's left-to-right sum over it -- the
+// rounding of the host's CSR product without a fused multiply-add. Larger
+// matrices take the thread-per-row product from memory.
+constexpr int kPowThreads = 1024;
+constexpr int kPowMax = 4096;    // longest vector of an instance the kernel takes
+constexpr int kPowSlots = kPowMax / kPowThreads;  // rows a thread owns: t + 1024 s
+constexpr int kPowCache = 16;    // matrix entries per thread held in registers
+constexpr int kPowRed = 4 * (kPowThreads / kWave);
+constexpr int kPowLds = 20480 - kPowRed;  // doubles for vectors and products: 160 KiB in all
+
+__device__ double pow_block_sum(double v, double *red, int slot)
+{
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    v = wave_sum(v);
+    if (lane == 0) red[slot * (kPowThreads / kWave) + wid] = v;
+    __syncthreads();
+    double t = red[slot * (kPowThreads / kWave)];
+#pragma unroll
+    for (int w = 1; w < kPowThreads / kWave; ++w) t += red[slot * (kPowThreads / kWave) + w];
+    return t;
+}
+
+// y = M x over nrow rows of a CSR instance view, x in LDS; f(r, y_r).
+template <class F>
+__device__ void pow_spmv(const Mat &M, int nrow, const double *xs, const F &f)
+{
+    for (int r = threadIdx.x; r < nrow; r += kPowThreads) {
+        const int k0 = M.rp[r], k1 = M.rp[r + 1];
+        double acc = 0.0;
+        int k = k0;
+        for (; k + 8 <= k1; k += 8) {
+            int c[8];
+            double v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                c[u] = M.ci[k + u];
+                v[u] = M.val[k + u];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) acc += v[u] * xs[c[u]];
+        }
+        for (; k < k1; ++k) acc += M.val[k] * xs[M.ci[k]];
+        f(r, acc);
+    }
+}
+
+// f(r, x[idx(lo)] + x[idx(lo + 1)] + ... left to right) for the rows of a
+// CSR view owned by this thread, row pointers rpl in LDS.
+template <class Idx, class F>
+__device__ void pow_row_sums(const int *rpl, const double *x, int nrow, const Idx &idx, const F &f)
+{
+    for (int r = threadIdx.x; r < nrow; r += kPowThreads) {
+        const int k1 = rpl[r + 1];
+        double acc = 0.0;
+        for (int k = rpl[r]; k < k1; ++k) acc += x[idx(k)];
+        f(r, acc);
+    }
+}
+
+// The power iteration of wg_power over vectors in LDS; prod(epi) forms
+// M q (q complete in LDS) and calls epi(r, (M q)_r) once for every r < len.
+template <class Prod>
+__device__ double pow_loop(int len, double *q, double *z, double *red, const Prod &prod)
+{
+    const int t = threadIdx.x;
+    double s = 0.0;
+    const double len1 = (double)len + 1.0;
+    for (int i = t; i < len; i += kPowThreads) {
+        const double v = 1.0 + ((double)i + 1.0) / len1;
+        z[i] = v;
+        s += v * v;
+    }
+    double qn = sqrt(pow_block_sum(s, red, 0));
+    double lam = 1.0;
+    for (int it = 0; it < 5000; ++it) {
+        if (qn <= 1e-16) {
+            lam = 0.0;
+            break;
+        }
+        for (int i = t; i < len; i += kPowThreads) q[i] = z[i] / qn;
+        __syncthreads();
+        double s0 = 0.0, s1 = 0.0;
+        prod([&](int r, double v) {
+            z[r] = v;
+            s0 += q[r] * v;
+            s1 += v * v;
+        });
+        lam = pow_block_sum(s0, red, 1);
+        const double zn = sqrt(pow_block_sum(s1, red, 2));
+        double r2 = 0.0;
+        for (int i = t; i < len; i += kPowThreads) {
+            const double r = z[i] - lam * q[i];
+            q[i] = r;
+            r2 += r * r;
+        }
+        const double err = sqrt(pow_block_sum(r2, red, 3)) / (zn + fabs(lam) + 1e-16);
+        __syncthreads();  // red, q, z and the products are rewritten by the next step
+        if (err < 1e-4) break;
+        qn = zn;
+    }
+    return lam < 0.0 ? 0.0 : lam;
+}
+
+// Doubles of LDS for n ints (rounded up).
+__device__ inline int pow_ints(int n) { return (n + 1) / 2; }
+
+// lambda_max(A A'): z = A (A' q) over vectors of length m.
+__device__ double pow_a(const Mat &A, const Mat &AT, const int *at_perm, int n, int m, double *lds, double *red)
+{
+    if (m == 0) return 0.0;
+    const int t = threadIdx.x, Z = A.rp[m];
+    // cached layout: q | z = tmp (tmp lives from the A' q sums to the A tmp
+    // products, z from the row sums to the next q) | P | A' perm | A' rp | A rp
+    const int zt = n > m ? n : m;
+    if (Z > kPowThreads * kPowCache || m + zt + Z + pow_ints(Z) + pow_ints(n + 1) + pow_ints(m + 1) > kPowLds) {
+        double *q = lds, *z = lds + m, *tmp = lds + 2 * m;
+        return pow_loop(m, q, z, red, [&](auto &&epi) {
+            pow_spmv(AT, n, q, [&](int r, double v) { tmp[r] = v; });
+            __syncthreads();
+            pow_spmv(A, m, tmp, epi);
+        });
+    }
+    double *q = lds, *z = lds + m, *tmp = z, *P = z + zt;
+    int *perm = reinterpret_cast<int *>(P + Z);
+    int *at_rp = perm + 2 * pow_ints(Z), *a_rp = at_rp + 2 * pow_ints(n + 1);
+    for (int k = t; k < Z; k += kPowThreads) perm[k] = at_perm[k];
+    for (int r = t; r <= n; r += kPowThreads) at_rp[r] = AT.rp[r];
+    for (int r = t; r <= m; r += kPowThreads) a_rp[r] = A.rp[r];
+    // entry e = t + 1024 u of A: value and row | col << 16 (n, m <= 4096)
+    double v[kPowCache];
+    int rc[kPowCache];
+#pragma unroll
+    for (int u = 0; u < kPowCache; ++u) {
+        const int e = t + u * kPowThreads;
+        v[u] = 0.0;
+        rc[u] = 0;
+        if (e < Z) {
+            int lo = 0, hi = m;  // A.rp[lo] <= e < A.rp[hi]
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (A.rp[mid] <= e) lo = mid;
+                else hi = mid;
+            }
+            v[u] = A.val[e];
+            rc[u] = lo | (A.ci[e] << 16);
+        }
+    }
+    __syncthreads();
+    return pow_loop(m, q, z, red, [&](auto &&epi) {
+        // A' q: products in A order, each A' row (a column of A) summed in
+        // A' order through the permutation
+#pragma unroll
+        for (int u = 0; u < kPowCache; ++u)
+            if (t + u * kPowThreads < Z) P[t + u * kPowThreads] = v[u] * q[rc[u] & 0xffff];
+        __syncthreads();
+        pow_row_sums(at_rp, P, n, [&](int k) { return perm[k]; }, [&](int j, double sj) { tmp[j] = sj; });
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < kPowCache; ++u)
+            if (t + u * kPowThreads < Z) P[t + u * kPowThreads] = v[u] * tmp[rc[u] >> 16];
+        __syncthreads();
+        pow_row_sums(a_rp, P, m, [](int k) { return k; }, epi);
+    });
+}
+
+// lambda_max(Q): z = Q q over vectors of length n.
+__device__ double pow_q(const Mat &Q, int n, double *lds, double *red)
+{
+    const int t = threadIdx.x, Z = Q.rp[n];
+    double *q = lds, *z = lds + n, *P = lds + 2 * n;
+    if (Z > kPowThreads * kPowCache || 2 * n + Z + pow_ints(n + 1) > kPowLds)
+        return pow_loop(n, q, z, red, [&](auto &&epi) { pow_spmv(Q, n, q, epi); });
+    int *q_rp = reinterpret_cast<int *>(P + Z);
+    for (int r = t; r <= n; r += kPowThreads) q_rp[r] = Q.rp[r];
+    double v[kPowCache];
+    int c[kPowCache];
+#pragma unroll
+    for (int u = 0; u < kPowCache; ++u) {
+        const int e = t + u * kPowThreads;
+        v[u] = e < Z ? Q.val[e] : 0.0;
+        c[u] = e < Z ? Q.ci[e] : 0;
+    }
+    __syncthreads();
+    return pow_loop(n, q, z, red, [&](auto &&epi) {
+#pragma unroll
+        for (int u = 0; u < kPowCache; ++u)
+            if (t + u * kPowThreads < Z) P[t + u * kPowThreads] = v[u] * q[c[u]];
+        __syncthreads();
+        pow_row_sums(q_rp, P, n, [](int k) { return k; }, epi);
+    });
+}
+
+// lambda_max(A A') and, for a non-diagonal Q, lambda_max(Q) of every CSR
+// instance, x 1.05, into its state; runs between setup phases 1 and 3 (the
+// matrices are scaled and A' holds its values).
+__global__ __launch_bounds__(kPowThreads) void batch_power_kernel(Arrays ar, int count)
+{
+    __shared__ double lds[kPowLds];
+    __shared__ double red[kPowRed];
+    if ((int)blockIdx.x >= count) return;
+    const int k = ar.order[blockIdx.x];
+    const Inst *I = ar.inst + k;
+    if (I->dense) return;
+    const Mat Q{ar.rp + I->q_rp, ar.ci + I->q_nz, ar.val + I->q_nz, nullptr, 0, nullptr, 0, 0, 0};
+    const Mat A{ar.rp + I->a_rp, ar.ci + ar.a_base + I->a_nz, ar.val + ar.a_base + I->a_nz, nullptr, 0, nullptr, 0, 0, 0};
+    const Mat AT{ar.rp + I->at_rp, ar.ci + ar.at_ci_base + I->a_nz, ar.val + ar.at_ci_base + I->a_nz, nullptr, 0,
+                 nullptr, 0, 0, 0};
+    const int diag = ar.state[k].Q_is_diag;
+    const double lam_a = pow_a(A, AT, ar.perm + I->a_nz, I->n, I->m, lds, red) * 1.05;
+    const double lam_q = diag ? 0.0 : pow_q(Q, I->n, lds, red) * 1.05;
+    if (threadIdx.x == 0) {
+        ar.state[k].lam_a = lam_a;
+        ar.state[k].lam_q = lam_q;
+    }
+}
+
 template <bool D>
 // phase 0: the whole setup; 1: up to the power iterations (the state is
-// stored); 2: from the power iterations on (the state is loaded). Dense
-// batches run 1, dense_gram_kernel, 2.
+// stored); 2: from the power iterations on (the state is loaded); 3: as 2
+// with the lambdas batch_power_kernel stored. Dense batches run 1,
+// dense_gram_kernel, 2; CSR batches whose vectors fit in LDS run 1,
+// batch_power_kernel, 3.
 __global__ __launch_bounds__(kThreads) void batch_setup_kernel(Arrays ar, Params P, int count, int phase)
 {
     __shared__ double prod[kStreamCap];
@@ -1352,7 +1582,7 @@ __global__ __launch_bounds__(kThreads) void batch_setup_kernel(Arrays ar, Params
     WGv<D> g;
     wg_bind(g, ar, P, k, &S, prod, red);
     if (threadIdx.x == 0) {
-        if (phase == 2) {
+        if (phase >= 2) {
             S = ar.state[k];
         } else {
             S = State{};
@@ -1367,12 +1597,12 @@ __global__ __launch_bounds__(kThreads) void batch_setup_kernel(Arrays ar, Params
     }
     __syncthreads();
     double sums[kMaxTerms] = {0}, maxs[kMaxTerms] = {0};
-    if (phase != 2) wg_setup_pre(g, sums, maxs);
+    if (phase < 2) wg_setup_pre(g, sums, maxs);
     if (phase == 1) {
         if (threadIdx.x == 0) ar.state[k] = S;
         return;
     }
-    wg_setup_post(g, sums, maxs);
+    wg_setup_post(g, sums, maxs, phase == 3);
     // iteration-0 residuals (solver_backend.c:396)
     wg_residuals(g, 0, sums, maxs);
     if (threadIdx.x == 0) {
@@ -1755,8 +1985,24 @@ namespace {
 // Common end of batch creation: instance table upload, state and stream,
 // then the setup kernels (per-instance scaling, power iteration, sigma0,
 // iteration-0 residuals). Deletes the batch on failure.
+// HPRQP_BATCH_TRACE=1 reports the stages of batch creation on stderr
+// (diagnostics): wall time since the previous mark.
+struct StageClock {
+    bool on = std::getenv("HPRQP_BATCH_TRACE") != nullptr;
+    std::chrono::steady_clock::time_point t = std::chrono::steady_clock::now();
+    void mark(const char *what)
+    {
+        if (!on) return;
+        const auto now = std::chrono::steady_clock::now();
+        std::fprintf(stderr, "hprqp batch: %-28s %9.3f ms\n", what,
+                     std::chrono::duration<double, std::milli>(now - t).count());
+        t = now;
+    }
+};
+
 HPRQPStatus batch_finish(HPRQPHipBatch *b)
 {
+    StageClock clock;
     using namespace hprqp::batch;
     const size_t count = b->count;
     // largest instances first, so the short ones fill the tail
@@ -1798,8 +2044,21 @@ HPRQPStatus batch_finish(HPRQPHipBatch *b)
         }
         ar.gram = b->d_gram;
     }
-    if (b->any_sparse)
-        batch_setup_kernel<false><<<(unsigned)count, hprqp::kThreads, 0, b->stream>>>(ar, b->P, (int)count, 0);
+    clock.mark("device state and order");
+    if (b->any_sparse) {
+        bool fits = true;
+        for (size_t k = 0; k < count; ++k)
+            if (!b->inst[k].dense && (b->inst[k].n > kPowMax || b->inst[k].m > kPowMax)) fits = false;
+        if (fits) {
+            batch_setup_kernel<false><<<(unsigned)count, hprqp::kThreads, 0, b->stream>>>(ar, b->P, (int)count, 1);
+            if (clock.on && hipStreamSynchronize(b->stream) == hipSuccess) clock.mark("setup: scaling (CSR)");
+            batch_power_kernel<<<(unsigned)count, kPowThreads, 0, b->stream>>>(ar, (int)count);
+            if (clock.on && hipStreamSynchronize(b->stream) == hipSuccess) clock.mark("setup: power iterations");
+            batch_setup_kernel<false><<<(unsigned)count, hprqp::kThreads, 0, b->stream>>>(ar, b->P, (int)count, 3);
+        } else {
+            batch_setup_kernel<false><<<(unsigned)count, hprqp::kThreads, 0, b->stream>>>(ar, b->P, (int)count, 0);
+        }
+    }
     if (b->any_dense) {
         batch_setup_kernel<true><<<(unsigned)count, hprqp::kThreads, 0, b->stream>>>(ar, b->P, (int)count, 1);
         dense_gram_kernel<<<(unsigned)count, hprqp::kThreads, 0, b->stream>>>(ar, (int)count);
@@ -1809,6 +2068,7 @@ HPRQPStatus batch_finish(HPRQPHipBatch *b)
         delete b;
         return HPRQP_STATUS_NOT_IMPLEMENTED;
     }
+    clock.mark("setup: residuals (all)");
     if (b->any_sparse) {
         // Wave threshold for the CSR engine: HPRQP_BATCH_WAVE_ROW, or the
         // fastest of a few candidates on a timed probe of one iteration's
@@ -1857,6 +2117,7 @@ HPRQPStatus batch_finish(HPRQPHipBatch *b)
             delete b;
             return HPRQP_STATUS_NOT_IMPLEMENTED;
         }
+        clock.mark("wave threshold probes");
     }
     return HPRQP_STATUS_OK;
 }
@@ -1874,9 +2135,11 @@ HPRQPStatus hprqp_hip_batch_create(const HPRQPModel *models, size_t count, const
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return HPRQP_STATUS_NOT_IMPLEMENTED;
     if (params->device_number < 0 || params->device_number >= ndev) return HPRQP_STATUS_INVALID_ARGUMENT;
+    StageClock clock;
     HPRQP_HIP(hipSetDevice(params->device_number));
     auto *b = new (std::nothrow) HPRQPHipBatch();
     if (!b) return HPRQP_STATUS_ALLOCATION_FAILED;
+    clock.mark("device selection");
     b->count = count;
     b->time_limit = params->time_limit;
     b->P = Params{params->stoptol, (long long)std::min(params->max_iter, (size_t)INT64_MAX),
@@ -1957,6 +2220,7 @@ HPRQPStatus hprqp_hip_batch_create(const HPRQPModel *models, size_t count, const
     // placeholders of empty arrays (one entry each) are written here
     ci[0] = perm[0] = 0;
     val[0] = dval[0] = in_m[0] = in_m[1] = 0.0;
+    clock.mark("layout and staging");
     auto pack = [&](size_t k) {
         const HPRQPModel &md = models[k];
         Inst &I = b->inst[k];
@@ -2038,7 +2302,9 @@ HPRQPStatus hprqp_hip_batch_create(const HPRQPModel *models, size_t count, const
         b->m.push_back(models[k].A.nrow);
         b->work.push_back((double)(models[k].Q.nnz + 2 * models[k].A.nnz + models[k].Q.ncol + models[k].A.nrow));
     }
+    clock.mark("packing (threads)");
     std::vector<int> blocks = plan_all_blocks(b, rp.data(), kWaveRowDefault);
+    clock.mark("row-block plan");
     HPRQPStatus st = HPRQP_STATUS_OK;
     auto upload = [&](auto *&dst, const auto *src, size_t count) -> bool {
         using T = typename std::remove_const<typename std::remove_pointer<decltype(src)>::type>::type;
@@ -2080,6 +2346,7 @@ HPRQPStatus hprqp_hip_batch_create(const HPRQPModel *models, size_t count, const
     b->ar.at_ci_base = at_base;
     b->ar.a_base = a_base;
     b->ar.qt_perm_base = sum_a;
+    clock.mark("upload");
     st = batch_finish(b);
     if (st == HPRQP_STATUS_OK) *out = b;
     return st;
