@@ -81,6 +81,37 @@ __device__ inline double wave_sum(double v)
     return v;
 }
 
+// The same sum through DPP lane moves, which stay in the vector ALU instead
+// of the LDS crossbar that __shfl_down uses: pairs, quads, then rotations
+// within each row of 16 lanes leave the row's sum in all of its lanes; the
+// four rows are then read out and added, (r0 + r1) + (r2 + r3). The result
+// is uniform over the wavefront. Every lane must be active.
+template <int CTRL>
+__device__ inline double dpp_move(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+__device__ inline double read_lane(double v, int lane)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, lane);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+__device__ inline double wave_sum_dpp(double v)
+{
+    v += dpp_move<0xb1>(v);   // quad_perm [1, 0, 3, 2]
+    v += dpp_move<0x4e>(v);   // quad_perm [2, 3, 0, 1]
+    v += dpp_move<0x124>(v);  // row_ror:4
+    v += dpp_move<0x128>(v);  // row_ror:8
+    return (read_lane(v, 0) + read_lane(v, 16)) + (read_lane(v, 32) + read_lane(v, 48));
+}
+
 __device__ inline double wave_max(double v)
 {
 #pragma unroll

@@ -1343,26 +1343,53 @@ __global__ __launch_bounds__(kThreads) void batch_adjoint_kernel(Arrays ar, Para
 }
 
 // ------------------------------------------------ power iterations (CSR)
-// eigs.c:19-153 for CSR instances with n, m <= kPowMax, one 1024-thread
-// workgroup per instance, vectors in LDS. When a matrix has at most
-// kPowCache entries per thread and its products fit in LDS, each thread
-// holds entries e = t + 1024 u in registers for the whole iteration, so a
-// power step touches no memory: the products val * x go to an LDS array in
-// the order of the target CSR rows (A' order for A' q, A order for A tmp)
-// and each row is then one thread's left-to-right sum over it -- the
-// rounding of the host's CSR product without a fused multiply-add. Larger
-// matrices take the thread-per-row product from memory.
-constexpr int kPowThreads = 1024;
+// eigs.c:19-153 for CSR instances with n, m <= kPowMax, one 512-thread
+// workgroup per instance (256 VGPRs a thread), vectors in LDS. When a matrix
+// has at most kPowCache entries per thread, thread t holds the contiguous
+// run of entries [t C, (t + 1) C) (C = ceil(nnz / 512), row-major order) in
+// registers for the whole iteration, so a power step reads no memory: each
+// thread sums the rows of its run left to right (product, then add: the
+// rounding of the host's CSR product without a fused multiply-add) and
+// writes the rows that lie wholly in its run; a row split over several runs
+// is finished by the thread where it starts, adding the pieces the later
+// runs left in LDS in order. Larger matrices take the thread-per-row
+// product from memory.
+constexpr int kPowThreads = 512;
 constexpr int kPowMax = 4096;    // longest vector of an instance the kernel takes
-constexpr int kPowSlots = kPowMax / kPowThreads;  // rows a thread owns: t + 1024 s
-constexpr int kPowCache = 16;    // matrix entries per thread held in registers
+constexpr int kPowSlots = kPowMax / kPowThreads;  // rows a thread owns: t + kPowThreads s
+constexpr int kPowCache = 24;    // matrix entries per thread held in registers
+constexpr int kPowGather = 8;    // gathers a thread issues together
+static_assert(kPowCache % kPowGather == 0, "runs are walked in whole gather groups");
 constexpr int kPowRed = 4 * (kPowThreads / kWave);
-constexpr int kPowLds = 20480 - kPowRed;  // doubles for vectors and products: 160 KiB in all
+// doubles of LDS: q, z and A' q (n, m <= kPowMax), the run pieces and two
+// sets of mid flags
+constexpr int kPowLds = 3 * kPowMax + 2 * kPowThreads;
+
+// Both workgroup sums of (a, b) in one pass; slots 2 * slot and 2 * slot + 1.
+__device__ double2 pow_block_sum2(double a, double b, double *red, int slot)
+{
+    constexpr int W = kPowThreads / kWave;
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    a = wave_sum_dpp(a);
+    b = wave_sum_dpp(b);
+    if (lane == 0) {
+        red[2 * slot * W + wid] = a;
+        red[(2 * slot + 1) * W + wid] = b;
+    }
+    __syncthreads();
+    double2 t{red[2 * slot * W], red[(2 * slot + 1) * W]};
+#pragma unroll
+    for (int w = 1; w < W; ++w) {
+        t.x += red[2 * slot * W + w];
+        t.y += red[(2 * slot + 1) * W + w];
+    }
+    return t;
+}
 
 __device__ double pow_block_sum(double v, double *red, int slot)
 {
     const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
-    v = wave_sum(v);
+    v = wave_sum_dpp(v);
     if (lane == 0) red[slot * (kPowThreads / kWave) + wid] = v;
     __syncthreads();
     double t = red[slot * (kPowThreads / kWave)];
@@ -1395,21 +1422,124 @@ __device__ void pow_spmv(const Mat &M, int nrow, const double *xs, const F &f)
     }
 }
 
-// f(r, x[idx(lo)] + x[idx(lo + 1)] + ... left to right) for the rows of a
-// CSR view owned by this thread, row pointers rpl in LDS.
-template <class Idx, class F>
-__device__ void pow_row_sums(const int *rpl, const double *x, int nrow, const Idx &idx, const F &f)
-{
-    for (int r = threadIdx.x; r < nrow; r += kPowThreads) {
-        const int k1 = rpl[r + 1];
-        double acc = 0.0;
-        for (int k = rpl[r]; k < k1; ++k) acc += x[idx(k)];
-        f(r, acc);
-    }
-}
+// One thread's run of a CSR view (R rows, Z entries), held in registers:
+// entries [t C, t C + cnt) with cnt = min(C, Z - t C).
+struct PowRun {
+    double v[kPowCache];
+    int rc[kPowCache];  // row | col << 16 (R, columns <= 4096)
+    // first row | last row << 12 | head << 24 | tail << 25: rows of the
+    // first and last held entry; head: the first row began in an earlier
+    // run; tail: the last row goes on past the run
+    unsigned meta;
+    unsigned empty;  // bit s: row t + kPowThreads s has no entries
 
-// The power iteration of wg_power over vectors in LDS; prod(epi) forms
-// M q (q complete in LDS) and calls epi(r, (M q)_r) once for every r < len.
+    __device__ int first() const { return meta & 0xfff; }
+    __device__ int last() const { return (meta >> 12) & 0xfff; }
+    __device__ bool head() const { return meta >> 24 & 1u; }
+    __device__ bool tail() const { return meta >> 25 & 1u; }
+    __device__ static int count(int Z, int C)
+    {
+        const int left = Z - (int)threadIdx.x * C;
+        return left < C ? (left > 0 ? left : 0) : C;
+    }
+
+    // mid[t] (LDS): the run lies inside one row that goes on past it.
+    __device__ void load(const Mat &M, int R, int Z, int C, int *mid)
+    {
+        const int t = threadIdx.x, lo = t * C, cnt = count(Z, C);
+        unsigned empty = 0;
+#pragma unroll
+        for (int s = 0; s < kPowSlots; ++s) {
+            const int r = t + s * kPowThreads;
+            if (r < R && M.rp[r] == M.rp[r + 1]) empty |= 1u << s;
+        }
+        int row = 0;
+        if (cnt > 0) {
+            int hi = R;  // M.rp[row] <= lo < M.rp[hi]
+            while (hi - row > 1) {
+                const int mid_r = (row + hi) >> 1;
+                if (M.rp[mid_r] <= lo) row = mid_r;
+                else hi = mid_r;
+            }
+        }
+        const int first_row = row;
+#pragma unroll
+        for (int u = 0; u < kPowCache; ++u) {
+            v[u] = 0.0;
+            rc[u] = 0;
+            if (u < cnt) {
+                const int e = lo + u;
+                while (M.rp[row + 1] <= e) ++row;
+                v[u] = M.val[e];
+                rc[u] = row | (M.ci[e] << 16);
+            }
+        }
+        const bool head = cnt > 0 && M.rp[first_row] < lo, tail = cnt > 0 && M.rp[row + 1] > lo + cnt;
+        meta = (unsigned)first_row | (unsigned)row << 12 | (unsigned)head << 24 | (unsigned)tail << 25;
+        this->empty = empty;
+        mid[t] = head && tail && first_row == row;
+    }
+
+    // y = M x for the rows this run finishes; pieces of split rows go to
+    // H[t] (a row begun in an earlier run) or `own` (the row this run
+    // begins). combine() finishes the split rows after a barrier.
+    __device__ void run(const double *__restrict__ x, double *__restrict__ y, double *__restrict__ H, double &own,
+                        int Z, int C) const
+    {
+        const int t = threadIdx.x, cnt = count(Z, C);
+#pragma unroll
+        for (int s = 0; s < kPowSlots; ++s)
+            if (empty >> s & 1u) y[t + s * kPowThreads] = 0.0;
+        if (cnt == 0) return;
+        double acc = 0.0;
+        int cur = first();
+#pragma unroll
+        for (int u0 = 0; u0 < kPowCache; u0 += kPowGather) {
+            // the gathers of a group go out together (an entry past the run
+            // reads x[0] and is not used); the empty asm keeps the compiler
+            // from holding every gather address in a register across steps
+            double xv[kPowGather];
+#pragma unroll
+            for (int j = 0; j < kPowGather; ++j) {
+                int rcu = rc[u0 + j];
+                asm volatile("" : "+v"(rcu));
+                xv[j] = x[u0 + j < cnt ? rcu >> 16 : 0];
+            }
+#pragma unroll
+            for (int j = 0; j < kPowGather; ++j) {
+                const int u = u0 + j;
+                if (u < cnt) {
+                    const int r = rc[u] & 0xffff;
+                    if (r != cur) {  // cur is complete; only the first row can be a split piece here
+                        if (head() && cur == first()) H[t] = acc;
+                        else y[cur] = acc;
+                        acc = 0.0;
+                        cur = r;
+                    }
+                    const double p = v[u] * xv[j];
+                    acc += p;
+                }
+            }
+        }
+        if (head() && cur == first()) H[t] = acc;
+        else if (tail()) own = acc;
+        else y[cur] = acc;
+    }
+
+    __device__ void combine(double *y, const double *H, const int *mid, double own, int Z, int C) const
+    {
+        if (count(Z, C) == 0 || !tail() || (head() && first() == last())) return;
+        double total = own;
+        for (int j = threadIdx.x + 1;; ++j) {
+            total += H[j];
+            if (!mid[j]) break;
+        }
+        y[last()] = total;
+    }
+};
+
+// The power iteration of wg_power over vectors in LDS; prod() writes z = M q
+// (q complete in LDS).
 template <class Prod>
 __device__ double pow_loop(int len, double *q, double *z, double *red, const Prod &prod)
 {
@@ -1430,21 +1560,24 @@ __device__ double pow_loop(int len, double *q, double *z, double *red, const Pro
         }
         for (int i = t; i < len; i += kPowThreads) q[i] = z[i] / qn;
         __syncthreads();
+        prod();
+        __syncthreads();
         double s0 = 0.0, s1 = 0.0;
-        prod([&](int r, double v) {
-            z[r] = v;
-            s0 += q[r] * v;
+        for (int i = t; i < len; i += kPowThreads) {
+            const double v = z[i];
+            s0 += q[i] * v;
             s1 += v * v;
-        });
-        lam = pow_block_sum(s0, red, 1);
-        const double zn = sqrt(pow_block_sum(s1, red, 2));
+        }
+        const double2 ss = pow_block_sum2(s0, s1, red, 1);  // slots 2 and 3
+        lam = ss.x;
+        const double zn = sqrt(ss.y);
         double r2 = 0.0;
         for (int i = t; i < len; i += kPowThreads) {
             const double r = z[i] - lam * q[i];
             q[i] = r;
             r2 += r * r;
         }
-        const double err = sqrt(pow_block_sum(r2, red, 3)) / (zn + fabs(lam) + 1e-16);
+        const double err = sqrt(pow_block_sum(r2, red, 1)) / (zn + fabs(lam) + 1e-16);
         __syncthreads();  // red, q, z and the products are rewritten by the next step
         if (err < 1e-4) break;
         qn = zn;
@@ -1452,92 +1585,57 @@ __device__ double pow_loop(int len, double *q, double *z, double *red, const Pro
     return lam < 0.0 ? 0.0 : lam;
 }
 
-// Doubles of LDS for n ints (rounded up).
-__device__ inline int pow_ints(int n) { return (n + 1) / 2; }
-
 // lambda_max(A A'): z = A (A' q) over vectors of length m.
-__device__ double pow_a(const Mat &A, const Mat &AT, const int *at_perm, int n, int m, double *lds, double *red)
+__device__ double pow_a(const Mat &A, const Mat &AT, int n, int m, double *lds, double *red)
 {
     if (m == 0) return 0.0;
-    const int t = threadIdx.x, Z = A.rp[m];
-    // cached layout: q | z = tmp (tmp lives from the A' q sums to the A tmp
-    // products, z from the row sums to the next q) | P | A' perm | A' rp | A rp
-    const int zt = n > m ? n : m;
-    if (Z > kPowThreads * kPowCache || m + zt + Z + pow_ints(Z) + pow_ints(n + 1) + pow_ints(m + 1) > kPowLds) {
-        double *q = lds, *z = lds + m, *tmp = lds + 2 * m;
-        return pow_loop(m, q, z, red, [&](auto &&epi) {
-            pow_spmv(AT, n, q, [&](int r, double v) { tmp[r] = v; });
+    const int Z = A.rp[m];
+    double *q = lds, *z = lds + m, *tmp = lds + 2 * m;
+    auto store = [](double *y) { return [y](int r, double v) { y[r] = v; }; };
+    if (Z > kPowThreads * kPowCache)
+        return pow_loop(m, q, z, red, [&] {
+            pow_spmv(AT, n, q, store(tmp));
             __syncthreads();
-            pow_spmv(A, m, tmp, epi);
+            pow_spmv(A, m, tmp, store(z));
         });
-    }
-    double *q = lds, *z = lds + m, *tmp = z, *P = z + zt;
-    int *perm = reinterpret_cast<int *>(P + Z);
-    int *at_rp = perm + 2 * pow_ints(Z), *a_rp = at_rp + 2 * pow_ints(n + 1);
-    for (int k = t; k < Z; k += kPowThreads) perm[k] = at_perm[k];
-    for (int r = t; r <= n; r += kPowThreads) at_rp[r] = AT.rp[r];
-    for (int r = t; r <= m; r += kPowThreads) a_rp[r] = A.rp[r];
-    // entry e = t + 1024 u of A: value and row | col << 16 (n, m <= 4096)
-    double v[kPowCache];
-    int rc[kPowCache];
-#pragma unroll
-    for (int u = 0; u < kPowCache; ++u) {
-        const int e = t + u * kPowThreads;
-        v[u] = 0.0;
-        rc[u] = 0;
-        if (e < Z) {
-            int lo = 0, hi = m;  // A.rp[lo] <= e < A.rp[hi]
-            while (hi - lo > 1) {
-                const int mid = (lo + hi) >> 1;
-                if (A.rp[mid] <= e) lo = mid;
-                else hi = mid;
-            }
-            v[u] = A.val[e];
-            rc[u] = lo | (A.ci[e] << 16);
-        }
-    }
+    // q | z | tmp | run pieces | mid flags of A' | mid flags of A
+    double *H = tmp + n;
+    int *mid_at = reinterpret_cast<int *>(H + kPowThreads), *mid_a = mid_at + kPowThreads;
+    const int C = (Z + kPowThreads - 1) / kPowThreads;
+    PowRun at, a;
+    at.load(AT, n, Z, C, mid_at);
+    a.load(A, m, Z, C, mid_a);
     __syncthreads();
-    return pow_loop(m, q, z, red, [&](auto &&epi) {
-        // A' q: products in A order, each A' row (a column of A) summed in
-        // A' order through the permutation
-#pragma unroll
-        for (int u = 0; u < kPowCache; ++u)
-            if (t + u * kPowThreads < Z) P[t + u * kPowThreads] = v[u] * q[rc[u] & 0xffff];
+    return pow_loop(m, q, z, red, [&] {
+        double own = 0.0;
+        at.run(q, tmp, H, own, Z, C);
         __syncthreads();
-        pow_row_sums(at_rp, P, n, [&](int k) { return perm[k]; }, [&](int j, double sj) { tmp[j] = sj; });
+        at.combine(tmp, H, mid_at, own, Z, C);
         __syncthreads();
-#pragma unroll
-        for (int u = 0; u < kPowCache; ++u)
-            if (t + u * kPowThreads < Z) P[t + u * kPowThreads] = v[u] * tmp[rc[u] >> 16];
+        a.run(tmp, z, H, own, Z, C);
         __syncthreads();
-        pow_row_sums(a_rp, P, m, [](int k) { return k; }, epi);
+        a.combine(z, H, mid_a, own, Z, C);
     });
 }
 
 // lambda_max(Q): z = Q q over vectors of length n.
 __device__ double pow_q(const Mat &Q, int n, double *lds, double *red)
 {
-    const int t = threadIdx.x, Z = Q.rp[n];
-    double *q = lds, *z = lds + n, *P = lds + 2 * n;
-    if (Z > kPowThreads * kPowCache || 2 * n + Z + pow_ints(n + 1) > kPowLds)
-        return pow_loop(n, q, z, red, [&](auto &&epi) { pow_spmv(Q, n, q, epi); });
-    int *q_rp = reinterpret_cast<int *>(P + Z);
-    for (int r = t; r <= n; r += kPowThreads) q_rp[r] = Q.rp[r];
-    double v[kPowCache];
-    int c[kPowCache];
-#pragma unroll
-    for (int u = 0; u < kPowCache; ++u) {
-        const int e = t + u * kPowThreads;
-        v[u] = e < Z ? Q.val[e] : 0.0;
-        c[u] = e < Z ? Q.ci[e] : 0;
-    }
+    const int Z = Q.rp[n];
+    double *q = lds, *z = lds + n;
+    if (Z > kPowThreads * kPowCache)
+        return pow_loop(n, q, z, red, [&] { pow_spmv(Q, n, q, [z](int r, double v) { z[r] = v; }); });
+    double *H = z + n;
+    int *mid = reinterpret_cast<int *>(H + kPowThreads);
+    const int C = (Z + kPowThreads - 1) / kPowThreads;
+    PowRun run;
+    run.load(Q, n, Z, C, mid);
     __syncthreads();
-    return pow_loop(n, q, z, red, [&](auto &&epi) {
-#pragma unroll
-        for (int u = 0; u < kPowCache; ++u)
-            if (t + u * kPowThreads < Z) P[t + u * kPowThreads] = v[u] * q[c[u]];
+    return pow_loop(n, q, z, red, [&] {
+        double own = 0.0;
+        run.run(q, z, H, own, Z, C);
         __syncthreads();
-        pow_row_sums(q_rp, P, n, [](int k) { return k; }, epi);
+        run.combine(z, H, mid, own, Z, C);
     });
 }
 
@@ -1557,7 +1655,7 @@ __global__ __launch_bounds__(kPowThreads) void batch_power_kernel(Arrays ar, int
     const Mat AT{ar.rp + I->at_rp, ar.ci + ar.at_ci_base + I->a_nz, ar.val + ar.at_ci_base + I->a_nz, nullptr, 0,
                  nullptr, 0, 0, 0};
     const int diag = ar.state[k].Q_is_diag;
-    const double lam_a = pow_a(A, AT, ar.perm + I->a_nz, I->n, I->m, lds, red) * 1.05;
+    const double lam_a = pow_a(A, AT, I->n, I->m, lds, red) * 1.05;
     const double lam_q = diag ? 0.0 : pow_q(Q, I->n, lds, red) * 1.05;
     if (threadIdx.x == 0) {
         ar.state[k].lam_a = lam_a;
