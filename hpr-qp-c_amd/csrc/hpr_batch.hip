@@ -1944,6 +1944,10 @@ struct HPRQPHipBatch {
     double *d_gram = nullptr;  // G = A A^T of the dense instances (setup)
     mutable hipEvent_t caller_done = nullptr;  // last work queued on a caller's stream that reads this batch
     std::vector<int> h_rp;  // host copy of all CSR row pointers (block planning)
+    // CSR instances' A row and column offsets in the block-diagonal matrix of
+    // all of them (A' entries and the A', Q' permutations are formed on the
+    // device at setup); empty when the host formed them
+    std::vector<int2> tr_off;
     hprqp::batch::Inst *d_inst = nullptr;
     hprqp::batch::State *d_state = nullptr;
     std::vector<hprqp::batch::State> state;
@@ -1995,26 +1999,6 @@ std::vector<int> plan_batch_blocks(const int *rp, int nrow, int wave_row)
     starts.push_back(nrow);
     if (nrow == 0) starts = {0, 0};
     return starts;
-}
-
-// Host counting-sort transpose permutation (column-major order, rows
-// ascending within a column): perm[k] = source entry of output entry k.
-void transpose_perm(size_t nrow, size_t ncol, const size_t *rp, const size_t *ci, std::vector<int> &trp,
-                    std::vector<int> &tci, std::vector<int> &perm)
-{
-    const size_t nnz = rp[nrow];
-    trp.assign(ncol + 1, 0);
-    tci.resize(nnz);
-    perm.resize(nnz);
-    for (size_t k = 0; k < nnz; ++k) trp[ci[k] + 1] += 1;
-    for (size_t j = 0; j < ncol; ++j) trp[j + 1] += trp[j];
-    std::vector<int> fill(trp.begin(), trp.end() - 1);
-    for (size_t r = 0; r < nrow; ++r)
-        for (size_t k = rp[r]; k < rp[r + 1]; ++k) {
-            const int dst = fill[ci[k]]++;
-            tci[dst] = (int)r;
-            perm[dst] = (int)k;
-        }
 }
 
 // Row blocks of every CSR instance (Q, A, A^T) for wave threshold
@@ -2080,9 +2064,88 @@ bool upload_plan(HPRQPHipBatch *b, const std::vector<int> &blocks)
 
 namespace {
 
-// Common end of batch creation: instance table upload, state and stream,
-// then the setup kernels (per-instance scaling, power iteration, sigma0,
-// iteration-0 residuals). Deletes the batch on failure.
+// ---------------------------------------------------- device transposes
+// The A' column indices and the A' -> A and Q' -> Q entry permutations of
+// every CSR instance (the host counting-sort transpose: columns ascending,
+// rows ascending within a column) from one stable device transpose of the
+// block-diagonal matrix of all CSR instances: an instance's columns are one
+// contiguous range of it, so its transposed entries land at its own entry
+// offsets and only the row and entry numbering needs making local again.
+
+// which = 0: A (rows m, columns n), 1: Q (rows n, columns n). off[k] =
+// (row offset of A, column offset), among the CSR instances only.
+__global__ __launch_bounds__(hprqp::kThreads) void batch_global_csr_kernel(hprqp::batch::Arrays ar, const int2 *off, int which,
+                                                                          int *grp, int *gci, int total_rows,
+                                                                          int total_nnz)
+{
+    const int k = blockIdx.x;
+    if (k == 0 && threadIdx.x == 0) grp[total_rows] = total_nnz;
+    const hprqp::batch::Inst I = ar.inst[k];
+    if (I.dense) return;
+    const int rows = which ? I.n : I.m;
+    const int *rp = ar.rp + (which ? I.q_rp : I.a_rp);
+    const long long nz = which ? I.q_nz : I.a_nz;
+    const int *ci = ar.ci + (which ? 0 : ar.a_base) + nz;
+    const int row_off = which ? off[k].y : off[k].x, col_off = off[k].y;
+    for (int i = threadIdx.x; i < rows; i += hprqp::kThreads) grp[row_off + i] = (int)nz + rp[i];
+    const int cnt = rp[rows];
+    for (int e = threadIdx.x; e < cnt; e += hprqp::kThreads) gci[nz + e] = col_off + ci[e];
+}
+
+__global__ __launch_bounds__(hprqp::kThreads) void batch_localize_kernel(hprqp::batch::Arrays ar, const int2 *off, int which)
+{
+    const int k = blockIdx.x;
+    const hprqp::batch::Inst I = ar.inst[k];
+    if (I.dense) return;
+    if (which == 0) {
+        const int cnt = ar.rp[I.a_rp + I.m];
+        int *tci = ar.ci + ar.at_ci_base + I.a_nz, *perm = ar.perm + I.a_nz;
+        for (int e = threadIdx.x; e < cnt; e += hprqp::kThreads) {
+            tci[e] -= off[k].x;
+            perm[e] -= (int)I.a_nz;
+        }
+    } else {
+        const int cnt = ar.rp[I.q_rp + I.n];
+        int *perm = ar.perm + ar.qt_perm_base + I.q_nz;
+        for (int e = threadIdx.x; e < cnt; e += hprqp::kThreads) perm[e] -= (int)I.q_nz;
+    }
+}
+
+HPRQPStatus batch_device_transposes(HPRQPHipBatch *b)
+{
+    using namespace hprqp::batch;
+    const int count = (int)b->count;
+    long long rows_a = 0, cols = 0;
+    for (int k = 0; k < count; ++k)
+        if (!b->inst[k].dense) {
+            rows_a += b->inst[k].m;
+            cols += b->inst[k].n;
+        }
+    const long long nnz_a = b->ar.at_ci_base - b->ar.a_base, nnz_q = b->ar.a_base;
+    hprqp::DeviceBuffer<int2> off;
+    hprqp::DeviceBuffer<int> grp, gci, trp;
+    HPRQP_HIP(off.alloc((size_t)count));
+    HPRQP_HIP(hipMemcpyAsync(off.p, b->tr_off.data(), sizeof(int2) * count, hipMemcpyHostToDevice, b->stream));
+    HPRQP_HIP(grp.alloc((size_t)std::max(rows_a, cols) + 1));
+    HPRQP_HIP(gci.alloc((size_t)std::max(nnz_a, nnz_q)));
+    HPRQP_HIP(trp.alloc((size_t)cols + 1));
+    for (int which = 0; which < 2; ++which) {
+        const long long rows = which ? cols : rows_a, nnz = which ? nnz_q : nnz_a;
+        batch_global_csr_kernel<<<count, hprqp::kThreads, 0, b->stream>>>(b->ar, off.p, which, grp.p, gci.p, (int)rows,
+                                                                          (int)nnz);
+        HPRQP_HIP(hipGetLastError());
+        int *tci = which ? nullptr : b->ar.ci + b->ar.at_ci_base;
+        int *perm = which ? b->ar.perm + b->ar.qt_perm_base : b->ar.perm;
+        const HPRQPStatus st = hprqp_hip_csr_transpose((int)rows, (int)cols, (int)nnz, grp.p, gci.p, nullptr, trp.p,
+                                                       tci, nullptr, perm, b->stream);
+        if (st != HPRQP_STATUS_OK) return st;
+        batch_localize_kernel<<<count, hprqp::kThreads, 0, b->stream>>>(b->ar, off.p, which);
+        HPRQP_HIP(hipGetLastError());
+    }
+    HPRQP_HIP(hipStreamSynchronize(b->stream));
+    return HPRQP_STATUS_OK;
+}
+
 // HPRQP_BATCH_TRACE=1 reports the stages of batch creation on stderr
 // (diagnostics): wall time since the previous mark.
 struct StageClock {
@@ -2098,6 +2161,9 @@ struct StageClock {
     }
 };
 
+// Common end of batch creation: instance table upload, state and stream,
+// then the setup kernels (per-instance scaling, power iteration, sigma0,
+// iteration-0 residuals). Deletes the batch on failure.
 HPRQPStatus batch_finish(HPRQPHipBatch *b)
 {
     StageClock clock;
@@ -2143,6 +2209,14 @@ HPRQPStatus batch_finish(HPRQPHipBatch *b)
         ar.gram = b->d_gram;
     }
     clock.mark("device state and order");
+    if (!b->tr_off.empty()) {
+        const HPRQPStatus tst = batch_device_transposes(b);
+        if (tst != HPRQP_STATUS_OK) {
+            delete b;
+            return tst;
+        }
+        clock.mark("transposes (device)");
+    }
     if (b->any_sparse) {
         bool fits = true;
         for (size_t k = 0; k < count; ++k)
@@ -2302,21 +2376,22 @@ HPRQPStatus hprqp_hip_batch_create(const HPRQPModel *models, size_t count, const
     // cleared on the device; only the five input vectors travel.
     const size_t n_rp = (size_t)std::max(sum_rp, 1LL), n_nz = (size_t)std::max(sum_q + 2 * sum_a, 1LL);
     const size_t n_perm = (size_t)std::max(sum_a + sum_q, 1LL), n_d = (size_t)std::max(sum_d, 1LL);
+    const size_t n_host = (size_t)std::max(at_base, 1LL);  // Q and A entries; A' is formed on the device
     const size_t sm1 = (size_t)std::max(sum_m, 1LL);
-    std::unique_ptr<int[]> ci(new (std::nothrow) int[n_nz]), perm(new (std::nothrow) int[n_perm]);
-    std::unique_ptr<double[]> val(new (std::nothrow) double[n_nz]), dval(new (std::nothrow) double[n_d]);
+    std::unique_ptr<int[]> ci(new (std::nothrow) int[n_host]);
+    std::unique_ptr<double[]> val(new (std::nothrow) double[n_host]), dval(new (std::nothrow) double[n_d]);
     std::unique_ptr<double[]> in_n(new (std::nothrow) double[(size_t)sum_n * 3]), in_m(new (std::nothrow) double[sm1 * 2]);
     std::vector<int> rp;
     try {
         rp.resize(n_rp);
     } catch (const std::bad_alloc &) {
     }
-    if (!ci || !perm || !val || !dval || !in_n || !in_m || rp.size() != n_rp) {
+    if (!ci || !val || !dval || !in_n || !in_m || rp.size() != n_rp) {
         delete b;
         return HPRQP_STATUS_ALLOCATION_FAILED;
     }
     // placeholders of empty arrays (one entry each) are written here
-    ci[0] = perm[0] = 0;
+    ci[0] = 0;
     val[0] = dval[0] = in_m[0] = in_m[1] = 0.0;
     clock.mark("layout and staging");
     auto pack = [&](size_t k) {
@@ -2350,29 +2425,32 @@ HPRQPStatus hprqp_hip_batch_create(const HPRQPModel *models, size_t count, const
             for (size_t r = 0; r < m; ++r)
                 for (size_t e = md.A.row_ptr[r]; e < md.A.row_ptr[r + 1]; ++e) Ad[r * n + md.A.col_ind[e]] += md.A.values[e];
         } else {
-            std::vector<int> qrp(n + 1), arp(m + 1), trp, tci, tperm;
+            // Q and A as they are; A' and Q' row pointers by column counts
+            // (their entries and permutations are formed on the device)
+            int *qrp = rp.data() + I.q_rp, *arp = rp.data() + I.a_rp;
+            int *atrp = rp.data() + I.at_rp, *qtrp = rp.data() + I.qt_rp;
             for (size_t r = 0; r <= n; ++r) qrp[r] = (int)md.Q.row_ptr[r];
             for (size_t r = 0; r <= m; ++r) arp[r] = (int)md.A.row_ptr[r];
-            std::copy(qrp.begin(), qrp.end(), rp.begin() + I.q_rp);
-            std::copy(arp.begin(), arp.end(), rp.begin() + I.a_rp);
-            std::fill(val.get() + at_base + I.a_nz, val.get() + at_base + I.a_nz + md.A.nnz, 0.0);  // A^T: on the device
+            std::fill(atrp, atrp + n + 1, 0);
+            std::fill(qtrp, qtrp + n + 1, 0);
+            int *qci = ci.get() + I.q_nz, *aci = ci.get() + a_base + I.a_nz;
+            double *qv = val.get() + I.q_nz, *av = val.get() + a_base + I.a_nz;
             for (size_t e = 0; e < md.Q.nnz; ++e) {
-                ci[I.q_nz + e] = (int)md.Q.col_ind[e];
-                val[I.q_nz + e] = md.Q.values[e];
+                const int c = (int)md.Q.col_ind[e];
+                qci[e] = c;
+                qv[e] = md.Q.values[e];
+                ++qtrp[c + 1];
             }
             for (size_t e = 0; e < md.A.nnz; ++e) {
-                ci[a_base + I.a_nz + e] = (int)md.A.col_ind[e];
-                val[a_base + I.a_nz + e] = md.A.values[e];
+                const int c = (int)md.A.col_ind[e];
+                aci[e] = c;
+                av[e] = md.A.values[e];
+                ++atrp[c + 1];
             }
-            transpose_perm(m, n, md.A.row_ptr, md.A.col_ind, trp, tci, tperm);
-            std::copy(trp.begin(), trp.end(), rp.begin() + I.at_rp);
-            for (size_t e = 0; e < md.A.nnz; ++e) {
-                ci[at_base + I.a_nz + e] = tci[e];
-                perm[I.a_nz + e] = tperm[e];
+            for (size_t j = 0; j < n; ++j) {
+                atrp[j + 1] += atrp[j];
+                qtrp[j + 1] += qtrp[j];
             }
-            transpose_perm(n, n, md.Q.row_ptr, md.Q.col_ind, trp, tci, tperm);
-            std::copy(trp.begin(), trp.end(), rp.begin() + I.qt_rp);
-            for (size_t e = 0; e < md.Q.nnz; ++e) perm[sum_a + I.q_nz + e] = tperm[e];
         }
         for (size_t i = 0; i < n; ++i) {
             in_n[I.vn + i] = md.c[i];
@@ -2420,8 +2498,16 @@ HPRQPStatus hprqp_hip_batch_create(const HPRQPModel *models, size_t count, const
                 return false;
         return true;
     };
-    if (!upload(b->d_rp, rp.data(), n_rp) || !upload(b->d_ci, ci.get(), n_nz) || !upload(b->d_val, val.get(), n_nz) ||
-        !upload(b->d_perm, perm.get(), n_perm) || !upload(b->d_blocks, blocks.data(), blocks.size()) ||
+    // ci and val hold Q, A, then A' (formed on the device); perm is formed on the device
+    auto upload_part = [&](auto *&dst, const auto *src, size_t count, size_t part) -> bool {
+        using T = typename std::remove_const<typename std::remove_pointer<decltype(src)>::type>::type;
+        if (hipMalloc((void **)&dst, sizeof(T) * count) != hipSuccess) return false;
+        return hipMemcpy(dst, src, sizeof(T) * part, hipMemcpyHostToDevice) == hipSuccess;
+    };
+    if (!upload(b->d_rp, rp.data(), n_rp) || !upload_part(b->d_ci, ci.get(), n_nz, n_host) ||
+        !upload_part(b->d_val, val.get(), n_nz, n_host) ||
+        hipMalloc((void **)&b->d_perm, sizeof(int) * n_perm) != hipSuccess ||
+        !upload(b->d_blocks, blocks.data(), blocks.size()) ||
         !vectors(b->d_vn, (size_t)sum_n, N_COUNT, in_n.get(), {N_c, N_l, N_u}) ||
         !vectors(b->d_vm, sm1, M_COUNT, in_m.get(), {M_AL, M_AU}) || !upload(b->d_dval, dval.get(), n_d))
         st = HPRQP_STATUS_ALLOCATION_FAILED;
@@ -2444,6 +2530,14 @@ HPRQPStatus hprqp_hip_batch_create(const HPRQPModel *models, size_t count, const
     b->ar.at_ci_base = at_base;
     b->ar.a_base = a_base;
     b->ar.qt_perm_base = sum_a;
+    b->tr_off.resize(count);
+    for (size_t k = 0, ra = 0, cn = 0; k < count; ++k) {
+        b->tr_off[k] = make_int2((int)ra, (int)cn);
+        if (!b->inst[k].dense) {
+            ra += b->inst[k].m;
+            cn += b->inst[k].n;
+        }
+    }
     clock.mark("upload");
     st = batch_finish(b);
     if (st == HPRQP_STATUS_OK) *out = b;
