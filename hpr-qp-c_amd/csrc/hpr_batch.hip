@@ -19,6 +19,8 @@
 #include <cstring>
 #include <memory>
 #include <thread>
+
+#include <sys/mman.h>
 #include <vector>
 
 #include "device_common.h"
@@ -1924,6 +1926,25 @@ __global__ __launch_bounds__(kThreads) void batch_unscale_kernel(const double *_
 
 // ================================================================ host side
 
+// Host staging arrays of batch creation: 2 MiB aligned and marked for
+// transparent huge pages, so that their first touches fault once per 2 MiB
+// instead of once per 4 KiB page. Not zero-filled.
+struct HostFree {
+    void operator()(void *p) const { std::free(p); }
+};
+template <class T>
+using HostArray = std::unique_ptr<T[], HostFree>;
+
+template <class T>
+HostArray<T> host_array(size_t count)
+{
+    constexpr size_t kHuge = size_t(2) << 20;
+    const size_t bytes = (sizeof(T) * std::max<size_t>(count, 1) + kHuge - 1) / kHuge * kHuge;
+    void *p = std::aligned_alloc(kHuge, bytes);
+    if (p) (void)madvise(p, bytes, MADV_HUGEPAGE);
+    return HostArray<T>(static_cast<T *>(p));
+}
+
 struct HPRQPHipBatch {
     size_t count = 0;
     bool any_dense = false, any_sparse = false;  // which kernel variants have work
@@ -1943,11 +1964,12 @@ struct HPRQPHipBatch {
     double *d_adj_resid = nullptr;  // adjoint final relative residuals of the last adjoint
     double *d_gram = nullptr;  // G = A A^T of the dense instances (setup)
     mutable hipEvent_t caller_done = nullptr;  // last work queued on a caller's stream that reads this batch
-    std::vector<int> h_rp;  // host copy of all CSR row pointers (block planning)
+    HostArray<int> h_rp;  // host copy of all CSR row pointers (block planning)
     // CSR instances' A row and column offsets in the block-diagonal matrix of
     // all of them (A' entries and the A', Q' permutations are formed on the
     // device at setup); empty when the host formed them
     std::vector<int2> tr_off;
+    std::vector<int> max_row;  // longest CSR row of each instance (Q, A and A'); empty: unknown
     hprqp::batch::Inst *d_inst = nullptr;
     hprqp::batch::State *d_state = nullptr;
     std::vector<hprqp::batch::State> state;
@@ -2247,6 +2269,15 @@ HPRQPStatus batch_finish(HPRQPHipBatch *b)
         // products (the plan changes only the order of long-row sums).
         const char *force = std::getenv("HPRQP_BATCH_WAVE_ROW");
         std::vector<int> cands = {16, 32, 64, 128, 1 << 20};
+        if (!b->max_row.empty()) {
+            // thresholds above every row length give one and the same plan
+            const int longest = *std::max_element(b->max_row.begin(), b->max_row.end());
+            std::vector<int> kept;
+            for (int T : cands)
+                if (T <= longest) kept.push_back(T);
+            kept.push_back(1 << 20);
+            cands = kept;
+        }
         if (force && std::atoi(force) > 0) cands = {std::atoi(force)};
         int best = kWaveRowDefault;
         float best_ms = 1e30f;
@@ -2256,7 +2287,7 @@ HPRQPStatus batch_finish(HPRQPHipBatch *b)
         bool ok = hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess;
         for (size_t c = 0; ok && c < cands.size(); ++c) {
             const int T = cands[c];
-            std::vector<int> plan = plan_all_blocks(b, b->h_rp.data(), T);
+            std::vector<int> plan = plan_all_blocks(b, b->h_rp.get(), T);
             ok = upload_plan(b, plan);
             if (!ok || cands.size() == 1) {
                 best = T;
@@ -2378,21 +2409,17 @@ HPRQPStatus hprqp_hip_batch_create(const HPRQPModel *models, size_t count, const
     const size_t n_perm = (size_t)std::max(sum_a + sum_q, 1LL), n_d = (size_t)std::max(sum_d, 1LL);
     const size_t n_host = (size_t)std::max(at_base, 1LL);  // Q and A entries; A' is formed on the device
     const size_t sm1 = (size_t)std::max(sum_m, 1LL);
-    std::unique_ptr<int[]> ci(new (std::nothrow) int[n_host]);
-    std::unique_ptr<double[]> val(new (std::nothrow) double[n_host]), dval(new (std::nothrow) double[n_d]);
-    std::unique_ptr<double[]> in_n(new (std::nothrow) double[(size_t)sum_n * 3]), in_m(new (std::nothrow) double[sm1 * 2]);
-    std::vector<int> rp;
-    try {
-        rp.resize(n_rp);
-    } catch (const std::bad_alloc &) {
-    }
-    if (!ci || !val || !dval || !in_n || !in_m || rp.size() != n_rp) {
+    HostArray<int> ci = host_array<int>(n_host), rp = host_array<int>(n_rp);
+    HostArray<double> val = host_array<double>(n_host), dval = host_array<double>(n_d);
+    HostArray<double> in_n = host_array<double>((size_t)sum_n * 3), in_m = host_array<double>(sm1 * 2);
+    if (!ci || !val || !dval || !in_n || !in_m || !rp) {
         delete b;
         return HPRQP_STATUS_ALLOCATION_FAILED;
     }
     // placeholders of empty arrays (one entry each) are written here
-    ci[0] = 0;
+    ci[0] = rp[0] = 0;
     val[0] = dval[0] = in_m[0] = in_m[1] = 0.0;
+    b->max_row.assign(count, 0);
     clock.mark("layout and staging");
     auto pack = [&](size_t k) {
         const HPRQPModel &md = models[k];
@@ -2427,8 +2454,8 @@ HPRQPStatus hprqp_hip_batch_create(const HPRQPModel *models, size_t count, const
         } else {
             // Q and A as they are; A' and Q' row pointers by column counts
             // (their entries and permutations are formed on the device)
-            int *qrp = rp.data() + I.q_rp, *arp = rp.data() + I.a_rp;
-            int *atrp = rp.data() + I.at_rp, *qtrp = rp.data() + I.qt_rp;
+            int *qrp = rp.get() + I.q_rp, *arp = rp.get() + I.a_rp;
+            int *atrp = rp.get() + I.at_rp, *qtrp = rp.get() + I.qt_rp;
             for (size_t r = 0; r <= n; ++r) qrp[r] = (int)md.Q.row_ptr[r];
             for (size_t r = 0; r <= m; ++r) arp[r] = (int)md.A.row_ptr[r];
             std::fill(atrp, atrp + n + 1, 0);
@@ -2447,10 +2474,14 @@ HPRQPStatus hprqp_hip_batch_create(const HPRQPModel *models, size_t count, const
                 av[e] = md.A.values[e];
                 ++atrp[c + 1];
             }
+            int longest = 0;
             for (size_t j = 0; j < n; ++j) {
+                longest = std::max({longest, atrp[j + 1], qrp[j + 1] - qrp[j]});
                 atrp[j + 1] += atrp[j];
                 qtrp[j + 1] += qtrp[j];
             }
+            for (size_t r = 0; r < m; ++r) longest = std::max(longest, arp[r + 1] - arp[r]);
+            b->max_row[k] = longest;
         }
         for (size_t i = 0; i < n; ++i) {
             in_n[I.vn + i] = md.c[i];
@@ -2479,7 +2510,7 @@ HPRQPStatus hprqp_hip_batch_create(const HPRQPModel *models, size_t count, const
         b->work.push_back((double)(models[k].Q.nnz + 2 * models[k].A.nnz + models[k].Q.ncol + models[k].A.nrow));
     }
     clock.mark("packing (threads)");
-    std::vector<int> blocks = plan_all_blocks(b, rp.data(), kWaveRowDefault);
+    std::vector<int> blocks = plan_all_blocks(b, rp.get(), kWaveRowDefault);
     clock.mark("row-block plan");
     HPRQPStatus st = HPRQP_STATUS_OK;
     auto upload = [&](auto *&dst, const auto *src, size_t count) -> bool {
@@ -2504,7 +2535,7 @@ HPRQPStatus hprqp_hip_batch_create(const HPRQPModel *models, size_t count, const
         if (hipMalloc((void **)&dst, sizeof(T) * count) != hipSuccess) return false;
         return hipMemcpy(dst, src, sizeof(T) * part, hipMemcpyHostToDevice) == hipSuccess;
     };
-    if (!upload(b->d_rp, rp.data(), n_rp) || !upload_part(b->d_ci, ci.get(), n_nz, n_host) ||
+    if (!upload(b->d_rp, rp.get(), n_rp) || !upload_part(b->d_ci, ci.get(), n_nz, n_host) ||
         !upload_part(b->d_val, val.get(), n_nz, n_host) ||
         hipMalloc((void **)&b->d_perm, sizeof(int) * n_perm) != hipSuccess ||
         !upload(b->d_blocks, blocks.data(), blocks.size()) ||
