@@ -1190,14 +1190,20 @@ __device__ double adj_dot(WG &g, const double *an, const double *am, const doubl
     return v;
 }
 
-// y += a x over both parts
-template <class WG>
-__device__ void adj_axpy(WG &g, double a, const double *xn, const double *xm, double *yn, double *ym)
+// Sum over the 256-thread workgroup (DPP wavefront sums, then the four
+// wavefronts in order); valid in every thread. red: kThreads / kWave doubles
+// used by nothing else.
+__device__ double adj_block_sum(double v, double *red)
 {
-    const int n = g.I->n, m = g.I->m;
-    for (int i = threadIdx.x; i < n; i += kThreads) yn[i] += a * xn[i];
-    for (int i = threadIdx.x; i < m; i += kThreads) ym[i] += a * xm[i];
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    v = wave_sum_dpp(v);
+    if (lane == 0) red[wid] = v;
     __syncthreads();
+    double t = red[0];
+#pragma unroll
+    for (int w = 1; w < kThreads / kWave; ++w) t += red[w];
+    __syncthreads();  // red is rewritten by the next sum
+    return t;
 }
 
 template <bool D>
@@ -1211,6 +1217,7 @@ __global__ __launch_bounds__(kThreads) void batch_adjoint_kernel(Arrays ar, Para
 {
     __shared__ double prod[kStreamCap];
     __shared__ double red[kMaxTerms];
+    __shared__ double adj_red[kThreads / kWave];
     __shared__ State S;
     // dense instances: x_m staging of the one-pass operator (adj_op_dense)
     __shared__ double dense_lds[D ? kDenseMax : 1];
@@ -1243,50 +1250,85 @@ __global__ __launch_bounds__(kThreads) void batch_adjoint_kernel(Arrays ar, Para
     // recurrence, rotated); w, w1, w2 (update directions, rotated).
     double *xn = g.N(N_pq), *xm = g.M(M_pq);
     double *vn = g.N(N_pz), *vm = g.M(M_pz);
-    double *ln[3] = {g.N(N_dx), g.N(N_dw), g.N(N_tempv)};
-    double *lm[3] = {g.M(M_tr), g.M(M_dy), g.M(M_e0)};
-    double *wn[3] = {g.N(N_dQw), g.N(N_QATdy), g.N(N_Qx)};
-    double *wm[3] = {g.M(M_e1), g.M(M_e2), g.M(M_e3)};
-    int ir1 = 0, ir2 = 1, iy = 2;  // r1, r2, y in ln / lm
-    int iw = 0, iw1 = 1, iw2 = 2;  // w, w1, w2 in wn / wm
+    // rotated by swapping pointers (no indexed pointer arrays: those would
+    // live in scratch memory)
+    double *r1n = g.N(N_dx), *r2n = g.N(N_dw), *yn = g.N(N_tempv);
+    double *r1m = g.M(M_tr), *r2m = g.M(M_dy), *ym = g.M(M_e0);
+    double *wn = g.N(N_dQw), *w1n = g.N(N_QATdy), *w2n = g.N(N_Qx);
+    double *wm = g.M(M_e1), *w1m = g.M(M_e2), *w2m = g.M(M_e3);
     for (int i = t; i < n; i += kThreads) {
         const double b = mk.free_var(i) ? gk[i] / d[i] : 0.0;
         bn[i] = b;
         xn[i] = 0.0;
-        ln[ir1][i] = b;
-        ln[ir2][i] = b;
-        wn[iw][i] = 0.0;
-        wn[iw2][i] = 0.0;
+        r1n[i] = b;
+        r2n[i] = b;
+        wn[i] = 0.0;
+        w2n[i] = 0.0;
     }
     for (int i = t; i < m; i += kThreads) {
         xm[i] = 0.0;
-        lm[ir1][i] = 0.0;
-        lm[ir2][i] = 0.0;
-        wm[iw][i] = 0.0;
-        wm[iw2][i] = 0.0;
+        r1m[i] = 0.0;
+        r2m[i] = 0.0;
+        wm[i] = 0.0;
+        w2m[i] = 0.0;
     }
     __syncthreads();
-    const double beta1 = sqrt(adj_dot(g, bn, lm[ir1], bn, lm[ir1], sums, maxs));
+    const double beta1 = sqrt(adj_dot(g, bn, r1m, bn, r1m, sums, maxs));
     double oldb = 0.0, beta = beta1, dbar = 0.0, epsln = 0.0, phibar = beta1, cs = -1.0, sn = 0.0;
     int step = 0;
     if (beta1 > 0.0) {
-        for (step = 1; step <= max_steps; ++step) {
+        // Each thread owns the same indices in every elementwise pass, so a
+        // pass needs no barrier before the next one reads what it wrote;
+        // barriers stand only before the operator (it reads v at every
+        // index) and inside the two sums. A step is then: the operator,
+        // (y -= (beta/oldb) r1, alfa = v.y), (y -= (alfa/beta) r2,
+        // beta = |y|), (w and x updates, next v = y/beta).
+        {
             const double sc = 1.0 / beta;
-            for (int i = t; i < n; i += kThreads) vn[i] = sc * ln[ir2][i];
-            for (int i = t; i < m; i += kThreads) vm[i] = sc * lm[ir2][i];
+            for (int i = t; i < n; i += kThreads) vn[i] = sc * r2n[i];
+            for (int i = t; i < m; i += kThreads) vm[i] = sc * r2m[i];
+        }
+        __syncthreads();
+        for (step = 1; step <= max_steps; ++step) {
+            apply(mk, vn, vm, yn, ym);
             __syncthreads();
-            apply(mk, vn, vm, ln[iy], lm[iy]);
-            if (step >= 2) adj_axpy(g, -beta / oldb, ln[ir1], lm[ir1], ln[iy], lm[iy]);
-            const double alfa = adj_dot(g, vn, vm, ln[iy], lm[iy], sums, maxs);
-            adj_axpy(g, -alfa / beta, ln[ir2], lm[ir2], ln[iy], lm[iy]);
+            double pa = 0.0;
+            {
+                const double a1 = step >= 2 ? -beta / oldb : 0.0;
+                for (int i = t; i < n; i += kThreads) {
+                    const double y = step >= 2 ? yn[i] + a1 * r1n[i] : yn[i];
+                    yn[i] = y;
+                    pa += vn[i] * y;
+                }
+                for (int i = t; i < m; i += kThreads) {
+                    const double y = step >= 2 ? ym[i] + a1 * r1m[i] : ym[i];
+                    ym[i] = y;
+                    pa += vm[i] * y;
+                }
+            }
+            const double alfa = adj_block_sum(pa, adj_red);
+            double pb = 0.0;
+            {
+                const double a2 = -alfa / beta;
+                for (int i = t; i < n; i += kThreads) {
+                    const double y = yn[i] + a2 * r2n[i];
+                    yn[i] = y;
+                    pb += y * y;
+                }
+                for (int i = t; i < m; i += kThreads) {
+                    const double y = ym[i] + a2 * r2m[i];
+                    ym[i] = y;
+                    pb += y * y;
+                }
+            }
             {  // r1 <- r2, r2 <- y; the next y reuses r1's buffer
-                const int o = ir1;
-                ir1 = ir2;
-                ir2 = iy;
-                iy = o;
+                double *on = r1n, *om = r1m;
+                r1n = r2n, r1m = r2m;
+                r2n = yn, r2m = ym;
+                yn = on, ym = om;
             }
             oldb = beta;
-            beta = sqrt(adj_dot(g, ln[ir2], lm[ir2], ln[ir2], lm[ir2], sums, maxs));
+            beta = sqrt(adj_block_sum(pb, adj_red));
             const double oldeps = epsln;
             const double delta = cs * dbar + sn * alfa;
             const double gbar = sn * dbar - cs * alfa;
@@ -1298,21 +1340,23 @@ __global__ __launch_bounds__(kThreads) void batch_adjoint_kernel(Arrays ar, Para
             const double phi = cs * phibar;
             phibar = sn * phibar;
             {  // w1 <- w2, w2 <- w; the new w goes into w1's old buffer
-                const int o = iw1;
-                iw1 = iw2;
-                iw2 = iw;
-                iw = o;
+                double *on = w1n, *om = w1m;
+                w1n = w2n, w1m = w2m;
+                w2n = wn, w2m = wm;
+                wn = on, wm = om;
             }
-            const double rg = 1.0 / gamma;
+            const double rg = 1.0 / gamma, sc = beta > 0.0 ? 1.0 / beta : 0.0;
             for (int i = t; i < n; i += kThreads) {
-                const double w = (vn[i] - oldeps * wn[iw1][i] - delta * wn[iw2][i]) * rg;
-                wn[iw][i] = w;
+                const double w = (vn[i] - oldeps * w1n[i] - delta * w2n[i]) * rg;
+                wn[i] = w;
                 xn[i] += phi * w;
+                vn[i] = sc * r2n[i];
             }
             for (int i = t; i < m; i += kThreads) {
-                const double w = (vm[i] - oldeps * wm[iw1][i] - delta * wm[iw2][i]) * rg;
-                wm[iw][i] = w;
+                const double w = (vm[i] - oldeps * w1m[i] - delta * w2m[i]) * rg;
+                wm[i] = w;
                 xm[i] += phi * w;
+                vm[i] = sc * r2m[i];
             }
             __syncthreads();
             if (phibar <= tol * beta1 || !(beta > 0.0)) break;  // converged, or the Krylov space is exhausted
@@ -1322,11 +1366,12 @@ __global__ __launch_bounds__(kThreads) void batch_adjoint_kernel(Arrays ar, Para
     // the true relative residual ||b - K x|| / ||b|| of the returned iterate
     double rel = 0.0;
     if (beta1 > 0.0) {
-        apply(mk, xn, xm, ln[iy], lm[iy]);
-        for (int i = t; i < n; i += kThreads) ln[iy][i] = bn[i] - ln[iy][i];
-        for (int i = t; i < m; i += kThreads) lm[iy][i] = -lm[iy][i];
+        apply(mk, xn, xm, yn, ym);
         __syncthreads();
-        rel = sqrt(adj_dot(g, ln[iy], lm[iy], ln[iy], lm[iy], sums, maxs)) / beta1;
+        for (int i = t; i < n; i += kThreads) yn[i] = bn[i] - yn[i];
+        for (int i = t; i < m; i += kThreads) ym[i] = -ym[i];
+        __syncthreads();
+        rel = sqrt(adj_dot(g, yn, ym, yn, ym, sums, maxs)) / beta1;
     }
     // bound multipliers: v~_B = b~_B - (Q~ u~ + A~' v~)_B with b~ = g/d
     double *tn = g.N(N_Qw_bar), *tmp = g.N(N_pt);
@@ -2247,7 +2292,7 @@ HPRQPStatus batch_finish(HPRQPHipBatch *b)
             batch_setup_kernel<false><<<(unsigned)count, hprqp::kThreads, 0, b->stream>>>(ar, b->P, (int)count, 1);
             if (clock.on && hipStreamSynchronize(b->stream) == hipSuccess) clock.mark("setup: scaling (CSR)");
             batch_power_kernel<<<(unsigned)count, kPowThreads, 0, b->stream>>>(ar, (int)count);
-            if (clock.on && hipStreamSynchronize(b->stream) == hipSuccess) clock.mark("setup: power iterations");
+            if (clock.on && hipStreamSynchronize(b->stream) == hipSuccess) clock.mark("setup: power iterations (CSR)");
             batch_setup_kernel<false><<<(unsigned)count, hprqp::kThreads, 0, b->stream>>>(ar, b->P, (int)count, 3);
         } else {
             batch_setup_kernel<false><<<(unsigned)count, hprqp::kThreads, 0, b->stream>>>(ar, b->P, (int)count, 0);
@@ -2255,14 +2300,16 @@ HPRQPStatus batch_finish(HPRQPHipBatch *b)
     }
     if (b->any_dense) {
         batch_setup_kernel<true><<<(unsigned)count, hprqp::kThreads, 0, b->stream>>>(ar, b->P, (int)count, 1);
+        if (clock.on && hipStreamSynchronize(b->stream) == hipSuccess) clock.mark("setup: scaling (dense)");
         dense_gram_kernel<<<(unsigned)count, hprqp::kThreads, 0, b->stream>>>(ar, (int)count);
+        if (clock.on && hipStreamSynchronize(b->stream) == hipSuccess) clock.mark("setup: G = A A' (dense)");
         batch_setup_kernel<true><<<(unsigned)count, hprqp::kThreads, 0, b->stream>>>(ar, b->P, (int)count, 2);
     }
     if (hipGetLastError() != hipSuccess || hipStreamSynchronize(b->stream) != hipSuccess) {
         delete b;
         return HPRQP_STATUS_NOT_IMPLEMENTED;
     }
-    clock.mark("setup: residuals (all)");
+    clock.mark("setup: last phase (residuals; dense: power iterations)");
     if (b->any_sparse) {
         // Wave threshold for the CSR engine: HPRQP_BATCH_WAVE_ROW, or the
         // fastest of a few candidates on a timed probe of one iteration's
@@ -2931,6 +2978,22 @@ hipError_t ensure_adjoint_outputs(HPRQPHipBatch *b)
     return hipSuccess;
 }
 
+// Launches the adjoint kernels (one workgroup per instance) on s.
+void launch_adjoint(HPRQPHipBatch *b, hipStream_t s, const double *g, double *u, double *v_rows, double *v_bounds,
+                    double tol, int max_steps)
+{
+    using namespace hprqp::batch;
+    const unsigned grid = (unsigned)b->count;
+    if (b->any_sparse)
+        batch_adjoint_kernel<false><<<grid, hprqp::kThreads, 0, s>>>(b->ar, b->P, (int)b->count, g, u, v_rows,
+                                                                     v_bounds, tol, max_steps, b->d_steps,
+                                                                     b->d_adj_resid, 0);
+    if (b->any_dense)
+        batch_adjoint_kernel<true><<<grid, hprqp::kThreads, adj_dense_lds(b), s>>>(
+            b->ar, b->P, (int)b->count, g, u, v_rows, v_bounds, tol, max_steps, b->d_steps, b->d_adj_resid,
+            adj_dense_mld(b));
+}
+
 // Marks the end of work queued on a caller's stream that reads the batch, so
 // that destroying the batch waits for it.
 hipError_t mark_caller_use(const HPRQPHipBatch *b, hipStream_t s)
@@ -2990,13 +3053,7 @@ HPRQPStatus hprqp_hip_batch_adjoint(HPRQPHipBatch *b, const double *dl_dx, doubl
     HPRQP_HIP(d_vr.alloc(sm));
     HPRQP_HIP(hipMemcpy(d_g.p, dl_dx, sizeof(double) * sn, hipMemcpyHostToDevice));
     HPRQP_HIP(hipMemsetAsync(d_vr.p, 0, sizeof(double) * std::max<size_t>(sm, 1), b->stream));
-    if (b->any_sparse)
-        batch_adjoint_kernel<false><<<(unsigned)b->count, hprqp::kThreads, 0, b->stream>>>(
-            b->ar, b->P, (int)b->count, d_g.p, d_u.p, d_vr.p, d_vb.p, tol, max_steps, b->d_steps, b->d_adj_resid, 0);
-    if (b->any_dense)
-        batch_adjoint_kernel<true><<<(unsigned)b->count, hprqp::kThreads, adj_dense_lds(b), b->stream>>>(
-            b->ar, b->P, (int)b->count, d_g.p, d_u.p, d_vr.p, d_vb.p, tol, max_steps, b->d_steps, b->d_adj_resid,
-            adj_dense_mld(b));
+    launch_adjoint(b, b->stream, d_g.p, d_u.p, d_vr.p, d_vb.p, tol, max_steps);
     HPRQP_HIP(hipGetLastError());
     HPRQP_HIP(hipStreamSynchronize(b->stream));
     HPRQP_HIP(hipMemcpy(u, d_u.p, sizeof(double) * sn, hipMemcpyDeviceToHost));
@@ -3023,14 +3080,7 @@ HPRQPStatus hprqp_hip_batch_adjoint_device(HPRQPHipBatch *b, const double *dl_dx
     if (real_m && !v_rows) return HPRQP_STATUS_INVALID_ARGUMENT;
     HPRQP_HIP(ensure_adjoint_outputs(b));
     hipStream_t s = (hipStream_t)stream;
-    if (b->any_sparse)
-        batch_adjoint_kernel<false><<<(unsigned)b->count, hprqp::kThreads, 0, s>>>(b->ar, b->P, (int)b->count, dl_dx,
-                                                                                 u, v_rows, v_bounds, tol, max_steps,
-                                                                                 b->d_steps, b->d_adj_resid, 0);
-    if (b->any_dense)
-        batch_adjoint_kernel<true><<<(unsigned)b->count, hprqp::kThreads, adj_dense_lds(b), s>>>(
-            b->ar, b->P, (int)b->count, dl_dx, u, v_rows, v_bounds, tol, max_steps, b->d_steps, b->d_adj_resid,
-            adj_dense_mld(b));
+    launch_adjoint(b, s, dl_dx, u, v_rows, v_bounds, tol, max_steps);
     HPRQP_HIP(hipGetLastError());
     HPRQP_HIP(mark_caller_use(b, s));
     if (steps) {
