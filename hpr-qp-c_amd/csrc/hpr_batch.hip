@@ -136,6 +136,13 @@ struct Mat {
 
 // Largest dense dimension: x is staged whole in the LDS product buffer.
 constexpr int kDenseMax = kStreamCap;
+// CSR row blocks of the batched engine hold at most kBatchStreamCap entries
+// (the products of a block are staged in LDS); the CSR solve kernel stages
+// the gathered vector of each iteration product in LDS too when it has at
+// most kStageMax entries, so that the gathers read LDS instead of missing L2
+// again once the vector's lines are evicted.
+constexpr int kBatchStreamCap = 1024;
+constexpr int kStageMax = 2048;
 
 // Column-form dot of a dense view: sum_{i<K} S[i*ld + j] xs[i], i ascending.
 // CACHED: ordinary loads, 16 in flight per thread, for products that re-read
@@ -229,6 +236,22 @@ __device__ void wg_spmv(const Mat &M, const double *__restrict__ x, const Epi &e
     __syncthreads();
 }
 
+// An iteration product. A staging workgroup (CSR, every vector of the batch
+// at most kStageMax long) copies the gathered vector x to LDS first and the
+// product gathers from there: the same sums in the same order.
+template <class WG, class Epi>
+__device__ void wg_product(WG &g, const Mat &M, const double *x, int xlen, const Epi &epi, double *sums,
+                           double *maxs)
+{
+    if constexpr (WG::kStage) {
+        for (int i = threadIdx.x; i < xlen; i += kThreads) g.xs[i] = x[i];
+        __syncthreads();
+        wg_spmv<false>(M, g.xs, epi, sums, maxs, g.prod);
+    } else {
+        wg_spmv<WG::kDense>(M, x, epi, sums, maxs, g.prod);
+    }
+}
+
 template <class F>
 __device__ void wg_map(int len, const F &f, double *sums, double *maxs)
 {
@@ -268,10 +291,12 @@ struct DenseDiagInfo {
 __device__ __forceinline__ double dclamp(double v, double lo, double hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
 
-// One workgroup's view of its instance; D: the instance's matrices are dense.
-template <bool D>
+// One workgroup's view of its instance; D: the instance's matrices are dense;
+// ST: the iteration products stage their gathered vector in LDS (xs).
+template <bool D, bool ST = false>
 struct WGv {
     static constexpr bool kDense = D;
+    static constexpr bool kStage = ST;
     const Inst *I;
     State *S;  // in LDS
     Mat Q, A, AT;
@@ -282,6 +307,7 @@ struct WGv {
     double *vm[M_COUNT];
     double *prod;   // LDS staging
     double *red;    // LDS reduction results
+    double *xs = nullptr;  // ST: LDS copy of a product's gathered vector (<= kStageMax entries)
     Params P;
 
     __device__ double *N(int v) const { return vn[v]; }
@@ -886,21 +912,21 @@ __device__ void wg_step_t(WG &g, double h1, double h2, double *sums, double *max
     const int m = g.I->m;
     const double sg = S.sigma;
     const double f2s = 1.0 / (1.0 + sg * S.lam_q), f1s = 1.0 - f2s;
-    wg_spmv<WG::kDense>(g.Q, g.N(N_w), EpiZxw1<DIAG, CHECK>{{}, g.N(N_Qw), g.N(N_x), g.N(N_x_bar), g.N(N_x_hat), g.N(N_w_bar), g.N(N_dx),
+    wg_product(g, g.Q, g.N(N_w), g.I->n, EpiZxw1<DIAG, CHECK>{{}, g.N(N_Qw), g.N(N_x), g.N(N_x_bar), g.N(N_x_hat), g.N(N_w_bar), g.N(N_dx),
                                  g.N(N_z_bar), g.N(N_w), g.N(N_ATy), g.N(N_c), g.N(N_l), g.N(N_u), g.N(N_last_x),
                                  g.N(N_fact1), g.N(N_fact2), sg, f1s, f2s, h1, h2},
-            sums, maxs, g.prod);
+            sums, maxs);
     if (m > 0) {
-        wg_spmv<WG::kDense>(g.Q, g.N(N_w_bar), EpiTempv{{}, g.N(N_tempv), g.N(N_x_hat), g.N(N_Qw), sg}, sums, maxs, g.prod);
+        wg_product(g, g.Q, g.N(N_w_bar), g.I->n, EpiTempv{{}, g.N(N_tempv), g.N(N_x_hat), g.N(N_Qw), sg}, sums, maxs);
         const double f1 = S.lam_a * sg, f2 = 1.0 / f1;
-        wg_spmv<WG::kDense>(g.A, g.N(N_tempv), EpiY<CHECK>{{}, g.M(M_y), g.M(M_y_bar), g.M(M_s), g.M(M_dy), g.M(M_AL), g.M(M_AU), g.M(M_last_y), f1,
+        wg_product(g, g.A, g.N(N_tempv), g.I->n, EpiY<CHECK>{{}, g.M(M_y), g.M(M_y_bar), g.M(M_s), g.M(M_dy), g.M(M_AL), g.M(M_AU), g.M(M_last_y), f1,
                             f2, h1, h2},
-                sums, maxs, g.prod);
+                sums, maxs);
     }
     const double fs = sg / (1.0 + sg * S.lam_q);
-    wg_spmv<WG::kDense>(g.AT, g.M(M_y_bar), EpiW2<DIAG, CHECK>{{}, g.N(N_w_bar), g.N(N_w), g.N(N_ATy), g.N(N_ATy_bar), g.N(N_dw), g.N(N_ATdy),
+    wg_product(g, g.AT, g.M(M_y_bar), m, EpiW2<DIAG, CHECK>{{}, g.N(N_w_bar), g.N(N_w), g.N(N_ATy), g.N(N_ATy_bar), g.N(N_dw), g.N(N_ATdy),
                                g.N(N_last_w), g.N(N_last_ATy), g.N(N_fact), fs, h1, h2},
-            sums, maxs, g.prod);
+            sums, maxs);
 }
 
 // iteration.c:19-83 and 176-216 (noC: A^T y first, Halpern on w and Qw)
@@ -911,17 +937,17 @@ __device__ void wg_step_noc_t(WG &g, double h1, double h2, double *sums, double 
     const int m = g.I->m;
     const double sg = S.sigma;
     const double f2s = 1.0 / (1.0 + sg * S.lam_q), f1s = 1.0 - f2s;
-    wg_spmv<WG::kDense>(g.AT, g.M(M_y), EpiZxw1NoC<DIAG, CHECK>{{}, g.N(N_ATy), g.N(N_x), g.N(N_x_bar), g.N(N_x_hat), g.N(N_w), g.N(N_w_bar),
+    wg_product(g, g.AT, g.M(M_y), m, EpiZxw1NoC<DIAG, CHECK>{{}, g.N(N_ATy), g.N(N_x), g.N(N_x_bar), g.N(N_x_hat), g.N(N_w), g.N(N_w_bar),
                                     g.N(N_dx), g.N(N_dw), g.N(N_z_bar), g.N(N_Qw), g.N(N_c), g.N(N_l), g.N(N_u),
                                     g.N(N_last_x), g.N(N_last_w), g.N(N_fact1), g.N(N_fact2), sg, f1s, f2s, h1, h2},
-            sums, maxs, g.prod);
-    wg_spmv<WG::kDense>(g.Q, g.N(N_w_bar), EpiQwNoC{{}, g.N(N_Qw), g.N(N_Qw_bar), g.N(N_tempv), g.N(N_last_Qw), g.N(N_x_hat), sg, h1, h2}, sums,
-            maxs, g.prod);
+            sums, maxs);
+    wg_product(g, g.Q, g.N(N_w_bar), g.I->n, EpiQwNoC{{}, g.N(N_Qw), g.N(N_Qw_bar), g.N(N_tempv), g.N(N_last_Qw), g.N(N_x_hat), sg, h1, h2}, sums,
+            maxs);
     if (m > 0) {
         const double f1 = S.lam_a * sg, f2 = 1.0 / f1;
-        wg_spmv<WG::kDense>(g.A, g.N(N_tempv), EpiY<CHECK>{{}, g.M(M_y), g.M(M_y_bar), g.M(M_s), g.M(M_dy), g.M(M_AL), g.M(M_AU), g.M(M_last_y), f1,
+        wg_product(g, g.A, g.N(N_tempv), g.I->n, EpiY<CHECK>{{}, g.M(M_y), g.M(M_y_bar), g.M(M_s), g.M(M_dy), g.M(M_AL), g.M(M_AU), g.M(M_last_y), f1,
                             f2, h1, h2},
-                sums, maxs, g.prod);
+                sums, maxs);
     }
 }
 
@@ -1763,10 +1789,14 @@ __global__ __launch_bounds__(kThreads) void batch_setup_kernel(Arrays ar, Params
 // needs more registers than four waves per SIMD leave; CSR instances are
 // latency-bound (gathers, barriers) and run six waves per SIMD, which beats
 // the spills that the smaller register budget costs.
-template <bool D>
+template <bool D, bool ST>
 __global__ __launch_bounds__(kThreads, D ? 2 : 6) void batch_solve_kernel(Arrays ar, Params P, int count, int bodies)
 {
-    __shared__ double prod[kStreamCap];
+    // CSR: row-block products (and with ST the staged gathered vector: 25 KB
+    // in all, six workgroups per CU); dense: x staged for the column products
+    static_assert(!(D && ST), "dense products stage x in prod");
+    __shared__ double prod[D ? kStreamCap : kBatchStreamCap];
+    __shared__ double xs[ST ? kStageMax : 1];
     __shared__ double red[kMaxTerms];
     __shared__ State S;
     if ((int)blockIdx.x >= count) return;
@@ -1775,8 +1805,9 @@ __global__ __launch_bounds__(kThreads, D ? 2 : 6) void batch_solve_kernel(Arrays
     if (threadIdx.x == 0) S = ar.state[k];
     __syncthreads();
     if (S.finished) return;
-    WGv<D> g;
+    WGv<D, ST> g;
     wg_bind(g, ar, P, k, &S, prod, red);
+    g.xs = xs;
     double sums[kMaxTerms] = {0}, maxs[kMaxTerms] = {0};
     for (int b = 0; b < bodies; ++b)
         if (wg_body(g, sums, maxs)) break;
@@ -1993,6 +2024,7 @@ HostArray<T> host_array(size_t count)
 struct HPRQPHipBatch {
     size_t count = 0;
     bool any_dense = false, any_sparse = false;  // which kernel variants have work
+    bool stage_x = false;  // every CSR instance has n, m <= kStageMax: the solve stages gathers in LDS
     std::vector<double> work;                    // per-instance size estimate (dispatch order)
     int *d_order = nullptr;
     size_t blocks_cap = 0;                       // ints allocated at d_blocks
@@ -2042,7 +2074,7 @@ namespace {
 
 // Row blocks for the workgroup engine: runs of rows with >= wave_row entries
 // (up to 8 per block, one wavefront per row), otherwise stream blocks of
-// <= kThreads rows and <= kStreamCap entries.
+// <= kThreads rows and <= kBatchStreamCap entries.
 std::vector<int> plan_batch_blocks(const int *rp, int nrow, int wave_row)
 {
     const int kWaveRow = wave_row;
@@ -2058,7 +2090,7 @@ std::vector<int> plan_batch_blocks(const int *rp, int nrow, int wave_row)
         int nnz = 0;
         while (r < nrow && r - first < hprqp::kThreads) {
             const int len = rp[r + 1] - rp[r];
-            if (len >= kWaveRow || nnz + len > hprqp::kStreamCap) break;
+            if (len >= kWaveRow || nnz + len > hprqp::batch::kBatchStreamCap) break;
             nnz += len;
             ++r;
         }
@@ -2260,6 +2292,9 @@ HPRQPStatus batch_finish(HPRQPHipBatch *b)
     ar.order = b->d_order;
     ar.wave_row = kWaveRowDefault;
     for (size_t k = 0; k < count; ++k) (b->inst[k].dense ? b->any_dense : b->any_sparse) = true;
+    b->stage_x = !std::getenv("HPRQP_BATCH_NO_STAGE");
+    for (size_t k = 0; k < count; ++k)
+        if (!b->inst[k].dense && (b->inst[k].n > kStageMax || b->inst[k].m > kStageMax)) b->stage_x = false;
     if (b->any_dense) {
         // G = A A^T per dense instance (m x m each)
         long long gsum = 0;
@@ -2637,11 +2672,14 @@ HPRQPStatus hprqp_hip_batch_solve(HPRQPHipBatch *b, size_t bodies_per_launch, si
     const bool use_graph = !(graph_env && std::atoi(graph_env) == 0);
     auto enqueue_chunk = [&]() {
         batch_zero_counter_kernel<<<1, 1, 0, b->stream>>>(b->d_active);
-        if (b->any_sparse)
-            batch_solve_kernel<false><<<(unsigned)b->count, hprqp::kThreads, 0, b->stream>>>(
+        if (b->any_sparse && b->stage_x)
+            batch_solve_kernel<false, true><<<(unsigned)b->count, hprqp::kThreads, 0, b->stream>>>(
+                b->ar, b->P, (int)b->count, (int)bodies_per_launch);
+        else if (b->any_sparse)
+            batch_solve_kernel<false, false><<<(unsigned)b->count, hprqp::kThreads, 0, b->stream>>>(
                 b->ar, b->P, (int)b->count, (int)bodies_per_launch);
         if (b->any_dense)
-            batch_solve_kernel<true><<<(unsigned)b->count, hprqp::kThreads, 0, b->stream>>>(
+            batch_solve_kernel<true, false><<<(unsigned)b->count, hprqp::kThreads, 0, b->stream>>>(
                 b->ar, b->P, (int)b->count, (int)bodies_per_launch);
     };
     if (use_graph && (!b->graph || b->graph_bodies != bodies_per_launch)) {
