@@ -551,11 +551,28 @@ static int scale_model(Model *md, int use_scaling, Scaling *sc)
 /* ------------------------------------------------------------------------ */
 /* Workspace: src/cpu/workspace.c:26-248                                     */
 
+/* Equality-row (Gram) metric state; kind 0 when the metric is off. */
+enum { GRAM_NONE = 0, GRAM_QP = 1, GRAM_ROW = 2 };
+typedef struct Gram {
+    int kind;
+    size_t e;        /* metric rows */
+    size_t *rows;    /* row index of each metric row */
+    long *map;       /* row -> metric index, -1 for the other rows (m entries) */
+    double *b;       /* right-hand side of each metric row */
+    double *G;       /* the Gram matrix, dense row-major (kept for inspection) */
+    double *L;       /* its Cholesky factor, dense row-major lower */
+    Csr ET;          /* A_E^T: n rows, column = row index of A */
+    double beta, backward, relative;
+    size_t gnnz;     /* nonzeros of the assembled G */
+    double *r, *step, *tmp;
+} Gram;
+
 typedef struct Work {
     size_t n, m;
     double sigma, lambda_max_A, lambda_max_Q;
     size_t power_steps_A, power_steps_Q; /* power-iteration products taken (0: not run) */
     int Q_is_diag, noC, to_check, soc;
+    Gram g;
     double *x, *x_bar, *x_hat, *w, *w_bar, *y, *y_bar, *y_hat;
     double *Ax, *ATy, *ATy_bar, *ATdy, *Qw, *Qw_bar, *Qx, *dQw, *QATdy;
     double *dx, *dy, *dw, *Rp, *Rd, *s, *z_bar, *tempv;
@@ -568,8 +585,23 @@ typedef struct Work {
     X(fact) X(fact_M) X(last_x) X(last_Qw) X(last_w) X(last_ATy)
 #define OR_WORK_M_VECTORS(X) X(y) X(y_bar) X(y_hat) X(Ax) X(dy) X(Rp) X(s) X(last_y)
 
+static void gram_free(Gram *g)
+{
+    free(g->rows);
+    free(g->map);
+    free(g->b);
+    free(g->G);
+    free(g->L);
+    csr_free(&g->ET);
+    free(g->r);
+    free(g->step);
+    free(g->tmp);
+    memset(g, 0, sizeof(*g));
+}
+
 static void work_free(Work *w)
 {
+    gram_free(&w->g);
 #define OR_FREE(v) free(w->v);
     OR_WORK_N_VECTORS(OR_FREE)
     OR_WORK_M_VECTORS(OR_FREE)
@@ -656,6 +688,8 @@ static double power_iteration(const Model *md, int which, size_t *steps)
     return lam < 0.0 ? 0.0 : lam;
 }
 
+static void gram_setup(Work *w, const Model *md, int mode);
+
 /* workspace.c:191-248 (+ control.cu:189-220 for cone models) */
 static int work_setup(Work *w, const Model *md, const OrParams *p)
 {
@@ -692,6 +726,10 @@ static int work_setup(Work *w, const Model *md, const OrParams *p)
         w->noC = 0;
         if (md->m > 0 && w->lambda_max_A <= 0.0) w->lambda_max_A = 1.0;
     }
+    /* control.cu:222-239: the row metric may change sigma, so the diagonal
+     * factors are formed again after it */
+    gram_setup(w, md, p->gram_mode);
+    if (w->g.kind && w->Q_is_diag) diag_factors(w);
     return OR_OK;
 }
 
@@ -730,6 +768,395 @@ static void project_mixed(double *out, const double *in, const double *lo, const
     size_t lin = ncones ? off[0] : len;
     for (size_t i = 0; i < lin; ++i) out[i] = fmin(fmax(in[i], lo[i]), hi[i]);
     for (size_t k = 0; k < ncones; ++k) or_soc_project(out + off[k], in + off[k], off[k + 1] - off[k]);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Equality-row (Gram) metric. The reference runs it on the GPU only:        */
+/* src/gpu/qp_row_metric.cu:94-149 (plain QPs, gram_mode REQUEST),           */
+/* src/gpu/row_metric.cu:68-130 (the homogeneous equality prefix of row-cone */
+/* models) and src/shared/gram_policy.cpp:6-50 (the AUTO rule). With        */
+/* A = [E; C], G = E E' and beta >= ||C||_2^2 majorised by row and column    */
+/* sums, the y update of the metric rows becomes y - G^{-1}(E t - b)/(2 sigma) */
+/* and the other rows keep theirs with lambda_A = 2 beta. The reference      */
+/* factors G with a sparse direct solver; the oracle uses a dense Cholesky.  */
+
+/* gram_policy.cpp:6-50 on the built, unscaled model. */
+static int gram_auto_eligible(const Model *md)
+{
+    size_t e, n = md->n, g;
+    size_t *degree, *location, *state;
+    double *q;
+    int ok = 1;
+    if (md->nvc || !md->nrc) return 0;
+    e = md->row_off[0];
+    g = (size_t)sqrt((double)e);
+    if (g < 2 || g * g != e || e > n) return 0;
+    for (size_t k = 0; k < md->nrc; ++k)
+        if (md->row_off[k + 1] - md->row_off[k] != 3) return 0;
+    for (size_t i = 0; i < e; ++i)
+        if (md->AL[i] != 0.0 || md->AU[i] != 0.0 || md->soc_rhs[i] != 0.0) return 0;
+    degree = (size_t *)calloc(n ? n : 1, sizeof(size_t));
+    location = (size_t *)malloc((n ? n : 1) * sizeof(size_t));
+    state = (size_t *)malloc((e ? e : 1) * sizeof(size_t));
+    q = (double *)calloc(n ? n : 1, sizeof(double));
+    if (!degree || !location || !state || !q) ok = 0;
+    if (ok) {
+        size_t multi = 0;
+        for (size_t j = 0; j < n; ++j) location[j] = e;
+        for (size_t i = 0; i < e; ++i) state[i] = n;
+        for (size_t k = 0; k < md->A.ptr[e]; ++k) degree[md->A.ind[k]] += 1;
+        for (size_t j = 0; j < n; ++j) multi += degree[j] > 1;
+        if (multi != e) ok = 0;
+        /* each stencil row owns one shared column, with coefficient 4 */
+        for (size_t i = 0; ok && i < e; ++i) {
+            for (size_t k = md->A.ptr[i]; ok && k < md->A.ptr[i + 1]; ++k) {
+                size_t j = md->A.ind[k];
+                if (degree[j] > 1 && md->A.val[k] > 0.0) {
+                    if (state[i] != n || location[j] != e || md->A.val[k] != 4.0) ok = 0;
+                    state[i] = j;
+                    location[j] = i;
+                }
+            }
+            if (state[i] == n) ok = 0;
+        }
+        /* the shared columns form the five-point stencil of a g x g grid */
+        for (size_t i = 0; ok && i < e; ++i) {
+            size_t count = 0;
+            for (size_t k = md->A.ptr[i]; ok && k < md->A.ptr[i + 1]; ++k) {
+                size_t j = location[md->A.ind[k]];
+                int nb;
+                if (j == e) continue;
+                nb = (i >= g && j == i - g) || (i + g < e && j == i + g) || (i % g && j == i - 1) ||
+                     (i % g + 1 < g && j == i + 1);
+                if (md->A.val[k] != (i == j ? 4.0 : (nb ? -1.0 : 0.0))) ok = 0;
+                count += 1;
+            }
+            if (ok && count != (size_t)(1 + (i >= g) + (i + g < e) + (i % g > 0) + (i % g + 1 < g))) ok = 0;
+        }
+        /* Q diagonal with a positive finite entry on every state column */
+        for (size_t i = 0; ok && i < n; ++i)
+            for (size_t k = md->Q.ptr[i]; ok && k < md->Q.ptr[i + 1]; ++k) {
+                if (md->Q.ind[k] != i && md->Q.val[k] != 0.0) ok = 0;
+                if (md->Q.ind[k] == i) q[i] += md->Q.val[k];
+            }
+        for (size_t i = 0; ok && i < e; ++i)
+            if (!(q[state[i]] > 0.0) || !isfinite(q[state[i]])) ok = 0;
+    }
+    free(degree);
+    free(location);
+    free(state);
+    free(q);
+    return ok;
+}
+
+/* Dense Cholesky G = L L' (row-major, lower); 0 when a pivot is not
+ * positive and finite. */
+static int gram_cholesky(const double *G, double *L, size_t e)
+{
+    memset(L, 0, e * e * sizeof(double));
+    for (size_t j = 0; j < e; ++j) {
+        double d = G[j * e + j];
+        for (size_t t = 0; t < j; ++t) d -= L[j * e + t] * L[j * e + t];
+        if (!(d > 0.0) || !isfinite(d)) return 0;
+        d = sqrt(d);
+        L[j * e + j] = d;
+        for (size_t i = j + 1; i < e; ++i) {
+            double v = G[i * e + j];
+            for (size_t t = 0; t < j; ++t) v -= L[i * e + t] * L[j * e + t];
+            L[i * e + j] = v / d;
+        }
+    }
+    return 1;
+}
+
+/* x = G^{-1} r by forward then backward substitution with L. */
+static void gram_solve(const Gram *g, const double *r, double *x)
+{
+    size_t e = g->e;
+    for (size_t i = 0; i < e; ++i) {
+        double v = r[i];
+        for (size_t t = 0; t < i; ++t) v -= g->L[i * e + t] * x[t];
+        x[i] = v / g->L[i * e + i];
+    }
+    for (size_t i = e; i-- > 0;) {
+        double v = x[i];
+        for (size_t t = i + 1; t < e; ++t) v -= g->L[t * e + i] * x[t];
+        x[i] = v / g->L[i * e + i];
+    }
+}
+
+typedef struct GramPair {
+    size_t col, seq;
+    double val;
+} GramPair;
+
+static int gram_pair_cmp(const void *a, const void *b)
+{
+    const GramPair *p = (const GramPair *)a, *q = (const GramPair *)b;
+    if (p->col != q->col) return p->col < q->col ? -1 : 1;
+    return p->seq < q->seq ? -1 : (p->seq > q->seq);
+}
+
+/* Shared by both metrics: given the metric rows (g->rows, g->map, g->e) and
+ * beta, build E' and G (qp_row_metric.cu:109-116 / row_metric.cu:83-93), the
+ * factor and its validation (:120-127 / :98-103). Returns 1 when accepted. */
+static int gram_build(Gram *g, const Model *md)
+{
+    size_t e = g->e, n = md->n, nnz = 0;
+    size_t *cnt;
+    unsigned long long work = 0;
+    GramPair *pairs = NULL;
+    size_t cap = 0;
+    double *t = NULL, *x = NULL, err = 0.0, an = 0.0, xn = 0.0, bn = 0.0;
+    cnt = (size_t *)calloc(n + 1, sizeof(size_t));
+    if (!cnt) return 0;
+    for (size_t q = 0; q < e; ++q) {
+        size_t i = g->rows[q];
+        for (size_t k = md->A.ptr[i]; k < md->A.ptr[i + 1]; ++k) cnt[md->A.ind[k] + 1] += 1;
+    }
+    for (size_t j = 0; j < n; ++j) work += (unsigned long long)cnt[j + 1] * cnt[j + 1];
+    if (work > 100000000ULL) {
+        free(cnt);
+        return 0;
+    }
+    for (size_t j = 0; j < n; ++j) cnt[j + 1] += cnt[j];
+    if (csr_alloc(&g->ET, n, md->m, cnt[n]) != OR_OK) {
+        free(cnt);
+        return 0;
+    }
+    memcpy(g->ET.ptr, cnt, (n + 1) * sizeof(size_t));
+    for (size_t q = 0; q < e; ++q) {
+        size_t i = g->rows[q];
+        for (size_t k = md->A.ptr[i]; k < md->A.ptr[i + 1]; ++k) {
+            size_t z = cnt[md->A.ind[k]]++;
+            g->ET.ind[z] = i;
+            g->ET.val[z] = md->A.val[k];
+        }
+    }
+    free(cnt);
+    g->G = (double *)calloc(e * e, sizeof(double));
+    g->L = (double *)malloc(e * e * sizeof(double));
+    g->r = (double *)calloc(e, sizeof(double));
+    g->step = (double *)calloc(e, sizeof(double));
+    g->tmp = (double *)calloc(n ? n : 1, sizeof(double));
+    if (!g->G || !g->L || !g->r || !g->step || !g->tmp) return 0;
+    /* row q of G: products of row i with every metric row sharing a column,
+     * gathered in entry order and summed per column in that order */
+    for (size_t q = 0; q < e; ++q) {
+        size_t i = g->rows[q], cntp = 0;
+        for (size_t k = md->A.ptr[i]; k < md->A.ptr[i + 1]; ++k) {
+            size_t j = md->A.ind[k];
+            size_t need = cntp + (g->ET.ptr[j + 1] - g->ET.ptr[j]);
+            if (need > cap) {
+                GramPair *np;
+                cap = need * 2;
+                np = (GramPair *)realloc(pairs, cap * sizeof(GramPair));
+                if (!np) {
+                    free(pairs);
+                    return 0;
+                }
+                pairs = np;
+            }
+            for (size_t z = g->ET.ptr[j]; z < g->ET.ptr[j + 1]; ++z) {
+                pairs[cntp].col = (size_t)g->map[g->ET.ind[z]];
+                pairs[cntp].seq = cntp;
+                pairs[cntp].val = md->A.val[k] * g->ET.val[z];
+                cntp += 1;
+            }
+        }
+        if (cntp) qsort(pairs, cntp, sizeof(GramPair), gram_pair_cmp);
+        for (size_t z = 0; z < cntp;) {
+            size_t col = pairs[z].col;
+            double sum = 0.0;
+            do {
+                sum += pairs[z++].val;
+            } while (z < cntp && pairs[z].col == col);
+            if (sum != 0.0) {
+                g->G[q * e + col] = sum;
+                nnz += 1;
+            }
+        }
+    }
+    free(pairs);
+    g->gnnz = nnz;
+    if (!gram_cholesky(g->G, g->L, e)) return 0;
+    /* validation solve (qp_row_metric.cu:120-127) */
+    t = (double *)calloc(e, sizeof(double));
+    x = (double *)calloc(e, sizeof(double));
+    if (!t || !x) {
+        free(t);
+        free(x);
+        return 0;
+    }
+    for (size_t i = 0; i < e; ++i) t[i] = sin(.173 * (double)(i + 1)) + .2 * cos(.719 * (double)(i + 1));
+    gram_solve(g, t, x);
+    for (size_t i = 0; i < e; ++i) {
+        long double z = 0.0L;
+        double rs = 0.0;
+        if (!isfinite(x[i])) {
+            free(t);
+            free(x);
+            return 0;
+        }
+        for (size_t k = 0; k < e; ++k) {
+            double v = g->G[i * e + k];
+            if (v != 0.0) {
+                z += (long double)v * x[k];
+                rs += fabs(v);
+            }
+        }
+        err = fmax(err, (double)fabsl(z - (long double)t[i]));
+        an = fmax(an, rs);
+        xn = fmax(xn, fabs(x[i]));
+        bn = fmax(bn, fabs(t[i]));
+    }
+    free(t);
+    free(x);
+    g->backward = err / (an * xn + bn);
+    g->relative = err / bn;
+    return g->backward <= 1e-10 && (e > 5000 || g->relative <= 1e-8);
+}
+
+/* qp_row_metric.cu:94-149: plain QPs, every row with AL == AU finite. */
+static int gram_setup_qp(Work *w, const Model *md)
+{
+    Gram *g = &w->g;
+    size_t m = md->m, n = md->n;
+    double *cols, rowmax = 0.0, cmax = 0.0, normb, normc, *sol;
+    long double rest = 0.0L, bq = 0.0L;
+    if (!m || !n || md->A.nnz == 0) return 0;
+    g->map = (long *)malloc(m * sizeof(long));
+    g->rows = (size_t *)malloc(m * sizeof(size_t));
+    g->b = (double *)malloc(m * sizeof(double));
+    cols = (double *)calloc(n, sizeof(double));
+    if (!g->map || !g->rows || !g->b || !cols) {
+        free(cols);
+        return 0;
+    }
+    for (size_t i = 0; i < m; ++i) {
+        g->map[i] = -1;
+        if (isfinite(md->AL[i]) && md->AL[i] == md->AU[i]) {
+            g->map[i] = (long)g->e;
+            g->rows[g->e] = i;
+            g->b[g->e] = md->AL[i];
+            g->e += 1;
+        }
+    }
+    if (g->e == 0 || g->e > n) {
+        free(cols);
+        return 0;
+    }
+    for (size_t i = 0; i < m; ++i) {
+        double row = 0.0, lo, hi, v;
+        if (g->map[i] >= 0) continue;
+        for (size_t k = md->A.ptr[i]; k < md->A.ptr[i + 1]; ++k) {
+            row += fabs(md->A.val[k]);
+            cols[md->A.ind[k]] += fabs(md->A.val[k]);
+        }
+        rowmax = fmax(rowmax, row);
+        lo = isfinite(md->AL[i]) ? fabs(md->AL[i]) : 0.0;
+        hi = isfinite(md->AU[i]) ? fabs(md->AU[i]) : 0.0;
+        v = fmax(lo, hi);
+        rest += (long double)v * v;
+    }
+    for (size_t j = 0; j < n; ++j) cmax = fmax(cmax, cols[j]);
+    free(cols);
+    g->beta = (1 + 1e-12) * rowmax * cmax;
+    if (g->beta == 0.0) g->beta = 1.0;
+    if (!isfinite(g->beta)) return 0;
+    if (!gram_build(g, md)) return 0;
+    /* sigma from the metric norm of b (:128-133) */
+    sol = (double *)malloc(g->e * sizeof(double));
+    if (!sol) return 0;
+    gram_solve(g, g->b, sol);
+    for (size_t q = 0; q < g->e; ++q) bq += (long double)g->b[q] * sol[q];
+    free(sol);
+    if (!isfinite((double)bq) || bq < -1e-10L) return 0;
+    normb = sqrt((double)((bq > 0.0L ? bq : 0.0L) + rest / g->beta));
+    normc = nrm2(md->c, n);
+    w->sigma = (normb > 1e-16 && normc > 1e-16 && normb < 1e16 && normc < 1e16) ? normb / normc : 1.0;
+    w->lambda_max_A = 2.0 * g->beta;
+    w->noC = 0;
+    g->kind = GRAM_QP;
+    return 1;
+}
+
+/* row_metric.cu:68-130: the linear prefix of a row-cone model, all
+ * homogeneous equalities; diagonal Q, no variable cones. */
+static int gram_setup_row(Work *w, const Model *md)
+{
+    Gram *g = &w->g;
+    size_t m = md->m, n = md->n, l;
+    double *cols, rowmax = 0.0, cmax = 0.0;
+    if (!md->nrc || md->nvc || !w->Q_is_diag || w->noC) return 0;
+    l = md->row_off[0];
+    if (!l) return 0;
+    for (size_t i = 0; i < l; ++i)
+        if (md->AL[i] != 0.0 || md->AU[i] != 0.0 || md->soc_rhs[i] != 0.0) return 0;
+    cols = (double *)calloc(n ? n : 1, sizeof(double));
+    if (!cols) return 0;
+    for (size_t i = l; i < m; ++i) {
+        double row = 0.0;
+        for (size_t k = md->A.ptr[i]; k < md->A.ptr[i + 1]; ++k) {
+            row += fabs(md->A.val[k]);
+            cols[md->A.ind[k]] += fabs(md->A.val[k]);
+        }
+        rowmax = fmax(rowmax, row);
+    }
+    for (size_t j = 0; j < n; ++j) cmax = fmax(cmax, cols[j]);
+    free(cols);
+    g->beta = (1 + 1e-12) * rowmax * cmax;
+    if (!(g->beta > 0.0)) return 0;
+    g->e = l;
+    g->map = (long *)malloc(m * sizeof(long));
+    g->rows = (size_t *)malloc(l * sizeof(size_t));
+    g->b = (double *)calloc(l, sizeof(double));
+    if (!g->map || !g->rows || !g->b) return 0;
+    for (size_t i = 0; i < m; ++i) g->map[i] = i < l ? (long)i : -1;
+    for (size_t i = 0; i < l; ++i) g->rows[i] = i;
+    if (!gram_build(g, md)) return 0;
+    w->sigma /= sqrt(g->beta);
+    w->lambda_max_A = 2.0 * g->beta;
+    g->kind = GRAM_ROW;
+    return 1;
+}
+
+/* control.cu:222-225 (setup) with the declined metric leaving the workspace
+ * as it was (qp_row_metric.cu:140-147, row_metric.cu:133-146). */
+static void gram_setup(Work *w, const Model *md, int mode)
+{
+    double sigma = w->sigma, lam = w->lambda_max_A;
+    int noC = w->noC, ok = 0;
+    if (mode == 0) return;
+    if (!md->nvc && !md->nrc) ok = gram_setup_qp(w, md);
+    else if (md->nrc) ok = gram_setup_row(w, md);
+    if (!ok) {
+        gram_free(&w->g);
+        w->sigma = sigma;
+        w->lambda_max_A = lam;
+        w->noC = noC;
+    }
+}
+
+/* The metric's y_bar on its rows (qp_row_metric.cu:55-63, row_metric.cu:46-55). */
+static void gram_step(Work *w)
+{
+    Gram *g = &w->g;
+    for (size_t q = 0; q < g->e; ++q) g->r[q] = w->Ax[g->rows[q]] - g->b[q];
+    gram_solve(g, g->r, g->step);
+}
+
+/* 2 ||E' dy_E||^2 + 2 beta ||dy_rest||^2 (qp_row_metric.cu:64-73,
+ * row_metric.cu:56-66). */
+static double gram_quadratic(Work *w)
+{
+    Gram *g = &w->g;
+    double eq, rest = 0.0;
+    spmv(&g->ET, w->dy, g->tmp);
+    eq = dot(g->tmp, g->tmp, w->n);
+    for (size_t i = 0; i < w->m; ++i)
+        if (g->map[i] < 0) rest += w->dy[i] * w->dy[i];
+    return 2.0 * eq + 2.0 * g->beta * rest;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -869,8 +1296,11 @@ static void step_soc(Work *w, const Model *md, double a, double b)
         spmv(&md->A, w->tempv, w->Ax);
         for (size_t i = 0; i < m; ++i) w->Rp[i] = w->Ax[i] - (md->nrc ? md->soc_rhs[i] : 0.0) - tau * w->y[i];
         project_mixed(w->s, w->Rp, md->AL, md->AU, m, md->nrc, md->row_off);
+        if (w->g.kind) gram_step(w);
         for (size_t i = 0; i < m; ++i) {
-            double y = w->y[i], yb = (w->s[i] - w->Rp[i]) / tau;
+            double y = w->y[i];
+            double yb = (w->g.kind && w->g.map[i] >= 0) ? y - w->g.step[w->g.map[i]] / (2.0 * w->sigma)
+                                                         : (w->s[i] - w->Rp[i]) / tau;
             w->dy[i] = yb - y;
             w->y_bar[i] = yb;
             w->y[i] = a * w->last_y[i] + b * (2.0 * yb - y);
@@ -885,7 +1315,7 @@ static int iteration_step(Work *w, const Model *md, size_t inner)
     double h1 = 1.0 / ((double)inner + 2.0);
     double h2 = 1.0 - h1;
     int st;
-    if (w->soc) {
+    if (w->soc || w->g.kind) {
         step_soc(w, md, h1, h2);
         return OR_OK;
     }
@@ -923,6 +1353,8 @@ static void residuals(Work *w, const Model *md, const Scaling *sc, Resid *r, siz
     cx = dot(md->c, w->x_bar, n);
     xqx = dot(w->x_bar, w->Qx, n);
     zx = dot(w->z_bar, w->x_bar, n);
+    if (w->g.kind == GRAM_QP)
+        for (size_t i = 0; i < m; ++i) w->s[i] = clamp(w->Ax[i], md->AL[i], md->AU[i]);
     if (m > 0) ys = dot(w->y_bar, w->s, m);
     if (md->nrc) yb = dot(w->y_bar, md->soc_rhs, m);
     r->pobj = scale * (cx + 0.5 * xqx) + md->obj_constant;
@@ -1019,7 +1451,7 @@ static double m_norm(Work *w, const Model *md)
     spmv(&md->Q, w->dw, w->dQw);
     m2 -= 2.0 * dot(w->dQw, w->dx, n);
     if (m > 0) {
-        m1 = w->sigma * w->lambda_max_A * dot(w->dy, w->dy, m);
+        m1 = w->g.kind ? w->sigma * gram_quadratic(w) : w->sigma * w->lambda_max_A * dot(w->dy, w->dy, m);
         spmv(&md->AT, w->dy, w->ATdy);
         spmv(&md->Q, w->ATdy, w->QATdy);
         m1 -= 2.0 * w->sigma * dot(w->dQw, w->ATdy, n);
@@ -1174,7 +1606,7 @@ static void sigma_update(Restart *r, Work *w, const Model *md, const Resid *res)
     if (m > 0) {
         for (size_t i = 0; i < m; ++i) w->dy[i] = w->y_bar[i] - w->last_y[i];
         for (size_t i = 0; i < n; ++i) w->ATdy[i] = w->ATy_bar[i] - w->last_ATy[i];
-        a = w->lambda_max_A * dot(w->dy, w->dy, m);
+        a = w->g.kind ? gram_quadratic(w) : w->lambda_max_A * dot(w->dy, w->dy, m);
         a -= 2.0 * dot(w->dQw, w->ATdy, n);
         spmv(&md->Q, w->ATdy, w->QATdy);
     }
@@ -1386,6 +1818,7 @@ void or_params_defaults(OrParams *p)
     p->use_scaling = 1;
     p->lambda_A_given = 0.0;
     p->lambda_Q_given = 0.0;
+    p->gram_mode = 1;
 }
 
 int or_session_create(size_t n, size_t m,
@@ -1406,8 +1839,12 @@ int or_session_create(size_t n, size_t m,
     s->p = *params;
     st = model_build(&s->md, n, m, q_nnz, q_ptr, q_ind, q_val, a_nnz, a_ptr, a_ind, a_val, c, AL, AU, l, u,
                      obj_constant, num_var_cones, var_offsets, num_row_cones, row_offsets, soc_rhs);
+    /* solver.cu:145-151: AUTO becomes REQUEST when the built model passes
+     * the structural rule, else OFF */
+    if (st == OR_OK && (s->p.gram_mode < 0 || s->p.gram_mode > 2)) st = OR_INVALID;
+    if (st == OR_OK && s->p.gram_mode == 1) s->p.gram_mode = gram_auto_eligible(&s->md) ? 2 : 0;
     if (st == OR_OK) st = scale_model(&s->md, params->use_scaling, &s->sc);
-    if (st == OR_OK) st = work_setup(&s->w, &s->md, params);
+    if (st == OR_OK) st = work_setup(&s->w, &s->md, &s->p);
     if (st != OR_OK) {
         or_session_destroy(s);
         return st;
@@ -1498,6 +1935,10 @@ size_t or_session_read(const OrSession *s, const char *name, double *out, size_t
     if (!src && strcmp(name, "soc_rhs") == 0 && s->md.soc_rhs) { src = s->md.soc_rhs; len = s->md.m; }
     if (!src && strcmp(name, "row_norm") == 0) { src = s->sc.row_norm; len = s->md.m; }
     if (!src && strcmp(name, "col_norm") == 0) { src = s->sc.col_norm; len = s->md.n; }
+    if (!src && s->w.g.kind && strcmp(name, "gram_G") == 0) { src = s->w.g.G; len = s->w.g.e * s->w.g.e; }
+    if (!src && s->w.g.kind && strcmp(name, "gram_r") == 0) { src = s->w.g.r; len = s->w.g.e; }
+    if (!src && s->w.g.kind && strcmp(name, "gram_step") == 0) { src = s->w.g.step; len = s->w.g.e; }
+    if (!src && s->w.g.kind && strcmp(name, "gram_b") == 0) { src = s->w.g.b; len = s->w.g.e; }
     if (!src) return 0;
     if (out) memcpy(out, src, (len < cap ? len : cap) * sizeof(double));
     return len;
@@ -1528,6 +1969,12 @@ double or_session_scalar(const OrSession *s, const char *name)
     if (!strcmp(name, "dobj")) return s->res.dobj;
     if (!strcmp(name, "current_gap")) return s->rs.current_gap;
     if (!strcmp(name, "last_gap")) return s->rs.last_gap;
+    if (!strcmp(name, "gram_active")) return s->w.g.kind;
+    if (!strcmp(name, "gram_rows")) return (double)s->w.g.e;
+    if (!strcmp(name, "gram_beta")) return s->w.g.beta;
+    if (!strcmp(name, "gram_nnz")) return (double)s->w.g.gnnz;
+    if (!strcmp(name, "gram_backward")) return s->w.g.backward;
+    if (!strcmp(name, "gram_mode")) return s->p.gram_mode;
     return NAN;
 }
 
@@ -1622,4 +2069,16 @@ int or_session_residuals(OrSession *s, size_t iter, double *out6)
 double or_session_m_norm(OrSession *s)
 {
     return s ? m_norm(&s->w, &s->md) : NAN;
+}
+
+double or_session_gram_quadratic(OrSession *s)
+{
+    return s && s->w.g.kind ? gram_quadratic(&s->w) : NAN;
+}
+
+size_t or_session_gram_map(const OrSession *s, long *out, size_t cap)
+{
+    if (!s || !s->w.g.kind) return 0;
+    if (out) memcpy(out, s->w.g.map, (s->w.m < cap ? s->w.m : cap) * sizeof(long));
+    return s->w.m;
 }

@@ -34,6 +34,8 @@ typedef struct OrParams {
      * iteration (only the CPU-baseline timing uses this, to bound setup). */
     double lambda_A_given;
     double lambda_Q_given;
+    /* equality-row (Gram) metric: 0 OFF, 1 AUTO (default), 2 REQUEST */
+    int gram_mode;
 } OrParams;
 
 typedef struct OrResults {
@@ -117,6 +119,15 @@ int or_session_write(OrSession *s, const char *name, const double *in, size_t le
 int or_session_step(OrSession *s, size_t inner, int to_check);
 int or_session_residuals(OrSession *s, size_t iter, double *out6);
 double or_session_m_norm(OrSession *s);
+
+/* Equality-row (Gram) metric: the quadratic 2||E'dy_E||^2 + 2 beta
+ * ||dy_rest||^2 at the current dy (NaN when the metric is off), and the row
+ * map (metric index or -1 per row; returns m, 0 when off). Vectors
+ * "gram_G", "gram_r", "gram_step", "gram_b" and scalars "gram_active" (0 off,
+ * 1 plain-QP metric, 2 row-cone prefix metric), "gram_rows", "gram_beta",
+ * "gram_nnz", "gram_backward", "gram_mode" are readable by name. */
+double or_session_gram_quadratic(OrSession *s);
+size_t or_session_gram_map(const OrSession *s, long *out, size_t cap);
 
 /* Lorentz-cone projection of one block (t = in[0], v = in[1:len]). */
 void or_soc_project(double *out, const double *in, size_t len);

@@ -200,7 +200,7 @@ def soc_qp(seed=0, n_box=100, cones=10, cone_dim=10, m=50, a_row_nnz=10, box=5.0
 class OrParams(C.Structure):
     _fields_ = [("stoptol", C.c_double), ("max_iter", C.c_size_t), ("time_limit", C.c_double),
                 ("check_iter", C.c_size_t), ("print_frequency", C.c_int), ("use_scaling", C.c_int),
-                ("lambda_A_given", C.c_double), ("lambda_Q_given", C.c_double)]
+                ("lambda_A_given", C.c_double), ("lambda_Q_given", C.c_double), ("gram_mode", C.c_int)]
 
 
 class OrResults(C.Structure):
@@ -248,6 +248,10 @@ def oracle_lib():
         L.or_session_residuals.argtypes = [C.c_void_p, C.c_size_t, c_double_p]
         L.or_session_m_norm.argtypes = [C.c_void_p]
         L.or_session_m_norm.restype = C.c_double
+        L.or_session_gram_quadratic.argtypes = [C.c_void_p]
+        L.or_session_gram_quadratic.restype = C.c_double
+        L.or_session_gram_map.argtypes = [C.c_void_p, C.POINTER(C.c_long), C.c_size_t]
+        L.or_session_gram_map.restype = C.c_size_t
         L.or_preprocess.argtypes = ([C.c_size_t, C.c_size_t] + [C.c_size_t, c_size_p, c_size_p, c_double_p] * 2
                                     + [c_double_p, c_double_p, c_size_p, c_size_p, c_size_p, c_double_p,
                                        c_size_p, c_size_p, c_size_p, c_size_p, c_double_p])
@@ -608,6 +612,83 @@ def soc_simple_models():
     out["constant_cone_infeasible"] = (qp_dense_soc([[1.0]], np.zeros((2, 1)), [-1], [-inf] * 2, [inf] * 2,
                                                     [-inf], [inf], row_offsets=[0, 2], rhs=[1.0, 0.0]), None)
     return out
+
+
+def gram_fixture_qp(kind):
+    """The analytic QPs of the reference's Gram contract test
+    (tests/polishing/fixtures.py:6-17, used by tests/polishing/test_gpu.py:14-66):
+    two variables in [-1, 1]^2, Q = [[2, .3], [.3, 1]] and c = -Q x* with
+    x* = (0.25, 0.5), under five row sets -- 'affine' (one equality row),
+    'mixed' (an equality between two interval rows), 'dependent' (two
+    proportional equality rows, so G is singular), 'no_equalities' and 'box'
+    (no rows). Returns (qp, optimal objective)."""
+    Qd = np.array([[2.0, 0.3], [0.3, 1.0]])
+    opt = np.array([0.25, 0.5])
+    c = -Qd @ opt
+    if kind == "affine":
+        Ad = np.array([[1.0, 2.0]])
+        lo = hi = Ad @ opt
+    elif kind == "mixed":
+        Ad = np.array([[0.0, 1.0], [1.0, 2.0], [1.0, 0.0]])
+        lo, hi = np.array([0.0, 1.25, -1.0]), np.array([1.0, 1.25, 1.0])
+    elif kind == "dependent":
+        Ad = np.array([[1.0, 2.0], [2.0, 4.0]])
+        lo = hi = Ad @ opt
+    elif kind == "no_equalities":
+        Ad = np.eye(2)
+        lo, hi = np.zeros(2), np.ones(2)
+    elif kind == "box":
+        Ad = np.zeros((0, 2))
+        lo = hi = np.zeros(0)
+    else:
+        raise ValueError(kind)
+    qp = qp_from_dense(Qd, Ad, c, lo, hi, [-1.0, -1.0], [1.0, 1.0])
+    return qp, float(0.5 * opt @ Qd @ opt + c @ opt)
+
+
+def gram_fixture_spatial(g):
+    """The boundary-control model on a g x g grid that the reference's AUTO
+    rule recognises (tests/polishing/fixtures.py:19-29, src/shared/gram_policy.cpp):
+    variables [controls u (k = 4g) | states y (g^2) | slacks v (k)]; g^2
+    homogeneous equality rows K y - B u = 0 with K the five-point Laplacian
+    (4 on the diagonal, -1 to each grid neighbour) and B placing control j on
+    boundary state br[j]; then one 3-dimensional row cone per control,
+    (1 - C_j, -1 - C_j, 2 u_j) with C_j = u_j - y_br[j] - v_j. Q = diag(0 on u,
+    h^2 on y, h/100 on v), c = -h^2 d on y for the target
+    d = 3 - 2 |p - (1/2, 1/2)|^2 at grid point p, h = 1/(g+1)."""
+    h = 1.0 / (g + 1)
+    e, k = g * g, 4 * g
+    n, m = 2 * k + e, e + 3 * k
+    A = np.zeros((m, n))
+    for r in range(g):
+        for col in range(g):
+            i = r * g + col
+            A[i, k + i] = 4.0
+            for rr, cc in ((r - 1, col), (r + 1, col), (r, col - 1), (r, col + 1)):
+                if 0 <= rr < g and 0 <= cc < g:
+                    A[i, k + rr * g + cc] = -1.0
+    br = np.concatenate([np.arange(g), np.arange(e - g, e), np.arange(0, e, g), np.arange(g - 1, e, g)])
+    for j in range(k):
+        A[br[j], j] -= 1.0                  # - B u on the state rows
+        t = e + 3 * j
+        # rows t, t+1: -C_j = -u_j + y_br[j] + v_j ; row t+2: 2 u_j
+        for rr in (t, t + 1):
+            A[rr, j] = -1.0
+            A[rr, k + br[j]] = 1.0
+            A[rr, k + e + j] = 1.0
+        A[t + 2, j] = 2.0
+    v = np.arange(1, g + 1) * h
+    d = (3.0 - 2.0 * ((v[:, None] - 0.5) ** 2 + (v[None, :] - 0.5) ** 2)).ravel()
+    qd = np.concatenate([np.zeros(k), np.full(e, h * h), np.full(k, 0.01 * h)])
+    c = np.concatenate([np.zeros(k), -h * h * d, np.zeros(k)])
+    l = np.concatenate([np.zeros(k + e), np.full(k, 3.7)])
+    u = np.concatenate([np.full(k + e, 2.071), np.full(k, 4.5)])
+    AL = np.concatenate([np.zeros(e), np.full(3 * k, -math.inf)])
+    AU = np.concatenate([np.zeros(e), np.full(3 * k, math.inf)])
+    rhs = np.concatenate([np.zeros(e), np.tile([-1.0, 1.0, 0.0], k)])
+    qp = qp_dense_soc(np.diag(qd), A, c, AL, AU, l, u, row_offsets=np.arange(e, m + 1, 3), rhs=rhs)
+    qp.obj_constant = float(0.5 * h * h * (d @ d))
+    return qp
 
 
 def cone_project(v):
