@@ -249,8 +249,10 @@ struct EpiResA {
     static constexpr int NS = 1, NM = 2;
     double *partials;
     double *Ax;
-    const double *AL, *AU, *row_norm, *y_bar, *s;
+    const double *AL, *AU, *row_norm, *y_bar;
+    double *s;
     double b_scale;
+    int reproject;  // 1: s <- the projection of A x_bar (residuals.cu:28-50)
     __device__ void row(int i, double ax, double *sm, double *mx) const
     {
         Ax[i] = ax;
@@ -258,7 +260,8 @@ struct EpiResA {
         const double f = row_norm[i] * b_scale;
         mx[0] = fmax(mx[0], fabs((p - ax) * f));
         mx[1] = fmax(mx[1], fabs(ax * f));
-        sm[0] += y_bar[i] * s[i];
+        if (reproject) s[i] = p;
+        sm[0] += y_bar[i] * (reproject ? p : s[i]);
     }
 };
 

@@ -597,8 +597,7 @@ HPRQPStatus build_from_QAbc_soc(const HPRQPMatrixCSR *Q, const HPRQPMatrixCSR *A
     return build_impl(Q, A, c, AL, AU, l, u, obj_constant, soc, model);
 }
 
-// solver.c:133-157. use_gpu defaults to 1 (this library is the GPU backend);
-// gram_mode defaults to OFF because the Gram path does not exist here.
+// solver.c:133-157. use_gpu defaults to 1 (this library is the GPU backend).
 void params_set_defaults(HPRQPParams *params)
 {
     if (!params) return;
@@ -614,7 +613,7 @@ void params_set_defaults(HPRQPParams *params)
     params->verbose = 1;
     params->spmv_mode_q = HPRQP_SPMV_MODE_AUTO;
     params->spmv_mode_a = HPRQP_SPMV_MODE_AUTO;
-    params->gram_mode = HPRQP_GRAM_OFF;
+    params->gram_mode = HPRQP_GRAM_AUTO;
     params->polishing = 1;
     params->polish_max_attempts = 12;
     params->polish_episode_seconds = 20.0;

@@ -26,6 +26,7 @@
 
 #include "device_common.h"
 #include "epilogues.h"
+#include "gram.h"
 #include "hprqp.h"
 #include "hprqp_hip.h"
 #include "solve_loop.h"
@@ -479,13 +480,20 @@ struct EpiSocY : NoRed {
     double *Rp, *s;
     const double *rhs, *AL, *AU;
     int lin_m;
+    // equality-row metric (null when off): rows with gmap >= 0 hand
+    // A t - b to the metric solve instead of finishing here
+    const int *gmap = nullptr;
+    double *gr = nullptr;
+    const double *gb = nullptr;
     __device__ void row(int i, double ax, double *, double *) const
     {
         const double raw = ax - (rhs ? rhs[i] : 0.0) - fin.tau * fin.y[i];
         Rp[i] = raw;
         if (i < lin_m) {
             s[i] = box(raw, AL[i], AU[i]);
-            fin.apply(i);
+            const int q = gmap ? gmap[i] : -1;
+            if (q >= 0) gr[q] = ax - gb[q];
+            else fin.apply(i);
         }
     }
 };
@@ -873,6 +881,11 @@ public:
     }
     hipStream_t stream() const { return stream_; }
     double scalar(const char *name) const;
+    int gram_active() const override { return gram_ && gram_->kind() ? 1 : 0; }
+    size_t gram_read(const char *name, double *out, size_t cap) const
+    {
+        return gram_ ? gram_->read(name, out, cap, stream_) : 0;
+    }
     HPRQPResiduals last_res_{};
     size_t restart_times_ = 0;
 
@@ -897,6 +910,8 @@ private:
     int *QT_rp_ = nullptr, *QT_perm_ = nullptr;
     int *var_off_ = nullptr, *row_off_ = nullptr;
     size_t num_var_cones_ = 0, num_row_cones_ = 0;
+    int gram_mode_ = 0;                 // 2: the equality-row metric is requested (AUTO resolved)
+    std::unique_ptr<GramMetric> gram_;  // set when the metric was accepted
     double *arena_ = nullptr;
     double *partials_ = nullptr;  // [4 regions][max_grid * kMaxTerms]
     size_t region_ = 0;
@@ -1011,6 +1026,7 @@ private:
 void HipBackend::release()
 {
     if (stream_) (void)hipStreamSynchronize(stream_);
+    gram_.reset();
     free_panels(Qp_);
     free_panels(Ap_);
     free_panels(ATp_);
@@ -1411,6 +1427,49 @@ HPRQPStatus HipBackend::workspace_setup(const HPRQPModel *md)
         noC_ = 0;
         if (m_ > 0 && lambda_A_ <= 0.0) lambda_A_ = 1.0;
     }
+    if (gram_mode_ == 2) {
+        // control.cu:222-239: the metric is set up on the scaled model; a
+        // declined metric leaves sigma, lambda_A and noC as they are
+        std::vector<int> rp(m_ + 1), ci(A_.nnz);
+        std::vector<double> val(A_.nnz);
+        HPRQP_HIP(hipMemcpyAsync(rp.data(), A_.rp, sizeof(int) * (m_ + 1), hipMemcpyDeviceToHost, stream_));
+        if (A_.nnz) {
+            HPRQP_HIP(hipMemcpyAsync(ci.data(), A_.ci, sizeof(int) * A_.nnz, hipMemcpyDeviceToHost, stream_));
+            HPRQP_HIP(hipMemcpyAsync(val.data(), A_.val, sizeof(double) * A_.nnz, hipMemcpyDeviceToHost, stream_));
+        }
+        HPRQP_HIP(hipStreamSynchronize(stream_));
+        GramInputs in;
+        in.n = (int)n_;
+        in.m = (int)m_;
+        in.rp = rp.data();
+        in.ci = ci.data();
+        in.val = val.data();
+        in.AL = al.data();
+        in.AU = au.data();
+        in.c = c.data();
+        in.soc_rhs = num_row_cones_ ? rhs.data() : nullptr;
+        in.num_var_cones = num_var_cones_;
+        in.num_row_cones = num_row_cones_;
+        in.lin_m = (int)lin_m_;
+        in.Q_is_diag = Q_is_diag_;
+        in.noC = noC_;
+        gram_.reset(new GramMetric());
+        double sg = sigma_, lam = lambda_A_;
+        const char *reason = nullptr;
+        st = gram_->setup(in, stream_, &sg, &lam, &reason);
+        if (st != HPRQP_STATUS_OK) return st;
+        if (gram_->kind()) {
+            sigma_ = sg;
+            lambda_A_ = lam;
+            noC_ = 0;
+            if (Q_is_diag_)
+                map(n_, MapDiagFactors{{}, vec(V_fact1), vec(V_fact2), vec(V_fact), vec(V_fact_M), vec(V_diag_Q), sigma_});
+        } else {
+            gram_.reset();
+            const char *v = std::getenv("HPRQP_DIAGNOSTICS");
+            if (v && !std::strcmp(v, "1")) std::fprintf(stderr, "GRAM_SKIP reason=%s\n", reason ? reason : "declined");
+        }
+    }
     to_check_ = 1;
     HPRQP_HIP(hipGetLastError());
     return HPRQP_STATUS_OK;
@@ -1425,6 +1484,14 @@ HPRQPStatus HipBackend::setup(const HPRQPModel *model, const HPRQPParams *params
     }
     device_ = params->device_number;
     if (device_ < 0 || device_ >= ndev) return HPRQP_STATUS_INVALID_ARGUMENT;
+    // solver.cu:145-151: AUTO asks for the metric only when the built model
+    // passes the structural rule of gram_policy.cpp
+    if (params->gram_mode < HPRQP_GRAM_OFF || params->gram_mode > HPRQP_GRAM_REQUEST)
+        return HPRQP_STATUS_INVALID_ARGUMENT;
+    gram_mode_ = (params->gram_mode == HPRQP_GRAM_REQUEST ||
+                  (params->gram_mode == HPRQP_GRAM_AUTO && gram_auto_eligible(model)))
+                     ? 2
+                     : 0;
     HPRQP_HIP(hipSetDevice(device_));
     HPRQP_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     obj_constant_ = model->obj_constant;
@@ -1454,7 +1521,7 @@ HPRQPStatus HipBackend::iteration_step(size_t inner)
     const double f2s = 1.0 / (1.0 + sigma_ * lambda_Q_);
     const double f1s = 1.0 - f2s;
     const bool chk = to_check_ != 0, diag = Q_is_diag_ != 0;
-    if (num_var_cones_ || num_row_cones_) return soc_iteration_step(inner);
+    if (num_var_cones_ || num_row_cones_ || gram_) return soc_iteration_step(inner);
     // the check variant of the w2 step stores A^T y_bar (reused by the residuals)
     aty_bar_valid_ = chk && !noC_;
     if (noC_) {
@@ -1538,7 +1605,10 @@ HPRQPStatus HipBackend::compute_residuals(HPRQPResiduals *res, size_t iter)
         }
     }
     if (m_ > 0) {
-        spmv(A_, vec(V_x_bar), EpiResA{nullptr, vec(V_Ax), vec(V_AL), vec(V_AU), vec(V_row_norm), vec(V_y_bar), vec(V_s), b_scale_}, 2);
+        // with the plain-QP metric the dual objective pairs y_bar with the
+        // projection of A x_bar (the reference GPU's convention,
+        // residuals.cu:28-50, 138-147); otherwise with the iteration's s
+        spmv(A_, vec(V_x_bar), EpiResA{nullptr, vec(V_Ax), vec(V_AL), vec(V_AU), vec(V_row_norm), vec(V_y_bar), vec(V_s), b_scale_, gram_ ? 1 : 0}, 2);
         finish(2, red_blocks(A_), 1, 2, 6);
     }
     if (iter == 0) {
@@ -1606,6 +1676,11 @@ HPRQPStatus HipBackend::soc_iteration_step(size_t inner)
         ey.AL = vec(V_AL);
         ey.AU = vec(V_AU);
         ey.lin_m = (int)lin_m_;
+        if (gram_) {
+            ey.gmap = gram_->map();
+            ey.gr = gram_->rhs();
+            ey.gb = gram_->b();
+        }
         spmv(A_, vec(V_tempv), ey, 0, T_Y);
         if (num_row_cones_) {
             if (profiling_) prof_begin(T_CONE_Y);
@@ -1613,6 +1688,11 @@ HPRQPStatus HipBackend::soc_iteration_step(size_t inner)
                                       0, stream_>>>((int)num_row_cones_, row_off_, vec(V_Rp_soc()), vec(V_s));
             map(m_ - lin_m_, MapSocFinishY{{}, fy, (int)lin_m_});
             if (profiling_) prof_end();
+        }
+        // qp_row_metric.cu:55-63 / row_metric.cu:46-55: the metric rows
+        if (gram_) {
+            HPRQPStatus st = gram_->finish_y(stream_, vec(V_y), vec(V_y_bar), vec(V_dy), vec(V_last_y), sigma_, a, b);
+            if (st != HPRQP_STATUS_OK) return st;
         }
     }
     const double fs = sigma_ / (1.0 + sigma_ * lambda_Q_);
@@ -1695,6 +1775,8 @@ double HipBackend::compute_m_norm()
     }
     if (hipGetLastError() != hipSuccess || read_red(9) != HPRQP_STATUS_OK) return INFINITY;
     const double *r = red_host_;
+    double gram_q = 0.0;  // control.cu:273-279: the metric's quadratic in dy
+    if (gram_ && m_ > 0 && gram_->quadratic(stream_, vec(V_dy), &gram_q) != HPRQP_STATUS_OK) return INFINITY;
     const double dx_sq = r[0], dqw_dx = r[1], dw_dqw = r[2], dqw_sq = r[3];
     const double sg = sigma_;
     double m1 = 0.0, m3 = 0.0;
@@ -1702,7 +1784,7 @@ double HipBackend::compute_m_norm()
     m2 -= 2.0 * dqw_dx;
     if (m_ > 0) {
         const double atdy_dx = r[4], dqw_atdy = r[5], atdy_qatdy = r[6], fm_term = r[7], dy_sq = r[8];
-        m1 = sg * lambda_A_ * dy_sq;
+        m1 = gram_ ? sg * gram_q : sg * lambda_A_ * dy_sq;
         m1 -= 2.0 * sg * dqw_atdy;
         m2 += 2.0 * atdy_dx;
         if (diag) {
@@ -1746,7 +1828,13 @@ HPRQPStatus HipBackend::update_sigma(HPRQPRestart *restart, const HPRQPResiduals
     double b = r[0], a = 0.0, cterm = 0.0, dterm = 0.0;
     const double dy_sq = r[1], dqw_atdy = r[2], dqw_sq = r[3], dw_dqw = r[4], atdy_qatdy = r[5];
     if (m_ > 0) {
-        a = lambda_A_ * dy_sq;
+        if (gram_) {
+            // control.cu:400-404
+            st = gram_->quadratic(stream_, vec(V_dy), &a);
+            if (st != HPRQP_STATUS_OK) return st;
+        } else {
+            a = lambda_A_ * dy_sq;
+        }
         a -= 2.0 * dqw_atdy;
     }
     if (diag) {
@@ -1931,6 +2019,10 @@ double HipBackend::scalar(const char *name) const
     if (!std::strcmp(name, "lds_pad_Q")) return Qp_.lds_pad;
     if (!std::strcmp(name, "lds_pad_A")) return Ap_.lds_pad;
     if (!std::strcmp(name, "lds_pad_AT")) return ATp_.lds_pad;
+    if (!std::strcmp(name, "gram_active")) return gram_ ? gram_->kind() : 0;
+    if (!std::strcmp(name, "gram_rows")) return gram_ ? gram_->rows() : 0;
+    if (!std::strcmp(name, "gram_beta")) return gram_ ? gram_->beta() : 0.0;
+    if (!std::strcmp(name, "gram_mode")) return gram_mode_;
     return NAN;
 }
 
@@ -2008,6 +2100,7 @@ HPRQPStatus hprqp_hip_session_advance(HPRQPHipSession *s, size_t bodies, int *fi
 size_t hprqp_hip_session_read(const HPRQPHipSession *s, const char *name, double *out, size_t capacity)
 {
     if (!s || !name) return 0;
+    if (!std::strncmp(name, "gram_", 5)) return s->backend.gram_read(name, out, capacity);
     for (int v = 0; v < hprqp::V_M_END; ++v) {
         if (std::strcmp(hprqp::kVecNames[v], name) != 0) continue;
         size_t len = s->backend.vec_len(v);

@@ -345,6 +345,7 @@ HPRQPStatus SolveLoop::collect(HPRQPResults *results)
     results->status = finished_ ? final_status_ : HPRQP_STATUS_CONTINUE;
     results->polish_attempts = polisher_ ? polisher_->attempts() : polish_attempts_;
     results->polish_accepted = polisher_ ? (int)polisher_->accepted() : (int)polish_accepted_;
+    results->gram_active = b_->gram_active();
     return HPRQP_STATUS_OK;
 }
 

@@ -31,6 +31,8 @@ public:
     virtual HPRQPStatus restart_apply(HPRQPRestart *restart) = 0;
     virtual double compute_m_norm() = 0;
     virtual HPRQPStatus iteration_step(size_t inner) = 0;
+    // 1 when the equality-row (Gram) metric runs (results.gram_active)
+    virtual int gram_active() const { return 0; }
 };
 
 // Restart bookkeeping, src/cpu/control.c:129-142, 231-305.

@@ -159,6 +159,7 @@ class Result:
     total_time: float = 0.0
     polish_attempts: int = 0
     polish_accepted: bool = False
+    gram_active: bool = False
 
 
 class Parameters:
@@ -335,7 +336,7 @@ def _vec(p, n):
 def _result_of(r: CResults, st: int, n: int, m: int) -> Result:
     return Result(st, status_name(st), r.iter, r.iter_4, r.iter_6, r.time, r.time_4, r.time_6, r.primal_obj,
                   r.residuals, r.gap, _vec(r.x, n), _vec(r.y, m), _vec(r.z, n), _vec(r.w, n), r.total_time,
-                  int(r.polish_attempts), bool(r.polish_accepted))
+                  int(r.polish_attempts), bool(r.polish_accepted), bool(r.gram_active))
 
 
 def optimize(model: Model, params: Parameters | None = None) -> Result:
