@@ -22,6 +22,9 @@ JSON line ("workloads"), the other workloads the metric names:
                 against sequential plain GPU solves and host-core solves
   layer_1024    forward + backward ms of the differentiable layer on 1,024
                 dense QPs (n = 256, m = 320) held as ROCm tensors
+  gram_grid63   time to 1e-8 of the grid-control model (g = 63) with the
+                equality-row (Gram) metric that AUTO takes, against the same
+                model with the metric off
 Each carries its own roofline and cpu_baseline; --workload NAME runs one.
 
   roofline      the dominant kernel's algorithmic bytes per launch / its mean
@@ -98,9 +101,17 @@ def banded_variant(qp):
 
 
 def build_model(H, qp):
-    soc = {"var_offsets": qp.var_offsets} if qp.num_var_cones else None
+    soc = None
+    if qp.num_var_cones or qp.num_row_cones:
+        soc = {}
+        if qp.num_var_cones:
+            soc["var_offsets"] = qp.var_offsets
+        if qp.num_row_cones:
+            soc["row_offsets"] = qp.row_offsets
+            soc["rhs"] = qp.soc_rhs
     return H.build_from_qabc((qp.n, qp.n, qp.q_ptr, qp.q_ind, qp.q_val), qp.c,
-                             (qp.m, qp.n, qp.a_ptr, qp.a_ind, qp.a_val), qp.AL, qp.AU, qp.l, qp.u, soc=soc)
+                             (qp.m, qp.n, qp.a_ptr, qp.a_ind, qp.a_val), qp.AL, qp.AU, qp.l, qp.u,
+                             qp.obj_constant, soc=soc)
 
 
 def profile_kernels(H, model, iters):
@@ -417,6 +428,74 @@ def bench_batched(H, Q, args):
 
 # ----------------------------------------------------------------- layer
 
+def bench_gram(H, Q, args, steps, warmup, g=63):
+    """The equality-row (Gram) metric on the grid-control model at g = 63
+    (3,969 metric rows, n = 4,473, m = 4,725, 252 row cones), the size of the
+    reference's live AUTO case: time to 1e-8 with the metric (AUTO takes it)
+    against the same model with the metric off; the roofline of the metric's
+    solve (two triangular matrix-vector products over W = L^-1 and W', each
+    e(e+1)/2 entries, streamed from HBM)."""
+    qp = Q.gram_fixture_spatial(g)
+    model = build_model(H, qp)
+    if args.profile_only:
+        for p in profile_kernels(H, model, args.profile_iters):
+            log(f"kernel {p['kernel']}: {p['mean_ms'] * 1e3:.1f} us/launch, {p['bytes'] / 1e6:.2f} MB, "
+                f"{p['gbs']:.0f} GB/s")
+        return None
+    for w in range(warmup):
+        H.optimize(model, H.Parameters(verbose=False, max_iter=50, stoptol=args.stoptol))
+    results = []
+    for k in range(steps):
+        r = H.optimize(model, H.Parameters(verbose=False, stoptol=args.stoptol))
+        log(f"gram_grid{g} step {k}: {r.status} gram_active={r.gram_active} iter={r.iter} solve={r.time:.4f}s "
+            f"total={r.total_time:.3f}s kkt={r.residual:.3e} obj={r.objective:.12g}")
+        results.append(r)
+    off = H.optimize(model, H.Parameters(verbose=False, stoptol=args.stoptol, gram_mode=0, max_iter=200000))
+    log(f"gram_grid{g} metric off: {off.status} iter={off.iter} solve={off.time:.4f}s obj={off.objective:.12g}")
+    value = float(np.mean([r.time for r in results]))
+    prof = profile_kernels(H, model, args.profile_iters)
+    for p in prof:
+        log(f"gram_grid{g} kernel {p['kernel']}: {p['mean_ms'] * 1e3:.1f} us/launch, {p['bytes'] / 1e6:.2f} MB, "
+            f"{p['gbs']:.0f} GB/s")
+    roofline = None
+    gram = [p for p in prof if p["kernel"] == "gram_solve"]
+    if gram:
+        p = gram[0]
+        roofline = {"bound": "hbm", "kernel": "gram_solve (W and W' triangular products + y finish)",
+                    "achieved": round(p["gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(p["gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
+                    "kernel_us": round(p["mean_ms"] * 1e3, 2),
+                    "iteration_us": round(sum(q["mean_ms"] for q in prof) * 1e3, 1),
+                    "kernels_us": {q["kernel"]: round(q["mean_ms"] * 1e3, 2) for q in prof}}
+    cpu = None
+    if not args.no_cpu:
+        # the oracle (one core) with the same metric: its setup (a dense
+        # Cholesky of G) and then loop bodies 1..100, scaled to the GPU's
+        # iteration count
+        t0 = time.time()
+        o = Q.OracleSession(qp, stoptol=args.stoptol)
+        setup = time.time() - t0
+        o.advance(1)
+        t1 = time.time()
+        o.advance(100)
+        per = (time.time() - t1) / 100
+        iters = int(results[-1].iter)
+        cpu = dict(value=round(per * iters, 3), unit="s (est. solve loop to 1e-8)", cores=1, kind="port",
+                   sample=f"oracle with the Gram metric (dense Cholesky of G, {setup:.1f} s setup), loop bodies "
+                          f"1..100: {per * 1e3:.2f} ms/iteration x {iters} iterations", host=host_info())
+        cpu["speedup"] = round(cpu["value"] / value, 2) if value > 0 else None
+        log(f"gram_grid{g} cpu_baseline:", cpu)
+    r = results[-1]
+    return {"value": round(value, 5), "unit": "s", "ms_per_step": round(value * 1e3, 3), "steps": steps,
+            "config": {"workload": f"gram_grid{g}", "n": qp.n, "m": qp.m, "metric_rows": g * g,
+                       "row_cones": qp.num_row_cones, "gram_active": bool(r.gram_active), "stoptol": args.stoptol,
+                       "iterations": int(r.iter), "status": r.status, "objective": r.objective, "kkt": r.residual,
+                       "metric_off": {"status": off.status, "iterations": int(off.iter), "solve_s": round(off.time, 4),
+                                      "objective": off.objective},
+                       "speedup_vs_metric_off": round(off.time / value, 2) if value > 0 else None},
+            "roofline": roofline, "cpu_baseline": cpu}
+
+
 def bench_layer(H, Q, args, steps, warmup):
     """1024 dense QPs (n=256, m=320) held as FP64 tensors on the GPU (the
     zero-copy path: no array crosses PCIe); forward to 1e-8 and backward
@@ -501,7 +580,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="all",
                     choices=["all", "plain_n1M", "plain_n2k", "plain_n1M_banded", "plain_n1M_no_heavy",
-                             "plain_n1M_only_heavy", "soc_n220k", "batched_4096", "layer_1024"])
+                             "plain_n1M_only_heavy", "soc_n220k", "batched_4096", "layer_1024", "gram_grid63"])
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--seq-sample", type=int, default=32)
     ap.add_argument("--stoptol", type=float, default=1e-8)
@@ -541,6 +620,11 @@ def main():
     if w == "layer_1024":
         print(json.dumps(emit("QP layer fwd+bwd ms (1 GPU)", bench_layer(H, Q, args, args.steps, args.warmup),
                               False)), flush=True)
+        return
+    if w == "gram_grid63":
+        res = bench_gram(H, Q, args, args.steps, args.warmup)
+        if res:
+            print(json.dumps(emit("time-to-1e-8 KKT (Gram metric, 1 GPU)", res, False)), flush=True)
         return
     if w == "soc_n220k":
         qp = Q.soc_qp(**Q.SOC_N220K)
@@ -590,6 +674,8 @@ def main():
     subs["batched_4096"] = {"metric": "batched QPs/s at 1e-6 (1 GPU)", "higher_is_better": True, **r}
     r = bench_layer(H, Q, args, 3, 1)
     subs["layer_1024"] = {"metric": "QP layer fwd+bwd ms (1 GPU)", "higher_is_better": False, **r}
+    r = bench_gram(H, Q, args, 3, 1)
+    subs["gram_grid63"] = {"metric": "time-to-1e-8 KKT (Gram metric, 1 GPU)", "higher_is_better": False, **r}
     line["workloads"] = subs
     print(json.dumps(line), flush=True)
 

@@ -891,12 +891,13 @@ public:
 
     // Per-kernel timing of the iteration kernels with HIP events recorded on
     // this backend's own stream (profile()).
-    enum Tag { T_ZXW1, T_TEMPV, T_Y, T_W2, T_ZXW1_NOC, T_QW_NOC, T_CONE_X, T_CONE_Y, T_COUNT };
+    enum Tag { T_ZXW1, T_TEMPV, T_Y, T_W2, T_ZXW1_NOC, T_QW_NOC, T_CONE_X, T_CONE_Y, T_GRAM, T_COUNT };
     HPRQPStatus profile(size_t iterations, double *mean_ms, double *bytes, size_t *launches);
     double algorithmic_bytes(int tag) const;
     static const char *tag_name(int tag)
     {
-        static const char *names[] = {"zxw1_q", "tempv_q", "y_a", "w2_at", "zxw1_noc_at", "qw_noc_q", "cone_x", "cone_y"};
+        static const char *names[] = {"zxw1_q",   "tempv_q", "y_a",    "w2_at",     "zxw1_noc_at",
+                                      "qw_noc_q", "cone_x",  "cone_y", "gram_solve"};
         return names[tag];
     }
 
@@ -1691,7 +1692,9 @@ HPRQPStatus HipBackend::soc_iteration_step(size_t inner)
         }
         // qp_row_metric.cu:55-63 / row_metric.cu:46-55: the metric rows
         if (gram_) {
+            if (profiling_) prof_begin(T_GRAM);
             HPRQPStatus st = gram_->finish_y(stream_, vec(V_y), vec(V_y_bar), vec(V_dy), vec(V_last_y), sigma_, a, b);
+            if (profiling_) prof_end();
             if (st != HPRQP_STATUS_OK) return st;
         }
     }
@@ -1955,6 +1958,11 @@ double HipBackend::algorithmic_bytes(int tag) const
     case T_CONE_Y:  // cone range of y: projection (Rp in, s out), then the finish
         // (reads y, s, Rp, last_y; writes dy, y_bar, y)
         return 8.0 * (m - (double)lin_m_) * (2.0 + 4.0 + 3.0) + 4.0 * (num_row_cones_ + 1);
+    case T_GRAM: {  // the two triangles of W and W' (e(e+1)/2 entries each), r, u, step;
+        // the finish reads y, last_y and writes y, y_bar, dy of the metric rows
+        const double e = gram_ ? (double)gram_->rows() : 0.0;
+        return 8.0 * e * (e + 1.0) + 8.0 * e * (3.0 + 2.0 + 3.0) + 4.0 * e;
+    }
     default:
         return 0.0;
     }
