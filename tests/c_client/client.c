@@ -114,6 +114,21 @@ static int model_api_equality(void)
            "y0=%.17g\n",
            hprqp_status_name(st), hprqp_status_name((HPRQPStatus)res.status), res.iter, res.primal_obj, res.residuals,
            res.gap, res.x ? res.x[0] : NAN, res.x ? res.x[1] : NAN, res.y ? res.y[0] : NAN);
+    /* the equality-row metric asked for, as solve_qp --gram request does
+     * (examples/solve_qp.c:223-227): taken for the one equality row, the
+     * same optimum; AUTO declines a plain QP */
+    for (int mode = HPRQP_GRAM_AUTO; mode <= HPRQP_GRAM_REQUEST; ++mode) {
+        HPRQPResults g;
+        results_init(&g);
+        params.gram_mode = (HPRQPGramMode)mode;
+        st = solve(&model, &params, &g);
+        printf("model_api_gram_%s rc=%s status=%s gram_active=%d objective=%.17g x0=%.17g x1=%.17g\n",
+               mode == HPRQP_GRAM_AUTO ? "auto" : "request", hprqp_status_name(st),
+               hprqp_status_name((HPRQPStatus)g.status), g.gram_active, g.primal_obj, g.x ? g.x[0] : NAN,
+               g.x ? g.x[1] : NAN);
+        results_free(&g);
+    }
+    params.gram_mode = HPRQP_GRAM_AUTO;
     /* the CPU path is the reference's, not this library's */
     params.use_gpu = 0;
     HPRQPResults cpu;

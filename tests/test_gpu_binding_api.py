@@ -58,6 +58,14 @@ def test_c_client_cases(H):
     assert e["rc"] == e["status"] == "OPTIMAL"
     assert abs(float(e["objective"]) - 213 / 36) <= 1e-6
     assert abs(float(e["x0"]) - 7 / 6) <= 1e-6 and abs(float(e["x1"]) - 5 / 6) <= 1e-6
+    # gram_mode through the C struct: REQUEST takes the metric on the equality
+    # row and reaches the same optimum; AUTO leaves a plain QP alone
+    ga, gr = lines["model_api_gram_auto"], lines["model_api_gram_request"]
+    assert ga["status"] == gr["status"] == "OPTIMAL"
+    assert ga["gram_active"] == "0" and gr["gram_active"] == "1"
+    for g in (ga, gr):
+        assert abs(float(g["objective"]) - 213 / 36) <= 1e-6
+        assert abs(float(g["x0"]) - 7 / 6) <= 1e-6 and abs(float(g["x1"]) - 5 / 6) <= 1e-6
     assert lines["gpu_backend_available"]["value"] == "1"
     assert lines["model_api_cpu"]["rc"] == "NOT_IMPLEMENTED"
 

@@ -169,3 +169,22 @@ def test_gram_grid_auto_reaches_the_optimum(H, g):
     assert not off.gram_active and off.status == "OPTIMAL"
     assert abs(on.objective - off.objective) <= 1e-7 * (1 + abs(off.objective))
     assert on.iter < off.iter
+
+
+def test_gram_closed_forms_unscaled_mixed_gpu(H):
+    """'mixed' without scaling on the device: one metric row, G = [5], beta =
+    (1 + 1e-12), sigma = sqrt(1.25^2 / 5 + 2 / beta) / ||c||_2, lambda_A =
+    2 beta, and the first step's metric solve r / 5."""
+    import math
+    qp, _ = Q.gram_fixture_qp("mixed")
+    s = H.Session(build(H, qp), H.Parameters(verbose=False, gram_mode=2, use_scaling=0))
+    beta = 1 + 1e-12
+    assert s.scalar("gram_active") == 1 and s.scalar("gram_rows") == 1
+    assert s.scalar("gram_beta") == beta and s.scalar("lambda_max_A") == 2 * beta
+    c = -np.array([[2.0, 0.3], [0.3, 1.0]]) @ [0.25, 0.5]
+    want = math.sqrt(1.25 ** 2 / 5.0 + 2.0 / beta) / np.linalg.norm(c)
+    assert abs(s.scalar("sigma") - want) <= 1e-15 * want
+    s.advance(2)
+    r, step = s.read("gram_r"), s.read("gram_step")
+    assert r.shape == step.shape == (1,)
+    assert abs(step[0] - r[0] / 5.0) <= 1e-15 * (1 + abs(r[0]))
