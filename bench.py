@@ -461,9 +461,12 @@ def bench_gram(H, Q, args, steps, warmup, g=63):
     gram = [p for p in prof if p["kernel"] == "gram_solve"]
     if gram:
         p = gram[0]
+        data = _profile_json(f"gram_solve_gram_grid{g}.md")
         roofline = {"bound": "hbm", "kernel": "gram_solve (W and W' triangular products + y finish)",
                     "achieved": round(p["gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(p["gbs"] / HBM_PEAK_GBS, 4), "traffic": None,
+                    "frac": round(p["gbs"] / HBM_PEAK_GBS, 4),
+                    "traffic": data.get("traffic_bytes_per_step") if data else None,
+                    "traffic_source": f"profiles/gram_solve_gram_grid{g}.md" if data else None,
                     "kernel_us": round(p["mean_ms"] * 1e3, 2),
                     "iteration_us": round(sum(q["mean_ms"] for q in prof) * 1e3, 1),
                     "kernels_us": {q["kernel"]: round(q["mean_ms"] * 1e3, 2) for q in prof}}
