@@ -21,6 +21,7 @@
 #include <cstring>
 #include <memory>
 #include <mutex>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -1457,7 +1458,19 @@ HPRQPStatus HipBackend::workspace_setup(const HPRQPModel *md)
         gram_.reset(new GramMetric());
         double sg = sigma_, lam = lambda_A_;
         const char *reason = nullptr;
-        st = gram_->setup(in, stream_, &sg, &lam, &reason);
+        // qp_row_metric.cu:136-147, row_metric.cu:133-146: running out of host
+        // or device memory declines the metric; other device errors end the solve
+        try {
+            st = gram_->setup(in, stream_, &sg, &lam, &reason);
+        } catch (const std::bad_alloc &) {
+            st = HPRQP_STATUS_ALLOCATION_FAILED;
+        }
+        if (st == HPRQP_STATUS_ALLOCATION_FAILED) {
+            (void)hipGetLastError();
+            gram_.reset(new GramMetric());
+            reason = "allocation";
+            st = HPRQP_STATUS_OK;
+        }
         if (st != HPRQP_STATUS_OK) return st;
         if (gram_->kind()) {
             sigma_ = sg;

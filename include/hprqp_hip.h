@@ -32,12 +32,15 @@ void hprqp_hip_session_destroy(HPRQPHipSession *session);
 HPRQPStatus hprqp_hip_session_advance(HPRQPHipSession *session, size_t bodies, int *finished);
 /* Copies the named device vector (x, x_bar, w, w_bar, y, y_bar, z_bar, s, ATy,
  * Qw, x_hat, last_x, last_w, last_y, last_ATy, dx, dy, dw, c, l, u, AL, AU,
- * row_norm, col_norm) to host; returns its length (0: unknown name). */
+ * row_norm, col_norm; with the equality-row metric active also gram_r, the
+ * last step's E t - b of the metric rows, and gram_step, its G^{-1} solve)
+ * to host; returns its length (0: unknown name). */
 size_t hprqp_hip_session_read(const HPRQPHipSession *session, const char *name, double *out, size_t capacity);
 /* Named scalars (sigma, lambda_max_A, lambda_max_Q, power_steps_A and
  * power_steps_Q (products taken by the two power iterations), Q_is_diag, noC, iter,
  * inner, restart_times, kkt, err_rp, err_rd, gap, pobj, dobj, norm_b_org,
- * norm_c_org); NaN if unknown. */
+ * norm_c_org, gram_active (0 off, 1 plain-QP metric, 2 row-cone prefix),
+ * gram_rows, gram_beta, gram_mode); NaN if unknown. */
 double hprqp_hip_session_scalar(const HPRQPHipSession *session, const char *name);
 /* Scaled device CSR of "Q", "A" or "AT" copied to host (int32 indices
  * widened to size_t); returns nnz. */
@@ -50,7 +53,8 @@ HPRQPStatus hprqp_hip_session_results(HPRQPHipSession *session, HPRQPResults *re
  * session's iterate) with HIP events on the session's stream. For each
  * iteration kernel kind k < return value: names + 32*k (NUL-terminated),
  * mean_ms[k] per launch, bytes_per_launch[k] (algorithmic HBM bytes),
- * launches[k]. capacity >= 8. Returns -1 on error. */
+ * launches[k]. capacity >= 9 (the kinds: zxw1_q, tempv_q, y_a, w2_at,
+ * zxw1_noc_at, qw_noc_q, cone_x, cone_y, gram_solve). Returns -1 on error. */
 int hprqp_hip_session_profile(HPRQPHipSession *session, size_t iterations, int capacity, char *names,
                               double *mean_ms, double *bytes_per_launch, size_t *launches);
 
