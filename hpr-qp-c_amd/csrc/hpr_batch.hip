@@ -1113,16 +1113,18 @@ __device__ void adj_op(WG &g, const AdjMask &mk, const double *xn, const double 
 }
 
 // Sums of kAdjRows vectors over a wavefront in one pass (a reduce-scatter
-// by halves, then one exchange): v[u] holds lane's term of row u on entry;
-// on exit lane L holds the wavefront's sum of row L >> 1 in v[0]. Fixed
-// pairing, so deterministic.
-constexpr int kAdjRows = 32;  // rows of A per chunk: loads in flight per thread
+// by halves, then exchanges over the remaining low lane bits): v[u] holds
+// lane's term of row u on entry; on exit lane L holds the wavefront's sum
+// of row L / (kWave / kAdjRows) in v[0]. Fixed pairing, so deterministic.
+constexpr int kAdjRows = 16;  // rows of A per chunk: loads in flight per thread
+constexpr int kAdjStages = 4;  // log2(kAdjRows)
+static_assert((1 << kAdjStages) == kAdjRows && kAdjRows <= kWave, "reduce-scatter shape");
 
 __device__ inline void wave_reduce_scatter(double (&v)[kAdjRows], int lane)
 {
 #pragma unroll
-    for (int st = 0; st < 5; ++st) {
-        const int o = 32 >> st;
+    for (int st = 0; st < kAdjStages; ++st) {
+        const int o = (kWave / 2) >> st;
         const int h = kAdjRows >> (st + 1);
         const bool up = (lane & o) != 0;
 #pragma unroll
@@ -1132,7 +1134,8 @@ __device__ inline void wave_reduce_scatter(double (&v)[kAdjRows], int lane)
             v[k] = mine + __shfl_xor(send, o, kWave);
         }
     }
-    v[0] += __shfl_xor(v[0], 1, kWave);
+#pragma unroll
+    for (int o = (kWave / kAdjRows) / 2; o > 0; o >>= 1) v[0] += __shfl_xor(v[0], o, kWave);
 }
 
 // The same operator on a dense instance in one pass over A instead of two
@@ -1184,8 +1187,9 @@ __device__ void adj_op_dense(WG &g, const AdjMask &mk, const double *xn, const d
                     a[u] = __builtin_nontemporal_load(Ad + (long long)min(i0 + kAdjRows + u, m - 1) * n + jc);
             }
             wave_reduce_scatter(p, lane);
-            const int r = i0 + (lane >> 1);
-            if ((lane & 1) == 0 && r < m) wr[r] = p[0];
+            constexpr int kLanesPerRow = kWave / kAdjRows;
+            const int r = i0 + lane / kLanesPerRow;
+            if (lane % kLanesPerRow == 0 && r < m) wr[r] = p[0];
         }
         if (col) yn[j] = mk.free_var(j) ? qx + acc : 0.0;
         __syncthreads();
