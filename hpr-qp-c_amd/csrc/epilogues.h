@@ -463,6 +463,15 @@ struct MapScaleDiv : NoRed {
     __device__ void row(int i, double *, double *) const { q[i] = z[i] / d; }
 };
 
+// q = z / sqrt(||z||^2) with ||z||^2 read on the device (IEEE sqrt and
+// division, so the same bits as dividing by the host's std::sqrt of it)
+struct MapScaleDivNorm : NoRed {
+    double *q;
+    const double *z;
+    const double *z2;
+    __device__ void row(int i, double *, double *) const { q[i] = z[i] / sqrt(*z2); }
+};
+
 struct EpiPowerZ {
     static constexpr int NS = 2, NM = 0;
     double *partials;
@@ -476,16 +485,20 @@ struct EpiPowerZ {
     }
 };
 
+// ||z - lambda q||^2 of one power step, and q = z / sqrt(||z||^2) for the
+// next step in the same pass (the residual itself is not kept); lz points to
+// the step's (lambda, ||z||^2) on the device.
 struct MapPowerResid {
     static constexpr int NS = 1, NM = 0;
     double *partials;
     double *q;
     const double *z;
-    const double *lambda;  // device scalar
+    const double *lz;
     __device__ void row(int i, double *s, double *) const
     {
-        const double r = z[i] - (*lambda) * q[i];
-        q[i] = r;
+        const double zi = z[i];
+        const double r = zi - lz[0] * q[i];
+        q[i] = zi / sqrt(lz[1]);
         s[0] += r * r;
     }
 };

@@ -474,7 +474,8 @@ __device__ double wg_power(WG &g, bool a_matrix, double *sums, double *maxs)
             lam = 0.0;
             break;
         }
-        wg_map(len, MapScaleDiv{{}, q, z, qn}, sums, maxs);
+        // later steps' q = z / qn comes out of the previous residual pass
+        if (it == 0) wg_map(len, MapScaleDiv{{}, q, z, qn}, sums, maxs);
         if (a_matrix) {
             if constexpr (WG::kDense) {
                 // z = G q with G = A A^T formed once: the same iteration as

@@ -66,6 +66,13 @@ namespace {
 
 inline int grid_for(size_t len) { return (int)((len + kThreads - 1) / kThreads); }
 
+// red_ holds the reduction results: indices below kPowSlots for the residuals,
+// the sigma update and the setup; from kPowSlots one slot of three per power
+// step of a run.
+constexpr int kPowRun = 16;  // power-iteration steps issued per host read
+constexpr int kPowSlots = 64;
+constexpr int kRedLen = kPowSlots + 3 * kPowRun;
+
 // ------------------------------------------------------------------ device CSR
 
 struct DevCsr {
@@ -1145,8 +1152,8 @@ HPRQPStatus HipBackend::upload(const HPRQPModel *md)
                                 (size_t)m_grid(), (size_t)1});
     region_ = max_grid * kMaxTerms;
     HPRQP_HIP(hipMalloc(&partials_, sizeof(double) * region_ * 4));
-    HPRQP_HIP(hipMalloc(&red_, sizeof(double) * 64));
-    HPRQP_HIP(hipHostMalloc(reinterpret_cast<void **>(&red_host_), sizeof(double) * 64, hipHostMallocDefault));
+    HPRQP_HIP(hipMalloc(&red_, sizeof(double) * kRedLen));
+    HPRQP_HIP(hipHostMalloc(reinterpret_cast<void **>(&red_host_), sizeof(double) * kRedLen, hipHostMallocDefault));
     return HPRQP_STATUS_OK;
 }
 
@@ -1323,6 +1330,15 @@ HPRQPStatus HipBackend::autotune_panels()
 }
 
 // eigs.c:19-153: power iteration on A A^T (q_matrix = false) or Q.
+//
+// The steps are issued in runs of kPowRun without waiting for the host: the
+// residual pass of each step also forms the next step's q = z / ||z|| from
+// ||z||^2 on the device (the same correctly rounded square root the host
+// takes), and every step leaves (lambda, ||z||^2, ||z - lambda q||^2) in its
+// own slot of red_. After a run the host reads the slots and replays the
+// stopping rule step by step, so lambda and the step count are those of the
+// one-step-at-a-time loop; the steps issued past the stopping step only
+// overwrite the scratch vectors.
 HPRQPStatus HipBackend::power_iteration(bool q_matrix, double *lambda, int *steps)
 {
     const size_t len = q_matrix ? n_ : m_;
@@ -1341,26 +1357,45 @@ HPRQPStatus HipBackend::power_iteration(bool q_matrix, double *lambda, int *step
     if (st != HPRQP_STATUS_OK) return st;
     double qn = std::sqrt(red_host_[0]);
     double lam = 1.0;
-    for (int it = 0; it < 5000; ++it) {
+    const DevCsr &M = q_matrix ? Q_ : A_;
+    map(len, MapScaleDivNorm{{}, q, z, red_});  // the first q, from the probe's ||z||^2
+    constexpr int kMaxSteps = 5000;
+    bool done = false;
+    for (int it0 = 0; it0 < kMaxSteps && !done; it0 += kPowRun) {
         if (qn <= 1e-16) {
             lam = 0.0;
             break;
         }
-        map(len, MapScaleDiv{{}, q, z, qn});
-        *steps = it + 1;
-        const DevCsr &M = q_matrix ? Q_ : A_;
-        if (!q_matrix) spmv(AT_, q, EpiStore{{}, tmp});
-        spmv(M, q_matrix ? q : tmp, EpiPowerZ{nullptr, z, q}, 0);
-        finish(0, red_blocks(M), 2, 0, 0);
-        map(len, MapPowerResid{nullptr, q, z, red_ + 0}, 1);
-        finish(1, g, 1, 0, 2);
-        st = read_red(3);
-        if (st != HPRQP_STATUS_OK) return st;
-        lam = red_host_[0];
-        const double zn = std::sqrt(red_host_[1]);
-        const double err = std::sqrt(red_host_[2]) / (zn + std::fabs(lam) + 1e-16);
-        if (err < 1e-4) break;
-        qn = zn;
+        const int run = std::min(kPowRun, kMaxSteps - it0);
+        for (int j = 0; j < run; ++j) {
+            const int slot = kPowSlots + 3 * j;
+            if (!q_matrix) spmv(AT_, q, EpiStore{{}, tmp});
+            spmv(M, q_matrix ? q : tmp, EpiPowerZ{nullptr, z, q}, 0);
+            finish(0, red_blocks(M), 2, 0, slot);
+            map(len, MapPowerResid{nullptr, q, z, red_ + slot}, 1);
+            finish(1, g, 1, 0, slot + 2);
+        }
+        HPRQP_HIP(hipGetLastError());
+        HPRQP_HIP(hipMemcpyAsync(red_host_ + kPowSlots, red_ + kPowSlots, sizeof(double) * 3 * run,
+                                 hipMemcpyDeviceToHost, stream_));
+        HPRQP_HIP(hipStreamSynchronize(stream_));
+        for (int j = 0; j < run; ++j) {
+            if (qn <= 1e-16) {  // the step after this one divided by ~0: its values are discarded
+                lam = 0.0;
+                done = true;
+                break;
+            }
+            const double *r = red_host_ + kPowSlots + 3 * j;
+            *steps = it0 + j + 1;
+            lam = r[0];
+            const double zn = std::sqrt(r[1]);
+            const double err = std::sqrt(r[2]) / (zn + std::fabs(lam) + 1e-16);
+            if (err < 1e-4) {
+                done = true;
+                break;
+            }
+            qn = zn;
+        }
     }
     *lambda = lam < 0.0 ? 0.0 : lam;
     return HPRQP_STATUS_OK;
