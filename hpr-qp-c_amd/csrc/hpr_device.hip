@@ -14,6 +14,7 @@
 #include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -72,6 +73,24 @@ inline int grid_for(size_t len) { return (int)((len + kThreads - 1) / kThreads);
 constexpr int kPowRun = 16;  // power-iteration steps issued per host read
 constexpr int kPowSlots = 64;
 constexpr int kRedLen = kPowSlots + 3 * kPowRun;
+
+// HPRQP_SETUP_TRACE=1 reports the wall time of each setup stage on stderr,
+// waiting for the stream at each mark (diagnostics only).
+struct SetupClock {
+    hipStream_t s;
+    bool on = std::getenv("HPRQP_SETUP_TRACE") != nullptr;
+    std::chrono::steady_clock::time_point t = std::chrono::steady_clock::now();
+    explicit SetupClock(hipStream_t stream) : s(stream) {}
+    void mark(const char *what)
+    {
+        if (!on) return;
+        (void)hipStreamSynchronize(s);
+        const auto now = std::chrono::steady_clock::now();
+        std::fprintf(stderr, "hprqp setup: %-34s %9.3f ms\n", what,
+                     std::chrono::duration<double, std::milli>(now - t).count());
+        t = now;
+    }
+};
 
 // ------------------------------------------------------------------ device CSR
 
@@ -1443,8 +1462,10 @@ HPRQPStatus HipBackend::workspace_setup(const HPRQPModel *md)
     Q_is_diag_ = n_ == 0 || red_host_[0] <= 1e-15;
     if (Q_is_diag_) map(n_, MapDiagFactors{{}, vec(V_fact1), vec(V_fact2), vec(V_fact), vec(V_fact_M), vec(V_diag_Q), sigma_});
     const bool cones = num_var_cones_ || num_row_cones_;
+    SetupClock clock(stream_);
     st = power_iteration(false, &lambda_A_, &power_steps_A_);
     if (st != HPRQP_STATUS_OK) return st;
+    clock.mark("  power iteration on A A'");
     lambda_A_ *= 1.05;
     if (Q_is_diag_) {
         std::vector<double> d(n_);
@@ -1458,6 +1479,7 @@ HPRQPStatus HipBackend::workspace_setup(const HPRQPModel *md)
     } else {
         st = power_iteration(true, &lambda_Q_, &power_steps_Q_);
         if (st != HPRQP_STATUS_OK) return st;
+        clock.mark("  power iteration on Q");
         lambda_Q_ *= 1.05;
     }
     if (cones) {
@@ -1554,10 +1576,15 @@ HPRQPStatus HipBackend::setup(const HPRQPModel *model, const HPRQPParams *params
     }
     norm_c_org_ = 0.0;
     for (size_t i = 0; i < model->Q.ncol; ++i) norm_c_org_ = std::max(norm_c_org_, std::fabs(model->c[i]));
+    SetupClock clock(stream_);
     HPRQPStatus st = upload(model);
+    clock.mark("upload, transpose, row blocks");
     if (st == HPRQP_STATUS_OK) st = scale(params);
+    clock.mark("scaling");
     if (st == HPRQP_STATUS_OK) st = autotune_panels();
+    clock.mark("panel and residency autotune");
     if (st == HPRQP_STATUS_OK) st = workspace_setup(model);
+    clock.mark("workspace (power iterations, sigma)");
     return st;
 }
 
