@@ -119,10 +119,11 @@ __device__ inline double wave_max(double v)
     return v;
 }
 
-// Sum over the 256-thread workgroup; the result is valid in every thread.
+// Sum over the workgroup of NTH threads; the result is valid in every thread.
+template <int NTH = kThreads>
 __device__ inline double block_sum_all(double v)
 {
-    __shared__ double part[kThreads / kWave];
+    __shared__ double part[NTH / kWave];
     const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
     v = wave_sum(v);
     __syncthreads();
@@ -130,17 +131,17 @@ __device__ inline double block_sum_all(double v)
     __syncthreads();
     double total = part[0];
 #pragma unroll
-    for (int w = 1; w < kThreads / kWave; ++w) total += part[w];
+    for (int w = 1; w < NTH / kWave; ++w) total += part[w];
     return total;
 }
 
-// Reduce NS sums and NM maxima over the workgroup and write them, in that
-// order, to out[0 .. NS+NM) (thread 0 writes).
-template <int NS, int NM>
+// Reduce NS sums and NM maxima over the workgroup of NTH threads and write
+// them, in that order, to out[0 .. NS+NM) (threads 0 .. NS+NM-1 write).
+template <int NS, int NM, int NTH = kThreads>
 __device__ inline void block_reduce_store(const double *sums, const double *maxs, double *out)
 {
     constexpr int NT = NS + NM;
-    __shared__ double part[kThreads / kWave][NT > 0 ? NT : 1];
+    __shared__ double part[NTH / kWave][NT > 0 ? NT : 1];
     const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
 #pragma unroll
     for (int t = 0; t < NS; ++t) {
@@ -156,7 +157,7 @@ __device__ inline void block_reduce_store(const double *sums, const double *maxs
     if (threadIdx.x < NT) {
         const int t = threadIdx.x;
         double acc = part[0][t];
-        for (int w = 1; w < kThreads / kWave; ++w) acc = t < NS ? acc + part[w][t] : fmax(acc, part[w][t]);
+        for (int w = 1; w < NTH / kWave; ++w) acc = t < NS ? acc + part[w][t] : fmax(acc, part[w][t]);
         out[t] = acc;
     }
 }

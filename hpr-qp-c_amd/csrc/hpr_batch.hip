@@ -143,6 +143,7 @@ constexpr int kDenseMax = kStreamCap;
 // again once the vector's lines are evicted.
 constexpr int kBatchStreamCap = 1024;
 constexpr int kStageMax = 2048;
+constexpr int kLoneThreads = 1024;  // workgroup of an instance in the batch's tail
 
 // Column-form dot of a dense view: sum_{i<K} S[i*ld + j] xs[i], i ascending.
 // CACHED: ordinary loads, 16 in flight per thread, for products that re-read
@@ -173,15 +174,66 @@ __device__ __forceinline__ double dense_column_dot(const Mat &M, int j, const do
 
 // Dense views run as "blocks" of kThreads outputs through the same epilogue
 // call as the CSR stream mode, so the epilogue is inlined once per mode.
-template <bool DENSE, bool CACHED = false, class Epi>
-__device__ void wg_spmv(const Mat &M, const double *__restrict__ x, const Epi &epi, double *sums, double *maxs,
+// NT > kThreads (CSR only): the workgroup runs as NT / kThreads groups of
+// kThreads threads, group g taking row blocks g, g + NT / kThreads, ... with
+// a staging area of its own (prod holds NT / kThreads areas); every row is
+// summed exactly as with kThreads threads, only the threads that hold the
+// epilogue's reduction terms change.
+template <bool DENSE, bool CACHED = false, int NT = kThreads, class Epi>
+__device__ __forceinline__ void wg_spmv(const Mat &M, const double *__restrict__ x, const Epi &epi, double *sums, double *maxs,
                         double *prod)
 {
     constexpr bool dense = DENSE;
+    static_assert(!DENSE || NT == kThreads, "dense views run with kThreads threads");
     const int *__restrict__ rp = M.rp;
     const int *__restrict__ ci = M.ci;
     const double *__restrict__ val = M.val;
     const int *__restrict__ blk = M.blk;
+    if constexpr (NT > kThreads) {
+        constexpr int NG = NT / kThreads;
+        const int t = threadIdx.x % kThreads, grp = threadIdx.x / kThreads;
+        const int lane = t & (kWave - 1), wid = t / kWave;
+        double *gprod = prod + (size_t)grp * kBatchStreamCap;
+        for (int b0 = 0; b0 < M.nblk; b0 += NG) {
+            const int b = b0 + grp;
+            const bool have = b < M.nblk;
+            int r0 = 0, r1 = 0, k0 = 0, k1 = 0;
+            if (have) {
+                r0 = blk[b], r1 = blk[b + 1];
+                k0 = rp[r0], k1 = rp[r1];
+            }
+            const bool wave_blk = have && (long long)(k1 - k0) >= (long long)M.wave_row * (r1 - r0);
+            // every thread reaches both barriers, whatever its group's block kind
+            if (wave_blk) {
+                for (int r = r0 + wid; r < r1; r += kThreads / kWave) {
+                    const int e = rp[r + 1];
+                    double a0 = 0.0, a1 = 0.0;
+                    int k = rp[r] + lane;
+                    for (; k + kWave < e; k += 2 * kWave) {
+                        a0 = fma(val[k], x[ci[k]], a0);
+                        a1 = fma(val[k + kWave], x[ci[k + kWave]], a1);
+                    }
+                    if (k < e) a0 = fma(val[k], x[ci[k]], a0);
+                    const double acc = wave_sum(a0 + a1);
+                    if (lane == 0) epi.row(r, acc, sums, maxs);
+                }
+            } else if (have) {
+                const int cnt = k1 - k0;
+                for (int k = t; k < cnt; k += kThreads) gprod[k] = val[k0 + k] * x[ci[k0 + k]];
+            }
+            __syncthreads();
+            const int r = r0 + t;
+            if (have && !wave_blk && r < r1) {
+                double acc = 0.0;
+                const int e = rp[r + 1] - k0;
+                for (int k = rp[r] - k0; k < e; ++k) acc += gprod[k];
+                epi.row(r, acc, sums, maxs);
+            }
+            __syncthreads();
+        }
+        __syncthreads();
+        return;
+    }
     const int t = threadIdx.x;
     const int lane = t & (kWave - 1), wid = t / kWave;
     int nblk = M.nblk;
@@ -240,30 +292,30 @@ __device__ void wg_spmv(const Mat &M, const double *__restrict__ x, const Epi &e
 // at most kStageMax long) copies the gathered vector x to LDS first and the
 // product gathers from there: the same sums in the same order.
 template <class WG, class Epi>
-__device__ void wg_product(WG &g, const Mat &M, const double *x, int xlen, const Epi &epi, double *sums,
+__device__ __forceinline__ void wg_product(WG &g, const Mat &M, const double *x, int xlen, const Epi &epi, double *sums,
                            double *maxs)
 {
     if constexpr (WG::kStage) {
-        for (int i = threadIdx.x; i < xlen; i += kThreads) g.xs[i] = x[i];
+        for (int i = threadIdx.x; i < xlen; i += WG::kNT) g.xs[i] = x[i];
         __syncthreads();
-        wg_spmv<false>(M, g.xs, epi, sums, maxs, g.prod);
+        wg_spmv<false, false, WG::kNT>(M, g.xs, epi, sums, maxs, g.prod);
     } else {
-        wg_spmv<WG::kDense>(M, x, epi, sums, maxs, g.prod);
+        wg_spmv<WG::kDense, false, WG::kNT>(M, x, epi, sums, maxs, g.prod);
     }
 }
 
-template <class F>
-__device__ void wg_map(int len, const F &f, double *sums, double *maxs)
+template <int NT = kThreads, class F>
+__device__ __forceinline__ void wg_map(int len, const F &f, double *sums, double *maxs)
 {
-    for (int i = threadIdx.x; i < len; i += kThreads) f.row(i, sums, maxs);
+    for (int i = threadIdx.x; i < len; i += NT) f.row(i, sums, maxs);
     __syncthreads();
 }
 
 // Reduce per-thread terms; every thread then reads out[0 .. NS+NM).
-template <int NS, int NM>
-__device__ void wg_reduce(double *sums, double *maxs, double *out)
+template <int NS, int NM, int NT = kThreads>
+__device__ __forceinline__ void wg_reduce(double *sums, double *maxs, double *out)
 {
-    block_reduce_store<NS, NM>(sums, maxs, out);
+    block_reduce_store<NS, NM, NT>(sums, maxs, out);
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < (NS > 0 ? NS : 1); ++i) sums[i] = 0.0;
@@ -293,10 +345,11 @@ __device__ __forceinline__ double dclamp(double v, double lo, double hi) { retur
 
 // One workgroup's view of its instance; D: the instance's matrices are dense;
 // ST: the iteration products stage their gathered vector in LDS (xs).
-template <bool D, bool ST = false>
+template <bool D, bool ST = false, int NT = kThreads>
 struct WGv {
     static constexpr bool kDense = D;
     static constexpr bool kStage = ST;
+    static constexpr int kNT = NT;  // threads of the workgroup
     const Inst *I;
     State *S;  // in LDS
     Mat Q, A, AT;
@@ -699,23 +752,23 @@ __device__ int d_check_next(const Params &p, const Restart &r, long long next)
 
 // residuals.c:88-189 (CPU convention for the dual objective)
 template <class WG>
-__device__ void wg_residuals(WG &g, long long iter, double *sums, double *maxs)
+__device__ __forceinline__ void wg_residuals(WG &g, long long iter, double *sums, double *maxs)
 {
     const int n = g.I->n, m = g.I->m;
     State &S = *g.S;
-    wg_spmv<WG::kDense>(g.Q, g.N(N_x_bar), EpiResQ{nullptr, g.N(N_Qx), g.N(N_x_bar), g.N(N_c), g.N(N_z_bar)}, sums, maxs, g.prod);
-    wg_reduce<3, 0>(sums, maxs, g.red);
+    wg_spmv<WG::kDense, false, WG::kNT>(g.Q, g.N(N_x_bar), EpiResQ{nullptr, g.N(N_Qx), g.N(N_x_bar), g.N(N_c), g.N(N_z_bar)}, sums, maxs, g.prod);
+    wg_reduce<3, 0, WG::kNT>(sums, maxs, g.red);
     const double cx = g.red[0], xqx = g.red[1], zx = g.red[2];
     __syncthreads();
-    wg_spmv<WG::kDense>(g.AT, g.M(M_y_bar), EpiResAT{nullptr, g.N(N_ATdy), g.N(N_Qx), g.N(N_c), g.N(N_z_bar), g.N(N_col), 1.0}, sums, maxs, g.prod);
-    wg_reduce<0, 3>(sums, maxs, g.red);
+    wg_spmv<WG::kDense, false, WG::kNT>(g.AT, g.M(M_y_bar), EpiResAT{nullptr, g.N(N_ATdy), g.N(N_Qx), g.N(N_c), g.N(N_z_bar), g.N(N_col), 1.0}, sums, maxs, g.prod);
+    wg_reduce<0, 3, WG::kNT>(sums, maxs, g.red);
     const double rd = g.red[0], qxm = g.red[1], atym = g.red[2];
     __syncthreads();
     double ys = 0.0, rp = 0.0, axm = 0.0;
     if (m > 0) {
-        wg_spmv<WG::kDense>(g.A, g.N(N_x_bar), EpiResA{nullptr, g.M(M_pz), g.M(M_AL), g.M(M_AU), g.M(M_row), g.M(M_y_bar), g.M(M_s), 1.0}, sums,
+        wg_spmv<WG::kDense, false, WG::kNT>(g.A, g.N(N_x_bar), EpiResA{nullptr, g.M(M_pz), g.M(M_AL), g.M(M_AU), g.M(M_row), g.M(M_y_bar), g.M(M_s), 1.0}, sums,
                 maxs, g.prod);
-        wg_reduce<1, 2>(sums, maxs, g.red);
+        wg_reduce<1, 2, WG::kNT>(sums, maxs, g.red);
         ys = g.red[0];
         rp = g.red[1];
         axm = g.red[2];
@@ -723,9 +776,9 @@ __device__ void wg_residuals(WG &g, long long iter, double *sums, double *maxs)
     }
     double viol = 0.0;
     if (iter == 0) {
-        wg_map(n, MapBoundViolation{nullptr, g.N(N_dx), g.N(N_l), g.N(N_u), g.N(N_x_bar), g.N(N_col), 1.0}, sums,
+        wg_map<WG::kNT>(n, MapBoundViolation{nullptr, g.N(N_dx), g.N(N_l), g.N(N_u), g.N(N_x_bar), g.N(N_col), 1.0}, sums,
                maxs);
-        wg_reduce<0, 1>(sums, maxs, g.red);
+        wg_reduce<0, 1, WG::kNT>(sums, maxs, g.red);
         viol = g.red[0];
         __syncthreads();
     }
@@ -743,31 +796,31 @@ __device__ void wg_residuals(WG &g, long long iter, double *sums, double *maxs)
 
 // control.c:155-229
 template <class WG>
-__device__ double wg_m_norm(WG &g, double *sums, double *maxs)
+__device__ __forceinline__ double wg_m_norm(WG &g, double *sums, double *maxs)
 {
     const int m = g.I->m;
     const State &S = *g.S;
     const int diag = S.Q_is_diag;
-    wg_spmv<WG::kDense>(g.Q, g.N(N_dw), EpiMnQdw{nullptr, g.N(N_dQw), g.N(N_dx), g.N(N_dw)},
+    wg_spmv<WG::kDense, false, WG::kNT>(g.Q, g.N(N_dw), EpiMnQdw{nullptr, g.N(N_dQw), g.N(N_dx), g.N(N_dw)},
             sums, maxs, g.prod);
-    wg_reduce<4, 0>(sums, maxs, g.red);
+    wg_reduce<4, 0, WG::kNT>(sums, maxs, g.red);
     const double dx_sq = g.red[0], dqw_dx = g.red[1], dw_dqw = g.red[2], dqw_sq = g.red[3];
     __syncthreads();
     const double sg = S.sigma;
     double m1 = 0.0, m3 = 0.0, m2 = (1.0 / sg) * dx_sq;
     m2 -= 2.0 * dqw_dx;
     if (m > 0) {
-        wg_spmv<WG::kDense>(g.AT, g.M(M_dy), EpiMnATdy{nullptr, g.N(N_ATdy), g.N(N_dx)},
+        wg_spmv<WG::kDense, false, WG::kNT>(g.AT, g.M(M_dy), EpiMnATdy{nullptr, g.N(N_ATdy), g.N(N_dx)},
                 sums, maxs, g.prod);
-        wg_reduce<1, 0>(sums, maxs, g.red);
+        wg_reduce<1, 0, WG::kNT>(sums, maxs, g.red);
         const double atdy_dx = g.red[0];
         __syncthreads();
-        wg_spmv<WG::kDense>(g.Q, g.N(N_ATdy), EpiMnQATdy{nullptr, g.N(N_QATdy), g.N(N_dQw), g.N(N_ATdy), g.N(N_fact_M), diag}, sums, maxs, g.prod);
-        wg_reduce<3, 0>(sums, maxs, g.red);
+        wg_spmv<WG::kDense, false, WG::kNT>(g.Q, g.N(N_ATdy), EpiMnQATdy{nullptr, g.N(N_QATdy), g.N(N_dQw), g.N(N_ATdy), g.N(N_fact_M), diag}, sums, maxs, g.prod);
+        wg_reduce<3, 0, WG::kNT>(sums, maxs, g.red);
         const double dqw_atdy = g.red[0], atdy_qatdy = g.red[1], fm_term = g.red[2];
         __syncthreads();
-        wg_map(m, MapDot{nullptr, g.M(M_dy), g.M(M_dy)}, sums, maxs);
-        wg_reduce<1, 0>(sums, maxs, g.red);
+        wg_map<WG::kNT>(m, MapDot{nullptr, g.M(M_dy), g.M(M_dy)}, sums, maxs);
+        wg_reduce<1, 0, WG::kNT>(sums, maxs, g.red);
         const double dy_sq = g.red[0];
         __syncthreads();
         m1 = sg * S.lam_a * dy_sq;
@@ -789,14 +842,14 @@ __device__ double wg_m_norm(WG &g, double *sums, double *maxs)
 
 // control.c:307-411 (diagonal Q: golden search over the workgroup)
 template <class WG>
-__device__ double wg_golden_diag(WG &g, double a, double b)
+__device__ __forceinline__ double wg_golden_diag(WG &g, double a, double b)
 {
     const int n = g.I->n;
     const double *dq = g.N(N_diag), *c = g.N(N_ATdy), *d = g.N(N_QATdy);
     auto eval = [&](double x) {
         double acc = 0.0;
-        for (int i = threadIdx.x; i < n; i += kThreads) acc += c[i] * (d[i] / (1.0 + x * dq[i]));
-        const double tot = block_sum_all(acc);
+        for (int i = threadIdx.x; i < n; i += WG::kNT) acc += c[i] * (d[i] / (1.0 + x * dq[i]));
+        const double tot = block_sum_all<WG::kNT>(acc);
         return a * x + b / x + x * x * tot;
     };
     double lo = 1e-12, hi = 1e12;
@@ -824,34 +877,34 @@ __device__ double wg_golden_diag(WG &g, double a, double b)
 }
 
 template <class WG>
-__device__ void wg_update_sigma(WG &g, double *sums, double *maxs)
+__device__ __forceinline__ void wg_update_sigma(WG &g, double *sums, double *maxs)
 {
     State &S = *g.S;
     if (S.rs.flag < 1) return;
     const int n = g.I->n, m = g.I->m;
     const int diag = S.Q_is_diag;
     const double old = S.sigma;
-    wg_map(n, MapSigmaDiffN{nullptr, g.N(N_dx), g.N(N_dw), g.N(N_ATdy), g.N(N_x_bar), g.N(N_last_x), g.N(N_w_bar),
+    wg_map<WG::kNT>(n, MapSigmaDiffN{nullptr, g.N(N_dx), g.N(N_dw), g.N(N_ATdy), g.N(N_x_bar), g.N(N_last_x), g.N(N_w_bar),
                             g.N(N_last_w), g.N(N_ATy_bar), g.N(N_last_ATy), m > 0 ? 1 : 0},
            sums, maxs);
-    wg_reduce<1, 0>(sums, maxs, g.red);
+    wg_reduce<1, 0, WG::kNT>(sums, maxs, g.red);
     double b = g.red[0];
     __syncthreads();
     double dy_sq = 0.0;
     if (m > 0) {
-        wg_map(m, MapSigmaDiffM{nullptr, g.M(M_dy), g.M(M_y_bar), g.M(M_last_y)}, sums, maxs);
-        wg_reduce<1, 0>(sums, maxs, g.red);
+        wg_map<WG::kNT>(m, MapSigmaDiffM{nullptr, g.M(M_dy), g.M(M_y_bar), g.M(M_last_y)}, sums, maxs);
+        wg_reduce<1, 0, WG::kNT>(sums, maxs, g.red);
         dy_sq = g.red[0];
         __syncthreads();
     }
-    wg_spmv<WG::kDense>(g.Q, g.N(N_dw), EpiSigmaQdw{nullptr, g.N(N_dQw), g.N(N_ATdy), g.N(N_dw)}, sums, maxs, g.prod);
-    wg_reduce<3, 0>(sums, maxs, g.red);
+    wg_spmv<WG::kDense, false, WG::kNT>(g.Q, g.N(N_dw), EpiSigmaQdw{nullptr, g.N(N_dQw), g.N(N_ATdy), g.N(N_dw)}, sums, maxs, g.prod);
+    wg_reduce<3, 0, WG::kNT>(sums, maxs, g.red);
     const double dqw_atdy = g.red[0], dqw_sq = g.red[1], dw_dqw = g.red[2];
     __syncthreads();
     double atdy_qatdy = 0.0;
     if (m > 0) {
-        wg_spmv<WG::kDense>(g.Q, g.N(N_ATdy), EpiSigmaQATdy{nullptr, g.N(N_QATdy), g.N(N_ATdy)}, sums, maxs, g.prod);
-        wg_reduce<1, 0>(sums, maxs, g.red);
+        wg_spmv<WG::kDense, false, WG::kNT>(g.Q, g.N(N_ATdy), EpiSigmaQATdy{nullptr, g.N(N_QATdy), g.N(N_ATdy)}, sums, maxs, g.prod);
+        wg_reduce<1, 0, WG::kNT>(sums, maxs, g.red);
         atdy_qatdy = g.red[0];
         __syncthreads();
     }
@@ -879,23 +932,23 @@ __device__ void wg_update_sigma(WG &g, double *sums, double *maxs)
     __syncthreads();
     if (threadIdx.x == 0) S.sigma = sg;
     if (diag && fabs(sg - old) > 1e-15)
-        wg_map(n, MapDiagFactors{{}, g.N(N_fact1), g.N(N_fact2), g.N(N_fact), g.N(N_fact_M), g.N(N_diag), sg}, sums,
+        wg_map<WG::kNT>(n, MapDiagFactors{{}, g.N(N_fact1), g.N(N_fact2), g.N(N_fact), g.N(N_fact_M), g.N(N_diag), sg}, sums,
                maxs);
     __syncthreads();
 }
 
 // control.c:413-457
 template <class WG>
-__device__ void wg_restart_apply(WG &g, double *sums, double *maxs)
+__device__ __forceinline__ void wg_restart_apply(WG &g, double *sums, double *maxs)
 {
     State &S = *g.S;
     if (S.rs.flag <= 0) return;
     const int m = g.I->m;
-    wg_spmv<WG::kDense>(g.AT, g.M(M_y_bar), EpiRestart{{}, g.N(N_ATy_bar), g.N(N_ATy), g.N(N_last_ATy), g.N(N_x), g.N(N_w), g.N(N_last_x),
+    wg_spmv<WG::kDense, false, WG::kNT>(g.AT, g.M(M_y_bar), EpiRestart{{}, g.N(N_ATy_bar), g.N(N_ATy), g.N(N_last_ATy), g.N(N_x), g.N(N_w), g.N(N_last_x),
                        g.N(N_last_w), g.N(N_Qw), g.N(N_last_Qw), g.N(N_x_bar), g.N(N_w_bar), g.N(N_Qw_bar),
                        m > 0 ? 1 : 0, g.I->noC},
             sums, maxs, g.prod);
-    wg_map(m, MapRestartM{{}, g.M(M_y), g.M(M_last_y), g.M(M_y_bar)}, sums, maxs);
+    wg_map<WG::kNT>(m, MapRestartM{{}, g.M(M_y), g.M(M_last_y), g.M(M_y_bar)}, sums, maxs);
     if (threadIdx.x == 0) {
         S.rs.last_gap = S.rs.current_gap;
         S.rs.save_gap = INFINITY;
@@ -907,7 +960,7 @@ __device__ void wg_restart_apply(WG &g, double *sums, double *maxs)
 
 // iteration.c:250-293 (general variant)
 template <bool DIAG, bool CHECK, class WG>
-__device__ void wg_step_t(WG &g, double h1, double h2, double *sums, double *maxs)
+__device__ __forceinline__ void wg_step_t(WG &g, double h1, double h2, double *sums, double *maxs)
 {
     const State &S = *g.S;
     const int m = g.I->m;
@@ -932,7 +985,7 @@ __device__ void wg_step_t(WG &g, double h1, double h2, double *sums, double *max
 
 // iteration.c:19-83 and 176-216 (noC: A^T y first, Halpern on w and Qw)
 template <bool DIAG, bool CHECK, class WG>
-__device__ void wg_step_noc_t(WG &g, double h1, double h2, double *sums, double *maxs)
+__device__ __forceinline__ void wg_step_noc_t(WG &g, double h1, double h2, double *sums, double *maxs)
 {
     const State &S = *g.S;
     const int m = g.I->m;
@@ -953,7 +1006,7 @@ __device__ void wg_step_noc_t(WG &g, double h1, double h2, double *sums, double 
 }
 
 template <class WG>
-__device__ void wg_step(WG &g, long long inner, double *sums, double *maxs)
+__device__ __forceinline__ void wg_step(WG &g, long long inner, double *sums, double *maxs)
 {
     const double h1 = 1.0 / ((double)inner + 2.0), h2 = 1.0 - h1;
     const bool diag = g.S->Q_is_diag, chk = g.S->to_check;
@@ -978,7 +1031,7 @@ __device__ void wg_step(WG &g, long long inner, double *sums, double *maxs)
 
 // solver_backend.c:432-513, one body; returns true when the instance is done.
 template <class WG>
-__device__ bool wg_body(WG &g, double *sums, double *maxs)
+__device__ __forceinline__ bool wg_body(WG &g, double *sums, double *maxs)
 {
     State &S = *g.S;
     const long long iter = S.iter;
@@ -1032,7 +1085,7 @@ __device__ bool wg_body(WG &g, double *sums, double *maxs)
 }
 
 template <class WG>
-__device__ void wg_bind(WG &g, const Arrays &ar, const Params &P, int k, State *S, double *prod, double *red)
+__device__ __forceinline__ void wg_bind(WG &g, const Arrays &ar, const Params &P, int k, State *S, double *prod, double *red)
 {
     const Inst *I = ar.inst + k;
     g.I = I;
@@ -1822,6 +1875,40 @@ __global__ __launch_bounds__(kThreads, D ? 2 : 6) void batch_solve_kernel(Arrays
     }
 }
 
+// The tail of a batch: once at most one unfinished instance per CU is left,
+// each CSR instance runs on a workgroup of kLoneThreads threads, whose
+// kLoneThreads / kThreads groups stage and sum different row blocks of a
+// product at once (a lone instance's iteration is a chain of dependent block
+// loads, not a stream). Rows are summed as in batch_solve_kernel; the
+// reductions over the wider workgroup add their terms in another order. The
+// solve path's device functions are forced inline: within the 128 VGPRs of a
+// 1,024-thread workgroup the inliner would otherwise leave whole iteration
+// steps as calls (0.9-1.4 kB of scratch per lane; 64 B inlined).
+template <bool ST>
+__global__ __launch_bounds__(kLoneThreads) void batch_solve_lone_kernel(Arrays ar, Params P, int count, int bodies)
+{
+    __shared__ double prod[(kLoneThreads / kThreads) * kBatchStreamCap];
+    __shared__ double xs[ST ? kStageMax : 1];
+    __shared__ double red[kMaxTerms];
+    __shared__ State S;
+    if ((int)blockIdx.x >= count) return;
+    const int k = ar.order[blockIdx.x];
+    if (ar.inst[k].dense) return;
+    if (threadIdx.x == 0) S = ar.state[k];
+    __syncthreads();
+    if (S.finished) return;
+    WGv<false, ST, kLoneThreads> g;
+    wg_bind(g, ar, P, k, &S, prod, red);
+    g.xs = xs;
+    double sums[kMaxTerms] = {0}, maxs[kMaxTerms] = {0};
+    for (int b = 0; b < bodies; ++b)
+        if (wg_body(g, sums, maxs)) break;
+    if (threadIdx.x == 0) {
+        ar.state[k] = S;
+        if (!S.finished) atomicAdd(ar.active, 1);
+    }
+}
+
 // Timing probe for the CSR engine's wave threshold: the three products of an
 // iteration (Q w, A x, A^T y) of every sparse instance, into scratch vectors.
 __global__ __launch_bounds__(kThreads) void batch_probe_kernel(Arrays ar, Params P, int count)
@@ -2035,6 +2122,9 @@ struct HPRQPHipBatch {
     size_t blocks_cap = 0;                       // ints allocated at d_blocks
     hipGraphExec_t graph = nullptr;              // captured chunk (counter clear + solve launches)
     size_t graph_bodies = 0;
+    hipGraphExec_t graph_lone = nullptr;         // the same with the tail's lone-instance kernel
+    size_t graph_lone_bodies = 0;
+    int lone_max = 0;  // unfinished instances at or below which CSR instances run alone (the CU count)
     hprqp::batch::Arrays ar{};
     hprqp::batch::Params P{};
     std::vector<hprqp::batch::Inst> inst;
@@ -2066,6 +2156,7 @@ struct HPRQPHipBatch {
             (void)hipEventDestroy(caller_done);
         }
         if (graph) (void)hipGraphExecDestroy(graph);
+        if (graph_lone) (void)hipGraphExecDestroy(graph_lone);
         if (d_order) (void)hipFree(d_order);
         for (void *p : {(void *)d_vn, (void *)d_vm, (void *)d_val, (void *)d_dval, (void *)d_rp, (void *)d_ci, (void *)d_perm,
                         (void *)d_blocks, (void *)d_active, (void *)d_inst, (void *)d_state, (void *)d_steps,
@@ -2289,6 +2380,12 @@ HPRQPStatus batch_finish(HPRQPHipBatch *b)
     if (st != HPRQP_STATUS_OK) {
         delete b;
         return st;
+    }
+    {
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess)
+            b->lone_max = cus;
     }
     Arrays &ar = b->ar;
     ar.inst = b->d_inst;
@@ -2675,9 +2772,21 @@ HPRQPStatus hprqp_hip_batch_solve(HPRQPHipBatch *b, size_t bodies_per_launch, si
     // launches it eagerly instead).
     const char *graph_env = std::getenv("HPRQP_BATCH_GRAPH");
     const bool use_graph = !(graph_env && std::atoi(graph_env) == 0);
-    auto enqueue_chunk = [&]() {
+    // The tail: with at most lone_max unfinished instances (one per CU) the
+    // CSR instances continue in batch_solve_lone_kernel. HPRQP_BATCH_LONE=0
+    // keeps the shared kernel to the end, =1 runs the lone kernel from the
+    // start.
+    const char *lone_env = std::getenv("HPRQP_BATCH_LONE");
+    const int lone_mode = lone_env ? std::atoi(lone_env) : -1;
+    auto enqueue_chunk = [&](bool lone) {
         batch_zero_counter_kernel<<<1, 1, 0, b->stream>>>(b->d_active);
-        if (b->any_sparse && b->stage_x)
+        if (b->any_sparse && lone && b->stage_x)
+            batch_solve_lone_kernel<true><<<(unsigned)b->count, kLoneThreads, 0, b->stream>>>(
+                b->ar, b->P, (int)b->count, (int)bodies_per_launch);
+        else if (b->any_sparse && lone)
+            batch_solve_lone_kernel<false><<<(unsigned)b->count, kLoneThreads, 0, b->stream>>>(
+                b->ar, b->P, (int)b->count, (int)bodies_per_launch);
+        else if (b->any_sparse && b->stage_x)
             batch_solve_kernel<false, true><<<(unsigned)b->count, hprqp::kThreads, 0, b->stream>>>(
                 b->ar, b->P, (int)b->count, (int)bodies_per_launch);
         else if (b->any_sparse)
@@ -2687,24 +2796,30 @@ HPRQPStatus hprqp_hip_batch_solve(HPRQPHipBatch *b, size_t bodies_per_launch, si
             batch_solve_kernel<true, false><<<(unsigned)b->count, hprqp::kThreads, 0, b->stream>>>(
                 b->ar, b->P, (int)b->count, (int)bodies_per_launch);
     };
-    if (use_graph && (!b->graph || b->graph_bodies != bodies_per_launch)) {
-        if (b->graph) HPRQP_HIP(hipGraphExecDestroy(b->graph));
-        b->graph = nullptr;
+    auto capture = [&](bool lone, hipGraphExec_t *exec, size_t *bodies) -> HPRQPStatus {
+        if (*exec && *bodies == bodies_per_launch) return HPRQP_STATUS_OK;
+        if (*exec) HPRQP_HIP(hipGraphExecDestroy(*exec));
+        *exec = nullptr;
         hipGraph_t g = nullptr;
         HPRQP_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-        enqueue_chunk();
+        enqueue_chunk(lone);
         HPRQP_HIP(hipStreamEndCapture(b->stream, &g));
-        const hipError_t ie = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+        const hipError_t ie = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
         (void)hipGraphDestroy(g);
         HPRQP_HIP(ie);
-        b->graph_bodies = bodies_per_launch;
-    }
-    int active = 1;
+        *bodies = bodies_per_launch;
+        return HPRQP_STATUS_OK;
+    };
+    int active = (int)b->count;
     while (active > 0) {
+        const bool lone = b->any_sparse && lone_mode != 0 && (lone_mode == 1 || active <= b->lone_max);
         if (use_graph) {
-            HPRQP_HIP(hipGraphLaunch(b->graph, b->stream));
+            const HPRQPStatus cs = lone ? capture(true, &b->graph_lone, &b->graph_lone_bodies)
+                                        : capture(false, &b->graph, &b->graph_bodies);
+            if (cs != HPRQP_STATUS_OK) return cs;
+            HPRQP_HIP(hipGraphLaunch(lone ? b->graph_lone : b->graph, b->stream));
         } else {
-            enqueue_chunk();
+            enqueue_chunk(lone);
         }
         HPRQP_HIP(hipGetLastError());
         HPRQP_HIP(hipMemcpyAsync(&active, b->d_active, sizeof(int), hipMemcpyDeviceToHost, b->stream));

@@ -213,6 +213,36 @@ def test_batch_graph_replays_equal_eager_launches(H, monkeypatch):
     assert out["1"]["status"] == out["0"]["status"] == ["OPTIMAL"] * len(models)
 
 
+def test_batch_lone_kernel_matches_shared_kernel(H, monkeypatch):
+    """The tail kernel (one 1,024-thread workgroup per instance, four row
+    blocks of a product in flight) against the shared 256-thread kernel on
+    the same CSR instances: every row is summed alike, the workgroup
+    reductions add their terms in another order, so iterates agree to
+    rounding at fixed iteration counts, and both solve to the same optimum.
+    HPRQP_BATCH_LONE=1 runs the tail kernel from the start, =0 never."""
+    qps = Q.batched_instances(count=12, seed=31, n_lo=40, n_hi=2000)
+    models = [build(H, qp) for qp in qps]
+    out = {}
+    for max_iter in (1, 37, 100, 100000):
+        for lone in ("1", "0"):
+            monkeypatch.setenv("HPRQP_BATCH_LONE", lone)
+            print("lone", lone, "max_iter", max_iter, flush=True)
+            b = H.Batch(models, H.Parameters(verbose=False, stoptol=1e-8, max_iter=max_iter))
+            b.solve(50)
+            out[lone] = b.solution()
+        a, c = out["1"], out["0"]
+        assert a["status"] == c["status"]
+        if max_iter < 100000:
+            assert (a["iter"] == c["iter"]).all()
+            for v in ("x", "y", "z"):
+                scale = max(1.0, float(np.max(np.abs(c[v]))))
+                assert np.max(np.abs(a[v] - c[v])) <= 1e-10 * scale, (v, max_iter)
+        else:
+            assert c["status"] == ["OPTIMAL"] * len(models)
+            rel = np.abs(a["objective"] - c["objective"]) / np.maximum(1.0, np.abs(c["objective"]))
+            assert float(rel.max()) <= 1e-7, float(rel.max())
+
+
 @pytest.mark.parametrize("wave_row", ["16", "1048576"])
 def test_batch_wave_threshold_choices_agree(H, monkeypatch, wave_row):
     """Every wave threshold the autotuner may pick solves the instances to
