@@ -1875,7 +1875,7 @@ __global__ __launch_bounds__(kThreads, D ? 2 : 6) void batch_solve_kernel(Arrays
     }
 }
 
-// The tail of a batch: once at most one unfinished instance per CU is left,
+// The tail of a batch: once few unfinished instances are left (three per CU),
 // each CSR instance runs on a workgroup of kLoneThreads threads, whose
 // kLoneThreads / kThreads groups stage and sum different row blocks of a
 // product at once (a lone instance's iteration is a chain of dependent block
@@ -2124,7 +2124,11 @@ struct HPRQPHipBatch {
     size_t graph_bodies = 0;
     hipGraphExec_t graph_lone = nullptr;         // the same with the tail's lone-instance kernel
     size_t graph_lone_bodies = 0;
-    int lone_max = 0;  // unfinished instances at or below which CSR instances run alone (the CU count)
+    // unfinished instances at or below which the CSR instances continue in the
+    // lone kernel: three per CU (two lone workgroups fit a CU at once; at
+    // 4,096 instances of n <= 2,000 the lone kernel already beats the shared
+    // one with 649 left: the solve 0.372 s with one per CU, 0.367 s with three)
+    int lone_max = 0;
     hprqp::batch::Arrays ar{};
     hprqp::batch::Params P{};
     std::vector<hprqp::batch::Inst> inst;
@@ -2385,7 +2389,7 @@ HPRQPStatus batch_finish(HPRQPHipBatch *b)
         int dev = 0, cus = 0;
         if (hipGetDevice(&dev) == hipSuccess &&
             hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess)
-            b->lone_max = cus;
+            b->lone_max = 3 * cus;
     }
     Arrays &ar = b->ar;
     ar.inst = b->d_inst;
@@ -2772,12 +2776,14 @@ HPRQPStatus hprqp_hip_batch_solve(HPRQPHipBatch *b, size_t bodies_per_launch, si
     // launches it eagerly instead).
     const char *graph_env = std::getenv("HPRQP_BATCH_GRAPH");
     const bool use_graph = !(graph_env && std::atoi(graph_env) == 0);
-    // The tail: with at most lone_max unfinished instances (one per CU) the
-    // CSR instances continue in batch_solve_lone_kernel. HPRQP_BATCH_LONE=0
-    // keeps the shared kernel to the end, =1 runs the lone kernel from the
-    // start.
+    // The tail: with at most lone_max unfinished instances the CSR instances
+    // continue in batch_solve_lone_kernel. HPRQP_BATCH_LONE=0 keeps the
+    // shared kernel to the end, =1 runs the lone kernel from the start;
+    // HPRQP_BATCH_LONE_MAX overrides the threshold.
     const char *lone_env = std::getenv("HPRQP_BATCH_LONE");
     const int lone_mode = lone_env ? std::atoi(lone_env) : -1;
+    const char *lone_max_env = std::getenv("HPRQP_BATCH_LONE_MAX");
+    const int lone_max = lone_max_env && std::atoi(lone_max_env) > 0 ? std::atoi(lone_max_env) : b->lone_max;
     auto enqueue_chunk = [&](bool lone) {
         batch_zero_counter_kernel<<<1, 1, 0, b->stream>>>(b->d_active);
         if (b->any_sparse && lone && b->stage_x)
@@ -2812,7 +2818,7 @@ HPRQPStatus hprqp_hip_batch_solve(HPRQPHipBatch *b, size_t bodies_per_launch, si
     };
     int active = (int)b->count;
     while (active > 0) {
-        const bool lone = b->any_sparse && lone_mode != 0 && (lone_mode == 1 || active <= b->lone_max);
+        const bool lone = b->any_sparse && lone_mode != 0 && (lone_mode == 1 || active <= lone_max);
         if (use_graph) {
             const HPRQPStatus cs = lone ? capture(true, &b->graph_lone, &b->graph_lone_bodies)
                                         : capture(false, &b->graph, &b->graph_bodies);
