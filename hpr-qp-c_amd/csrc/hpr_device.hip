@@ -492,15 +492,79 @@ struct SocFinishY {
     double *y, *y_bar, *dy;
     const double *s, *Rp, *last_y;
     double tau, a, b;
-    __device__ void apply(int i) const
+    __device__ void apply(int i) const { apply_s(i, s[i]); }
+    // the same with s_i = s[i] given
+    __device__ void apply_s(int i, double si) const
     {
         const double yi = y[i];
-        const double yb = (s[i] - Rp[i]) / tau;
+        const double yb = (si - Rp[i]) / tau;
         dy[i] = yb - yi;
         y_bar[i] = yb;
         y[i] = a * last_y[i] + b * (2.0 * yb - yi);
     }
 };
+
+// Cone projection fused with the elementwise finish, L lanes per cone (L a
+// power of two >= the longest cone of the range, <= the wavefront): lane j
+// loads and writes entry j of its cone, so the accesses of a wavefront are
+// contiguous. Every lane folds the squared scaled entries in entry order
+// with the compensated sum of soc_project_block (the values come over
+// lane shuffles), so the projection has soc_project_block's bits.
+struct ConeFinX {
+    SocFinishX fin;
+    __device__ void operator()(int i, double xb) const { fin.apply(i, xb); }
+};
+struct ConeFinY {  // s[i] is the projected value this lane has just stored
+    SocFinishY fin;
+    __device__ void operator()(int i, double si) const { fin.apply_s(i, si); }
+};
+
+template <int L, class Fin>
+__global__ __launch_bounds__(kThreads) void soc_cone_group_kernel(int count, const int *__restrict__ off,
+                                                                  const double *__restrict__ in, double *out,
+                                                                  Fin fin)
+{
+    static_assert(L >= 1 && L <= kWave && (L & (L - 1)) == 0, "L lanes per cone: a power of two <= 64");
+    const int k = (int)((blockIdx.x * (unsigned)kThreads + threadIdx.x) / L);
+    const int j = (int)(threadIdx.x % L);
+    int b = 0, len = 0;
+    if (k < count) {
+        b = off[k];
+        len = off[k + 1] - b;
+    }
+    const bool act = j < len;
+    const double v = act ? in[b + j] : 0.0;
+    const double t = __shfl(v, 0, L);
+    double mx = act && j >= 1 ? fabs(v) : 0.0;
+#pragma unroll
+    for (int o = L / 2; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, L));
+    const double sc = mx > 0.0 && act && j >= 1 ? v / mx : 0.0;
+    double sum = 0.0, comp = 0.0;
+#pragma unroll
+    for (int jj = 1; jj < L; ++jj) {
+        const double s_j = __shfl(sc, jj, L);
+        if (mx > 0.0 && jj < len) {
+            const double term = s_j * s_j - comp;
+            const double nx = sum + term;
+            comp = (nx - sum) - term;
+            sum = nx;
+        }
+    }
+    const double norm = mx * sqrt(sum);
+    double xo;
+    if (norm <= t) {
+        xo = v;
+    } else if (norm <= -t) {
+        xo = 0.0;
+    } else {
+        const double alpha = 0.5 + 0.5 * (t / norm);
+        xo = j == 0 ? 0.5 * norm + 0.5 * t : alpha * v;
+    }
+    if (act) {
+        out[b + j] = xo;
+        fin(b + j, xo);
+    }
+}
 
 struct EpiSocY : NoRed {
     SocFinishY fin;
@@ -937,6 +1001,11 @@ private:
     int *AT_perm_ = nullptr;  // A^T entry -> A entry
     int *QT_rp_ = nullptr, *QT_perm_ = nullptr;
     int *var_off_ = nullptr, *row_off_ = nullptr;
+    // lanes per cone of the fused cone kernels (0: the cones do not tile
+    // their range or one is longer than a wavefront; one thread per cone)
+    int var_lanes_ = 0, row_lanes_ = 0;
+    template <class Fin>
+    void cone_step(int lanes, size_t count, const int *off, const double *in, double *out, const Fin &fin);
     size_t num_var_cones_ = 0, num_row_cones_ = 0;
     int gram_mode_ = 0;                 // 2: the equality-row metric is requested (AUTO resolved)
     std::unique_ptr<GramMetric> gram_;  // set when the metric was accepted
@@ -1135,6 +1204,23 @@ HPRQPStatus HipBackend::upload(const HPRQPModel *md)
     num_row_cones_ = md->num_row_cones;
     lin_n_ = num_var_cones_ ? md->var_cone_offsets[0] : n_;
     lin_m_ = num_row_cones_ ? md->row_cone_offsets[0] : m_;
+    // lanes per cone for the fused kernels: cones that tile [lin, len) in
+    // order, each at most a wavefront long
+    auto lanes_for = [](const size_t *o, size_t count, size_t len) {
+        size_t longest = 0;
+        if (o[count] != len) return 0;
+        for (size_t k = 0; k < count; ++k) {
+            if (o[k + 1] < o[k]) return 0;
+            longest = std::max(longest, o[k + 1] - o[k]);
+        }
+        if (longest > (size_t)kWave) return 0;
+        int l = 1;
+        while ((size_t)l < longest) l <<= 1;
+        return l;
+    };
+    var_lanes_ = num_var_cones_ ? lanes_for(md->var_cone_offsets, num_var_cones_, n_) : 0;
+    row_lanes_ = num_row_cones_ ? lanes_for(md->row_cone_offsets, num_row_cones_, m_) : 0;
+    if (const char *v = std::getenv("HPRQP_CONE_LANES"); v && std::atoi(v) == 0) var_lanes_ = row_lanes_ = 0;
     if (num_var_cones_) {
         std::vector<int> off(num_var_cones_ + 1);
         for (size_t k = 0; k <= num_var_cones_; ++k) off[k] = (int)md->var_cone_offsets[k];
@@ -1418,6 +1504,21 @@ HPRQPStatus HipBackend::power_iteration(bool q_matrix, double *lambda, int *step
     }
     *lambda = lam < 0.0 ? 0.0 : lam;
     return HPRQP_STATUS_OK;
+}
+
+template <class Fin>
+void HipBackend::cone_step(int lanes, size_t count, const int *off, const double *in, double *out, const Fin &fin)
+{
+    const unsigned grid = (unsigned)((count * (size_t)lanes + kThreads - 1) / kThreads);
+    switch (lanes) {
+    case 1: soc_cone_group_kernel<1, Fin><<<grid, kThreads, 0, stream_>>>((int)count, off, in, out, fin); break;
+    case 2: soc_cone_group_kernel<2, Fin><<<grid, kThreads, 0, stream_>>>((int)count, off, in, out, fin); break;
+    case 4: soc_cone_group_kernel<4, Fin><<<grid, kThreads, 0, stream_>>>((int)count, off, in, out, fin); break;
+    case 8: soc_cone_group_kernel<8, Fin><<<grid, kThreads, 0, stream_>>>((int)count, off, in, out, fin); break;
+    case 16: soc_cone_group_kernel<16, Fin><<<grid, kThreads, 0, stream_>>>((int)count, off, in, out, fin); break;
+    case 32: soc_cone_group_kernel<32, Fin><<<grid, kThreads, 0, stream_>>>((int)count, off, in, out, fin); break;
+    default: soc_cone_group_kernel<64, Fin><<<grid, kThreads, 0, stream_>>>((int)count, off, in, out, fin); break;
+    }
 }
 
 // workspace.c:191-248 (+ control.cu:189-220 for cone models)
@@ -1735,9 +1836,13 @@ HPRQPStatus HipBackend::soc_iteration_step(size_t inner)
     spmv(Q_, vec(V_w), ex, 0, T_ZXW1);
     if (num_var_cones_) {
         if (profiling_) prof_begin(T_CONE_X);
-        soc_cone_project_kernel<<<(unsigned)((num_var_cones_ + kConeThreads - 1) / kConeThreads), kConeThreads, 0,
-                                  stream_>>>((int)num_var_cones_, var_off_, vec(V_tempv), vec(V_x_bar));
-        map(n_ - lin_n_, MapSocFinishX{{}, fx, vec(V_x_bar), (int)lin_n_});
+        if (var_lanes_) {
+            cone_step(var_lanes_, num_var_cones_, var_off_, vec(V_tempv), vec(V_x_bar), ConeFinX{fx});
+        } else {
+            soc_cone_project_kernel<<<(unsigned)((num_var_cones_ + kConeThreads - 1) / kConeThreads), kConeThreads,
+                                      0, stream_>>>((int)num_var_cones_, var_off_, vec(V_tempv), vec(V_x_bar));
+            map(n_ - lin_n_, MapSocFinishX{{}, fx, vec(V_x_bar), (int)lin_n_});
+        }
         if (profiling_) prof_end();
     }
     spmv(Q_, vec(V_w_bar), EpiTempv{{}, vec(V_tempv), vec(V_x_hat), vec(V_Qw), sigma_}, 0, T_TEMPV);
@@ -1760,9 +1865,14 @@ HPRQPStatus HipBackend::soc_iteration_step(size_t inner)
         spmv(A_, vec(V_tempv), ey, 0, T_Y);
         if (num_row_cones_) {
             if (profiling_) prof_begin(T_CONE_Y);
-            soc_cone_project_kernel<<<(unsigned)((num_row_cones_ + kConeThreads - 1) / kConeThreads), kConeThreads,
-                                      0, stream_>>>((int)num_row_cones_, row_off_, vec(V_Rp_soc()), vec(V_s));
-            map(m_ - lin_m_, MapSocFinishY{{}, fy, (int)lin_m_});
+            if (row_lanes_) {
+                cone_step(row_lanes_, num_row_cones_, row_off_, vec(V_Rp_soc()), vec(V_s), ConeFinY{fy});
+            } else {
+                soc_cone_project_kernel<<<(unsigned)((num_row_cones_ + kConeThreads - 1) / kConeThreads),
+                                          kConeThreads, 0, stream_>>>((int)num_row_cones_, row_off_, vec(V_Rp_soc()),
+                                                                      vec(V_s));
+                map(m_ - lin_m_, MapSocFinishY{{}, fy, (int)lin_m_});
+            }
             if (profiling_) prof_end();
         }
         // qp_row_metric.cu:55-63 / row_metric.cu:46-55: the metric rows

@@ -71,3 +71,34 @@ def test_soc_generated_full_solve(H):
     assert abs(r.objective - orr.primal_obj) <= 1e-7 * max(1.0, abs(orr.primal_obj))
     cert = Q.soc_certificate(qp, r.x, r.y, r.z)
     assert max(cert.values()) < 1e-6, cert
+
+
+@pytest.mark.parametrize("case", ["var_cones", "mixed", "row_cones"])
+def test_soc_fused_cone_kernel_matches_thread_per_cone(H, monkeypatch, case):
+    """The cone step with L lanes per cone (projection and finish fused,
+    coalesced) against one thread per cone followed by the finish map
+    (HPRQP_CONE_LANES=0): the same bits after every loop body, for variable
+    cones (dimension 10), the mixed model (variable and row cones of several
+    sizes) and row cones of the grid-control model."""
+    if case == "var_cones":
+        qp = Q.soc_qp(seed=4, n_box=200, cones=40, cone_dim=10, m=300, a_row_nnz=20)
+    elif case == "mixed":
+        qp = Q.soc_mixed_model(2)
+    else:
+        qp = Q.gram_fixture_spatial(6)
+    print(f"case {case}", flush=True)
+    runs = {}
+    for lanes in ("0", "auto"):
+        if lanes == "0":
+            monkeypatch.setenv("HPRQP_CONE_LANES", "0")
+        else:
+            monkeypatch.delenv("HPRQP_CONE_LANES", raising=False)
+        s = H.Session(build(H, qp), H.Parameters(verbose=False, gram_mode=0))
+        trace = []
+        for _ in range(60):
+            s.advance(1)
+            trace.append({v: s.read(v) for v in ("x", "y", "w", "x_bar", "y_bar")})
+        runs[lanes] = trace
+    for k, (a, b) in enumerate(zip(runs["0"], runs["auto"])):
+        for v in a:
+            assert np.array_equal(a[v], b[v]), (k, v)
