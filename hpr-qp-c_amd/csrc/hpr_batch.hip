@@ -145,16 +145,18 @@ constexpr int kBatchStreamCap = 1024;
 constexpr int kStageMax = 2048;
 constexpr int kLoneThreads = 1024;  // workgroup of an instance in the batch's tail
 
-// Column-form dot of a dense view: sum_{i<K} S[i*ld + j] xs[i], i ascending.
-// CACHED: ordinary loads, 16 in flight per thread, for products that re-read
-// one instance's matrix many times (power iteration) -- a lone instance's
-// matrices then stay in L2; otherwise streaming (non-temporal) loads, 32 in
-// flight per thread, for the solve, whose matrices are read once per
-// iteration (its dense kernel runs two waves per SIMD for the registers).
+// Column-form dot of a dense view: sum_{i<K} S[i*ld + j] xs[i], i ascending,
+// 32 loads in flight per thread. CACHED: ordinary loads, for products that
+// re-read one instance's matrix many times (power iteration) -- a lone
+// instance's matrices then stay in L2, where the loads in flight set its
+// rate (16 in flight: the layer workload's power phase 69.3 ms, 32: 66.7 ms);
+// otherwise streaming (non-temporal) loads, for the solve, whose matrices are
+// read once per iteration (its dense kernel runs two waves per SIMD for the
+// registers).
 template <bool CACHED = false>
 __device__ __forceinline__ double dense_column_dot(const Mat &M, int j, const double *xs)
 {
-    constexpr int U = CACHED ? 16 : 32;
+    constexpr int U = 32;
     const double *__restrict__ col = M.dv + j;
     double acc = 0.0;
     int i = 0;
