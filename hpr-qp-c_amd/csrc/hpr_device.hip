@@ -676,12 +676,54 @@ __global__ void scale_col_norm_kernel(int n, const int *__restrict__ atrp, const
     out[j] = acc == 0.0 ? 1.0 : acc;
 }
 
+// Over the matrix's row blocks (one workgroup each): rows of a stream block
+// one per thread; the long rows of a wave block in turn, the max by the
+// whole workgroup (order-free), the sum by one thread in entry order with
+// its loads issued eight at a time (the host's sequential sum).
 template <bool SUM>
-__global__ void scale_row_norm_kernel(int m, const int *__restrict__ rp, const double *__restrict__ val,
-                                      double *__restrict__ out)
+__global__ __launch_bounds__(kThreads) void scale_row_norm_kernel(const int *__restrict__ rp,
+                                                                  const double *__restrict__ val,
+                                                                  const int2 *__restrict__ blk,
+                                                                  double *__restrict__ out)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= m) return;
+    const int b = blockIdx.x, t = threadIdx.x;
+    const int2 lo = blk[b], hi = blk[b + 1];
+    const int r0 = lo.x, r1 = hi.x, nr = r1 - r0;
+    if (hi.y - lo.y > kLongRow * nr) {
+        for (int r = r0; r < r1; ++r) {
+            const int k0 = rp[r], k1 = rp[r + 1];
+            double acc = 0.0;
+            if (SUM) {
+                if (t == 0) {
+                    int k = k0;
+                    for (; k + 8 <= k1; k += 8) {
+                        double v[8];
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) v[u] = fabs(val[k + u]);
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) acc += v[u];
+                    }
+                    for (; k < k1; ++k) acc += fabs(val[k]);
+                }
+            } else {
+                for (int k = k0 + t; k < k1; k += kThreads) acc = fmax(acc, fabs(val[k]));
+                acc = wave_max(acc);
+                __shared__ double wmax[kThreads / kWave];
+                if ((t & (kWave - 1)) == 0) wmax[t / kWave] = acc;
+                __syncthreads();
+                if (t == 0)
+                    for (int w = 1; w < kThreads / kWave; ++w) acc = fmax(acc, wmax[w]);
+                __syncthreads();
+            }
+            if (t == 0) {
+                acc = sqrt(acc);
+                out[r] = acc == 0.0 ? 1.0 : acc;
+            }
+        }
+        return;
+    }
+    const int i = r0 + t;
+    if (i >= r1) return;
     double acc = 0.0;
     for (int k = rp[i]; k < rp[i + 1]; ++k) {
         double v = fabs(val[k]);
@@ -702,12 +744,33 @@ __global__ void cone_block_max_kernel(int count, const int *__restrict__ off, do
     for (int j = off[k]; j < off[k + 1]; ++j) v[j] = best;
 }
 
-__global__ void scale_matrix_kernel(int nrow, const int *__restrict__ rp, const int *__restrict__ ci,
-                                    double *__restrict__ val, const double *__restrict__ rdiv,
-                                    const double *__restrict__ cdiv)
+// v_k = (v_k / rdiv[row]) / cdiv[col] over the matrix's row blocks: rows
+// of a stream block one per thread, the entries of a wave block's long rows
+// strided over the workgroup.
+__global__ __launch_bounds__(kThreads) void scale_matrix_kernel(const int *__restrict__ rp,
+                                                                const int *__restrict__ ci,
+                                                                double *__restrict__ val,
+                                                                const int2 *__restrict__ blk,
+                                                                const double *__restrict__ rdiv,
+                                                                const double *__restrict__ cdiv)
 {
-    int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= nrow) return;
+    const int b = blockIdx.x, t = threadIdx.x;
+    const int2 lo = blk[b], hi = blk[b + 1];
+    const int r0 = lo.x, r1 = hi.x, nr = r1 - r0;
+    if (hi.y - lo.y > kLongRow * nr) {
+        for (int r = r0; r < r1; ++r) {
+            const double d = rdiv[r];
+            for (int k = rp[r] + t; k < rp[r + 1]; k += kThreads) {
+                double v = val[k];
+                v /= d;
+                v /= cdiv[ci[k]];
+                val[k] = v;
+            }
+        }
+        return;
+    }
+    const int r = r0 + t;
+    if (r >= r1) return;
     const double d = rdiv[r];
     for (int k = rp[r]; k < rp[r + 1]; ++k) {
         double v = val[k];
@@ -1279,8 +1342,12 @@ HPRQPStatus HipBackend::scale(const HPRQPParams *params)
                 scale_col_norm_kernel<false><<<grid_for(n_), kThreads, 0, stream_>>>(
                     n, AT_.rp, AT_perm_, A_.val, QT_rp_, QT_perm_, Q_.val, vec(V_tc));
             if (m) {
-                if (sum) scale_row_norm_kernel<true><<<grid_for(m_), kThreads, 0, stream_>>>(m, A_.rp, A_.val, vec(V_tr));
-                else scale_row_norm_kernel<false><<<grid_for(m_), kThreads, 0, stream_>>>(m, A_.rp, A_.val, vec(V_tr));
+                if (sum)
+                    scale_row_norm_kernel<true><<<A_.nblocks, kThreads, 0, stream_>>>(A_.rp, A_.val, A_.blocks,
+                                                                                     vec(V_tr));
+                else
+                    scale_row_norm_kernel<false><<<A_.nblocks, kThreads, 0, stream_>>>(A_.rp, A_.val, A_.blocks,
+                                                                                      vec(V_tr));
             }
             if (num_var_cones_)
                 cone_block_max_kernel<<<grid_for(num_var_cones_), kThreads, 0, stream_>>>((int)num_var_cones_,
@@ -1288,8 +1355,12 @@ HPRQPStatus HipBackend::scale(const HPRQPParams *params)
             if (num_row_cones_)
                 cone_block_max_kernel<<<grid_for(num_row_cones_), kThreads, 0, stream_>>>((int)num_row_cones_,
                                                                                          row_off_, vec(V_tr));
-            scale_matrix_kernel<<<grid_for(n_), kThreads, 0, stream_>>>(n, Q_.rp, Q_.ci, Q_.val, vec(V_tc), vec(V_tc));
-            if (m) scale_matrix_kernel<<<grid_for(m_), kThreads, 0, stream_>>>(m, A_.rp, A_.ci, A_.val, vec(V_tr), vec(V_tc));
+            if (Q_.nrow)
+                scale_matrix_kernel<<<Q_.nblocks, kThreads, 0, stream_>>>(Q_.rp, Q_.ci, Q_.val, Q_.blocks, vec(V_tc),
+                                                                          vec(V_tc));
+            if (m)
+                scale_matrix_kernel<<<A_.nblocks, kThreads, 0, stream_>>>(A_.rp, A_.ci, A_.val, A_.blocks, vec(V_tr),
+                                                                          vec(V_tc));
             scale_vectors_n_kernel<<<grid_for(n_), kThreads, 0, stream_>>>(n, vec(V_c), vec(V_l), vec(V_u),
                                                                            vec(V_col_norm), vec(V_tc));
             if (m)
