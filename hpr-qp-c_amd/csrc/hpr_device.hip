@@ -464,9 +464,11 @@ struct EpiSocX : NoRed {
     }
 };
 
-// Cone blocks (one thread per cone; small workgroups so that the few
-// thousand cones spread over every CU): x_bar = projection of the raw point.
-// The elementwise finish of the cone range runs as a coalesced map after it.
+// Cone blocks, one thread per cone (small workgroups so that the few
+// thousand cones spread over every CU): x_bar = projection of the raw point,
+// the elementwise finish of the cone range then a coalesced map. The path of
+// cones longer than a wavefront, or of cones that do not tile their range;
+// the others take soc_cone_group_kernel below.
 constexpr int kConeThreads = 64;
 
 __global__ __launch_bounds__(kConeThreads) void soc_cone_project_kernel(int count, const int *__restrict__ off,
