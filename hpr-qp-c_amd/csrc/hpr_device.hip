@@ -63,6 +63,41 @@ __global__ __launch_bounds__(kThreads) void reduce_partials_kernel(const double 
     }
 }
 
+// Several finished reductions in one launch: block b reduces term b -
+// first[j] of job j (the same arithmetic as reduce_partials_kernel, so the
+// same bits); up to kMaxRedJobs jobs, their terms numbered consecutively.
+constexpr int kMaxRedJobs = 4;
+struct RedJobs {
+    const double *partials[kMaxRedJobs];
+    double *out[kMaxRedJobs];
+    int nblk[kMaxRedJobs], nterms[kMaxRedJobs], ns[kMaxRedJobs], first[kMaxRedJobs];
+    int count;
+};
+
+__global__ __launch_bounds__(kThreads) void reduce_jobs_kernel(RedJobs jobs)
+{
+    int j = 0;
+    while (j + 1 < jobs.count && (int)blockIdx.x >= jobs.first[j + 1]) ++j;
+    const int term = (int)blockIdx.x - jobs.first[j];
+    const int stride = jobs.nterms[j];
+    const bool is_max = term >= jobs.ns[j];
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < jobs.nblk[j]; b += kThreads) {
+        double v = jobs.partials[j][(size_t)b * stride + term];
+        acc = is_max ? fmax(acc, v) : acc + v;
+    }
+    __shared__ double part[kThreads / kWave];
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    acc = is_max ? wave_max(acc) : wave_sum(acc);
+    if (lane == 0) part[wid] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double total = part[0];
+        for (int w = 1; w < kThreads / kWave; ++w) total = is_max ? fmax(total, part[w]) : total + part[w];
+        jobs.out[j][term] = total;
+    }
+}
+
 namespace {
 
 inline int grid_for(size_t len) { return (int)((len + kThreads - 1) / kThreads); }
@@ -1130,7 +1165,10 @@ private:
     HPRQPStatus spmv(const DevCsr &M, const double *x, Epi epi, int reg = 0, int tag = -1)
     {
         if (M.nrow == 0) return HPRQP_STATUS_OK;
-        if (Epi::NS + Epi::NM > 0) epi.partials = region(reg);
+        if (Epi::NS + Epi::NM > 0) {
+            if (pending_regions_ >> reg & 1u) flush_red();  // its partial rows are still queued
+            epi.partials = region(reg);
+        }
         const bool timed = profiling_ && tag >= 0;
         if (timed) prof_begin(tag);
         const Panels &pn = panels_of(M);
@@ -1160,7 +1198,10 @@ private:
     HPRQPStatus map(size_t len, F f, int reg = 0)
     {
         if (len == 0) return HPRQP_STATUS_OK;
-        if (F::NS + F::NM > 0) f.partials = region(reg);
+        if (F::NS + F::NM > 0) {
+            if (pending_regions_ >> reg & 1u) flush_red();  // its partial rows are still queued
+            f.partials = region(reg);
+        }
         map_kernel<F><<<grid_for(len), kThreads, 0, stream_>>>((int)len, f);
         return HPRQP_STATUS_OK;
     }
@@ -1171,8 +1212,36 @@ private:
         if (nblk <= 0 || ns + nm == 0) return;
         reduce_partials_kernel<<<ns + nm, kThreads, 0, stream_>>>(region(reg), nblk, ns + nm, ns, red_ + dst);
     }
+    // A check's reductions are finished together: finish_later() queues a
+    // region (its partial rows must stay untouched until the flush) and
+    // flush_red() finishes the queue in one launch; read_red() flushes first.
+    RedJobs pending_{};
+    unsigned pending_regions_ = 0;  // regions whose partial rows a queued job reads
+    void finish_later(int reg, int nblk, int ns, int nm, int dst)
+    {
+        if (nblk <= 0 || ns + nm == 0) return;
+        if (pending_.count == kMaxRedJobs) flush_red();
+        pending_regions_ |= 1u << reg;
+        RedJobs &q = pending_;
+        const int j = q.count++;
+        q.partials[j] = region(reg);
+        q.out[j] = red_ + dst;
+        q.nblk[j] = nblk;
+        q.nterms[j] = ns + nm;
+        q.ns[j] = ns;
+        q.first[j] = j ? q.first[j - 1] + q.nterms[j - 1] : 0;
+    }
+    void flush_red()
+    {
+        if (pending_.count == 0) return;
+        const unsigned blocks = (unsigned)(pending_.first[pending_.count - 1] + pending_.nterms[pending_.count - 1]);
+        reduce_jobs_kernel<<<blocks, kThreads, 0, stream_>>>(pending_);
+        pending_.count = 0;
+        pending_regions_ = 0;
+    }
     HPRQPStatus read_red(int count)
     {
+        flush_red();
         HPRQP_HIP(hipMemcpyAsync(red_host_, red_, sizeof(double) * count, hipMemcpyDeviceToHost, stream_));
         HPRQP_HIP(hipStreamSynchronize(stream_));
         return HPRQP_STATUS_OK;
@@ -1841,17 +1910,17 @@ HPRQPStatus HipBackend::compute_residuals(HPRQPResiduals *res, size_t iter)
     std::memset(res, 0, sizeof(*res));
     if (num_var_cones_ || num_row_cones_) return soc_residuals(res);
     spmv(Q_, vec(V_x_bar), EpiResQ{nullptr, vec(V_Qx), vec(V_x_bar), vec(V_c), vec(V_z_bar)}, 0);
-    finish(0, red_blocks(Q_), 3, 0, 0);
+    finish_later(0, red_blocks(Q_), 3, 0, 0);
     zero_red(3, 8);
     {
         const EpiResAT rat{nullptr, vec(V_ATdy), vec(V_Qx), vec(V_c), vec(V_z_bar), vec(V_col_norm), c_scale_};
         if (aty_bar_valid_) {
             // the check step's A^T y_bar product is reused (same kernel, same bits)
             map(n_, MapResAT{nullptr, rat, vec(V_ATy_bar)}, 1);
-            finish(1, n_grid(), 0, 3, 3);
+            finish_later(1, n_grid(), 0, 3, 3);
         } else {
             spmv(AT_, vec(V_y_bar), rat, 1);
-            finish(1, red_blocks(AT_), 0, 3, 3);
+            finish_later(1, red_blocks(AT_), 0, 3, 3);
         }
     }
     if (m_ > 0) {
@@ -1859,11 +1928,11 @@ HPRQPStatus HipBackend::compute_residuals(HPRQPResiduals *res, size_t iter)
         // projection of A x_bar (the reference GPU's convention,
         // residuals.cu:28-50, 138-147); otherwise with the iteration's s
         spmv(A_, vec(V_x_bar), EpiResA{nullptr, vec(V_Ax), vec(V_AL), vec(V_AU), vec(V_row_norm), vec(V_y_bar), vec(V_s), b_scale_, gram_ ? 1 : 0}, 2);
-        finish(2, red_blocks(A_), 1, 2, 6);
+        finish_later(2, red_blocks(A_), 1, 2, 6);
     }
     if (iter == 0) {
         map(n_, MapBoundViolation{nullptr, vec(V_dx), vec(V_l), vec(V_u), vec(V_x_bar), vec(V_col_norm), b_scale_}, 3);
-        finish(3, n_grid(), 0, 1, 9);
+        finish_later(3, n_grid(), 0, 1, 9);
     }
     HPRQP_HIP(hipGetLastError());
     HPRQPStatus st = read_red(10);
@@ -1975,31 +2044,31 @@ HPRQPStatus HipBackend::soc_residuals(HPRQPResiduals *res)
 {
     const int vc = (int)num_var_cones_, rc = (int)num_row_cones_;
     spmv(Q_, vec(V_x_bar), EpiResQ{nullptr, vec(V_Qx), vec(V_x_bar), vec(V_c), vec(V_z_bar)}, 0);
-    finish(0, red_blocks(Q_), 3, 0, 0);
+    finish_later(0, red_blocks(Q_), 3, 0, 0);
     zero_red(3, 8);
     {
         const EpiResAT rat{nullptr, vec(V_ATdy), vec(V_Qx), vec(V_c), vec(V_z_bar), vec(V_col_norm), c_scale_};
         if (aty_bar_valid_) {
             map(n_, MapResAT{nullptr, rat, vec(V_ATy_bar)}, 1);
-            finish(1, n_grid(), 0, 3, 3);
+            finish_later(1, n_grid(), 0, 3, 3);
         } else {
             spmv(AT_, vec(V_y_bar), rat, 1);
-            finish(1, red_blocks(AT_), 0, 3, 3);
+            finish_later(1, red_blocks(AT_), 0, 3, 3);
         }
     }
     if (m_ > 0) {
         spmv(A_, vec(V_x_bar), EpiSocResA{nullptr, vec(V_Ax), vec(V_Rp_soc()), rc ? vec(V_soc_rhs) : nullptr,
                                           vec(V_row_norm), vec(V_y_bar), vec(V_s), b_scale_}, 2);
-        finish(2, red_blocks(A_), 2, 1, 6);
+        finish_later(2, red_blocks(A_), 2, 1, 6);
         soc_project_mixed_kernel<<<grid_for(std::max(lin_m_, (size_t)rc)), kThreads, 0, stream_>>>(
             (int)m_, (int)lin_m_, vec(V_Rp_soc()), vec(V_AL), vec(V_AU), rc, row_off_, vec(V_tr));
         map(m_, MapSocRowResid{nullptr, vec(V_tr), vec(V_Rp_soc()), vec(V_row_norm), b_scale_}, 3);
-        finish(3, m_grid(), 0, 1, 9);
+        finish_later(3, m_grid(), 0, 1, 9);
     }
     soc_project_mixed_kernel<<<grid_for(std::max(lin_n_, (size_t)vc)), kThreads, 0, stream_>>>(
         (int)n_, (int)lin_n_, vec(V_x_bar), vec(V_l), vec(V_u), vc, var_off_, vec(V_tc));
     map(n_, MapSocVarResid{nullptr, vec(V_tc), vec(V_x_bar), vec(V_col_norm), b_scale_}, 0);
-    finish(0, n_grid(), 0, 1, 10);
+    finish_later(0, n_grid(), 0, 1, 10);
     HPRQP_HIP(hipGetLastError());
     HPRQPStatus st = read_red(11);
     if (st != HPRQP_STATUS_OK) return st;
@@ -2025,14 +2094,14 @@ double HipBackend::compute_m_norm()
 {
     const bool diag = Q_is_diag_ != 0;
     spmv(Q_, vec(V_dw), EpiMnQdw{nullptr, vec(V_dQw), vec(V_dx), vec(V_dw)}, 0);
-    finish(0, red_blocks(Q_), 4, 0, 0);
+    finish_later(0, red_blocks(Q_), 4, 0, 0);
     if (m_ > 0) {
         spmv(AT_, vec(V_dy), EpiMnATdy{nullptr, vec(V_ATdy), vec(V_dx)}, 1);
-        finish(1, red_blocks(AT_), 1, 0, 4);
+        finish_later(1, red_blocks(AT_), 1, 0, 4);
         spmv(Q_, vec(V_ATdy), EpiMnQATdy{nullptr, vec(V_QATdy), vec(V_dQw), vec(V_ATdy), vec(V_fact_M), diag ? 1 : 0}, 2);
-        finish(2, red_blocks(Q_), 3, 0, 5);
+        finish_later(2, red_blocks(Q_), 3, 0, 5);
         map(m_, MapDot{nullptr, vec(V_dy), vec(V_dy)}, 3);
-        finish(3, m_grid(), 1, 0, 8);
+        finish_later(3, m_grid(), 1, 0, 8);
     }
     if (hipGetLastError() != hipSuccess || read_red(9) != HPRQP_STATUS_OK) return INFINITY;
     const double *r = red_host_;
@@ -2070,17 +2139,17 @@ HPRQPStatus HipBackend::update_sigma(HPRQPRestart *restart, const HPRQPResiduals
     const bool diag = Q_is_diag_ != 0;
     map(n_, MapSigmaDiffN{nullptr, vec(V_dx), vec(V_dw), vec(V_ATdy), vec(V_x_bar), vec(V_last_x), vec(V_w_bar),
                           vec(V_last_w), vec(V_ATy_bar), vec(V_last_ATy), m_ > 0 ? 1 : 0}, 0);
-    finish(0, n_grid(), 1, 0, 0);
+    finish_later(0, n_grid(), 1, 0, 0);
     zero_red(1, 5);
     if (m_ > 0) {
         map(m_, MapSigmaDiffM{nullptr, vec(V_dy), vec(V_y_bar), vec(V_last_y)}, 1);
-        finish(1, m_grid(), 1, 0, 1);
+        finish_later(1, m_grid(), 1, 0, 1);
     }
     spmv(Q_, vec(V_dw), EpiSigmaQdw{nullptr, vec(V_dQw), vec(V_ATdy), vec(V_dw)}, 2);
-    finish(2, red_blocks(Q_), 3, 0, 2);
+    finish_later(2, red_blocks(Q_), 3, 0, 2);
     if (m_ > 0) {
         spmv(Q_, vec(V_ATdy), EpiSigmaQATdy{nullptr, vec(V_QATdy), vec(V_ATdy)}, 3);
-        finish(3, red_blocks(Q_), 1, 0, 5);
+        finish_later(3, red_blocks(Q_), 1, 0, 5);
     }
     HPRQP_HIP(hipGetLastError());
     HPRQPStatus st = read_red(6);
