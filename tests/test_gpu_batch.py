@@ -34,6 +34,26 @@ def test_batch_matches_plain_and_oracle(H):
         assert np.max(np.abs(r.x - x)) <= 1e-5 * max(1.0, np.max(np.abs(x))), k
 
 
+def test_batch_mixed_dense_and_csr_instances(H, monkeypatch):
+    """One batch holding dense-form instances (full Q and A: dense storage is
+    no larger than CSR) next to CSR instances, so that every chunk launches
+    both kernels, with the CSR ones in the tail kernel (HPRQP_BATCH_LONE=1)
+    and in the shared kernel (=0): each instance as its plain solve."""
+    Qd, c, A, AL, AU, l, u = Q.layer_instances(3, n=40, m_eq=10, m_in=20, seed=4)
+    models = [H.build_from_qabc(Qd[k], c[k], A[k], AL[k], AU[k], l[k], u[k]) for k in range(3)]
+    qps = Q.batched_instances(count=5, seed=17, n_lo=30, n_hi=500)
+    models += [build(H, qp) for qp in qps]
+    params = H.Parameters(verbose=False, stoptol=1e-8)
+    plain = [H.optimize(mo, params) for mo in models]
+    for lone in ("1", "0"):
+        monkeypatch.setenv("HPRQP_BATCH_LONE", lone)
+        print("mixed batch, lone", lone, flush=True)
+        res, _ = H.solve_batch(models, params, bodies_per_launch=30)
+        for k, (r, p_) in enumerate(zip(res, plain)):
+            assert r.status == p_.status == "OPTIMAL", (lone, k, r.status, p_.status)
+            assert abs(r.objective - p_.objective) <= 1e-7 * max(1.0, abs(p_.objective)), (lone, k)
+
+
 def test_batch_ragged_edge_instances(H):
     """One batch holding a one-variable QP, an LP (Q = 0), an unconstrained
     QP (m = 0) and a generated QP: each instance as the plain solve."""
