@@ -396,15 +396,23 @@ __device__ void wg_scale_dense(WG &g)
             acc = sqrt(acc);
             g.N(N_tc)[j] = acc == 0.0 ? 1.0 : acc;
         }
-        for (int i = t; i < m; i += kThreads) {
-            double acc = 0.0;
-            for (int j = 0; j < n; ++j) {
-                const double v = fabs(Ad[(long long)i * n + j]);
-                if (sum) acc += v;
-                else if (v > acc) acc = v;
+        if (sum) {
+            // the sum in entry order, one thread per row
+            for (int i = t; i < m; i += kThreads) {
+                double acc = 0.0;
+                for (int j = 0; j < n; ++j) acc += fabs(Ad[(long long)i * n + j]);
+                acc = sqrt(acc);
+                g.M(M_tr)[i] = acc == 0.0 ? 1.0 : acc;
             }
-            acc = sqrt(acc);
-            g.M(M_tr)[i] = acc == 0.0 ? 1.0 : acc;
+        } else {
+            // the max (order-free) by one wavefront per row: coalesced loads
+            const int lane = t & (kWave - 1);
+            for (int i = t / kWave; i < m; i += kThreads / kWave) {
+                double acc = 0.0;
+                for (int j = lane; j < n; j += kWave) acc = fmax(acc, fabs(Ad[(long long)i * n + j]));
+                acc = sqrt(wave_max(acc));
+                if (lane == 0) g.M(M_tr)[i] = acc == 0.0 ? 1.0 : acc;
+            }
         }
         __syncthreads();
         const double *tc = g.N(N_tc), *tr = g.M(M_tr);
