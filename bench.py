@@ -596,7 +596,7 @@ def main():
                          "(the reference kernel for counter calibration); no solve, no JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): every GPU box here has one GPU, and "
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): the solver runs on one GPU, and "
               "the multi-GPU sharding of this path is design-only.", file=sys.stderr)
         sys.exit(2)
 
