@@ -213,13 +213,16 @@ def test_batch_from_dense_rejects_rows_it_cannot_keep(H):
         H.Batch.from_dense(Qd, c, A, AL, AU, l, u)
 
 
-def test_batch_graph_replays_equal_eager_launches(H, monkeypatch):
+@pytest.mark.parametrize("lone", ["0", "1"])
+def test_batch_graph_replays_equal_eager_launches(H, monkeypatch, lone):
     """The captured chunk graph, replayed many times (5 bodies per replay,
     host synchronisation and counter read-back between replays), gives the
-    same bits as eager launches of the same kernels."""
+    same bits as eager launches of the same kernels: the shared 256-thread
+    kernel (HPRQP_BATCH_LONE=0) and the tail kernel (=1)."""
     qps = Q.batched_instances(count=12, seed=21, n_lo=30, n_hi=300)
     models = [build(H, qp) for qp in qps]
     monkeypatch.setenv("HPRQP_BATCH_WAVE_ROW", "32")
+    monkeypatch.setenv("HPRQP_BATCH_LONE", lone)
     out = {}
     for graph in ("1", "0"):
         monkeypatch.setenv("HPRQP_BATCH_GRAPH", graph)
