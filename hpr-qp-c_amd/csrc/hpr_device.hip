@@ -556,21 +556,13 @@ struct ConeFinY {  // s[i] is the projected value this lane has just stored
     __device__ void operator()(int i, double si) const { fin.apply_s(i, si); }
 };
 
-template <int L, class Fin>
-__global__ __launch_bounds__(kThreads) void soc_cone_group_kernel(int count, const int *__restrict__ off,
-                                                                  const double *__restrict__ in, double *out,
-                                                                  Fin fin)
+// The projection of one cone (entry j of it in lane j of an L-lane group;
+// lanes past the cone's len entries give v = 0 and get back 0).
+template <int L>
+__device__ __forceinline__ double cone_lanes_project(double v, int j, int len)
 {
     static_assert(L >= 1 && L <= kWave && (L & (L - 1)) == 0, "L lanes per cone: a power of two <= 64");
-    const int k = (int)((blockIdx.x * (unsigned)kThreads + threadIdx.x) / L);
-    const int j = (int)(threadIdx.x % L);
-    int b = 0, len = 0;
-    if (k < count) {
-        b = off[k];
-        len = off[k + 1] - b;
-    }
     const bool act = j < len;
-    const double v = act ? in[b + j] : 0.0;
     const double t = __shfl(v, 0, L);
     double mx = act && j >= 1 ? fabs(v) : 0.0;
 #pragma unroll
@@ -588,18 +580,74 @@ __global__ __launch_bounds__(kThreads) void soc_cone_group_kernel(int count, con
         }
     }
     const double norm = mx * sqrt(sum);
-    double xo;
-    if (norm <= t) {
-        xo = v;
-    } else if (norm <= -t) {
-        xo = 0.0;
-    } else {
-        const double alpha = 0.5 + 0.5 * (t / norm);
-        xo = j == 0 ? 0.5 * norm + 0.5 * t : alpha * v;
+    if (norm <= t) return v;
+    if (norm <= -t) return 0.0;
+    const double alpha = 0.5 + 0.5 * (t / norm);
+    return j == 0 ? 0.5 * norm + 0.5 * t : alpha * v;
+}
+
+template <int L, class Fin>
+__global__ __launch_bounds__(kThreads) void soc_cone_group_kernel(int count, const int *__restrict__ off,
+                                                                  const double *__restrict__ in, double *out,
+                                                                  Fin fin)
+{
+    const int k = (int)((blockIdx.x * (unsigned)kThreads + threadIdx.x) / L);
+    const int j = (int)(threadIdx.x % L);
+    int b = 0, len = 0;
+    if (k < count) {
+        b = off[k];
+        len = off[k + 1] - b;
     }
+    const bool act = j < len;
+    const double xo = cone_lanes_project<L>(act ? in[b + j] : 0.0, j, len);
     if (act) {
         out[b + j] = xo;
         fin(b + j, xo);
+    }
+}
+
+// The whole x step of a cone model whose Q is exactly diagonal (every row of
+// Q holds at most its diagonal entry): Q w is elementwise there, so the
+// interval prefix (one thread per row, the first nbox blocks) and the cones
+// (L lanes per cone) each form their row's Q w and raw point themselves and
+// finish in one launch. (Q w)_i = 0 + q_ii w_i, the CSR row product's
+// rounding, so the bits are those of the product, cone and finish kernels.
+template <int L>
+__global__ __launch_bounds__(kThreads) void soc_x_diag_kernel(int nbox, int count, const int *__restrict__ off,
+                                                              const int *__restrict__ qrp,
+                                                              const double *__restrict__ qval,
+                                                              const double *__restrict__ w, EpiSocX ex)
+{
+    auto qw_of = [&](int i) {
+        double acc = 0.0;
+        if (qrp[i] < qrp[i + 1]) acc += qval[qrp[i]] * w[i];
+        return acc;
+    };
+    if ((int)blockIdx.x < nbox) {
+        const int i = blockIdx.x * kThreads + threadIdx.x;
+        if (i < ex.lin_n) ex.row(i, qw_of(i), nullptr, nullptr);
+        return;
+    }
+    const int k = (int)(((blockIdx.x - nbox) * (unsigned)kThreads + threadIdx.x) / L);
+    const int j = (int)(threadIdx.x % L);
+    int b = 0, len = 0;
+    if (k < count) {
+        b = off[k];
+        len = off[k + 1] - b;
+    }
+    const bool act = j < len;
+    double v = 0.0;
+    if (act) {
+        const int i = b + j;
+        const double qw = qw_of(i);
+        ex.Qw[i] = qw;
+        v = ex.xv[i] + ex.fin.sigma * (-qw + ex.ATy[i] - ex.c[i]);
+        ex.tempv[i] = v;
+    }
+    const double xo = cone_lanes_project<L>(v, j, len);
+    if (act) {
+        ex.x_bar[b + j] = xo;
+        ex.fin.apply(b + j, xo);
     }
 }
 
@@ -1104,6 +1152,10 @@ private:
     // lanes per cone of the fused cone kernels (0: the cones do not tile
     // their range or one is longer than a wavefront; one thread per cone)
     int var_lanes_ = 0, row_lanes_ = 0;
+    bool q_diag_exact_ = false;  // every row of Q holds at most its diagonal entry
+    bool fused_x_off_ = std::getenv("HPRQP_CONE_LANES") && std::atoi(std::getenv("HPRQP_CONE_LANES")) == 0;
+    // the cone x step runs as soc_x_diag_kernel (Q w, prefix, cones, finish)
+    bool fused_x() const { return num_var_cones_ && var_lanes_ && q_diag_exact_ && Qp_.P <= 1 && !fused_x_off_; }
     template <class Fin>
     void cone_step(int lanes, size_t count, const int *off, const double *in, double *out, const Fin &fin);
     size_t num_var_cones_ = 0, num_row_cones_ = 0;
@@ -1353,6 +1405,11 @@ HPRQPStatus HipBackend::upload(const HPRQPModel *md)
         return l;
     };
     var_lanes_ = num_var_cones_ ? lanes_for(md->var_cone_offsets, num_var_cones_, n_) : 0;
+    q_diag_exact_ = true;
+    for (size_t r = 0; r < md->Q.nrow && q_diag_exact_; ++r) {
+        const size_t e0 = md->Q.row_ptr[r], e1 = md->Q.row_ptr[r + 1];
+        if (e1 - e0 > 1 || (e1 > e0 && md->Q.col_ind[e0] != r)) q_diag_exact_ = false;
+    }
     row_lanes_ = num_row_cones_ ? lanes_for(md->row_cone_offsets, num_row_cones_, m_) : 0;
     if (const char *v = std::getenv("HPRQP_CONE_LANES"); v && std::atoi(v) == 0) var_lanes_ = row_lanes_ = 0;
     if (num_var_cones_) {
@@ -1975,8 +2032,27 @@ HPRQPStatus HipBackend::soc_iteration_step(size_t inner)
     ex.l = vec(V_l);
     ex.u = vec(V_u);
     ex.lin_n = (int)lin_n_;
-    spmv(Q_, vec(V_w), ex, 0, T_ZXW1);
-    if (num_var_cones_) {
+    if (fused_x()) {
+        // Q w, the interval prefix, the cones and their finish in one launch
+        if (profiling_) prof_begin(T_ZXW1);
+        const int nbox = grid_for(lin_n_);
+        const unsigned cone_blocks = (unsigned)((num_var_cones_ * (size_t)var_lanes_ + kThreads - 1) / kThreads);
+        const unsigned grid = (unsigned)nbox + cone_blocks;
+        const int cnt = (int)num_var_cones_;
+        switch (var_lanes_) {
+        case 1: soc_x_diag_kernel<1><<<grid, kThreads, 0, stream_>>>(nbox, cnt, var_off_, Q_.rp, Q_.val, vec(V_w), ex); break;
+        case 2: soc_x_diag_kernel<2><<<grid, kThreads, 0, stream_>>>(nbox, cnt, var_off_, Q_.rp, Q_.val, vec(V_w), ex); break;
+        case 4: soc_x_diag_kernel<4><<<grid, kThreads, 0, stream_>>>(nbox, cnt, var_off_, Q_.rp, Q_.val, vec(V_w), ex); break;
+        case 8: soc_x_diag_kernel<8><<<grid, kThreads, 0, stream_>>>(nbox, cnt, var_off_, Q_.rp, Q_.val, vec(V_w), ex); break;
+        case 16: soc_x_diag_kernel<16><<<grid, kThreads, 0, stream_>>>(nbox, cnt, var_off_, Q_.rp, Q_.val, vec(V_w), ex); break;
+        case 32: soc_x_diag_kernel<32><<<grid, kThreads, 0, stream_>>>(nbox, cnt, var_off_, Q_.rp, Q_.val, vec(V_w), ex); break;
+        default: soc_x_diag_kernel<64><<<grid, kThreads, 0, stream_>>>(nbox, cnt, var_off_, Q_.rp, Q_.val, vec(V_w), ex); break;
+        }
+        if (profiling_) prof_end();
+    } else {
+        spmv(Q_, vec(V_w), ex, 0, T_ZXW1);
+    }
+    if (num_var_cones_ && !fused_x()) {
         if (profiling_) prof_begin(T_CONE_X);
         if (var_lanes_) {
             cone_step(var_lanes_, num_var_cones_, var_off_, vec(V_tempv), vec(V_x_bar), ConeFinX{fx});
@@ -2268,7 +2344,9 @@ double HipBackend::algorithmic_bytes(int tag) const
     const double diag = Q_is_diag_ ? 2.0 : 0.0;
     switch (tag) {
     case T_ZXW1:  // reads w(gather), w, ATy, c, l, u, x, last_x [+fact1,fact2]; writes Qw, x_bar, x_hat, x, w_bar
-        return 12.0 * nq + 4.0 * (n + 1) + 8.0 * n + 8.0 * n * (7.0 + diag + 5.0);
+        // (fused with the cone step: tempv, dx and z_bar of the cone range written too)
+        return 12.0 * nq + 4.0 * (n + 1) + 8.0 * n + 8.0 * n * (7.0 + diag + 5.0) +
+               (fused_x() ? 8.0 * (n - (double)lin_n_) * 3.0 + 4.0 * (num_var_cones_ + 1) : 0.0);
     case T_TEMPV:  // reads w_bar(gather), x_hat, Qw; writes tempv
         return 12.0 * nq + 4.0 * (n + 1) + 8.0 * n + 8.0 * n * 3.0;
     case T_Y:  // reads tempv(gather), y, last_y, AL, AU; writes y_bar, y
