@@ -610,22 +610,33 @@ __global__ __launch_bounds__(kThreads) void soc_cone_group_kernel(int count, con
 // Q holds at most its diagonal entry): Q w is elementwise there, so the
 // interval prefix (one thread per row, the first nbox blocks) and the cones
 // (L lanes per cone) each form their row's Q w and raw point themselves and
-// finish in one launch. (Q w)_i = 0 + q_ii w_i, the CSR row product's
-// rounding, so the bits are those of the product, cone and finish kernels.
+// finish in one launch, then the row's tempv for the y step (the product
+// Q w_bar, elementwise too). (Q v)_i = 0 + q_ii v_i, the CSR row product's
+// rounding, so the bits are those of the two products, the cone kernel and
+// the finish.
 template <int L>
 __global__ __launch_bounds__(kThreads) void soc_x_diag_kernel(int nbox, int count, const int *__restrict__ off,
                                                               const int *__restrict__ qrp,
                                                               const double *__restrict__ qval,
                                                               const double *__restrict__ w, EpiSocX ex)
 {
-    auto qw_of = [&](int i) {
+    auto q_times = [&](int i, double v) {
         double acc = 0.0;
-        if (qrp[i] < qrp[i + 1]) acc += qval[qrp[i]] * w[i];
+        if (qrp[i] < qrp[i + 1]) acc += qval[qrp[i]] * v;
         return acc;
+    };
+    // iteration.c:141-143 once the finish has written x_hat_i and w_bar_i
+    auto next_tempv = [&](int i, double qw) {
+        const double qwb = q_times(i, ex.fin.w_bar[i]);
+        ex.tempv[i] = ex.fin.x_hat[i] + ex.fin.sigma * (qw - qwb);
     };
     if ((int)blockIdx.x < nbox) {
         const int i = blockIdx.x * kThreads + threadIdx.x;
-        if (i < ex.lin_n) ex.row(i, qw_of(i), nullptr, nullptr);
+        if (i < ex.lin_n) {
+            const double qw = q_times(i, w[i]);
+            ex.row(i, qw, nullptr, nullptr);
+            next_tempv(i, qw);
+        }
         return;
     }
     const int k = (int)(((blockIdx.x - nbox) * (unsigned)kThreads + threadIdx.x) / L);
@@ -636,10 +647,10 @@ __global__ __launch_bounds__(kThreads) void soc_x_diag_kernel(int nbox, int coun
         len = off[k + 1] - b;
     }
     const bool act = j < len;
-    double v = 0.0;
+    double v = 0.0, qw = 0.0;
     if (act) {
         const int i = b + j;
-        const double qw = qw_of(i);
+        qw = q_times(i, w[i]);
         ex.Qw[i] = qw;
         v = ex.xv[i] + ex.fin.sigma * (-qw + ex.ATy[i] - ex.c[i]);
         ex.tempv[i] = v;
@@ -648,6 +659,7 @@ __global__ __launch_bounds__(kThreads) void soc_x_diag_kernel(int nbox, int coun
     if (act) {
         ex.x_bar[b + j] = xo;
         ex.fin.apply(b + j, xo);
+        next_tempv(b + j, qw);
     }
 }
 
@@ -1550,6 +1562,10 @@ HPRQPStatus HipBackend::autotune_panels()
         std::vector<int> cands = {1};
         if (forced == 1 || forced == 2 || forced == 4) {
             cands = {forced};
+        } else if (&M == &Q_ && q_diag_exact_) {
+            // at most the diagonal entry per row: the gather is a coalesced
+            // stream that no panel can shorten (and the cone models' fused
+            // x step wants one panel)
         } else if ((size_t)M.ncol * sizeof(double) > ((size_t)1 << 20) && M.nnz > 0) {
             if (M.ncol >= 2 * 32768) cands.push_back(2);
             if (M.ncol >= 4 * 32768) cands.push_back(4);
@@ -2063,7 +2079,8 @@ HPRQPStatus HipBackend::soc_iteration_step(size_t inner)
         }
         if (profiling_) prof_end();
     }
-    spmv(Q_, vec(V_w_bar), EpiTempv{{}, vec(V_tempv), vec(V_x_hat), vec(V_Qw), sigma_}, 0, T_TEMPV);
+    if (!fused_x())  // else the fused x step has written tempv already
+        spmv(Q_, vec(V_w_bar), EpiTempv{{}, vec(V_tempv), vec(V_x_hat), vec(V_Qw), sigma_}, 0, T_TEMPV);
     if (m_ > 0) {
         const double tau = sigma_ * lambda_A_;
         SocFinishY fy{vec(V_y), vec(V_y_bar), vec(V_dy), vec(V_s), vec(V_Rp_soc()), vec(V_last_y), tau, a, b};
@@ -2344,9 +2361,10 @@ double HipBackend::algorithmic_bytes(int tag) const
     const double diag = Q_is_diag_ ? 2.0 : 0.0;
     switch (tag) {
     case T_ZXW1:  // reads w(gather), w, ATy, c, l, u, x, last_x [+fact1,fact2]; writes Qw, x_bar, x_hat, x, w_bar
-        // (fused with the cone step: tempv, dx and z_bar of the cone range written too)
+        // (fused with the cone step and the tempv product: tempv, dx and z_bar
+        // of the cone range and the next tempv of every row written too)
         return 12.0 * nq + 4.0 * (n + 1) + 8.0 * n + 8.0 * n * (7.0 + diag + 5.0) +
-               (fused_x() ? 8.0 * (n - (double)lin_n_) * 3.0 + 4.0 * (num_var_cones_ + 1) : 0.0);
+               (fused_x() ? 8.0 * (n - (double)lin_n_) * 3.0 + 4.0 * (num_var_cones_ + 1) + 8.0 * n : 0.0);
     case T_TEMPV:  // reads w_bar(gather), x_hat, Qw; writes tempv
         return 12.0 * nq + 4.0 * (n + 1) + 8.0 * n + 8.0 * n * 3.0;
     case T_Y:  // reads tempv(gather), y, last_y, AL, AU; writes y_bar, y
