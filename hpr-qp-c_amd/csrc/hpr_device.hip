@@ -1167,7 +1167,13 @@ private:
     bool q_diag_exact_ = false;  // every row of Q holds at most its diagonal entry
     bool fused_x_off_ = std::getenv("HPRQP_CONE_LANES") && std::atoi(std::getenv("HPRQP_CONE_LANES")) == 0;
     // the cone x step runs as soc_x_diag_kernel (Q w, prefix, cones, finish)
-    bool fused_x() const { return num_var_cones_ && var_lanes_ && q_diag_exact_ && Qp_.P <= 1 && !fused_x_off_; }
+    // the cone step's x side as one launch (soc_x_diag_kernel): Q exactly
+    // diagonal, its cones (if any) tiled by lanes
+    bool fused_x() const
+    {
+        return (num_var_cones_ || num_row_cones_ || gram_) && (!num_var_cones_ || var_lanes_) && q_diag_exact_ &&
+               Qp_.P <= 1 && !fused_x_off_;
+    }
     template <class Fin>
     void cone_step(int lanes, size_t count, const int *off, const double *in, double *out, const Fin &fin);
     size_t num_var_cones_ = 0, num_row_cones_ = 0;
