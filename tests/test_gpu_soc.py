@@ -79,7 +79,11 @@ def test_soc_fused_cone_kernel_matches_thread_per_cone(H, monkeypatch, case):
     coalesced) against one thread per cone followed by the finish map
     (HPRQP_CONE_LANES=0): the same bits after every loop body, for variable
     cones (dimension 10), the mixed model (variable and row cones of several
-    sizes) and row cones of the grid-control model."""
+    sizes) and row cones of the grid-control model. The variable-cone and
+    grid-control models have an exactly diagonal Q (the latter with empty
+    rows too), so their default runs take the one-launch x step (Q w, the
+    projections, the finish and Q w_bar elementwise) against the separate
+    products of HPRQP_CONE_LANES=0; the mixed model's Q is not diagonal."""
     if case == "var_cones":
         qp = Q.soc_qp(seed=4, n_box=200, cones=40, cone_dim=10, m=300, a_row_nnz=20)
     elif case == "mixed":
