@@ -915,7 +915,7 @@ __global__ void fill_kernel(int n, double *__restrict__ v, double value)
 // steps have at most 2^kGoldenDepth - 1 candidate points (a binary tree,
 // node (l, j) at index 2^l - 1 + j, child (l + 1, 2j + b) for branch b: b = 0
 // when f1 > f2). One pass evaluates all of them -- a partial-sum kernel over
-// the variables, then a one-wavefront step kernel that adds the partials in
+// the variables, then a one-workgroup step kernel that adds the partials in
 // a fixed order, walks the tree along the branches the actual values take
 // (applying the search's own updates, so its points and values are those of
 // the one-probe-at-a-time search) and writes the next tree. A pass therefore
@@ -1042,19 +1042,22 @@ __global__ __launch_bounds__(kThreads) void golden_partials_kernel(const GoldenS
     block_reduce_store<kGoldenPoints, 0>(acc, nullptr, partials + (size_t)blockIdx.x * kGoldenPoints);
 }
 
-__global__ __launch_bounds__(kWave) void golden_step_kernel(GoldenState *__restrict__ st,
-                                                             const double *__restrict__ partials, int nblk)
+__global__ __launch_bounds__(kThreads) void golden_step_kernel(GoldenState *__restrict__ st,
+                                                                const double *__restrict__ partials, int nblk)
 {
     if (st->done) return;
-    // one wavefront adds each point's partials: lane-strided sums, then a fixed tree
+    // one wavefront per point adds that point's partials (wavefront w takes
+    // points w, w + 4, ...): lane-strided sums, then a fixed tree
     __shared__ double total[kGoldenPoints];
     const int np = st->npts;
-    for (int p = 0; p < np; ++p) {
+    const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+    for (int p = wv; p < np; p += kThreads / kWave) {
         double part = 0.0;
-        for (int k = threadIdx.x; k < nblk; k += kWave) part += partials[(size_t)k * kGoldenPoints + p];
+        for (int k = lane; k < nblk; k += kWave) part += partials[(size_t)k * kGoldenPoints + p];
         part = wave_sum(part);
-        if (threadIdx.x == 0) total[p] = part;
+        if (lane == 0) total[p] = part;
     }
+    __syncthreads();
     if (threadIdx.x != 0) return;
     GoldenState g = *st;
     auto f_of = [&](int p) {
@@ -2293,7 +2296,7 @@ HPRQPStatus HipBackend::update_sigma(HPRQPRestart *restart, const HPRQPResiduals
             for (int pass = 0; pass < kGoldenPasses; ++pass) {
                 golden_partials_kernel<<<gblk, kThreads, 0, stream_>>>(golden_, vec(V_diag_Q), vec(V_ATdy),
                                                                        vec(V_QATdy), (int)n_, golden_part_);
-                golden_step_kernel<<<1, kWave, 0, stream_>>>(golden_, golden_part_, gblk);
+                golden_step_kernel<<<1, kThreads, 0, stream_>>>(golden_, golden_part_, gblk);
             }
             HPRQP_HIP(hipStreamEndCapture(stream_, &graph));
             const hipError_t ie = hipGraphInstantiate(&golden_graph_, graph, nullptr, nullptr, 0);
